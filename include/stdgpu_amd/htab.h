@@ -1,0 +1,158 @@
+/*
+ * C ABI of the MI355X hashed-container library (libstdgpu_amd.so).
+ *
+ * Plain C: pointers, sizes and status codes only.  HIP stays inside the shared
+ * library; a stream is passed as an opaque `void*` (a hipStream_t, or NULL for the
+ * legacy default stream).  Every entry point returns 0 on success or a negative
+ * HTAB_E* code; htab_last_error() describes the last failure of the calling thread.
+ * Nothing prints, aborts or throws across this boundary.
+ *
+ * Each entry point names the reference interface it replaces (paths relative to the
+ * stdgpu source tree).  Bulk calls are asynchronous on the given stream; a caller that
+ * needs the reference's blocking semantics synchronises the stream (the C++ facade in
+ * include/stdgpu/unordered_{map,set}.cuh does).
+ */
+#ifndef STDGPU_AMD_HTAB_H
+#define STDGPU_AMD_HTAB_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define HTAB_OK 0
+#define HTAB_E_ARG (-1)     /* invalid argument (null handle, bad kind, size out of range) */
+#define HTAB_E_OOM (-2)     /* device allocation failed */
+#define HTAB_E_HIP (-3)     /* a HIP runtime call failed */
+#define HTAB_E_NODEV (-4)   /* no usable GPU */
+#define HTAB_E_LIMIT (-5)   /* request exceeds a documented maximum */
+
+/* container kinds: key type -> mapped type (value_type layout follows stdgpu::pair) */
+#define HTAB_SET_U32 0     /* unordered_set<uint32_t>             value_type  4 B */
+#define HTAB_SET_U64 1     /* unordered_set<uint64_t>             value_type  8 B */
+#define HTAB_MAP_U64_U32 2 /* unordered_map<uint64_t, uint32_t>   value_type 16 B (key, value, 4 B pad) */
+#define HTAB_MAP_U32_U32 3 /* unordered_map<uint32_t, uint32_t>   value_type  8 B */
+#define HTAB_MAP_U64_U64 4 /* unordered_map<uint64_t, uint64_t>   value_type 16 B */
+
+typedef struct htab htab; /* opaque table handle; owns all device memory of one table */
+
+/* Layout/launch knobs (all optional; zero fields take the defaults).  No reference
+ * counterpart: the reference sizes itself from the capacity alone. */
+typedef struct htab_tuning
+{
+    int32_t bucket_lanes;      /* lanes that probe one bucket together: 4, 8 or 16 (bucket = 16*lanes bytes); default 8 */
+    int32_t block_size;        /* threads per workgroup of the bulk kernels: 256, 512 or 1024; default 256 */
+    float slots_per_capacity;  /* physical slots = ceil(capacity * this); default 1.0 (load 0.8 at 0.8*capacity keys) */
+    int32_t reserved;
+} htab_tuning;
+
+/* Sizes a caller can query.  bucket_count / max_size follow the reference formulas so
+ * the logical API is unchanged; physical_slots is this library's own layout. */
+typedef struct htab_info
+{
+    int32_t kind;
+    int32_t bucket_lanes;
+    int64_t capacity;          /* as passed to htab_create */
+    int64_t bucket_count;      /* reference: bit_ceil(capacity)                   unordered_base_detail.cuh:1263-1266 */
+    int64_t max_size;          /* reference: bucket_count + excess_count          unordered_base_detail.cuh:1271-1273 */
+    int64_t physical_buckets;  /* buckets of 16*bucket_lanes bytes */
+    int64_t physical_slots;    /* physical_buckets * slots per bucket */
+    int64_t limit;             /* most keys the table accepts: min(max_size, physical_slots) */
+    int64_t value_bytes;       /* sizeof(value_type) */
+    int64_t device_bytes;      /* device memory owned by the table */
+} htab_info;
+
+/* Device view handed to user kernels by the C++ facade (native find/insert/erase). */
+typedef struct htab_view
+{
+    void* slots;               /* (physical_slots + 2) value_type entries, AoS */
+    int64_t* stripes;          /* size counter stripes, 16 int64 apart */
+    uint32_t* flags;           /* presence bits of the two reserved key values */
+    uint64_t nbuckets;         /* physical buckets */
+    uint64_t limit;            /* logical size limit */
+    uint32_t nstripes;         /* 1 when limit < physical_slots (exact limit), else 32 */
+    uint32_t bucket_lanes;
+    int64_t ref_bucket_count;  /* reference bucket_count(), for bucket() */
+    int64_t ref_max_size;      /* reference max_size() */
+} htab_view;
+
+/* unordered_map::createDeviceObject / unordered_set::createDeviceObject
+ * (unordered_map.cuh:106-120, unordered_set.cuh:95-109; engine unordered_base_detail.cuh:1254-1308) */
+int htab_create(int kind, int64_t capacity, const htab_tuning* tuning, htab** out);
+
+/* destroyDeviceObject (unordered_map.cuh:126-138; unordered_base_detail.cuh:1310-1348) */
+int htab_destroy(htab* table);
+
+/* Bulk insert of value_type entries (AoS, sizeof(value_type) each), never overwriting an
+ * existing key: insert(begin, end) (unordered_map.cuh:359-375; unordered_base_detail.cuh:996-1017).
+ * inserted[i] (optional) = 1 iff element i was inserted, the bool of device insert(). */
+int htab_insert(htab* table, const void* values, int64_t n, uint8_t* inserted, void* stream);
+
+/* Same as htab_insert with keys and mapped values in two arrays (vals is NULL for sets). */
+int htab_insert_kv(htab* table, const void* keys, const void* vals, int64_t n, uint8_t* inserted, void* stream);
+
+/* Bulk find: hits[i] = contains(keys[i]); vals_out[i] = find(keys[i])->second on a hit
+ * (untouched on a miss).  The reference has no bulk find; this is the device
+ * find/contains (unordered_map.cuh:288-335; unordered_base_detail.cuh:653-706) over a range. */
+int htab_find(htab* table, const void* keys, int64_t n, uint8_t* hits, void* vals_out, void* stream);
+
+/* Bulk erase: erase(begin, end) (unordered_map.cuh:390-406; unordered_base_detail.cuh:1045-1066).
+ * erased[i] (optional) = 1 iff keys[i] was removed by this call (device erase() == 1). */
+int htab_erase(htab* table, const void* keys, int64_t n, uint8_t* erased, void* stream);
+
+/* size() (unordered_base_detail.cuh:1110-1132): synchronises the stream, then reads the count. */
+int htab_size(htab* table, int64_t* out, void* stream);
+
+/* clear() (unordered_base_detail.cuh:1220-1252), as device work on the stream. */
+int htab_clear(htab* table, void* stream);
+
+/* device_range() (unordered_base_detail.cuh:160-208): writes the slot indices of all stored
+ * entries, in ascending slot order, to idx_out (capacity `cap` entries, device memory) and
+ * their number to *n_out.  Synchronises the stream.  Entry i is view.slots[idx_out[i]]. */
+int htab_range(htab* table, int64_t* idx_out, int64_t cap, int64_t* n_out, void* stream);
+
+/* Contents as two dense arrays (keys, mapped values; vals_out NULL for sets) in slot order,
+ * the data the reference's device_range() exposes.  Synchronises the stream. */
+int htab_contents(htab* table, void* keys_out, void* vals_out, int64_t cap, int64_t* n_out, void* stream);
+
+/* valid() (unordered_base_detail.cuh:1192-1218) for this layout: every stored key is found at
+ * its own slot, no key is stored twice, and the size counter equals the number of stored
+ * entries.  *out = 1 if valid.  Synchronises the stream. */
+int htab_valid(htab* table, int32_t* out, void* stream);
+
+/* bucket(key) (unordered_base_detail.cuh:547-564): the reference's Fibonacci bucket over
+ * bucket_count(), so callers that assert on bucket numbers keep working. */
+int htab_bucket(const htab* table, uint64_t key, int64_t* out);
+
+int htab_get_info(const htab* table, htab_info* out);
+int htab_get_view(const htab* table, htab_view* out);
+
+/* Message for the last non-zero status returned to the calling thread. */
+const char* htab_last_error(void);
+
+/* ------------------------------------------------------------------------------------------
+ * Free-list deque<uint32_t> (deque.cuh; deque_detail.cuh:202-260 push_back, 390-448 pop_front),
+ * used by voxel-block pipelines to hand out slot numbers.  Pushes and pops are aggregated to one
+ * atomic per wavefront.  Bulk calls only; size follows the reference's clamped size().
+ * ------------------------------------------------------------------------------------------ */
+typedef struct htab_deque htab_deque;
+
+int htab_deque_create(int64_t capacity, htab_deque** out);
+int htab_deque_destroy(htab_deque* q);
+/* push_back of n values; pushed[i] (optional) = 1 iff value i was stored (0 when full). */
+int htab_deque_push_back(htab_deque* q, const uint32_t* values, int64_t n, uint8_t* pushed, void* stream);
+/* pop_front of n values; popped[i] = 1 iff out[i] received a value (0 when empty). */
+int htab_deque_pop_front(htab_deque* q, uint32_t* out, int64_t n, uint8_t* popped, void* stream);
+/* Fill with 0, 1, ..., capacity-1 (front to back). */
+int htab_deque_fill_iota(htab_deque* q, void* stream);
+int htab_deque_size(htab_deque* q, int64_t* out, void* stream);
+/* Copies the stored values front to back into out (capacity entries); *n_out = size. */
+int htab_deque_contents(htab_deque* q, uint32_t* out, int64_t* n_out, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* STDGPU_AMD_HTAB_H */

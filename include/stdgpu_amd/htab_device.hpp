@@ -1,0 +1,328 @@
+// Device-side layout and probing primitives shared by the bulk kernels
+// (stdgpu_amd/csrc/htab.hip) and the native per-thread API of the C++ facade
+// (include/stdgpu/unordered_{map,set}.cuh).  gfx950 only: wave64, 16-byte lanes.
+//
+// Layout (replaces the reference's _values/_offsets/_occupied/_locks/excess arrays,
+// src/stdgpu/impl/unordered_base.cuh:526-554):
+//   * nbuckets buckets of 16*G bytes; a bucket holds W = 16*G/sizeof(value_type) slots
+//     of value_type stored AoS, so a probe of one bucket is a single coalesced load by G
+//     lanes of a wavefront (16 bytes per lane) and an iterator is still value_type*.
+//   * Occupancy lives in the key field: all-ones = EMPTY (never used), all-ones minus one
+//     = TOMB (erased).  Those two key values are still storable: they live in two side
+//     slots after the buckets, with a presence bit each in `flags`.
+//   * Collisions overflow into the next bucket (linear probing over buckets, wrapping).
+//     A probe sequence ends at the first bucket that holds an EMPTY slot.
+//   * Home bucket = (hi32(mix64(key)) * nbuckets) >> 32, mix64 = MurmurHash3's fmix64.
+#ifndef STDGPU_AMD_HTAB_DEVICE_HPP
+#define STDGPU_AMD_HTAB_DEVICE_HPP
+
+#include <hip/hip_runtime.h>
+
+#include <stdint.h>
+
+#include <stdgpu_amd/htab.h>
+
+namespace htab_dev
+{
+
+constexpr int kWave = 64;
+constexpr int kStripeStride = 16; // int64 elements between size stripes (128 B)
+constexpr uint32_t kMaxStripes = 32;
+
+// ------------------------------------------------------------------ slot types
+template <typename K, typename M>
+struct alignas(sizeof(K) > sizeof(M) ? sizeof(K) : sizeof(M)) kv_slot
+{
+    K first;
+    M second;
+};
+
+template <typename K, typename M>
+struct slot_traits
+{
+    using key_type = K;
+    using mapped_type = M;
+    using value_type = kv_slot<K, M>;
+    static constexpr bool is_map = true;
+    __host__ __device__ static K key(const value_type& v) { return v.first; }
+};
+
+template <typename K>
+struct slot_traits<K, void>
+{
+    using key_type = K;
+    using mapped_type = void;
+    using value_type = K;
+    static constexpr bool is_map = false;
+    __host__ __device__ static K key(const value_type& v) { return v; }
+};
+
+template <typename K>
+__host__ __device__ constexpr K
+empty_key()
+{
+    return static_cast<K>(~static_cast<K>(0));
+}
+
+template <typename K>
+__host__ __device__ constexpr K
+tomb_key()
+{
+    return static_cast<K>(~static_cast<K>(0) - 1);
+}
+
+// ------------------------------------------------------------------ hashing
+__host__ __device__ inline uint64_t
+mix64(uint64_t x)
+{
+    x ^= x >> 33;
+    x *= 0xff51afd7ed558ccdULL;
+    x ^= x >> 33;
+    x *= 0xc4ceb9fe1a85ec53ULL;
+    x ^= x >> 33;
+    return x;
+}
+
+__host__ __device__ inline uint64_t
+home_bucket(uint64_t key, uint64_t nbuckets)
+{
+    const uint64_t h = mix64(key) >> 32;
+    return (h * nbuckets) >> 32;
+}
+
+__host__ __device__ inline uint64_t
+next_bucket(uint64_t b, uint64_t nbuckets)
+{
+    return (b + 1 == nbuckets) ? 0 : b + 1;
+}
+
+// The reference's bucket(): Fibonacci hashing of the identity hash over bucket_count()
+// (src/stdgpu/impl/unordered_base_detail.cuh:56-85, functional_detail.h:60-80).
+__host__ __device__ inline int64_t
+reference_bucket(uint64_t hash, int64_t bucket_count)
+{
+    int w = 0;
+    for (uint64_t x = static_cast<uint64_t>(bucket_count); x != 0; x >>= 1) ++w;
+    w -= 1;
+    if (w <= 0) return 0;
+    const unsigned dropped = static_cast<unsigned>(64 - w);
+    const uint64_t improved = hash ^ (hash >> dropped);
+    return static_cast<int64_t>((11400714819323198485ULL * improved) >> dropped);
+}
+
+// ------------------------------------------------------------------ coherent accessors
+// Probes of insert/erase read keys at agent scope (bypassing the non-coherent vector L1),
+// so a compare-and-swap that lost a race is followed by a fresh view of the slot.
+template <typename T>
+__device__ inline T
+load_agent(const T* p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <typename T>
+__device__ inline T
+cas_agent(T* p, T expected, T desired)
+{
+    __hip_atomic_compare_exchange_strong(p, &expected, desired, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                         __HIP_MEMORY_SCOPE_AGENT);
+    return expected; // value seen by the CAS (== the old expected on success)
+}
+
+__device__ inline int64_t
+stripe_sum(const int64_t* stripes, uint32_t nstripes)
+{
+    int64_t s = 0;
+    for (uint32_t i = 0; i < nstripes; ++i) s += load_agent(stripes + static_cast<uint64_t>(i) * kStripeStride);
+    return s;
+}
+
+// ------------------------------------------------------------------ typed view
+template <typename K, typename M>
+struct table_ref
+{
+    using traits = slot_traits<K, M>;
+    using value_type = typename traits::value_type;
+    static constexpr int SB = static_cast<int>(sizeof(value_type));
+
+    value_type* slots;
+    int64_t* stripes;
+    uint32_t* flags;
+    uint64_t nbuckets;
+    uint64_t limit;
+    uint32_t nstripes;
+    uint32_t W; // slots per bucket
+
+    __host__ __device__ static table_ref from_view(const htab_view& v)
+    {
+        table_ref t;
+        t.slots = static_cast<value_type*>(v.slots);
+        t.stripes = v.stripes;
+        t.flags = v.flags;
+        t.nbuckets = v.nbuckets;
+        t.limit = v.limit;
+        t.nstripes = v.nstripes;
+        t.W = 16U * v.bucket_lanes / static_cast<uint32_t>(SB);
+        return t;
+    }
+
+    __host__ __device__ uint64_t nslots() const { return nbuckets * W; }
+    __host__ __device__ value_type* end_ptr() const { return slots + nslots() + 2; }
+    __host__ __device__ static bool is_reserved(K k) { return k == empty_key<K>() || k == tomb_key<K>(); }
+    __host__ __device__ uint64_t side_index(K k) const { return nslots() + (k == empty_key<K>() ? 0 : 1); }
+    __host__ __device__ static uint32_t side_bit(K k) { return k == empty_key<K>() ? 1U : 2U; }
+    __device__ K* key_ptr(uint64_t slot) const { return reinterpret_cast<K*>(slots + slot); }
+
+    __device__ int64_t size() const { return stripe_sum(stripes, nstripes); }
+
+    // ---- native per-thread operations (user kernels; the facade's find/insert/erase) ----
+
+    // find (unordered_base_detail.cuh:653-682): pointer to the entry or end_ptr().
+    __device__ value_type* find(K key) const
+    {
+        if (is_reserved(key))
+        {
+            return (load_agent(flags) & side_bit(key)) ? slots + side_index(key) : end_ptr();
+        }
+        uint64_t b = home_bucket(static_cast<uint64_t>(key), nbuckets);
+        for (uint64_t step = 0; step < nbuckets; ++step)
+        {
+            bool saw_empty = false;
+            value_type* base = slots + b * W;
+            for (uint32_t j = 0; j < W; ++j)
+            {
+                const K k = load_agent(key_ptr(b * W + j));
+                if (k == key) return base + j;
+                saw_empty |= (k == empty_key<K>());
+            }
+            if (saw_empty) break;
+            b = next_bucket(b, nbuckets);
+        }
+        return end_ptr();
+    }
+
+    __device__ bool count_add(int64_t delta) const
+    {
+        // returns false if the exact limit would be exceeded (only tracked when nstripes == 1)
+        const uint32_t s = nstripes == 1 ? 0U : (static_cast<uint32_t>(blockIdx.x) & (nstripes - 1));
+        const int64_t old = __hip_atomic_fetch_add(stripes + static_cast<uint64_t>(s) * kStripeStride, delta,
+                                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (nstripes == 1 && delta > 0 && old + delta > static_cast<int64_t>(limit))
+        {
+            __hip_atomic_fetch_add(stripes, -delta, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            return false;
+        }
+        return true;
+    }
+
+    // insert (unordered_base_detail.cuh:708-809, 967-994): never overwrites; claims only EMPTY
+    // slots, so it stays duplicate-free when it runs concurrently with erase() in one kernel.
+    // Returns {slot pointer, true} on insertion, {end_ptr(), false} otherwise.
+    template <typename WriteValue>
+    __device__ value_type* insert(K key, const WriteValue& write_value, bool* inserted) const
+    {
+        *inserted = false;
+        if (nstripes == 1 && load_agent(stripes) >= static_cast<int64_t>(limit)) return end_ptr();
+        if (is_reserved(key))
+        {
+            const uint32_t bit = side_bit(key);
+            const uint32_t old = __hip_atomic_fetch_or(flags, bit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (old & bit) return end_ptr();
+            if (!count_add(1))
+            {
+                __hip_atomic_fetch_and(flags, ~bit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                return end_ptr();
+            }
+            value_type* p = slots + side_index(key);
+            write_value(p);
+            *inserted = true;
+            return p;
+        }
+        uint64_t b = home_bucket(static_cast<uint64_t>(key), nbuckets);
+        uint32_t j0 = 0;
+        for (uint64_t step = 0; step <= nbuckets;)
+        {
+            bool advanced = true;
+            for (uint32_t j = j0; j < W; ++j)
+            {
+                const uint64_t s = b * W + j;
+                K k = load_agent(key_ptr(s));
+                if (k == key) return end_ptr();
+                if (k == empty_key<K>())
+                {
+                    const K seen = cas_agent(key_ptr(s), empty_key<K>(), key);
+                    if (seen == empty_key<K>())
+                    {
+                        if (!count_add(1))
+                        {
+                            __hip_atomic_store(key_ptr(s), tomb_key<K>(), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                            return end_ptr();
+                        }
+                        write_value(slots + s);
+                        *inserted = true;
+                        return slots + s;
+                    }
+                    if (seen == key) return end_ptr();
+                    // lost the slot to another key: keep scanning from the next slot
+                }
+            }
+            if (advanced)
+            {
+                b = next_bucket(b, nbuckets);
+                j0 = 0;
+                ++step;
+            }
+        }
+        return end_ptr();
+    }
+
+    // erase (unordered_base_detail.cuh:811-913, 1019-1043): marks the slot TOMB; exactly one of
+    // several concurrent erasers of one key succeeds.
+    __device__ bool erase(K key) const
+    {
+        if (is_reserved(key))
+        {
+            const uint32_t bit = side_bit(key);
+            const uint32_t old = __hip_atomic_fetch_and(flags, ~bit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (!(old & bit)) return false;
+            count_add(-1);
+            return true;
+        }
+        uint64_t b = home_bucket(static_cast<uint64_t>(key), nbuckets);
+        for (uint64_t step = 0; step < nbuckets; ++step)
+        {
+            bool saw_empty = false;
+            for (uint32_t j = 0; j < W; ++j)
+            {
+                const uint64_t s = b * W + j;
+                const K k = load_agent(key_ptr(s));
+                if (k == key)
+                {
+                    if (cas_agent(key_ptr(s), key, tomb_key<K>()) != key) return false;
+                    count_add(-1);
+                    return true;
+                }
+                saw_empty |= (k == empty_key<K>());
+            }
+            if (saw_empty) break;
+            b = next_bucket(b, nbuckets);
+        }
+        return false;
+    }
+
+    // bucket_size(n) analogue for this layout: stored entries in physical bucket n.
+    __device__ int32_t physical_bucket_size(uint64_t n) const
+    {
+        int32_t c = 0;
+        for (uint32_t j = 0; j < W; ++j)
+        {
+            const K k = load_agent(key_ptr(n * W + j));
+            c += (k != empty_key<K>() && k != tomb_key<K>()) ? 1 : 0;
+        }
+        return c;
+    }
+};
+
+} // namespace htab_dev
+
+#endif // STDGPU_AMD_HTAB_DEVICE_HPP

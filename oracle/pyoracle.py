@@ -1,0 +1,187 @@
+"""ctypes binding of oracle/liboracle.so, the CPU restatement of stdgpu's hash-table engine
+(oracle/htab_oracle.cpp).  Test infrastructure only: the parity checker and the timed CPU
+baseline; the product never calls it.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+KIND = {("u32", None): 0, ("u64", None): 1, ("u64", "u32"): 2, ("u32", "u32"): 3, ("u64", "u64"): 4}
+NP = {"u32": np.uint32, "u64": np.uint64}
+
+_lib = None
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        subprocess.run(["make", "-s", "-C", HERE], check=True)
+    lib = ctypes.CDLL(LIB_PATH)
+    P, I64, I32, U64 = ctypes.c_void_p, ctypes.c_int64, ctypes.c_int, ctypes.c_uint64
+    sig = {
+        "orc_create": (P, [I32, I64]),
+        "orc_destroy": (None, [P]),
+        "orc_insert": (None, [P, P, P, I64, P, I32]),
+        "orc_find": (None, [P, P, I64, P, P, I32]),
+        "orc_erase": (None, [P, P, I64, P, I32]),
+        "orc_size": (I64, [P]),
+        "orc_contents": (I64, [P, P, P]),
+        "orc_valid": (I32, [P]),
+        "orc_clear": (None, [P]),
+        "orc_bucket_count": (I64, [P]),
+        "orc_max_size": (I64, [P]),
+        "orc_excess_free": (I64, [P]),
+        "orc_bucket": (I64, [P, U64]),
+        "orc_fibonacci_hashing": (I64, [U64, I64]),
+        "orc_sizing": (None, [I64, ctypes.POINTER(I64), ctypes.POINTER(I64)]),
+        "orc_deque_create": (P, [I64]),
+        "orc_deque_destroy": (None, [P]),
+        "orc_deque_size": (I64, [P]),
+        "orc_deque_push_back": (None, [P, P, I64, P, I32]),
+        "orc_deque_pop_front": (None, [P, P, I64, P, I32]),
+        "orc_deque_contents": (I64, [P, P]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def fibonacci_hashing(h: int, bucket_count: int) -> int:
+    return load().orc_fibonacci_hashing(ctypes.c_uint64(h & 0xFFFFFFFFFFFFFFFF), bucket_count)
+
+
+def sizing(capacity: int):
+    bc, ex = ctypes.c_int64(), ctypes.c_int64()
+    load().orc_sizing(capacity, ctypes.byref(bc), ctypes.byref(ex))
+    return bc.value, ex.value
+
+
+class Table:
+    """unordered_set<K> / unordered_map<K, M> on the CPU, reference algorithm."""
+
+    def __init__(self, key: str, mapped: str | None, capacity: int):
+        self.key, self.mapped = key, mapped
+        self._t = load().orc_create(KIND[(key, mapped)], capacity)
+        if not self._t:
+            raise ValueError("oracle: bad kind or capacity")
+
+    def close(self):
+        if self._t:
+            load().orc_destroy(self._t)
+            self._t = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def insert(self, keys, vals=None, threads: int = 1):
+        keys = np.ascontiguousarray(keys, dtype=NP[self.key])
+        if self.mapped is not None:
+            vals = np.ascontiguousarray(vals, dtype=NP[self.mapped])
+        ok = np.zeros(len(keys), dtype=np.uint8)
+        load().orc_insert(self._t, _p(keys), _p(vals), len(keys), _p(ok), threads)
+        return ok
+
+    def find(self, keys, threads: int = 1):
+        keys = np.ascontiguousarray(keys, dtype=NP[self.key])
+        hits = np.zeros(len(keys), dtype=np.uint8)
+        vals = None if self.mapped is None else np.zeros(len(keys), dtype=NP[self.mapped])
+        load().orc_find(self._t, _p(keys), len(keys), _p(hits), _p(vals), threads)
+        return hits, vals
+
+    def erase(self, keys, threads: int = 1):
+        keys = np.ascontiguousarray(keys, dtype=NP[self.key])
+        ok = np.zeros(len(keys), dtype=np.uint8)
+        load().orc_erase(self._t, _p(keys), len(keys), _p(ok), threads)
+        return ok
+
+    def size(self) -> int:
+        return load().orc_size(self._t)
+
+    def contents(self):
+        n = self.size()
+        keys = np.zeros(max(n, 1), dtype=NP[self.key])
+        vals = None if self.mapped is None else np.zeros(max(n, 1), dtype=NP[self.mapped])
+        c = load().orc_contents(self._t, _p(keys), _p(vals))
+        assert c == n, (c, n)
+        return keys[:c], (None if vals is None else vals[:c])
+
+    def sorted_contents(self):
+        k, v = self.contents()
+        o = np.argsort(k, kind="stable")
+        return k[o], (None if v is None else v[o])
+
+    def valid(self) -> bool:
+        return bool(load().orc_valid(self._t))
+
+    def clear(self):
+        load().orc_clear(self._t)
+
+    def bucket_count(self) -> int:
+        return load().orc_bucket_count(self._t)
+
+    def max_size(self) -> int:
+        return load().orc_max_size(self._t)
+
+    def excess_free(self) -> int:
+        return load().orc_excess_free(self._t)
+
+    def bucket(self, key: int) -> int:
+        return load().orc_bucket(self._t, ctypes.c_uint64(key & 0xFFFFFFFFFFFFFFFF))
+
+
+class Deque:
+    """deque<uint32_t> on the CPU, reference algorithm (push_back / pop_front)."""
+
+    def __init__(self, capacity: int):
+        self._d = load().orc_deque_create(capacity)
+        self.capacity = capacity
+
+    def close(self):
+        if self._d:
+            load().orc_deque_destroy(self._d)
+            self._d = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def push_back(self, values, threads: int = 1):
+        v = np.ascontiguousarray(values, dtype=np.uint32)
+        ok = np.zeros(len(v), dtype=np.uint8)
+        load().orc_deque_push_back(self._d, _p(v), len(v), _p(ok), threads)
+        return ok
+
+    def pop_front(self, n: int, threads: int = 1):
+        out = np.zeros(n, dtype=np.uint32)
+        ok = np.zeros(n, dtype=np.uint8)
+        load().orc_deque_pop_front(self._d, _p(out), n, _p(ok), threads)
+        return out, ok
+
+    def size(self) -> int:
+        return load().orc_deque_size(self._d)
+
+    def contents(self):
+        out = np.zeros(max(self.capacity, 1), dtype=np.uint32)
+        n = load().orc_deque_contents(self._d, _p(out))
+        return out[:n]

@@ -1,0 +1,125 @@
+"""ctypes binding of libstdgpu_amd.so (the C ABI declared in include/stdgpu_amd/htab.h).
+
+The library is built in-tree by `python -m stdgpu_amd.build`.  Loading fails loudly when it
+is missing: there is no CPU fallback for any operation.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libstdgpu_amd.so")
+
+HTAB_OK = 0
+HTAB_E_ARG = -1
+HTAB_E_OOM = -2
+HTAB_E_HIP = -3
+HTAB_E_NODEV = -4
+HTAB_E_LIMIT = -5
+
+HTAB_SET_U32 = 0
+HTAB_SET_U64 = 1
+HTAB_MAP_U64_U32 = 2
+HTAB_MAP_U32_U32 = 3
+HTAB_MAP_U64_U64 = 4
+
+
+class htab_tuning(ctypes.Structure):
+    _fields_ = [
+        ("bucket_lanes", ctypes.c_int32),
+        ("block_size", ctypes.c_int32),
+        ("slots_per_capacity", ctypes.c_float),
+        ("reserved", ctypes.c_int32),
+    ]
+
+
+class htab_info(ctypes.Structure):
+    _fields_ = [
+        ("kind", ctypes.c_int32),
+        ("bucket_lanes", ctypes.c_int32),
+        ("capacity", ctypes.c_int64),
+        ("bucket_count", ctypes.c_int64),
+        ("max_size", ctypes.c_int64),
+        ("physical_buckets", ctypes.c_int64),
+        ("physical_slots", ctypes.c_int64),
+        ("limit", ctypes.c_int64),
+        ("value_bytes", ctypes.c_int64),
+        ("device_bytes", ctypes.c_int64),
+    ]
+
+
+class htab_view(ctypes.Structure):
+    _fields_ = [
+        ("slots", ctypes.c_void_p),
+        ("stripes", ctypes.c_void_p),
+        ("flags", ctypes.c_void_p),
+        ("nbuckets", ctypes.c_uint64),
+        ("limit", ctypes.c_uint64),
+        ("nstripes", ctypes.c_uint32),
+        ("bucket_lanes", ctypes.c_uint32),
+        ("ref_bucket_count", ctypes.c_int64),
+        ("ref_max_size", ctypes.c_int64),
+    ]
+
+
+_P = ctypes.c_void_p
+_I64 = ctypes.c_int64
+_I32 = ctypes.c_int
+
+# name -> (restype, argtypes); every function declared in htab.h
+SIGNATURES = {
+    "htab_create": (_I32, [_I32, _I64, ctypes.POINTER(htab_tuning), ctypes.POINTER(_P)]),
+    "htab_destroy": (_I32, [_P]),
+    "htab_insert": (_I32, [_P, _P, _I64, _P, _P]),
+    "htab_insert_kv": (_I32, [_P, _P, _P, _I64, _P, _P]),
+    "htab_find": (_I32, [_P, _P, _I64, _P, _P, _P]),
+    "htab_erase": (_I32, [_P, _P, _I64, _P, _P]),
+    "htab_size": (_I32, [_P, ctypes.POINTER(_I64), _P]),
+    "htab_clear": (_I32, [_P, _P]),
+    "htab_range": (_I32, [_P, _P, _I64, ctypes.POINTER(_I64), _P]),
+    "htab_contents": (_I32, [_P, _P, _P, _I64, ctypes.POINTER(_I64), _P]),
+    "htab_valid": (_I32, [_P, ctypes.POINTER(ctypes.c_int32), _P]),
+    "htab_bucket": (_I32, [_P, ctypes.c_uint64, ctypes.POINTER(_I64)]),
+    "htab_get_info": (_I32, [_P, ctypes.POINTER(htab_info)]),
+    "htab_get_view": (_I32, [_P, ctypes.POINTER(htab_view)]),
+    "htab_last_error": (ctypes.c_char_p, []),
+    "htab_deque_create": (_I32, [_I64, ctypes.POINTER(_P)]),
+    "htab_deque_destroy": (_I32, [_P]),
+    "htab_deque_push_back": (_I32, [_P, _P, _I64, _P, _P]),
+    "htab_deque_pop_front": (_I32, [_P, _P, _I64, _P, _P]),
+    "htab_deque_fill_iota": (_I32, [_P, _P]),
+    "htab_deque_size": (_I32, [_P, ctypes.POINTER(_I64), _P]),
+    "htab_deque_contents": (_I32, [_P, _P, ctypes.POINTER(_I64), _P]),
+}
+
+_lib = None
+
+
+def load(path: str = LIB_PATH) -> ctypes.CDLL:
+    """Load the HIP library once; raise if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise ImportError(
+            f"{path} is missing: build it with `python -m stdgpu_amd.build` (hipcc, gfx950). "
+            "There is no CPU fallback."
+        )
+    lib = ctypes.CDLL(path)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+class HtabError(RuntimeError):
+    pass
+
+
+def check(rc: int, what: str) -> None:
+    if rc != HTAB_OK:
+        msg = load().htab_last_error()
+        raise HtabError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")

@@ -1,0 +1,71 @@
+"""Build step for the HIP library (libstdgpu_amd.so) and the CPU oracle.
+
+The library is compiled in-tree for gfx950 with hipcc; the oracle (test infrastructure)
+with g++ through oracle/Makefile.  `python -m stdgpu_amd.build` runs both.
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+PKG = os.path.join(ROOT, "stdgpu_amd")
+CSRC = os.path.join(PKG, "csrc")
+LIB = os.path.join(PKG, "libstdgpu_amd.so")
+SOURCES = ["htab.hip", "deque.hip"]
+HEADERS = [
+    os.path.join(ROOT, "include", "stdgpu_amd", "htab.h"),
+    os.path.join(ROOT, "include", "stdgpu_amd", "htab_device.hpp"),
+    os.path.join(CSRC, "htab_kernels.hpp"),
+    os.path.join(CSRC, "wave.hpp"),
+]
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = "gfx950"
+
+# -packed-fp32-ops: no packed FP32 instructions are generated (none are needed here).
+HIP_FLAGS = [
+    f"--offload-arch={ARCH}",
+    "-O3",
+    "-std=c++17",
+    "-fPIC",
+    "-Wall",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-I", os.path.join(ROOT, "include"),
+]
+
+
+def _run(cmd: list[str], cwd: str | None = None) -> None:
+    r = subprocess.run(cmd, cwd=cwd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout)
+        raise RuntimeError(f"build step failed ({r.returncode}): {' '.join(cmd)}")
+
+
+def build_library(out: str = LIB) -> str:
+    objs = []
+    build_dir = os.path.join(PKG, "_build")
+    os.makedirs(build_dir, exist_ok=True)
+    for src in SOURCES:
+        obj = os.path.join(build_dir, src.replace(".hip", ".o"))
+        _run([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj])
+        objs.append(obj)
+    tmp = out + ".tmp"
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs])
+    os.replace(tmp, out)
+    return out
+
+
+def build_oracle() -> str:
+    _run(["make", "-s", "-C", os.path.join(ROOT, "oracle")])
+    return os.path.join(ROOT, "oracle", "liboracle.so")
+
+
+def main() -> None:
+    print("library:", build_library())
+    print("oracle :", build_oracle())
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,286 @@
+"""Host-side mirror of stdgpu's unordered_map / unordered_set (Python), over the C ABI.
+
+Names, argument meaning and return values follow the reference's host API
+(src/stdgpu/unordered_map.cuh:70-538, src/stdgpu/unordered_set.cuh:61-525):
+createDeviceObject / destroyDeviceObject, bulk insert(begin, end) / erase(begin, end),
+size / empty / full / max_size / bucket_count / load_factor / max_load_factor, bucket(key),
+device_range(), clear(), valid().  The reference has no bulk find; find() / contains() here
+are its device find/contains applied to a range of keys.
+
+Device buffers are torch tensors on the GPU (PyTorch is used for device memory and streams
+only).  Unsigned keys travel in signed tensors of the same width (int32 for uint32_t, int64
+for uint64_t); the bits are unchanged.  Calls are asynchronous on the current torch stream
+unless a `stream` is given; size()/device_range()/valid() synchronise, like the reference's
+host-side size() (a device-to-host copy).
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+
+from . import _lib
+from ._lib import check
+
+_KINDS = {
+    ("u32", None): _lib.HTAB_SET_U32,
+    ("u64", None): _lib.HTAB_SET_U64,
+    ("u64", "u32"): _lib.HTAB_MAP_U64_U32,
+    ("u32", "u32"): _lib.HTAB_MAP_U32_U32,
+    ("u64", "u64"): _lib.HTAB_MAP_U64_U64,
+}
+
+_TORCH_DT = {"u32": "int32", "u64": "int64"}
+_BYTES = {"u32": 4, "u64": 8}
+
+
+def _torch():
+    import torch
+
+    return torch
+
+
+def _stream_ptr(stream) -> int:
+    torch = _torch()
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return int(s.cuda_stream)
+
+
+def _ptr(t) -> int | None:
+    return None if t is None else int(t.data_ptr())
+
+
+@dataclass
+class Tuning:
+    bucket_lanes: int = 8
+    block_size: int = 256
+    slots_per_capacity: float = 1.0
+
+    def to_c(self) -> _lib.htab_tuning:
+        return _lib.htab_tuning(self.bucket_lanes, self.block_size, self.slots_per_capacity, 0)
+
+
+class _unordered_base:
+    key: str = "u64"
+    mapped: str | None = None
+
+    def __init__(self, handle: int, capacity: int):
+        self._h = ctypes.c_void_p(handle)
+        self._capacity = capacity
+        info = _lib.htab_info()
+        check(_lib.load().htab_get_info(self._h, ctypes.byref(info)), "htab_get_info")
+        self.info = info
+
+    # ---- lifetime (unordered_base_detail.cuh:1254-1348) ----
+    @classmethod
+    def createDeviceObject(cls, capacity: int, tuning: Tuning | None = None):
+        lib = _lib.load()
+        kind = _KINDS[(cls.key, cls.mapped)]
+        h = ctypes.c_void_p()
+        tc = (tuning or Tuning()).to_c()
+        check(lib.htab_create(kind, int(capacity), ctypes.byref(tc), ctypes.byref(h)), "htab_create")
+        return cls(h.value, capacity)
+
+    @staticmethod
+    def destroyDeviceObject(obj) -> None:
+        if obj._h:
+            check(_lib.load().htab_destroy(obj._h), "htab_destroy")
+            obj._h = ctypes.c_void_p()
+
+    # ---- dtype helpers ----
+    @classmethod
+    def key_dtype(cls):
+        return getattr(_torch(), _TORCH_DT[cls.key])
+
+    @classmethod
+    def mapped_dtype(cls):
+        return None if cls.mapped is None else getattr(_torch(), _TORCH_DT[cls.mapped])
+
+    def _check_keys(self, keys):
+        if keys.dtype != self.key_dtype() or not keys.is_cuda or not keys.is_contiguous():
+            raise TypeError(f"keys must be a contiguous CUDA tensor of {self.key_dtype()}")
+
+    # ---- bulk operations ----
+    def insert(self, keys, values=None, inserted=None, stream=None):
+        """insert(begin, end): never overwrites; `inserted` (uint8, optional) receives the
+        per-element bool of device insert()."""
+        self._check_keys(keys)
+        n = keys.numel()
+        if self.mapped is not None:
+            if values is None or values.numel() != n or values.dtype != self.mapped_dtype():
+                raise TypeError("values must match keys and the mapped type")
+        check(
+            _lib.load().htab_insert_kv(self._h, _ptr(keys), _ptr(values), n, _ptr(inserted), _stream_ptr(stream)),
+            "htab_insert_kv",
+        )
+        return inserted
+
+    def insert_values(self, values_aos, n: int, inserted=None, stream=None):
+        """insert(begin, end) over value_type entries (AoS bytes, e.g. a uint8 tensor)."""
+        check(
+            _lib.load().htab_insert(self._h, _ptr(values_aos), int(n), _ptr(inserted), _stream_ptr(stream)),
+            "htab_insert",
+        )
+        return inserted
+
+    def find(self, keys, hits=None, values_out=None, stream=None):
+        """contains()/find() over a range: returns (hits uint8, values or None)."""
+        torch = _torch()
+        self._check_keys(keys)
+        n = keys.numel()
+        if hits is None:
+            hits = torch.empty(n, dtype=torch.uint8, device=keys.device)
+        if self.mapped is not None and values_out is None:
+            values_out = torch.zeros(n, dtype=self.mapped_dtype(), device=keys.device)
+        check(
+            _lib.load().htab_find(self._h, _ptr(keys), n, _ptr(hits), _ptr(values_out), _stream_ptr(stream)),
+            "htab_find",
+        )
+        return hits, values_out
+
+    def contains(self, keys, stream=None):
+        return self.find(keys, stream=stream)[0]
+
+    def erase(self, keys, erased=None, stream=None):
+        """erase(begin, end); `erased` (uint8, optional) receives device erase()'s 0/1."""
+        self._check_keys(keys)
+        check(
+            _lib.load().htab_erase(self._h, _ptr(keys), keys.numel(), _ptr(erased), _stream_ptr(stream)), "htab_erase"
+        )
+        return erased
+
+    def clear(self, stream=None):
+        check(_lib.load().htab_clear(self._h, _stream_ptr(stream)), "htab_clear")
+
+    # ---- queries ----
+    def size(self, stream=None) -> int:
+        out = ctypes.c_int64()
+        check(_lib.load().htab_size(self._h, ctypes.byref(out), _stream_ptr(stream)), "htab_size")
+        return out.value
+
+    def empty(self) -> bool:
+        return self.size() == 0
+
+    def full(self) -> bool:
+        return self.size() == self.max_size()
+
+    def max_size(self) -> int:
+        return self.info.max_size
+
+    def bucket_count(self) -> int:
+        return self.info.bucket_count
+
+    def load_factor(self) -> float:
+        # float(size) / float(bucket_count), as unordered_base_detail.cuh:1152-1158
+        import numpy as np
+
+        return float(np.float32(self.size()) / np.float32(self.bucket_count()))
+
+    def max_load_factor(self) -> float:
+        return 1.0
+
+    def bucket(self, key: int) -> int:
+        out = ctypes.c_int64()
+        check(_lib.load().htab_bucket(self._h, ctypes.c_uint64(key & 0xFFFFFFFFFFFFFFFF), ctypes.byref(out)), "htab_bucket")
+        return out.value
+
+    def valid(self, stream=None) -> bool:
+        out = ctypes.c_int32()
+        check(_lib.load().htab_valid(self._h, ctypes.byref(out), _stream_ptr(stream)), "htab_valid")
+        return bool(out.value)
+
+    def device_range(self, stream=None):
+        """Slot indices of all stored entries, ascending (int64 device tensor)."""
+        torch = _torch()
+        cap = self.info.limit + 2
+        idx = torch.empty(max(cap, 1), dtype=torch.int64, device="cuda")
+        n = ctypes.c_int64()
+        check(_lib.load().htab_range(self._h, _ptr(idx), cap, ctypes.byref(n), _stream_ptr(stream)), "htab_range")
+        return idx[: n.value]
+
+    def contents(self, stream=None):
+        """(keys, values) of all stored entries in slot order (device tensors)."""
+        torch = _torch()
+        cap = self.info.limit + 2
+        keys = torch.empty(max(cap, 1), dtype=self.key_dtype(), device="cuda")
+        vals = None if self.mapped is None else torch.empty(max(cap, 1), dtype=self.mapped_dtype(), device="cuda")
+        n = ctypes.c_int64()
+        check(
+            _lib.load().htab_contents(self._h, _ptr(keys), _ptr(vals), cap, ctypes.byref(n), _stream_ptr(stream)),
+            "htab_contents",
+        )
+        return keys[: n.value], (None if vals is None else vals[: n.value])
+
+    def view(self) -> _lib.htab_view:
+        v = _lib.htab_view()
+        check(_lib.load().htab_get_view(self._h, ctypes.byref(v)), "htab_get_view")
+        return v
+
+
+def _make(name: str, key: str, mapped: str | None):
+    return type(name, (_unordered_base,), {"key": key, "mapped": mapped})
+
+
+unordered_set_u32 = _make("unordered_set_u32", "u32", None)
+unordered_set_u64 = _make("unordered_set_u64", "u64", None)
+unordered_map_u64_u32 = _make("unordered_map_u64_u32", "u64", "u32")
+unordered_map_u32_u32 = _make("unordered_map_u32_u32", "u32", "u32")
+unordered_map_u64_u64 = _make("unordered_map_u64_u64", "u64", "u64")
+
+
+def unordered_map(key: str = "u64", mapped: str = "u32"):
+    return {("u64", "u32"): unordered_map_u64_u32, ("u32", "u32"): unordered_map_u32_u32,
+            ("u64", "u64"): unordered_map_u64_u64}[(key, mapped)]
+
+
+def unordered_set(key: str = "u32"):
+    return {"u32": unordered_set_u32, "u64": unordered_set_u64}[key]
+
+
+class deque_u32:
+    """deque<uint32_t> free list (deque.cuh) with bulk push_back / pop_front."""
+
+    def __init__(self, capacity: int):
+        h = ctypes.c_void_p()
+        check(_lib.load().htab_deque_create(int(capacity), ctypes.byref(h)), "htab_deque_create")
+        self._h = h
+        self.capacity = capacity
+
+    def destroy(self):
+        if self._h:
+            check(_lib.load().htab_deque_destroy(self._h), "htab_deque_destroy")
+            self._h = ctypes.c_void_p()
+
+    def fill_iota(self, stream=None):
+        check(_lib.load().htab_deque_fill_iota(self._h, _stream_ptr(stream)), "htab_deque_fill_iota")
+
+    def push_back(self, values, pushed=None, stream=None):
+        check(
+            _lib.load().htab_deque_push_back(self._h, _ptr(values), values.numel(), _ptr(pushed), _stream_ptr(stream)),
+            "htab_deque_push_back",
+        )
+        return pushed
+
+    def pop_front(self, n: int, out=None, popped=None, stream=None):
+        torch = _torch()
+        if out is None:
+            out = torch.empty(n, dtype=torch.int32, device="cuda")
+        if popped is None:
+            popped = torch.empty(n, dtype=torch.uint8, device="cuda")
+        check(
+            _lib.load().htab_deque_pop_front(self._h, _ptr(out), int(n), _ptr(popped), _stream_ptr(stream)),
+            "htab_deque_pop_front",
+        )
+        return out, popped
+
+    def size(self, stream=None) -> int:
+        out = ctypes.c_int64()
+        check(_lib.load().htab_deque_size(self._h, ctypes.byref(out), _stream_ptr(stream)), "htab_deque_size")
+        return out.value
+
+    def contents(self, stream=None):
+        torch = _torch()
+        out = torch.empty(max(self.capacity, 1), dtype=torch.int32, device="cuda")
+        n = ctypes.c_int64()
+        check(_lib.load().htab_deque_contents(self._h, _ptr(out), ctypes.byref(n), _stream_ptr(stream)),
+              "htab_deque_contents")
+        return out[: n.value]

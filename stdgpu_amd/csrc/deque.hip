@@ -1,0 +1,245 @@
+// Free-list deque<uint32_t> with wavefront-aggregated push_back / pop_front.
+//
+// Replaces the reference's per-element path (src/stdgpu/impl/deque_detail.cuh:202-260 push_back,
+// 390-448 pop_front: atomic size +-1, atomicInc of begin/end modulo capacity, then a spin on a
+// per-slot lock bit and an occupancy-bit flip).  Here a wave reserves room for all of its lanes
+// with one atomic on the size word and one on the begin/end counter, and lanes then read or write
+// their ring positions directly; no lock or occupancy bits are needed because one bulk launch only
+// pushes or only pops.  The ring position of counter value c is c mod capacity.
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include <stdgpu_amd/htab.h>
+
+#include "wave.hpp"
+
+using namespace htab_dev;
+
+namespace
+{
+
+constexpr int kSize = 0;   // int64 size word
+constexpr int kBegin = 16; // uint64 front counter (128 B apart)
+constexpr int kEnd = 32;   // uint64 back counter
+constexpr size_t kDequeCtlBytes = 512;
+
+int
+dq_fail(int code, const char* what)
+{
+    (void)what;
+    return code;
+}
+
+__global__ __launch_bounds__(256) void
+k_deque_push(uint32_t* __restrict__ data, int64_t* ctl, int64_t cap, const uint32_t* __restrict__ v, int64_t n,
+             uint8_t* __restrict__ pushed)
+{
+    const int lane = lane_id();
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const bool valid = i < n;
+    const uint64_t bal = ballot(valid);
+    const int64_t k = __builtin_popcountll(bal);
+    const int64_t rank = __builtin_popcountll(bal & ((1ULL << lane) - 1));
+    int64_t room = 0;
+    uint64_t pos0 = 0;
+    if (k > 0 && lane == 0)
+    {
+        const int64_t old = __hip_atomic_fetch_add(ctl + kSize, k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        room = cap - old;
+        room = room < 0 ? 0 : (room > k ? k : room);
+        if (room < k) __hip_atomic_fetch_add(ctl + kSize, -(k - room), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (room > 0)
+            pos0 = __hip_atomic_fetch_add(reinterpret_cast<uint64_t*>(ctl + kEnd), static_cast<uint64_t>(room),
+                                          __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    room = static_cast<int64_t>(shfl64(static_cast<uint64_t>(room), 0));
+    pos0 = shfl64(pos0, 0);
+    const bool ok = valid && rank < room;
+    if (ok) data[(pos0 + static_cast<uint64_t>(rank)) % static_cast<uint64_t>(cap)] = v[i];
+    if (valid && pushed) pushed[i] = ok ? 1 : 0;
+}
+
+__global__ __launch_bounds__(256) void
+k_deque_pop(const uint32_t* __restrict__ data, int64_t* ctl, int64_t cap, uint32_t* __restrict__ out, int64_t n,
+            uint8_t* __restrict__ popped)
+{
+    const int lane = lane_id();
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const bool valid = i < n;
+    const uint64_t bal = ballot(valid);
+    const int64_t k = __builtin_popcountll(bal);
+    const int64_t rank = __builtin_popcountll(bal & ((1ULL << lane) - 1));
+    int64_t avail = 0;
+    uint64_t pos0 = 0;
+    if (k > 0 && lane == 0)
+    {
+        const int64_t old = __hip_atomic_fetch_add(ctl + kSize, -k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        avail = old < 0 ? 0 : (old > k ? k : old);
+        if (avail < k) __hip_atomic_fetch_add(ctl + kSize, k - avail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (avail > 0)
+            pos0 = __hip_atomic_fetch_add(reinterpret_cast<uint64_t*>(ctl + kBegin), static_cast<uint64_t>(avail),
+                                          __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    avail = static_cast<int64_t>(shfl64(static_cast<uint64_t>(avail), 0));
+    pos0 = shfl64(pos0, 0);
+    const bool ok = valid && rank < avail;
+    if (valid)
+    {
+        out[i] = ok ? data[(pos0 + static_cast<uint64_t>(rank)) % static_cast<uint64_t>(cap)] : 0xFFFFFFFFU;
+        if (popped) popped[i] = ok ? 1 : 0;
+    }
+}
+
+__global__ __launch_bounds__(256) void
+k_deque_iota(uint32_t* __restrict__ data, int64_t* ctl, int64_t cap)
+{
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+    for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < static_cast<uint64_t>(cap);
+         i += stride)
+        data[i] = static_cast<uint32_t>(i);
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+    {
+        ctl[kSize] = cap;
+        ctl[kBegin] = 0;
+        ctl[kEnd] = cap;
+    }
+}
+
+__global__ __launch_bounds__(256) void
+k_deque_copy(const uint32_t* __restrict__ data, int64_t cap, uint64_t begin, int64_t n, uint32_t* __restrict__ out)
+{
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = data[(begin + static_cast<uint64_t>(i)) % static_cast<uint64_t>(cap)];
+}
+
+int
+dq_status()
+{
+    return hipGetLastError() == hipSuccess ? HTAB_OK : HTAB_E_HIP;
+}
+
+} // namespace
+
+struct htab_deque
+{
+    uint32_t* data = nullptr;
+    int64_t* ctl = nullptr;
+    int64_t capacity = 0;
+};
+
+extern "C" {
+
+int
+htab_deque_create(int64_t capacity, htab_deque** out)
+{
+    if (!out || capacity <= 0 || capacity > (int64_t(1) << 32)) return dq_fail(HTAB_E_ARG, "deque create");
+    *out = nullptr;
+    htab_deque* q = new (std::nothrow) htab_deque();
+    if (!q) return HTAB_E_OOM;
+    void* mem = nullptr;
+    if (hipMalloc(&mem, kDequeCtlBytes + static_cast<size_t>(capacity) * sizeof(uint32_t)) != hipSuccess)
+    {
+        delete q;
+        return HTAB_E_OOM;
+    }
+    q->ctl = static_cast<int64_t*>(mem);
+    q->data = reinterpret_cast<uint32_t*>(static_cast<char*>(mem) + kDequeCtlBytes);
+    q->capacity = capacity;
+    if (hipMemset(mem, 0, kDequeCtlBytes) != hipSuccess)
+    {
+        (void)hipFree(mem);
+        delete q;
+        return HTAB_E_HIP;
+    }
+    *out = q;
+    return HTAB_OK;
+}
+
+int
+htab_deque_destroy(htab_deque* q)
+{
+    if (!q) return HTAB_E_ARG;
+    const hipError_t e = hipFree(q->ctl);
+    delete q;
+    return e == hipSuccess ? HTAB_OK : HTAB_E_HIP;
+}
+
+int
+htab_deque_push_back(htab_deque* q, const uint32_t* values, int64_t n, uint8_t* pushed, void* stream)
+{
+    if (!q || n < 0 || (n > 0 && !values)) return HTAB_E_ARG;
+    if (n == 0) return HTAB_OK;
+    const int64_t grid = (n + 255) / 256;
+    if (grid > 0x7FFFFFFF) return HTAB_E_LIMIT;
+    hipLaunchKernelGGL(k_deque_push, dim3(static_cast<uint32_t>(grid)), dim3(256), 0, static_cast<hipStream_t>(stream),
+                       q->data, q->ctl, q->capacity, values, n, pushed);
+    return dq_status();
+}
+
+int
+htab_deque_pop_front(htab_deque* q, uint32_t* out, int64_t n, uint8_t* popped, void* stream)
+{
+    if (!q || n < 0 || (n > 0 && !out)) return HTAB_E_ARG;
+    if (n == 0) return HTAB_OK;
+    const int64_t grid = (n + 255) / 256;
+    if (grid > 0x7FFFFFFF) return HTAB_E_LIMIT;
+    hipLaunchKernelGGL(k_deque_pop, dim3(static_cast<uint32_t>(grid)), dim3(256), 0, static_cast<hipStream_t>(stream),
+                       q->data, q->ctl, q->capacity, out, n, popped);
+    return dq_status();
+}
+
+int
+htab_deque_fill_iota(htab_deque* q, void* stream)
+{
+    if (!q) return HTAB_E_ARG;
+    int64_t grid = (q->capacity + 255) / 256;
+    if (grid > 65536) grid = 65536;
+    hipLaunchKernelGGL(k_deque_iota, dim3(static_cast<uint32_t>(grid)), dim3(256), 0, static_cast<hipStream_t>(stream),
+                       q->data, q->ctl, q->capacity);
+    return dq_status();
+}
+
+static int
+deque_read_ctl(htab_deque* q, int64_t* size, uint64_t* begin, hipStream_t s)
+{
+    int64_t ctl[kEnd + 1];
+    if (hipMemcpyAsync(ctl, q->ctl, sizeof(ctl), hipMemcpyDeviceToHost, s) != hipSuccess) return HTAB_E_HIP;
+    if (hipStreamSynchronize(s) != hipSuccess) return HTAB_E_HIP;
+    int64_t n = ctl[kSize];
+    // the reference's size() clamps to [0, capacity] (deque_detail.cuh:482-512)
+    n = n < 0 ? 0 : (n > q->capacity ? q->capacity : n);
+    *size = n;
+    *begin = static_cast<uint64_t>(ctl[kBegin]);
+    return HTAB_OK;
+}
+
+int
+htab_deque_size(htab_deque* q, int64_t* out, void* stream)
+{
+    if (!q || !out) return HTAB_E_ARG;
+    uint64_t begin = 0;
+    return deque_read_ctl(q, out, &begin, static_cast<hipStream_t>(stream));
+}
+
+int
+htab_deque_contents(htab_deque* q, uint32_t* out, int64_t* n_out, void* stream)
+{
+    if (!q || !out || !n_out) return HTAB_E_ARG;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    int64_t n = 0;
+    uint64_t begin = 0;
+    int rc = deque_read_ctl(q, &n, &begin, s);
+    if (rc) return rc;
+    if (n > 0)
+    {
+        hipLaunchKernelGGL(k_deque_copy, dim3(static_cast<uint32_t>((n + 255) / 256)), dim3(256), 0, s, q->data,
+                           q->capacity, begin, n, out);
+        if (hipGetLastError() != hipSuccess) return HTAB_E_HIP;
+        if (hipStreamSynchronize(s) != hipSuccess) return HTAB_E_HIP;
+    }
+    *n_out = n;
+    return HTAB_OK;
+}
+
+} // extern "C"

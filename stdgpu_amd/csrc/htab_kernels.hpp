@@ -1,0 +1,882 @@
+// Bulk kernels of the hashed containers: wavefront-cooperative probing on gfx950.
+//
+// A wavefront owns 64 consecutive input elements, one per lane.  Lanes are split into
+// groups of G lanes; a group probes one bucket (16*G bytes) with a single 16-byte load per
+// lane, so one vector-memory instruction probes 64/G buckets.  A group works through the G
+// elements owned by its own lanes:
+//   1. first probes: the home buckets of all of the group's elements are loaded back to
+//      back (up to 8 loads in flight per lane), so a wave keeps 64 bucket lines in flight;
+//   2. each element is resolved against its home bucket with __ballot votes: a key match,
+//      an EMPTY slot (end of the probe sequence) or, for insert, a CAS on the first free
+//      slot elected with ctz over the ballot mask;
+//   3. elements the home bucket does not resolve (overflow, lost CAS) continue one group
+//      at a time along the probe sequence;
+//   4. results return to the owning lane and leave in one coalesced store; size changes are
+//      summed per workgroup into one atomic on one of 32 counter stripes.
+//
+// Reference call stacks these kernels replace (paths relative to the stdgpu source tree):
+//   find   : unordered_base_detail.cuh:653-706 via a for_each_index functor
+//   insert : unordered_base_detail.cuh:470-495, 708-809, 967-1017 (lock bits + excess stack)
+//   erase  : unordered_base_detail.cuh:497-522, 811-956, 1019-1066
+#ifndef STDGPU_AMD_HTAB_KERNELS_HPP
+#define STDGPU_AMD_HTAB_KERNELS_HPP
+
+#include <stdgpu_amd/htab_device.hpp>
+
+#include "wave.hpp"
+
+namespace htab_dev
+{
+
+// ------------------------------------------------------------------ 16-byte lane chunk of a bucket
+// A lane's chunk holds SPL = 16/sizeof(value_type) slots; (lo, hi) are its two 64-bit halves.
+template <typename K, typename M>
+struct chunk
+{
+    using traits = slot_traits<K, M>;
+    using value_type = typename traits::value_type;
+    static constexpr int SB = static_cast<int>(sizeof(value_type));
+    static constexpr int SPL = 16 / SB;
+    static_assert(SB == 4 || SB == 8 || SB == 16, "value_type must be 4, 8 or 16 bytes");
+
+    uint64_t lo, hi;
+
+    __device__ K key(int j) const
+    {
+        if constexpr (SB == 16)
+            return static_cast<K>(lo);
+        else if constexpr (SB == 8)
+            return static_cast<K>(j == 0 ? lo : hi);
+        else
+        {
+            const uint64_t w = (j < 2) ? lo : hi;
+            return static_cast<K>((j & 1) ? (w >> 32) : w);
+        }
+    }
+
+    // mapped value of slot j (maps only)
+    __device__ uint64_t mapped(int j) const
+    {
+        if constexpr (SB == 16)
+            return hi;
+        else if constexpr (SB == 8)
+            return (j == 0 ? lo : hi) >> (8 * sizeof(K));
+        else
+            return 0;
+    }
+
+    // plain (L1-cached) load: read-only phases
+    __device__ static chunk load_plain(const value_type* bucket, int li)
+    {
+        const uint4 q = *reinterpret_cast<const uint4*>(reinterpret_cast<const char*>(bucket) + 16 * li);
+        chunk c;
+        c.lo = static_cast<uint64_t>(q.x) | (static_cast<uint64_t>(q.y) << 32);
+        c.hi = static_cast<uint64_t>(q.z) | (static_cast<uint64_t>(q.w) << 32);
+        return c;
+    }
+
+    // coherent load of the key fields: phases that write the table
+    __device__ static chunk load_keys(const value_type* bucket, int li)
+    {
+        const uint64_t* p = reinterpret_cast<const uint64_t*>(reinterpret_cast<const char*>(bucket) + 16 * li);
+        chunk c;
+        c.lo = load_agent(p);
+        if constexpr (SB == 16)
+            c.hi = 0;
+        else
+            c.hi = load_agent(p + 1);
+        return c;
+    }
+
+    // bitmask over the chunk's slots j >= jmin_local for which pred(key) holds
+    template <typename Pred>
+    __device__ uint32_t mask(Pred pred, int jmin_local = 0) const
+    {
+        uint32_t m = 0;
+#pragma unroll
+        for (int j = 0; j < SPL; ++j)
+            if (j >= jmin_local && pred(key(j))) m |= 1U << j;
+        return m;
+    }
+};
+
+template <typename K, typename M>
+__device__ inline void
+store_mapped(typename slot_traits<K, M>::value_type* p, uint64_t v)
+{
+    if constexpr (slot_traits<K, M>::is_map) p->second = static_cast<M>(v);
+}
+
+// per-workgroup sum of `delta` (one value per thread) -> one atomic on a counter stripe.
+// Returns the stripe's previous value (only meaningful to callers that need the exact count).
+__device__ inline int64_t
+block_count_add(int64_t* stripes, uint32_t nstripes, int64_t delta, int64_t* s_wave, int64_t* s_old,
+                int64_t* wave_prefix_out)
+{
+    const int lane = lane_id();
+    const int wid = static_cast<int>(threadIdx.x) >> 6;
+    const int nw = static_cast<int>(blockDim.x) >> 6;
+    // wave-level sum via DPP-free shuffles (delta is 0/1 or 0/-1 in practice)
+    int64_t v = delta;
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += static_cast<int64_t>(shfl64(static_cast<uint64_t>(v), lane ^ off));
+    if (lane == 0) s_wave[wid] = v;
+    __syncthreads();
+    if (threadIdx.x == 0)
+    {
+        int64_t total = 0;
+        for (int w = 0; w < nw; ++w)
+        {
+            const int64_t c = s_wave[w];
+            s_wave[w] = total; // exclusive prefix
+            total += c;
+        }
+        int64_t old = 0;
+        if (total != 0)
+        {
+            const uint32_t s = (nstripes == 1) ? 0U : (static_cast<uint32_t>(blockIdx.x) & (nstripes - 1));
+            old = __hip_atomic_fetch_add(stripes + static_cast<uint64_t>(s) * kStripeStride, total, __ATOMIC_RELAXED,
+                                         __HIP_MEMORY_SCOPE_AGENT);
+        }
+        *s_old = old;
+    }
+    __syncthreads();
+    *wave_prefix_out = s_wave[wid];
+    return *s_old;
+}
+
+// ================================================================== find
+template <typename K, typename M, int G>
+__global__ __launch_bounds__(1024) void
+k_find(table_ref<K, M> t, const K* __restrict__ keys, int64_t n, uint8_t* __restrict__ hits,
+       void* __restrict__ vals_out)
+{
+    using C = chunk<K, M>;
+    using value_type = typename C::value_type;
+    constexpr int SPL = C::SPL;
+    constexpr int R = G < 8 ? G : 8; // first probes in flight per lane
+    const int lane = lane_id();
+    const int gbase = lane & ~(G - 1);
+    const int li = lane & (G - 1);
+    const uint64_t gmask = ((G == 64) ? ~0ULL : ((1ULL << G) - 1)) << gbase;
+    const uint32_t W = static_cast<uint32_t>(G * SPL);
+
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const bool valid = i < n;
+    const K mykey = valid ? keys[i] : empty_key<K>();
+    bool myhit = false;
+    uint64_t myval = 0;
+    const bool reserved = valid && table_ref<K, M>::is_reserved(mykey);
+    if (reserved)
+    {
+        if (__hip_atomic_load(t.flags, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & table_ref<K, M>::side_bit(mykey))
+        {
+            myhit = true;
+            if constexpr (slot_traits<K, M>::is_map) myval = static_cast<uint64_t>(t.slots[t.side_index(mykey)].second);
+        }
+    }
+    const bool probe = valid && !reserved;
+    uint32_t pend = 0; // group-uniform: elements not resolved by their home bucket
+
+#pragma unroll
+    for (int rb = 0; rb < G; rb += R)
+    {
+        C q[R];
+        K kr[R];
+        bool act[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+        {
+            kr[r] = shfl(mykey, gbase + rb + r);
+            act[r] = shfl32(probe ? 1U : 0U, gbase + rb + r) != 0;
+            if (act[r])
+            {
+                const uint64_t b = home_bucket(static_cast<uint64_t>(kr[r]), t.nbuckets);
+                q[r] = C::load_plain(t.slots + b * W, li);
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+        {
+            if (act[r])
+            {
+                const K k = kr[r];
+                const uint32_t m = q[r].mask([k](K x) { return x == k; });
+                const uint32_t e = q[r].mask([](K x) { return x == empty_key<K>(); });
+                const uint64_t bm = ballot(m != 0) & gmask;
+                const uint64_t be = ballot(e != 0) & gmask;
+                if (bm)
+                {
+                    const int src = ctz64(bm);
+                    const uint64_t cand = m ? q[r].mapped(__builtin_ctz(m)) : 0;
+                    const uint64_t v = shfl64(cand, src);
+                    if (li == rb + r)
+                    {
+                        myhit = true;
+                        myval = v;
+                    }
+                }
+                else if (!be)
+                {
+                    pend |= 1U << (rb + r);
+                }
+            }
+        }
+    }
+
+    // overflowed elements: walk the probe sequence one group element at a time
+    int cur = -1;
+    K ck = 0;
+    uint64_t b = 0, steps = 0;
+    while (true)
+    {
+        if (cur < 0 && pend)
+        {
+            cur = __builtin_ctz(pend);
+            pend &= pend - 1;
+            ck = shfl(mykey, gbase + cur);
+            b = next_bucket(home_bucket(static_cast<uint64_t>(ck), t.nbuckets), t.nbuckets);
+            steps = 1;
+        }
+        const bool active = cur >= 0;
+        if (ballot(active) == 0) break;
+        if (active)
+        {
+            const C q = C::load_plain(t.slots + b * W, li);
+            const K k = ck;
+            const uint32_t m = q.mask([k](K x) { return x == k; });
+            const uint32_t e = q.mask([](K x) { return x == empty_key<K>(); });
+            const uint64_t bm = ballot(m != 0) & gmask;
+            const uint64_t be = ballot(e != 0) & gmask;
+            if (bm)
+            {
+                const int src = ctz64(bm);
+                const uint64_t cand = m ? q.mapped(__builtin_ctz(m)) : 0;
+                const uint64_t v = shfl64(cand, src);
+                if (li == cur)
+                {
+                    myhit = true;
+                    myval = v;
+                }
+                cur = -1;
+            }
+            else if (be || ++steps >= t.nbuckets)
+            {
+                cur = -1;
+            }
+            else
+            {
+                b = next_bucket(b, t.nbuckets);
+            }
+        }
+    }
+
+    if (valid)
+    {
+        hits[i] = myhit ? 1 : 0;
+        if constexpr (slot_traits<K, M>::is_map)
+        {
+            if (vals_out && myhit) static_cast<M*>(vals_out)[i] = static_cast<M>(myval);
+        }
+    }
+}
+
+// ================================================================== insert
+// Elements come either as AoS value_type (`values`) or as split arrays (`keys`, `vals`).
+template <typename K, typename M>
+__device__ inline void
+fetch_element(const void* values, const void* keys, const void* vals, int64_t i, K* k, uint64_t* v)
+{
+    using value_type = typename slot_traits<K, M>::value_type;
+    if (values)
+    {
+        const value_type e = static_cast<const value_type*>(values)[i];
+        *k = slot_traits<K, M>::key(e);
+        if constexpr (slot_traits<K, M>::is_map)
+            *v = static_cast<uint64_t>(e.second);
+        else
+            *v = 0;
+    }
+    else
+    {
+        *k = static_cast<const K*>(keys)[i];
+        if constexpr (slot_traits<K, M>::is_map)
+            *v = vals ? static_cast<uint64_t>(static_cast<const M*>(vals)[i]) : 0;
+        else
+            *v = 0;
+    }
+}
+
+constexpr uint64_t kNoSlot = ~0ULL;
+
+template <typename K, typename M, int G>
+__global__ __launch_bounds__(1024) void
+k_insert(table_ref<K, M> t, const void* __restrict__ values, const void* __restrict__ keys,
+         const void* __restrict__ vals, int64_t n, uint8_t* __restrict__ inserted)
+{
+    using C = chunk<K, M>;
+    using value_type = typename C::value_type;
+    constexpr int SPL = C::SPL;
+    constexpr int R = G < 8 ? G : 8;
+    __shared__ int64_t s_wave[16];
+    __shared__ int64_t s_old;
+    const int lane = lane_id();
+    const int gbase = lane & ~(G - 1);
+    const int li = lane & (G - 1);
+    const uint64_t gmask = ((G == 64) ? ~0ULL : ((1ULL << G) - 1)) << gbase;
+    const uint32_t W = static_cast<uint32_t>(G * SPL);
+
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const bool valid = i < n;
+    K mykey = empty_key<K>();
+    uint64_t myv = 0;
+    if (valid) fetch_element<K, M>(values, keys, vals, i, &mykey, &myv);
+
+    // a table at its exact limit refuses every insert (reference: the !full() loop guard)
+    const bool at_limit = (t.nstripes == 1) && (load_agent(t.stripes) >= static_cast<int64_t>(t.limit));
+
+    bool myins = false;
+    uint64_t myslot = kNoSlot;
+    const bool reserved = valid && table_ref<K, M>::is_reserved(mykey);
+    if (reserved && !at_limit)
+    {
+        const uint32_t bit = table_ref<K, M>::side_bit(mykey);
+        const uint32_t old = __hip_atomic_fetch_or(t.flags, bit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (!(old & bit))
+        {
+            myslot = t.side_index(mykey);
+            store_mapped<K, M>(t.slots + myslot, myv);
+            myins = true;
+        }
+    }
+    const bool probe = valid && !reserved && !at_limit;
+    uint32_t pend = 0;
+
+#pragma unroll
+    for (int rb = 0; rb < G; rb += R)
+    {
+        C q[R];
+        K kr[R];
+        bool act[R];
+        uint64_t hb[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+        {
+            kr[r] = shfl(mykey, gbase + rb + r);
+            act[r] = shfl32(probe ? 1U : 0U, gbase + rb + r) != 0;
+            hb[r] = 0;
+            if (act[r])
+            {
+                hb[r] = home_bucket(static_cast<uint64_t>(kr[r]), t.nbuckets);
+                q[r] = C::load_keys(t.slots + hb[r] * W, li);
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+        {
+            if (act[r])
+            {
+                const K k = kr[r];
+                const uint32_t m = q[r].mask([k](K x) { return x == k; });
+                const uint32_t e = q[r].mask([](K x) { return x == empty_key<K>(); });
+                const uint32_t f = q[r].mask([](K x) { return x == empty_key<K>() || x == tomb_key<K>(); });
+                const uint64_t bm = ballot(m != 0) & gmask;
+                const uint64_t be = ballot(e != 0) & gmask;
+                const uint64_t bf = ballot(f != 0) & gmask;
+                if (bm)
+                {
+                    // key present: nothing to do (insert never overwrites)
+                }
+                else if (be)
+                {
+                    // probe sequence ends in the home bucket: claim its first free slot
+                    const int fl = ctz64(bf); // lane holding the first free slot
+                    // the value travels from the owner lane; the claiming lane stores it
+                    const uint64_t v = shfl64(myv, gbase + rb + r);
+                    uint64_t seen = 0;
+                    uint64_t slot = 0;
+                    if (lane == fl)
+                    {
+                        const int j = __builtin_ctz(f);
+                        const K obs = q[r].key(j);
+                        slot = hb[r] * W + static_cast<uint64_t>(li * SPL + j);
+                        const K got = cas_agent(t.key_ptr(slot), obs, k);
+                        seen = (got == obs) ? 1 : ((got == k) ? 2 : 0);
+                        if (seen == 1) store_mapped<K, M>(t.slots + slot, v);
+                    }
+                    seen = shfl64(seen, fl);
+                    slot = shfl64(slot, fl);
+                    if (seen == 1)
+                    {
+                        if (li == rb + r)
+                        {
+                            myins = true;
+                            myslot = slot;
+                        }
+                    }
+                    else if (seen == 0)
+                    {
+                        pend |= 1U << (rb + r);
+                    }
+                }
+                else
+                {
+                    pend |= 1U << (rb + r);
+                }
+            }
+        }
+    }
+
+    // slow path: full probe sequence with first-free tracking, CAS, and resume after a lost CAS
+    int cur = -1;
+    K ck = 0;
+    uint64_t cv = 0;
+    uint64_t b = 0, hb = 0, walked = 0, ff = kNoSlot;
+    K ffobs = 0;
+    uint32_t jmin = 0;
+    while (true)
+    {
+        if (cur < 0 && pend)
+        {
+            cur = __builtin_ctz(pend);
+            pend &= pend - 1;
+            ck = shfl(mykey, gbase + cur);
+            cv = shfl64(myv, gbase + cur);
+            hb = home_bucket(static_cast<uint64_t>(ck), t.nbuckets);
+            b = hb;
+            walked = 0;
+            ff = kNoSlot;
+            jmin = 0;
+        }
+        const bool active = cur >= 0;
+        if (ballot(active) == 0) break;
+        if (active)
+        {
+            const C q = C::load_keys(t.slots + b * W, li);
+            const K k = ck;
+            const int jloc = static_cast<int>(jmin) - li * SPL; // first slot of this chunk still in play
+            const uint32_t m = q.mask([k](K x) { return x == k; }, jloc);
+            const uint32_t e = q.mask([](K x) { return x == empty_key<K>(); }, jloc);
+            const uint32_t f = q.mask([](K x) { return x == empty_key<K>() || x == tomb_key<K>(); }, jloc);
+            const uint64_t bm = ballot(m != 0) & gmask;
+            const uint64_t be = ballot(e != 0) & gmask;
+            const uint64_t bf = ballot(f != 0) & gmask;
+            if (bm)
+            {
+                cur = -1; // present
+            }
+            else
+            {
+                if (ff == kNoSlot && bf)
+                {
+                    const int fl = ctz64(bf);
+                    uint64_t cand = 0, obs = 0;
+                    if (lane == fl)
+                    {
+                        const int j = __builtin_ctz(f);
+                        cand = b * W + static_cast<uint64_t>(li * SPL + j);
+                        obs = static_cast<uint64_t>(q.key(j));
+                    }
+                    ff = shfl64(cand, fl);
+                    ffobs = static_cast<K>(shfl64(obs, fl));
+                }
+                if (be)
+                {
+                    // end of the probe sequence: claim ff (the group's first lane issues the CAS)
+                    uint64_t res = 0;
+                    if (li == 0)
+                    {
+                        const K got = cas_agent(t.key_ptr(ff), ffobs, k);
+                        res = (got == ffobs) ? 1 : ((got == k) ? 2 : 0);
+                        if (res == 1) store_mapped<K, M>(t.slots + ff, cv);
+                    }
+                    res = shfl64(res, gbase);
+                    if (res == 1)
+                    {
+                        if (li == cur)
+                        {
+                            myins = true;
+                            myslot = ff;
+                        }
+                        cur = -1;
+                    }
+                    else if (res == 2)
+                    {
+                        cur = -1;
+                    }
+                    else
+                    {
+                        // lost ff to another key: every slot before ff holds another key (an insert-only
+                        // launch never frees a slot), so the scan resumes right after ff; ff only moves
+                        // forward, so this terminates
+                        b = ff / W;
+                        jmin = static_cast<uint32_t>(ff % W) + 1;
+                        ff = kNoSlot;
+                        if (jmin == W)
+                        {
+                            b = next_bucket(b, t.nbuckets);
+                            jmin = 0;
+                            if (b == hb) cur = -1; // wrapped around: no free slot left
+                        }
+                    }
+                }
+                else
+                {
+                    b = next_bucket(b, t.nbuckets);
+                    jmin = 0;
+                    ++walked;
+                    // every probe sequence is finite: give up after a full wrap, or early when the
+                    // table has reached its limit
+                    if (b == hb ||
+                        ((walked & 63) == 0 && stripe_sum(t.stripes, t.nstripes) >= static_cast<int64_t>(t.limit)))
+                    {
+                        cur = -1;
+                    }
+                }
+            }
+        }
+    }
+
+    int64_t wave_prefix = 0;
+    const int64_t old = block_count_add(t.stripes, t.nstripes, myins ? 1 : 0, s_wave, &s_old, &wave_prefix);
+    if (t.nstripes == 1)
+    {
+        // exact limit: inserts ranked past it are undone (their slot becomes TOMB)
+        const uint64_t bal = ballot(myins);
+        const int64_t rank = old + wave_prefix + __builtin_popcountll(bal & ((1ULL << lane) - 1));
+        if (myins && rank >= static_cast<int64_t>(t.limit))
+        {
+            if (myslot >= t.nslots())
+                __hip_atomic_fetch_and(t.flags, ~table_ref<K, M>::side_bit(mykey), __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+            else
+                __hip_atomic_store(t.key_ptr(myslot), tomb_key<K>(), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_fetch_add(t.stripes, static_cast<int64_t>(-1), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            myins = false;
+        }
+    }
+    if (valid && inserted) inserted[i] = myins ? 1 : 0;
+}
+
+// ================================================================== erase
+// Erase-only launch: a slot is reset to EMPTY when its bucket already holds an EMPTY slot (no
+// probe sequence runs through such a bucket, and none can start to while only erases run),
+// otherwise to TOMB so probe sequences through the bucket stay intact.
+template <typename K, typename M, int G>
+__global__ __launch_bounds__(1024) void
+k_erase(table_ref<K, M> t, const K* __restrict__ keys, int64_t n, uint8_t* __restrict__ erased)
+{
+    using C = chunk<K, M>;
+    constexpr int SPL = C::SPL;
+    constexpr int R = G < 8 ? G : 8;
+    __shared__ int64_t s_wave[16];
+    __shared__ int64_t s_old;
+    const int lane = lane_id();
+    const int gbase = lane & ~(G - 1);
+    const int li = lane & (G - 1);
+    const uint64_t gmask = ((G == 64) ? ~0ULL : ((1ULL << G) - 1)) << gbase;
+    const uint32_t W = static_cast<uint32_t>(G * SPL);
+
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const bool valid = i < n;
+    const K mykey = valid ? keys[i] : empty_key<K>();
+    bool mydel = false;
+    const bool reserved = valid && table_ref<K, M>::is_reserved(mykey);
+    if (reserved)
+    {
+        const uint32_t bit = table_ref<K, M>::side_bit(mykey);
+        const uint32_t old = __hip_atomic_fetch_and(t.flags, ~bit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        mydel = (old & bit) != 0;
+    }
+    const bool probe = valid && !reserved;
+    uint32_t pend = 0;
+
+#pragma unroll
+    for (int rb = 0; rb < G; rb += R)
+    {
+        C q[R];
+        K kr[R];
+        bool act[R];
+        uint64_t hb[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+        {
+            kr[r] = shfl(mykey, gbase + rb + r);
+            act[r] = shfl32(probe ? 1U : 0U, gbase + rb + r) != 0;
+            hb[r] = 0;
+            if (act[r])
+            {
+                hb[r] = home_bucket(static_cast<uint64_t>(kr[r]), t.nbuckets);
+                q[r] = C::load_keys(t.slots + hb[r] * W, li);
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+        {
+            if (act[r])
+            {
+                const K k = kr[r];
+                const uint32_t m = q[r].mask([k](K x) { return x == k; });
+                const uint32_t e = q[r].mask([](K x) { return x == empty_key<K>(); });
+                const uint64_t bm = ballot(m != 0) & gmask;
+                const uint64_t be = ballot(e != 0) & gmask;
+                if (bm)
+                {
+                    const int ml = ctz64(bm);
+                    uint64_t ok = 0;
+                    if (lane == ml)
+                    {
+                        const uint64_t slot = hb[r] * W + static_cast<uint64_t>(li * SPL + __builtin_ctz(m));
+                        const K repl = be ? empty_key<K>() : tomb_key<K>();
+                        ok = (cas_agent(t.key_ptr(slot), k, repl) == k) ? 1 : 0;
+                    }
+                    ok = shfl64(ok, ml);
+                    if (ok && li == rb + r) mydel = true;
+                }
+                else if (!be)
+                {
+                    pend |= 1U << (rb + r);
+                }
+            }
+        }
+    }
+
+    int cur = -1;
+    K ck = 0;
+    uint64_t b = 0, steps = 0;
+    while (true)
+    {
+        if (cur < 0 && pend)
+        {
+            cur = __builtin_ctz(pend);
+            pend &= pend - 1;
+            ck = shfl(mykey, gbase + cur);
+            b = next_bucket(home_bucket(static_cast<uint64_t>(ck), t.nbuckets), t.nbuckets);
+            steps = 1;
+        }
+        const bool active = cur >= 0;
+        if (ballot(active) == 0) break;
+        if (active)
+        {
+            const C q = C::load_keys(t.slots + b * W, li);
+            const K k = ck;
+            const uint32_t m = q.mask([k](K x) { return x == k; });
+            const uint32_t e = q.mask([](K x) { return x == empty_key<K>(); });
+            const uint64_t bm = ballot(m != 0) & gmask;
+            const uint64_t be = ballot(e != 0) & gmask;
+            if (bm)
+            {
+                const int ml = ctz64(bm);
+                uint64_t ok = 0;
+                if (lane == ml)
+                {
+                    const uint64_t slot = b * W + static_cast<uint64_t>(li * SPL + __builtin_ctz(m));
+                    const K repl = be ? empty_key<K>() : tomb_key<K>();
+                    ok = (cas_agent(t.key_ptr(slot), k, repl) == k) ? 1 : 0;
+                }
+                ok = shfl64(ok, ml);
+                if (ok && li == cur) mydel = true;
+                cur = -1;
+            }
+            else if (be || ++steps >= t.nbuckets)
+            {
+                cur = -1;
+            }
+            else
+            {
+                b = next_bucket(b, t.nbuckets);
+            }
+        }
+    }
+
+    int64_t wave_prefix = 0;
+    block_count_add(t.stripes, t.nstripes, mydel ? -1 : 0, s_wave, &s_old, &wave_prefix);
+    if (valid && erased) erased[i] = mydel ? 1 : 0;
+}
+
+// ================================================================== clear / fill
+template <typename K, typename M>
+__global__ __launch_bounds__(256) void
+k_clear(table_ref<K, M> t, uint64_t nchunks)
+{
+    // every key field becomes EMPTY: the whole slot area is filled with 0xFF bytes
+    uint4* p = reinterpret_cast<uint4*>(t.slots);
+    const uint4 ones = make_uint4(~0U, ~0U, ~0U, ~0U);
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+    for (uint64_t c = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; c < nchunks; c += stride) p[c] = ones;
+    if (blockIdx.x == 0 && threadIdx.x < kMaxStripes) t.stripes[static_cast<uint64_t>(threadIdx.x) * kStripeStride] = 0;
+    if (blockIdx.x == 0 && threadIdx.x == 0) *t.flags = 0;
+}
+
+// the side slot of the TOMB key value carries that key, so iterators to it read the right key
+template <typename K, typename M>
+__global__ void
+k_clear_tail(table_ref<K, M> t)
+{
+    using value_type = typename slot_traits<K, M>::value_type;
+    value_type* side = t.slots + t.nslots();
+    *reinterpret_cast<K*>(side + 0) = empty_key<K>();
+    *reinterpret_cast<K*>(side + 1) = tomb_key<K>();
+}
+
+// ================================================================== device_range / contents
+// Pass 1: per-workgroup count of stored slots over a contiguous tile of slots.
+template <typename K, typename M>
+__device__ inline bool
+slot_stored(const table_ref<K, M>& t, uint64_t s, uint32_t flags)
+{
+    const uint64_t ns = t.nslots();
+    if (s < ns)
+    {
+        const K k = *t.key_ptr(s);
+        return k != empty_key<K>() && k != tomb_key<K>();
+    }
+    return (flags & (s == ns ? 1U : 2U)) != 0;
+}
+
+constexpr int kRangeThreads = 256;
+constexpr int kRangePerThread = 16;
+constexpr uint64_t kRangeTile = static_cast<uint64_t>(kRangeThreads) * kRangePerThread;
+
+__device__ inline int64_t
+block_exclusive_scan(int64_t v, int64_t* s_tmp, int64_t* total)
+{
+    // LDS-staged scan: wave-level inclusive scan with shuffles, then wave totals in LDS
+    const int lane = lane_id();
+    const int wid = static_cast<int>(threadIdx.x) >> 6;
+    const int nw = static_cast<int>(blockDim.x) >> 6;
+    int64_t x = v;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1)
+    {
+        const int64_t y = static_cast<int64_t>(shfl64(static_cast<uint64_t>(x), lane >= off ? lane - off : lane));
+        if (lane >= off) x += y;
+    }
+    if (lane == 63) s_tmp[wid] = x;
+    __syncthreads();
+    if (threadIdx.x == 0)
+    {
+        int64_t run = 0;
+        for (int w = 0; w < nw; ++w)
+        {
+            const int64_t c = s_tmp[w];
+            s_tmp[w] = run;
+            run += c;
+        }
+        s_tmp[nw] = run;
+    }
+    __syncthreads();
+    const int64_t res = s_tmp[wid] + x - v;
+    *total = s_tmp[nw];
+    __syncthreads();
+    return res;
+}
+
+template <typename K, typename M>
+__global__ __launch_bounds__(kRangeThreads) void
+k_range_count(table_ref<K, M> t, uint64_t total_slots, int64_t* __restrict__ tile_counts)
+{
+    __shared__ int64_t s_tmp[17];
+    const uint32_t flags = *t.flags;
+    const uint64_t base = static_cast<uint64_t>(blockIdx.x) * kRangeTile + static_cast<uint64_t>(threadIdx.x) * kRangePerThread;
+    int64_t c = 0;
+    for (int j = 0; j < kRangePerThread; ++j)
+    {
+        const uint64_t s = base + j;
+        if (s < total_slots && slot_stored(t, s, flags)) ++c;
+    }
+    int64_t total = 0;
+    block_exclusive_scan(c, s_tmp, &total);
+    if (threadIdx.x == 0) tile_counts[blockIdx.x] = total;
+}
+
+// single-workgroup exclusive scan of the tile counts (in place); grand total at counts[ntiles]
+__global__ __launch_bounds__(1024) void
+k_scan_tiles(int64_t* counts, int64_t ntiles)
+{
+    __shared__ int64_t s_tmp[17];
+    __shared__ int64_t s_carry;
+    if (threadIdx.x == 0) s_carry = 0;
+    __syncthreads();
+    for (int64_t base = 0; base < ntiles; base += blockDim.x)
+    {
+        const int64_t i = base + threadIdx.x;
+        const int64_t v = i < ntiles ? counts[i] : 0;
+        int64_t total = 0;
+        const int64_t ex = block_exclusive_scan(v, s_tmp, &total);
+        if (i < ntiles) counts[i] = s_carry + ex;
+        __syncthreads();
+        if (threadIdx.x == 0) s_carry += total;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) counts[ntiles] = s_carry;
+}
+
+template <typename K, typename M>
+__global__ __launch_bounds__(kRangeThreads) void
+k_range_scatter(table_ref<K, M> t, uint64_t total_slots, const int64_t* __restrict__ tile_offsets,
+                int64_t* __restrict__ idx_out, K* __restrict__ keys_out, void* __restrict__ vals_out, int64_t cap)
+{
+    __shared__ int64_t s_tmp[17];
+    const uint32_t flags = *t.flags;
+    const uint64_t base = static_cast<uint64_t>(blockIdx.x) * kRangeTile + static_cast<uint64_t>(threadIdx.x) * kRangePerThread;
+    uint32_t mask = 0;
+    int64_t c = 0;
+    for (int j = 0; j < kRangePerThread; ++j)
+    {
+        const uint64_t s = base + j;
+        if (s < total_slots && slot_stored(t, s, flags))
+        {
+            mask |= 1U << j;
+            ++c;
+        }
+    }
+    int64_t total = 0;
+    int64_t pos = tile_offsets[blockIdx.x] + block_exclusive_scan(c, s_tmp, &total);
+    for (int j = 0; j < kRangePerThread; ++j)
+    {
+        if (!(mask & (1U << j))) continue;
+        const uint64_t s = base + j;
+        if (pos < cap)
+        {
+            if (idx_out) idx_out[pos] = static_cast<int64_t>(s);
+            if (keys_out) keys_out[pos] = slot_traits<K, M>::key(t.slots[s]);
+            if constexpr (slot_traits<K, M>::is_map)
+            {
+                if (vals_out) static_cast<M*>(vals_out)[pos] = t.slots[s].second;
+            }
+        }
+        ++pos;
+    }
+}
+
+__global__ void
+k_zero_u64(unsigned long long* p, int n)
+{
+    if (static_cast<int>(threadIdx.x) < n) p[threadIdx.x] = 0;
+}
+
+// ================================================================== valid()
+// Every stored key must be found at its own slot (reachable and unique); bad[0] counts violations,
+// bad[1] counts stored entries (compared with the size counter on the host).
+template <typename K, typename M>
+__global__ __launch_bounds__(256) void
+k_valid(table_ref<K, M> t, uint64_t total_slots, unsigned long long* bad)
+{
+    const uint32_t flags = *t.flags;
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+    unsigned long long nbad = 0, nstored = 0;
+    for (uint64_t s = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; s < total_slots; s += stride)
+    {
+        if (!slot_stored(t, s, flags)) continue;
+        ++nstored;
+        const K k = slot_traits<K, M>::key(t.slots[s]);
+        if (t.find(k) != t.slots + s) ++nbad;
+    }
+    if (nbad) atomicAdd(bad, nbad);
+    if (nstored) atomicAdd(bad + 1, nstored);
+}
+
+} // namespace htab_dev
+
+#endif // STDGPU_AMD_HTAB_KERNELS_HPP

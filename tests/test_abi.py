@@ -1,0 +1,63 @@
+"""CPU checks of the C ABI boundary: the HIP library loads and exports exactly what
+include/stdgpu_amd/htab.h declares (no compute calls: this container has no GPU)."""
+import ctypes
+import os
+import re
+
+import pytest
+
+from stdgpu_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "stdgpu_amd", "htab.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?\w+\**\s+\**(htab_\w+)\s*\(", text, flags=re.M)))
+
+
+def test_header_declares_the_api():
+    names = declared_functions()
+    for must in ["htab_create", "htab_destroy", "htab_insert", "htab_find", "htab_erase", "htab_size",
+                 "htab_range", "htab_clear", "htab_valid", "htab_bucket", "htab_last_error",
+                 "htab_deque_push_back", "htab_deque_pop_front"]:
+        assert must in names
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _lib.load()
+    for name in declared_functions():
+        assert hasattr(lib, name), name
+
+
+def test_python_binding_covers_header():
+    assert sorted(_lib.SIGNATURES) == declared_functions()
+
+
+def test_struct_layouts_match_header():
+    # field order/size of the C structs as seen by ctypes (plain 64-bit ABI)
+    assert ctypes.sizeof(_lib.htab_tuning) == 16
+    assert ctypes.sizeof(_lib.htab_info) == 8 + 8 * 8
+    assert ctypes.sizeof(_lib.htab_view) == 3 * 8 + 2 * 8 + 2 * 4 + 2 * 8
+
+
+def test_argument_errors_without_gpu():
+    lib = _lib.load()
+    h = ctypes.c_void_p()
+    assert lib.htab_create(99, 10, None, ctypes.byref(h)) == _lib.HTAB_E_ARG
+    assert lib.htab_create(_lib.HTAB_MAP_U64_U32, 0, None, ctypes.byref(h)) == _lib.HTAB_E_ARG
+    assert lib.htab_destroy(None) == _lib.HTAB_E_ARG
+    assert lib.htab_find(None, None, 0, None, None, None) == _lib.HTAB_E_ARG
+    assert b"null" in lib.htab_last_error()
+
+
+def test_missing_library_fails_loudly(tmp_path):
+    saved = _lib._lib
+    try:
+        _lib._lib = None
+        with pytest.raises(ImportError):
+            _lib.load(str(tmp_path / "nope.so"))
+    finally:
+        _lib._lib = saved

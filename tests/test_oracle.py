@@ -1,0 +1,250 @@
+"""Pins the CPU oracle (oracle/htab_oracle.cpp) before it is trusted as the parity checker.
+
+Each check is chosen so that a plausible slip in the restatement fails it:
+  * sizing and bucket() values printed by the compiled reference (OpenMP backend) in the
+    survey session, and an independent Python restatement of the closed forms;
+  * the reference tests' pinned colliding / non-colliding key pairs
+    (tests/stdgpu/unordered_datastructure.inc:899-934, vec_hash unordered_map.inc:181-206);
+  * brute-force set semantics against Python sets, single- and multi-threaded;
+  * the reference's valid() invariants after every step;
+  * the tiny-table capacity cases (unordered_datastructure.inc:1375-1739);
+  * the examples' closed-form sums (examples/openmp/unordered_map.cpp:99-108,
+    unordered_set.cpp:82-87);
+  * multiset conservation of the deque free list.
+"""
+from decimal import Decimal, getcontext
+import math
+
+import numpy as np
+import pytest
+
+from oracle import pyoracle as po
+
+GOLD = 11400714819323198485
+
+
+def py_fibonacci(h: int, bc: int) -> int:
+    """unordered_base_detail.cuh:56-85, restated with Python integers."""
+    w = bc.bit_length() - 1
+    if w <= 0:
+        return 0
+    d = 64 - w
+    h2 = h ^ (h >> d)
+    return ((GOLD * h2) & ((1 << 64) - 1)) >> d
+
+
+def py_sizing(capacity: int):
+    """unordered_base_detail.cuh:34-48, 1263-1273 with exact decimal arithmetic."""
+    want = math.ceil(float(np.float32(capacity)))
+    bc = 1
+    while bc < want:
+        bc <<= 1
+    getcontext().prec = 60
+    k, n = Decimal(bc), Decimal(capacity)
+    p = Decimal(1) if capacity == 1 else (1 - 1 / k) ** (int(capacity) - 1)
+    ec = n * (1 - p)
+    return bc, int(ec.to_integral_value(rounding="ROUND_CEILING")) + 1
+
+
+# values printed by the compiled reference (stdgpu OpenMP backend) in the survey session
+REFERENCE_KATS = [
+    # kind, capacity, bucket_count, max_size, {key: bucket}
+    (("u32", None), 1 << 21, 2097152, 3422806, {1: 1296111, 2: 495070, 3: 1791181, 0xFFFFFFFE: 547697}),
+    (("u64", "u32"), 100000, 131072, 184444,
+     {1: 81006, 2: 30941, 3: 111948, 0xFFFFFFFE: 34231, 0x0123456789ABCDEF: 28855}),
+    (("u64", "u32"), 1, 1, 2, {}),
+]
+
+
+@pytest.mark.parametrize("kind,cap,bc,mx,buckets", REFERENCE_KATS)
+def test_reference_kats(kind, cap, bc, mx, buckets):
+    t = po.Table(kind[0], kind[1], cap)
+    assert t.bucket_count() == bc
+    assert t.max_size() == mx
+    for k, b in buckets.items():
+        assert t.bucket(k) == b
+        assert py_fibonacci(k, bc) == b
+
+
+@pytest.mark.parametrize("cap", [1, 2, 3, 7, 100, 1000, 100000, 1 << 20, 1 << 21, (1 << 21) + 1, 83886080,
+                                 5_000_000, 12_345_678])
+def test_sizing_closed_form(cap):
+    bc, ex = po.sizing(cap)
+    pbc, pex = py_sizing(cap) if cap <= 1 << 21 else (None, None)
+    assert bc == 1 << (cap - 1).bit_length()
+    if pbc is not None:
+        assert (bc, ex) == (pbc, pex)
+
+
+def test_sizing_config_capacities():
+    # capacity for 2^26 keys at load 0.8 -> the reference's power-of-two buckets give load 0.5
+    bc, ex = po.sizing(83886080)
+    assert bc == 1 << 27
+    assert bc + ex == 173202826
+
+
+def test_fibonacci_random_against_python():
+    rng = np.random.default_rng(5)
+    for bc in [1, 2, 4, 1 << 10, 1 << 17, 1 << 21, 1 << 27, 1 << 30]:
+        for h in rng.integers(0, 2**63, size=200, dtype=np.int64):
+            hh = int(h) * 2 + 1
+            assert po.fibonacci_hashing(hh, bc) == py_fibonacci(hh, bc)
+
+
+def vec_hash(x: int, y: int, z: int) -> int:
+    """unordered_map.inc:181-206: int16 components sign-extended to size_t."""
+    m = (1 << 64) - 1
+    return ((x & m) * 73856093 ^ (y & m) * 19349669 ^ (z & m) * 83492791) & m
+
+
+def test_reference_test_collision_pairs():
+    bc, _ = po.sizing(100000)  # the test fixture's table (unordered_datastructure.inc:49-70)
+    b = lambda v: po.fibonacci_hashing(vec_hash(*v), bc)  # noqa: E731
+    # insert_collision asserts these pairs share a bucket
+    assert b((-7, -3, 15)) == b((7, 3, 15))
+    assert b((-5, -15, 13)) == b((5, 15, 13))
+    # insert_no_collision relies on these two not colliding
+    assert b((-7, -3, 15)) != b((-5, -15, 13))
+
+
+KINDS = [("u32", None), ("u64", None), ("u64", "u32"), ("u32", "u32"), ("u64", "u64")]
+
+
+def _rand_keys(rng, key, n, universe=None):
+    if key == "u32":
+        hi = universe or (1 << 32)
+        return rng.integers(0, hi, size=n, dtype=np.uint64).astype(np.uint32)
+    if universe:
+        return rng.integers(0, universe, size=n, dtype=np.uint64)
+    return rng.integers(0, 2**63, size=n, dtype=np.uint64) * np.uint64(2) + rng.integers(0, 2, size=n, dtype=np.uint64)
+
+
+def _val(key, keys, mapped):
+    if mapped is None:
+        return None
+    v = (keys.astype(np.uint64) * np.uint64(2654435761)) & np.uint64(0xFFFFFFFF)
+    return v.astype(np.uint32 if mapped == "u32" else np.uint64)
+
+
+@pytest.mark.parametrize("kind", KINDS)
+@pytest.mark.parametrize("threads", [1, 4])
+def test_set_semantics_bruteforce(kind, threads):
+    key, mapped = kind
+    rng = np.random.default_rng(11 + threads)
+    cap = 3000
+    t = po.Table(key, mapped, cap)
+    ref = {}
+    for rnd in range(4):
+        ins = _rand_keys(rng, key, 1500, universe=4000)
+        vals = _val(key, ins, mapped)
+        ok = t.insert(ins, vals, threads=threads)
+        # exactly one success per distinct new key
+        new = {}
+        for i, k in enumerate(ins.tolist()):
+            if k not in ref and k not in new:
+                new[k] = None if vals is None else int(vals[i])
+        for k in set(ins.tolist()):
+            idx = np.nonzero(ins == k)[0]
+            assert ok[idx].sum() == (1 if k in new else 0)
+        ref.update(new)
+        assert t.size() == len(ref)
+        assert t.valid()
+        q = _rand_keys(rng, key, 2000, universe=4000)
+        hits, got = t.find(q, threads=threads)
+        assert hits.tolist() == [1 if k in ref else 0 for k in q.tolist()]
+        if mapped is not None:
+            for k, h, v in zip(q.tolist(), hits.tolist(), got.tolist()):
+                if h:
+                    assert v == ref[k]
+        er = _rand_keys(rng, key, 700, universe=4000)
+        eok = t.erase(er, threads=threads)
+        gone = set()
+        for k in set(er.tolist()):
+            idx = np.nonzero(er == k)[0]
+            assert eok[idx].sum() == (1 if k in ref else 0)
+            if k in ref:
+                gone.add(k)
+        for k in gone:
+            del ref[k]
+        assert t.size() == len(ref)
+        assert t.valid()
+        sk, sv = t.sorted_contents()
+        assert sk.tolist() == sorted(ref)
+        if mapped is not None:
+            assert sv.tolist() == [ref[k] for k in sorted(ref)]
+
+
+def test_tiny_table_capacity_cases():
+    # unordered_datastructure.inc:1375-1443: createDeviceObject(1) holds exactly two keys
+    t = po.Table("u64", "u32", 1)
+    assert (t.bucket_count(), t.max_size()) == (1, 2)
+    assert t.insert(np.array([10], np.uint64), np.array([1], np.uint32)).tolist() == [1]
+    assert t.insert(np.array([20], np.uint64), np.array([2], np.uint32)).tolist() == [1]
+    assert t.size() == 2 and t.valid()
+    assert t.insert(np.array([30], np.uint64), np.array([3], np.uint32)).tolist() == [0]
+    assert t.size() == 2 and t.valid()
+    # insert_multiple_while_full: many threads, same key, none inserted
+    ok = t.insert(np.full(1000, 30, np.uint64), np.full(1000, 3, np.uint32), threads=8)
+    assert ok.sum() == 0 and t.size() == 2
+
+
+def test_tiny_table_parallel_one_free():
+    # unordered_datastructure.inc:1549-1603: one free place, many random keys -> exactly one wins
+    t = po.Table("u64", "u32", 1)
+    t.insert(np.array([123], np.uint64), np.array([1], np.uint32))
+    rng = np.random.default_rng(3)
+    keys = rng.integers(1, 2**40, size=5000, dtype=np.uint64)
+    ok = t.insert(keys, np.ones(5000, np.uint32), threads=8)
+    assert ok.sum() == 1
+    assert t.size() == 2 and t.valid()
+
+
+def test_excess_list_exhaustion_refuses_free_head():
+    # unordered_base_detail.cuh:980-981: an empty excess list refuses every insert, even one
+    # whose bucket head is free
+    t = po.Table("u64", None, 4)  # bc 4, excess = ceil(4(1-(3/4)^3)) + 1 = 4
+    bc = t.bucket_count()
+    ex = t.max_size() - bc
+    same = [k for k in range(1, 100000) if t.bucket(k) == 0][: ex + 1]
+    ok = t.insert(np.array(same, np.uint64))
+    assert ok.sum() == ex + 1  # one at the head, ex in the excess list
+    assert t.excess_free() == 0
+    other = next(k for k in range(1, 100000) if t.bucket(k) == 1)
+    assert t.insert(np.array([other], np.uint64)).tolist() == [0]
+
+
+def test_examples_closed_form_sums():
+    # examples/openmp/unordered_set.cpp:60-87 and unordered_map.cpp:70-108
+    n = 100
+    odd = np.arange(1, n + 1)[np.arange(1, n + 1) % 2 == 1]
+    neigh = np.concatenate([odd - 1, odd, odd + 1]).astype(np.uint32)
+    s = po.Table("u32", None, n)
+    s.insert(neigh, threads=4)
+    k, _ = s.contents()
+    assert s.size() == n + 1 and int(k.astype(np.int64).sum()) == n * (n + 1) // 2
+    m = po.Table("u32", "u32", n)
+    m.insert(neigh, (neigh.astype(np.int64) ** 2).astype(np.uint32), threads=4)
+    k, v = m.contents()
+    assert m.size() == n + 1
+    assert int(k.astype(np.int64).sum()) == n * (n + 1) // 2
+    assert int(v.astype(np.int64).sum()) == n * (n + 1) * (2 * n + 1) // 6
+
+
+def test_deque_multiset_conservation():
+    cap = 1 << 12
+    q = po.Deque(cap)
+    q.push_back(np.arange(cap, dtype=np.uint32))
+    assert q.size() == cap
+    assert q.push_back(np.array([7], np.uint32)).tolist() == [0]  # full
+    out, ok = q.pop_front(1000, threads=4)
+    assert ok.all()
+    rest = q.contents()
+    assert sorted(np.concatenate([out, rest]).tolist()) == list(range(cap))
+    # FIFO order single-threaded
+    out2, _ = q.pop_front(10)
+    assert out2.tolist() == rest[:10].tolist()
+    q.push_back(out)
+    assert q.size() == cap - 10
+    got, ok = q.pop_front(cap)
+    assert ok.sum() == cap - 10
+    assert sorted(np.concatenate([got[ok == 1], out2]).tolist()) == list(range(cap))
