@@ -113,6 +113,12 @@ int htab_clear(htab* table, void* stream);
  * their number to *n_out.  Synchronises the stream.  Entry i is view.slots[idx_out[i]]. */
 int htab_range(htab* table, int64_t* idx_out, int64_t cap, int64_t* n_out, void* stream);
 
+/* device_range() with the index buffer owned by the table, as the reference keeps _range_indices
+ * inside the table (unordered_base.cuh:526-554): *idx_out receives a device pointer to the
+ * ascending slot indices, valid until the next htab_range_owned call on this table or its
+ * destruction.  Synchronises the stream. */
+int htab_range_owned(htab* table, const int64_t** idx_out, int64_t* n_out, void* stream);
+
 /* Contents as two dense arrays (keys, mapped values; vals_out NULL for sets) in slot order,
  * the data the reference's device_range() exposes.  Synchronises the stream. */
 int htab_contents(htab* table, void* keys_out, void* vals_out, int64_t cap, int64_t* n_out, void* stream);
@@ -128,6 +134,33 @@ int htab_bucket(const htab* table, uint64_t key, int64_t* out);
 
 int htab_get_info(const htab* table, htab_info* out);
 int htab_get_view(const htab* table, htab_view* out);
+
+/* ------------------------------------------------------------------------------------------
+ * Batched form: many tables of one kind and bucket_lanes served by ONE persistent launch.  No
+ * reference counterpart (the reference serves one table per bulk call); every table gets the same
+ * results as a plain call on it alone.  Table t owns elements [offsets[t], offsets[t+1]) of the
+ * flattened device arrays keys / vals / flags / vals_out and runs ops[t] (device int8 array, one
+ * of HTAB_OP_*) or, when ops is NULL, `op` for every table.  flags receives inserted / hit /
+ * erased per element; vals_out the mapped values of find hits.  offsets is a device int64 array
+ * of ntables + 1 entries.  Elements of one table in one launch must share an operation.
+ * ------------------------------------------------------------------------------------------ */
+#define HTAB_OP_INSERT 0
+#define HTAB_OP_FIND 1
+#define HTAB_OP_ERASE 2
+
+typedef struct htab_batch htab_batch;
+typedef struct htab_graph htab_graph;
+
+int htab_batch_create(htab* const* tables, int32_t ntables, htab_batch** out);
+int htab_batch_destroy(htab_batch* batch);
+int htab_batch_run(htab_batch* batch, const int8_t* ops, int op, const int64_t* offsets, const void* keys,
+                   const void* vals, uint8_t* flags, void* vals_out, void* stream);
+/* Records htab_batch_run with these arguments into a HIP graph (captured on `stream`, which must not
+ * be the legacy default stream); htab_graph_launch replays it on any stream. */
+int htab_batch_capture(htab_batch* batch, const int8_t* ops, int op, const int64_t* offsets, const void* keys,
+                       const void* vals, uint8_t* flags, void* vals_out, void* stream, htab_graph** out);
+int htab_graph_launch(htab_graph* graph, void* stream);
+int htab_graph_destroy(htab_graph* graph);
 
 /* Message for the last non-zero status returned to the calling thread. */
 const char* htab_last_error(void);

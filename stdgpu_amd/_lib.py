@@ -24,6 +24,10 @@ HTAB_MAP_U64_U32 = 2
 HTAB_MAP_U32_U32 = 3
 HTAB_MAP_U64_U64 = 4
 
+HTAB_OP_INSERT = 0
+HTAB_OP_FIND = 1
+HTAB_OP_ERASE = 2
+
 
 class htab_tuning(ctypes.Structure):
     _fields_ = [
@@ -78,12 +82,19 @@ SIGNATURES = {
     "htab_size": (_I32, [_P, ctypes.POINTER(_I64), _P]),
     "htab_clear": (_I32, [_P, _P]),
     "htab_range": (_I32, [_P, _P, _I64, ctypes.POINTER(_I64), _P]),
+    "htab_range_owned": (_I32, [_P, ctypes.POINTER(_P), ctypes.POINTER(_I64), _P]),
     "htab_contents": (_I32, [_P, _P, _P, _I64, ctypes.POINTER(_I64), _P]),
     "htab_valid": (_I32, [_P, ctypes.POINTER(ctypes.c_int32), _P]),
     "htab_bucket": (_I32, [_P, ctypes.c_uint64, ctypes.POINTER(_I64)]),
     "htab_get_info": (_I32, [_P, ctypes.POINTER(htab_info)]),
     "htab_get_view": (_I32, [_P, ctypes.POINTER(htab_view)]),
     "htab_last_error": (ctypes.c_char_p, []),
+    "htab_batch_create": (_I32, [ctypes.POINTER(_P), ctypes.c_int32, ctypes.POINTER(_P)]),
+    "htab_batch_destroy": (_I32, [_P]),
+    "htab_batch_run": (_I32, [_P, _P, _I32, _P, _P, _P, _P, _P, _P]),
+    "htab_batch_capture": (_I32, [_P, _P, _I32, _P, _P, _P, _P, _P, _P, ctypes.POINTER(_P)]),
+    "htab_graph_launch": (_I32, [_P, _P]),
+    "htab_graph_destroy": (_I32, [_P]),
     "htab_deque_create": (_I32, [_I64, ctypes.POINTER(_P)]),
     "htab_deque_destroy": (_I32, [_P]),
     "htab_deque_push_back": (_I32, [_P, _P, _I64, _P, _P]),

@@ -1,0 +1,71 @@
+"""Batched form: many tables served by one persistent launch (C ABI htab_batch_*).
+
+A TableBatch groups tables of one kind and bucket width.  One call runs one operation per table
+over flattened device arrays: table t owns elements [offsets[t], offsets[t+1]).  `capture` records
+the launch into a HIP graph that replays with the same buffers.
+"""
+from __future__ import annotations
+
+import ctypes
+
+from . import _lib
+from ._lib import check
+from .containers import _ptr, _stream_ptr
+
+OPS = {"insert": _lib.HTAB_OP_INSERT, "find": _lib.HTAB_OP_FIND, "erase": _lib.HTAB_OP_ERASE}
+
+
+class Graph:
+    def __init__(self, handle: ctypes.c_void_p, keepalive):
+        self._g = handle
+        self._keep = keepalive  # buffers the graph reads and writes
+
+    def launch(self, stream=None):
+        check(_lib.load().htab_graph_launch(self._g, _stream_ptr(stream)), "htab_graph_launch")
+
+    def destroy(self):
+        if self._g:
+            check(_lib.load().htab_graph_destroy(self._g), "htab_graph_destroy")
+            self._g = ctypes.c_void_p()
+
+
+class TableBatch:
+    def __init__(self, tables):
+        self.tables = list(tables)
+        arr = (ctypes.c_void_p * len(self.tables))(*[t._h.value for t in self.tables])
+        h = ctypes.c_void_p()
+        check(_lib.load().htab_batch_create(arr, len(self.tables), ctypes.byref(h)), "htab_batch_create")
+        self._b = h
+
+    def destroy(self):
+        if self._b:
+            check(_lib.load().htab_batch_destroy(self._b), "htab_batch_destroy")
+            self._b = ctypes.c_void_p()
+
+    @staticmethod
+    def _op_args(op):
+        if isinstance(op, str):
+            return None, OPS[op]
+        return op, 0  # per-table int8 device tensor
+
+    def run(self, op, offsets, keys, vals=None, flags=None, vals_out=None, stream=None):
+        ops, opc = self._op_args(op)
+        check(
+            _lib.load().htab_batch_run(self._b, _ptr(ops), opc, _ptr(offsets), _ptr(keys), _ptr(vals), _ptr(flags),
+                                       _ptr(vals_out), _stream_ptr(stream)),
+            "htab_batch_run",
+        )
+
+    def capture(self, op, offsets, keys, vals=None, flags=None, vals_out=None, stream=None) -> Graph:
+        """Record run(...) into a HIP graph on `stream` (a created torch stream)."""
+        import torch
+
+        s = stream or torch.cuda.Stream()
+        ops, opc = self._op_args(op)
+        g = ctypes.c_void_p()
+        check(
+            _lib.load().htab_batch_capture(self._b, _ptr(ops), opc, _ptr(offsets), _ptr(keys), _ptr(vals),
+                                           _ptr(flags), _ptr(vals_out), int(s.cuda_stream), ctypes.byref(g)),
+            "htab_batch_capture",
+        )
+        return Graph(g, (ops, offsets, keys, vals, flags, vals_out, s))

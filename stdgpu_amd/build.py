@@ -13,6 +13,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 PKG = os.path.join(ROOT, "stdgpu_amd")
 CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(PKG, "libstdgpu_amd.so")
+ROOFLINE_LIB = os.path.join(PKG, "libstdgpu_amd_roofline.so")
 SOURCES = ["htab.hip", "deque.hip"]
 HEADERS = [
     os.path.join(ROOT, "include", "stdgpu_amd", "htab.h"),
@@ -43,11 +44,11 @@ def _run(cmd: list[str], cwd: str | None = None) -> None:
         raise RuntimeError(f"build step failed ({r.returncode}): {' '.join(cmd)}")
 
 
-def build_library(out: str = LIB) -> str:
+def build_library(out: str = LIB, sources=SOURCES) -> str:
     objs = []
     build_dir = os.path.join(PKG, "_build")
     os.makedirs(build_dir, exist_ok=True)
-    for src in SOURCES:
+    for src in sources:
         obj = os.path.join(build_dir, src.replace(".hip", ".o"))
         _run([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)
@@ -62,8 +63,31 @@ def build_oracle() -> str:
     return os.path.join(ROOT, "oracle", "liboracle.so")
 
 
+def build_roofline() -> str:
+    return build_library(ROOFLINE_LIB, ["roofline.hip"])
+
+
+CPP_TESTS = ["test_facade"]
+
+
+def build_cpp_tests() -> list[str]:
+    """hipcc builds of the C++ facade tests (tests/cpp/*.hip) against the in-tree library."""
+    out = []
+    src_dir = os.path.join(ROOT, "tests", "cpp")
+    bin_dir = os.path.join(src_dir, "_build")
+    os.makedirs(bin_dir, exist_ok=True)
+    for name in CPP_TESTS:
+        exe = os.path.join(bin_dir, name)
+        _run([HIPCC, *HIP_FLAGS, "-O2", os.path.join(src_dir, name + ".hip"), "-o", exe,
+              "-L", PKG, "-lstdgpu_amd", "-Wl,-rpath,$ORIGIN/../../../stdgpu_amd"])
+        out.append(exe)
+    return out
+
+
 def main() -> None:
     print("library:", build_library())
+    print("roofline:", build_roofline())
+    print("cpp tests:", build_cpp_tests())
     print("oracle :", build_oracle())
 
 

@@ -102,6 +102,24 @@ struct htab
     size_t bytes = 0;
     int64_t* scratch = nullptr; // range/valid scratch, grown on demand
     size_t scratch_bytes = 0;
+    int64_t* range_idx = nullptr; // table-owned device_range() indices (limit + 2 entries)
+};
+
+struct htab_batch
+{
+    int kind = 0;
+    int lanes = 8;
+    int32_t ntables = 0;
+    void* tables = nullptr;            // device array of table_ref<K, M>
+    int64_t* chunk_start = nullptr;    // ntables + 1
+    unsigned long long* counter = nullptr;
+    uint32_t grid = 0;                 // persistent workgroups
+};
+
+struct htab_graph
+{
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
 };
 
 namespace
@@ -136,6 +154,34 @@ struct ops
         hipLaunchKernelGGL((k_erase<K, M, G>), dim3(static_cast<uint32_t>(grid)), dim3(h->block), 0, s,
                            T::from_view(h->view), static_cast<const K*>(keys), n, del);
         return launch_status("erase launch");
+    }
+};
+
+template <typename K, typename M, int G>
+struct batch_ops
+{
+    static int build(htab_batch* b, htab* const* tables)
+    {
+        using T = table_ref<K, M>;
+        T* host = new (std::nothrow) T[b->ntables];
+        if (!host) return fail(HTAB_E_OOM, "htab_batch_create: host allocation");
+        for (int32_t t = 0; t < b->ntables; ++t) host[t] = T::from_view(tables[t]->view);
+        hipError_t e = hipMemcpy(b->tables, host, sizeof(T) * b->ntables, hipMemcpyHostToDevice);
+        delete[] host;
+        if (e != hipSuccess) return fail_hip(e, "htab_batch_create: descriptor copy");
+        return HTAB_OK;
+    }
+
+    static int run(htab_batch* b, const int8_t* ops, int op, const int64_t* offsets, const void* keys,
+                   const void* vals, uint8_t* flags, void* vals_out, hipStream_t s)
+    {
+        hipLaunchKernelGGL(k_batch_plan, dim3(1), dim3(1024), 0, s, offsets, b->ntables, b->chunk_start, b->counter);
+        int rc = launch_status("batch plan");
+        if (rc) return rc;
+        hipLaunchKernelGGL((k_batch<K, M, G>), dim3(b->grid), dim3(kBatchBlock), 0, s,
+                           static_cast<const table_ref<K, M>*>(b->tables), b->ntables, offsets, ops, op,
+                           b->chunk_start, b->counter, keys, vals, flags, vals_out);
+        return launch_status("batch launch");
     }
 };
 
@@ -381,6 +427,7 @@ htab_destroy(htab* h)
     hipError_t e1 = hipSuccess, e2 = hipSuccess;
     if (h->mem) e1 = hipFree(h->mem);
     if (h->scratch) e2 = hipFree(h->scratch);
+    if (h->range_idx && e2 == hipSuccess) e2 = hipFree(h->range_idx);
     delete h;
     if (e1 != hipSuccess) return fail_hip(e1, "htab_destroy");
     if (e2 != hipSuccess) return fail_hip(e2, "htab_destroy scratch");
@@ -469,6 +516,19 @@ htab_range(htab* h, int64_t* idx_out, int64_t cap, int64_t* n_out, void* stream)
 }
 
 int
+htab_range_owned(htab* h, const int64_t** idx_out, int64_t* n_out, void* stream)
+{
+    if (!h || !idx_out || !n_out) return fail(HTAB_E_ARG, "htab_range_owned: null argument");
+    const int64_t cap = h->info.limit + 2;
+    if (!h->range_idx)
+    {
+        HTAB_HIP_TRY(hipMalloc(&h->range_idx, static_cast<size_t>(cap) * sizeof(int64_t)), "htab_range_owned alloc");
+    }
+    *idx_out = h->range_idx;
+    return htab_range(h, h->range_idx, cap, n_out, stream);
+}
+
+int
 htab_contents(htab* h, void* keys_out, void* vals_out, int64_t cap, int64_t* n_out, void* stream)
 {
     if (!h || !n_out) return fail(HTAB_E_ARG, "htab_contents: null argument");
@@ -492,6 +552,125 @@ htab_bucket(const htab* h, uint64_t key, int64_t* out)
     if (!h || !out) return fail(HTAB_E_ARG, "htab_bucket: null argument");
     // hash<integer> is the identity, widened to size_t (signed keys are not part of the C ABI)
     *out = reference_bucket(key, h->view.ref_bucket_count);
+    return HTAB_OK;
+}
+
+int
+htab_batch_create(htab* const* tables, int32_t ntables, htab_batch** out)
+{
+    if (!out || !tables || ntables <= 0) return fail(HTAB_E_ARG, "htab_batch_create: bad argument");
+    *out = nullptr;
+    for (int32_t t = 0; t < ntables; ++t)
+    {
+        if (!tables[t]) return fail(HTAB_E_ARG, "htab_batch_create: null table");
+        if (tables[t]->kind != tables[0]->kind || tables[t]->lanes != tables[0]->lanes)
+            return fail(HTAB_E_ARG, "htab_batch_create: tables differ in kind or bucket_lanes");
+    }
+    htab_batch* b = new (std::nothrow) htab_batch();
+    if (!b) return fail(HTAB_E_OOM, "htab_batch_create: host allocation");
+    b->kind = tables[0]->kind;
+    b->lanes = tables[0]->lanes;
+    b->ntables = ntables;
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+        cus = 256;
+    b->grid = static_cast<uint32_t>(cus) * 8U;
+    const size_t desc = 64 * static_cast<size_t>(ntables); // >= sizeof(table_ref<K, M>)
+    static_assert(sizeof(table_ref<uint64_t, uint64_t>) <= 64, "descriptor size");
+    if (hipMalloc(&b->tables, desc) != hipSuccess ||
+        hipMalloc(&b->chunk_start, sizeof(int64_t) * (static_cast<size_t>(ntables) + 1)) != hipSuccess ||
+        hipMalloc(&b->counter, 256) != hipSuccess)
+    {
+        if (b->tables) (void)hipFree(b->tables);
+        if (b->chunk_start) (void)hipFree(b->chunk_start);
+        delete b;
+        return fail(HTAB_E_OOM, "htab_batch_create: device allocation");
+    }
+    const int rc = with_ops<batch_ops>(b->kind, b->lanes, [&](auto o) { return decltype(o)::build(b, tables); });
+    if (rc != HTAB_OK)
+    {
+        htab_batch_destroy(b);
+        return rc;
+    }
+    *out = b;
+    return HTAB_OK;
+}
+
+int
+htab_batch_destroy(htab_batch* b)
+{
+    if (!b) return fail(HTAB_E_ARG, "htab_batch_destroy: null handle");
+    if (b->tables) (void)hipFree(b->tables);
+    if (b->chunk_start) (void)hipFree(b->chunk_start);
+    if (b->counter) (void)hipFree(b->counter);
+    delete b;
+    return HTAB_OK;
+}
+
+int
+htab_batch_run(htab_batch* b, const int8_t* ops, int op, const int64_t* offsets, const void* keys, const void* vals,
+               uint8_t* flags, void* vals_out, void* stream)
+{
+    if (!b || !offsets || !keys) return fail(HTAB_E_ARG, "htab_batch_run: null argument");
+    if (!ops && (op < HTAB_OP_INSERT || op > HTAB_OP_ERASE)) return fail(HTAB_E_ARG, "htab_batch_run: bad op");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    return with_ops<batch_ops>(b->kind, b->lanes, [&](auto o) {
+        return decltype(o)::run(b, ops, op, offsets, keys, vals, flags, vals_out, s);
+    });
+}
+
+int
+htab_batch_capture(htab_batch* b, const int8_t* ops, int op, const int64_t* offsets, const void* keys,
+                   const void* vals, uint8_t* flags, void* vals_out, void* stream, htab_graph** out)
+{
+    if (!b || !out) return fail(HTAB_E_ARG, "htab_batch_capture: null argument");
+    if (!stream) return fail(HTAB_E_ARG, "htab_batch_capture: capture needs a created stream");
+    *out = nullptr;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    HTAB_HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "htab_batch_capture begin");
+    const int rc = htab_batch_run(b, ops, op, offsets, keys, vals, flags, vals_out, stream);
+    hipGraph_t graph = nullptr;
+    const hipError_t e = hipStreamEndCapture(s, &graph);
+    if (rc != HTAB_OK)
+    {
+        if (graph) (void)hipGraphDestroy(graph);
+        return rc;
+    }
+    if (e != hipSuccess) return fail_hip(e, "htab_batch_capture end");
+    htab_graph* g = new (std::nothrow) htab_graph();
+    if (!g)
+    {
+        (void)hipGraphDestroy(graph);
+        return fail(HTAB_E_OOM, "htab_batch_capture: host allocation");
+    }
+    g->graph = graph;
+    const hipError_t ei = hipGraphInstantiate(&g->exec, graph, nullptr, nullptr, 0);
+    if (ei != hipSuccess)
+    {
+        (void)hipGraphDestroy(graph);
+        delete g;
+        return fail_hip(ei, "htab_batch_capture instantiate");
+    }
+    *out = g;
+    return HTAB_OK;
+}
+
+int
+htab_graph_launch(htab_graph* g, void* stream)
+{
+    if (!g) return fail(HTAB_E_ARG, "htab_graph_launch: null handle");
+    HTAB_HIP_TRY(hipGraphLaunch(g->exec, static_cast<hipStream_t>(stream)), "htab_graph_launch");
+    return HTAB_OK;
+}
+
+int
+htab_graph_destroy(htab_graph* g)
+{
+    if (!g) return fail(HTAB_E_ARG, "htab_graph_destroy: null handle");
+    if (g->exec) (void)hipGraphExecDestroy(g->exec);
+    if (g->graph) (void)hipGraphDestroy(g->graph);
+    delete g;
     return HTAB_OK;
 }
 

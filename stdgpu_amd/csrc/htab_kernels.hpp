@@ -145,24 +145,60 @@ block_count_add(int64_t* stripes, uint32_t nstripes, int64_t delta, int64_t* s_w
     return *s_old;
 }
 
+__device__ inline int64_t
+block_exclusive_scan(int64_t v, int64_t* s_tmp, int64_t* total)
+{
+    // LDS-staged scan: wave-level inclusive scan with shuffles, then wave totals in LDS
+    const int lane = lane_id();
+    const int wid = static_cast<int>(threadIdx.x) >> 6;
+    const int nw = static_cast<int>(blockDim.x) >> 6;
+    int64_t x = v;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1)
+    {
+        const int64_t y = static_cast<int64_t>(shfl64(static_cast<uint64_t>(x), lane >= off ? lane - off : lane));
+        if (lane >= off) x += y;
+    }
+    if (lane == 63) s_tmp[wid] = x;
+    __syncthreads();
+    if (threadIdx.x == 0)
+    {
+        int64_t run = 0;
+        for (int w = 0; w < nw; ++w)
+        {
+            const int64_t c = s_tmp[w];
+            s_tmp[w] = run;
+            run += c;
+        }
+        s_tmp[nw] = run;
+    }
+    __syncthreads();
+    const int64_t res = s_tmp[wid] + x - v;
+    *total = s_tmp[nw];
+    __syncthreads();
+    return res;
+}
+
 // ================================================================== find
+// Rounds: every pending element of the group's sub-batch loads its current bucket (all loads of
+// a round are in flight together), then each is resolved by ballot: a key match (hit), an EMPTY
+// slot (miss) or neither (continue in the next bucket).  Elements still pending after
+// kBatchRounds buckets finish on the serial path.
+constexpr int kBatchRounds = 8;
+
 template <typename K, typename M, int G>
-__global__ __launch_bounds__(1024) void
-k_find(table_ref<K, M> t, const K* __restrict__ keys, int64_t n, uint8_t* __restrict__ hits,
-       void* __restrict__ vals_out)
+__device__ __forceinline__ void
+find_body(const table_ref<K, M>& t, const K* __restrict__ keys, const int64_t i, const bool valid,
+          uint8_t* __restrict__ hits, void* __restrict__ vals_out)
 {
     using C = chunk<K, M>;
-    using value_type = typename C::value_type;
-    constexpr int SPL = C::SPL;
-    constexpr int R = G < 8 ? G : 8; // first probes in flight per lane
+    constexpr int R = G < 4 ? G : 4; // elements probed together per sub-batch
     const int lane = lane_id();
     const int gbase = lane & ~(G - 1);
     const int li = lane & (G - 1);
     const uint64_t gmask = ((G == 64) ? ~0ULL : ((1ULL << G) - 1)) << gbase;
-    const uint32_t W = static_cast<uint32_t>(G * SPL);
+    const uint32_t W = static_cast<uint32_t>(G * C::SPL);
 
-    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    const bool valid = i < n;
     const K mykey = valid ? keys[i] : empty_key<K>();
     bool myhit = false;
     uint64_t myval = 0;
@@ -175,68 +211,81 @@ k_find(table_ref<K, M> t, const K* __restrict__ keys, int64_t n, uint8_t* __rest
             if constexpr (slot_traits<K, M>::is_map) myval = static_cast<uint64_t>(t.slots[t.side_index(mykey)].second);
         }
     }
-    const bool probe = valid && !reserved;
-    uint32_t pend = 0; // group-uniform: elements not resolved by their home bucket
+    const uint32_t probe = (valid && !reserved) ? 1U : 0U;
+    uint32_t slow = 0; // group-uniform: elements left to the serial path
 
 #pragma unroll
     for (int rb = 0; rb < G; rb += R)
     {
-        C q[R];
         K kr[R];
-        bool act[R];
+        uint64_t b[R];
+        uint32_t pend = 0;
 #pragma unroll
         for (int r = 0; r < R; ++r)
         {
             kr[r] = shfl(mykey, gbase + rb + r);
-            act[r] = shfl32(probe ? 1U : 0U, gbase + rb + r) != 0;
-            if (act[r])
-            {
-                const uint64_t b = home_bucket(static_cast<uint64_t>(kr[r]), t.nbuckets);
-                q[r] = C::load_plain(t.slots + b * W, li);
-            }
+            b[r] = home_bucket(static_cast<uint64_t>(kr[r]), t.nbuckets);
+            pend |= shfl32(probe, gbase + rb + r) << r;
         }
-#pragma unroll
-        for (int r = 0; r < R; ++r)
+        for (int round = 0;; ++round)
         {
-            if (act[r])
+            if (ballot(pend != 0) == 0) break;
+            if (round == kBatchRounds)
             {
-                const K k = kr[r];
-                const uint32_t m = q[r].mask([k](K x) { return x == k; });
-                const uint32_t e = q[r].mask([](K x) { return x == empty_key<K>(); });
-                const uint64_t bm = ballot(m != 0) & gmask;
-                const uint64_t be = ballot(e != 0) & gmask;
-                if (bm)
+                slow |= pend << rb;
+                break;
+            }
+            C q[R];
+#pragma unroll
+            for (int r = 0; r < R; ++r)
+                if (pend & (1U << r)) q[r] = C::load_plain(t.slots + b[r] * W, li);
+#pragma unroll
+            for (int r = 0; r < R; ++r)
+            {
+                if (pend & (1U << r))
                 {
-                    const int src = ctz64(bm);
-                    const uint64_t cand = m ? q[r].mapped(__builtin_ctz(m)) : 0;
-                    const uint64_t v = shfl64(cand, src);
-                    if (li == rb + r)
+                    const K k = kr[r];
+                    const uint32_t m = q[r].mask([k](K x) { return x == k; });
+                    const uint32_t e = q[r].mask([](K x) { return x == empty_key<K>(); });
+                    const uint64_t bm = ballot(m != 0) & gmask;
+                    const uint64_t be = ballot(e != 0) & gmask;
+                    if (bm)
                     {
-                        myhit = true;
-                        myval = v;
+                        const uint64_t cand = m ? q[r].mapped(__builtin_ctz(m)) : 0;
+                        const uint64_t v = shfl64(cand, ctz64(bm));
+                        if (li == rb + r)
+                        {
+                            myhit = true;
+                            myval = v;
+                        }
+                        pend &= ~(1U << r);
                     }
-                }
-                else if (!be)
-                {
-                    pend |= 1U << (rb + r);
+                    else if (be)
+                    {
+                        pend &= ~(1U << r);
+                    }
+                    else
+                    {
+                        b[r] = next_bucket(b[r], t.nbuckets);
+                    }
                 }
             }
         }
     }
 
-    // overflowed elements: walk the probe sequence one group element at a time
+    // serial path: one element of the group at a time, from its home bucket, with wrap detection
     int cur = -1;
     K ck = 0;
     uint64_t b = 0, steps = 0;
     while (true)
     {
-        if (cur < 0 && pend)
+        if (cur < 0 && slow)
         {
-            cur = __builtin_ctz(pend);
-            pend &= pend - 1;
+            cur = __builtin_ctz(slow);
+            slow &= slow - 1;
             ck = shfl(mykey, gbase + cur);
-            b = next_bucket(home_bucket(static_cast<uint64_t>(ck), t.nbuckets), t.nbuckets);
-            steps = 1;
+            b = home_bucket(static_cast<uint64_t>(ck), t.nbuckets);
+            steps = 0;
         }
         const bool active = cur >= 0;
         if (ballot(active) == 0) break;
@@ -250,9 +299,8 @@ k_find(table_ref<K, M> t, const K* __restrict__ keys, int64_t n, uint8_t* __rest
             const uint64_t be = ballot(e != 0) & gmask;
             if (bm)
             {
-                const int src = ctz64(bm);
                 const uint64_t cand = m ? q.mapped(__builtin_ctz(m)) : 0;
-                const uint64_t v = shfl64(cand, src);
+                const uint64_t v = shfl64(cand, ctz64(bm));
                 if (li == cur)
                 {
                     myhit = true;
@@ -309,15 +357,22 @@ fetch_element(const void* values, const void* keys, const void* vals, int64_t i,
 
 constexpr uint64_t kNoSlot = ~0ULL;
 
+// Insert-only launch.  Each pending element scans its probe sequence up to the first bucket
+// with an EMPTY slot (a key found on the way means "already present") while remembering the
+// first free slot (EMPTY or TOMB) it passed; it then claims that slot with one CAS.  All CASes
+// of a round are issued before any result is consumed.  A lost CAS (another key took the slot)
+// sends the element to the serial path, which rescans and resumes after the lost slot.
+// Uniqueness: free -> key is the only transition in an insert-only launch, so every key
+// inserted here sits at the first free slot of its probe sequence as of its CAS, and a racing
+// inserter of the same key either sees it there or loses the CAS on that very slot.
 template <typename K, typename M, int G>
-__global__ __launch_bounds__(1024) void
-k_insert(table_ref<K, M> t, const void* __restrict__ values, const void* __restrict__ keys,
-         const void* __restrict__ vals, int64_t n, uint8_t* __restrict__ inserted)
+__device__ __forceinline__ void
+insert_body(const table_ref<K, M>& t, const void* __restrict__ values, const void* __restrict__ keys,
+            const void* __restrict__ vals, const int64_t i, const bool valid, uint8_t* __restrict__ inserted)
 {
     using C = chunk<K, M>;
-    using value_type = typename C::value_type;
     constexpr int SPL = C::SPL;
-    constexpr int R = G < 8 ? G : 8;
+    constexpr int R = G < 4 ? G : 4;
     __shared__ int64_t s_wave[16];
     __shared__ int64_t s_old;
     const int lane = lane_id();
@@ -326,8 +381,6 @@ k_insert(table_ref<K, M> t, const void* __restrict__ values, const void* __restr
     const uint64_t gmask = ((G == 64) ? ~0ULL : ((1ULL << G) - 1)) << gbase;
     const uint32_t W = static_cast<uint32_t>(G * SPL);
 
-    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    const bool valid = i < n;
     K mykey = empty_key<K>();
     uint64_t myv = 0;
     if (valid) fetch_element<K, M>(values, keys, vals, i, &mykey, &myv);
@@ -349,85 +402,115 @@ k_insert(table_ref<K, M> t, const void* __restrict__ values, const void* __restr
             myins = true;
         }
     }
-    const bool probe = valid && !reserved && !at_limit;
-    uint32_t pend = 0;
+    const uint32_t probe = (valid && !reserved && !at_limit) ? 1U : 0U;
+    uint32_t slow = 0;
 
 #pragma unroll
     for (int rb = 0; rb < G; rb += R)
     {
-        C q[R];
         K kr[R];
-        bool act[R];
-        uint64_t hb[R];
+        uint64_t b[R], ff[R];
+        uint32_t pend = 0, ff_tomb = 0;
 #pragma unroll
         for (int r = 0; r < R; ++r)
         {
             kr[r] = shfl(mykey, gbase + rb + r);
-            act[r] = shfl32(probe ? 1U : 0U, gbase + rb + r) != 0;
-            hb[r] = 0;
-            if (act[r])
-            {
-                hb[r] = home_bucket(static_cast<uint64_t>(kr[r]), t.nbuckets);
-                q[r] = C::load_keys(t.slots + hb[r] * W, li);
-            }
+            b[r] = home_bucket(static_cast<uint64_t>(kr[r]), t.nbuckets);
+            ff[r] = kNoSlot;
+            pend |= shfl32(probe, gbase + rb + r) << r;
         }
-#pragma unroll
-        for (int r = 0; r < R; ++r)
+        for (int round = 0;; ++round)
         {
-            if (act[r])
+            if (ballot(pend != 0) == 0) break;
+            if (round == kBatchRounds)
             {
-                const K k = kr[r];
-                const uint32_t m = q[r].mask([k](K x) { return x == k; });
-                const uint32_t e = q[r].mask([](K x) { return x == empty_key<K>(); });
-                const uint32_t f = q[r].mask([](K x) { return x == empty_key<K>() || x == tomb_key<K>(); });
-                const uint64_t bm = ballot(m != 0) & gmask;
-                const uint64_t be = ballot(e != 0) & gmask;
-                const uint64_t bf = ballot(f != 0) & gmask;
-                if (bm)
+                slow |= pend << rb;
+                break;
+            }
+            C q[R];
+#pragma unroll
+            for (int r = 0; r < R; ++r)
+                if (pend & (1U << r)) q[r] = C::load_keys(t.slots + b[r] * W, li);
+            uint32_t claim = 0; // elements whose probe sequence ended this round
+#pragma unroll
+            for (int r = 0; r < R; ++r)
+            {
+                if (pend & (1U << r))
                 {
-                    // key present: nothing to do (insert never overwrites)
-                }
-                else if (be)
-                {
-                    // probe sequence ends in the home bucket: claim its first free slot
-                    const int fl = ctz64(bf); // lane holding the first free slot
-                    // the value travels from the owner lane; the claiming lane stores it
-                    const uint64_t v = shfl64(myv, gbase + rb + r);
-                    uint64_t seen = 0;
-                    uint64_t slot = 0;
-                    if (lane == fl)
+                    const K k = kr[r];
+                    const uint32_t m = q[r].mask([k](K x) { return x == k; });
+                    const uint32_t e = q[r].mask([](K x) { return x == empty_key<K>(); });
+                    const uint32_t f = q[r].mask([](K x) { return x == empty_key<K>() || x == tomb_key<K>(); });
+                    const uint64_t bm = ballot(m != 0) & gmask;
+                    const uint64_t be = ballot(e != 0) & gmask;
+                    const uint64_t bf = ballot(f != 0) & gmask;
+                    if (bm)
                     {
-                        const int j = __builtin_ctz(f);
-                        const K obs = q[r].key(j);
-                        slot = hb[r] * W + static_cast<uint64_t>(li * SPL + j);
-                        const K got = cas_agent(t.key_ptr(slot), obs, k);
-                        seen = (got == obs) ? 1 : ((got == k) ? 2 : 0);
-                        if (seen == 1) store_mapped<K, M>(t.slots + slot, v);
+                        pend &= ~(1U << r); // present: insert never overwrites
                     }
-                    seen = shfl64(seen, fl);
-                    slot = shfl64(slot, fl);
-                    if (seen == 1)
+                    else
                     {
+                        if (ff[r] == kNoSlot && bf)
+                        {
+                            const int fl = ctz64(bf);
+                            uint64_t cand = 0;
+                            uint32_t tomb = 0;
+                            if (lane == fl)
+                            {
+                                const int j = __builtin_ctz(f);
+                                cand = b[r] * W + static_cast<uint64_t>(li * SPL + j);
+                                tomb = (q[r].key(j) == tomb_key<K>()) ? 1U : 0U;
+                            }
+                            ff[r] = shfl64(cand, fl);
+                            ff_tomb |= shfl32(tomb, fl) << r;
+                        }
+                        if (be)
+                            claim |= 1U << r;
+                        else
+                            b[r] = next_bucket(b[r], t.nbuckets);
+                    }
+                }
+            }
+            // claims: the group's first lane issues every CAS of the round back to back
+            K got[R];
+#pragma unroll
+            for (int r = 0; r < R; ++r)
+            {
+                got[r] = 0;
+                if ((claim & (1U << r)) && li == 0)
+                {
+                    const K expect = (ff_tomb & (1U << r)) ? tomb_key<K>() : empty_key<K>();
+                    got[r] = cas_agent(t.key_ptr(ff[r]), expect, kr[r]);
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < R; ++r)
+            {
+                if (claim & (1U << r))
+                {
+                    const K seen = shfl(got[r], gbase);
+                    const K expect = (ff_tomb & (1U << r)) ? tomb_key<K>() : empty_key<K>();
+                    pend &= ~(1U << r);
+                    if (seen == expect)
+                    {
+                        const uint64_t v = shfl64(myv, gbase + rb + r);
+                        if (li == 0) store_mapped<K, M>(t.slots + ff[r], v);
                         if (li == rb + r)
                         {
                             myins = true;
-                            myslot = slot;
+                            myslot = ff[r];
                         }
                     }
-                    else if (seen == 0)
+                    else if (seen != kr[r])
                     {
-                        pend |= 1U << (rb + r);
+                        slow |= 1U << (rb + r); // lost the slot to another key
                     }
-                }
-                else
-                {
-                    pend |= 1U << (rb + r);
                 }
             }
         }
     }
 
-    // slow path: full probe sequence with first-free tracking, CAS, and resume after a lost CAS
+    // serial path: full probe sequence with first-free tracking, CAS, and resume after a lost CAS
     int cur = -1;
     K ck = 0;
     uint64_t cv = 0;
@@ -436,10 +519,10 @@ k_insert(table_ref<K, M> t, const void* __restrict__ values, const void* __restr
     uint32_t jmin = 0;
     while (true)
     {
-        if (cur < 0 && pend)
+        if (cur < 0 && slow)
         {
-            cur = __builtin_ctz(pend);
-            pend &= pend - 1;
+            cur = __builtin_ctz(slow);
+            slow &= slow - 1;
             ck = shfl(mykey, gbase + cur);
             cv = shfl64(myv, gbase + cur);
             hb = home_bucket(static_cast<uint64_t>(ck), t.nbuckets);
@@ -561,14 +644,16 @@ k_insert(table_ref<K, M> t, const void* __restrict__ values, const void* __restr
 // ================================================================== erase
 // Erase-only launch: a slot is reset to EMPTY when its bucket already holds an EMPTY slot (no
 // probe sequence runs through such a bucket, and none can start to while only erases run),
-// otherwise to TOMB so probe sequences through the bucket stay intact.
+// otherwise to TOMB so probe sequences through the bucket stay intact.  Exactly one of several
+// erasers of one key wins the CAS.
 template <typename K, typename M, int G>
-__global__ __launch_bounds__(1024) void
-k_erase(table_ref<K, M> t, const K* __restrict__ keys, int64_t n, uint8_t* __restrict__ erased)
+__device__ __forceinline__ void
+erase_body(const table_ref<K, M>& t, const K* __restrict__ keys, const int64_t i, const bool valid,
+           uint8_t* __restrict__ erased)
 {
     using C = chunk<K, M>;
     constexpr int SPL = C::SPL;
-    constexpr int R = G < 8 ? G : 8;
+    constexpr int R = G < 4 ? G : 4;
     __shared__ int64_t s_wave[16];
     __shared__ int64_t s_old;
     const int lane = lane_id();
@@ -577,8 +662,6 @@ k_erase(table_ref<K, M> t, const K* __restrict__ keys, int64_t n, uint8_t* __res
     const uint64_t gmask = ((G == 64) ? ~0ULL : ((1ULL << G) - 1)) << gbase;
     const uint32_t W = static_cast<uint32_t>(G * SPL);
 
-    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    const bool valid = i < n;
     const K mykey = valid ? keys[i] : empty_key<K>();
     bool mydel = false;
     const bool reserved = valid && table_ref<K, M>::is_reserved(mykey);
@@ -588,54 +671,84 @@ k_erase(table_ref<K, M> t, const K* __restrict__ keys, int64_t n, uint8_t* __res
         const uint32_t old = __hip_atomic_fetch_and(t.flags, ~bit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         mydel = (old & bit) != 0;
     }
-    const bool probe = valid && !reserved;
-    uint32_t pend = 0;
+    const uint32_t probe = (valid && !reserved) ? 1U : 0U;
+    uint32_t slow = 0;
 
 #pragma unroll
     for (int rb = 0; rb < G; rb += R)
     {
-        C q[R];
         K kr[R];
-        bool act[R];
-        uint64_t hb[R];
+        uint64_t b[R];
+        uint32_t pend = 0;
 #pragma unroll
         for (int r = 0; r < R; ++r)
         {
             kr[r] = shfl(mykey, gbase + rb + r);
-            act[r] = shfl32(probe ? 1U : 0U, gbase + rb + r) != 0;
-            hb[r] = 0;
-            if (act[r])
-            {
-                hb[r] = home_bucket(static_cast<uint64_t>(kr[r]), t.nbuckets);
-                q[r] = C::load_keys(t.slots + hb[r] * W, li);
-            }
+            b[r] = home_bucket(static_cast<uint64_t>(kr[r]), t.nbuckets);
+            pend |= shfl32(probe, gbase + rb + r) << r;
         }
-#pragma unroll
-        for (int r = 0; r < R; ++r)
+        for (int round = 0;; ++round)
         {
-            if (act[r])
+            if (ballot(pend != 0) == 0) break;
+            if (round == kBatchRounds)
             {
-                const K k = kr[r];
-                const uint32_t m = q[r].mask([k](K x) { return x == k; });
-                const uint32_t e = q[r].mask([](K x) { return x == empty_key<K>(); });
-                const uint64_t bm = ballot(m != 0) & gmask;
-                const uint64_t be = ballot(e != 0) & gmask;
-                if (bm)
+                slow |= pend << rb;
+                break;
+            }
+            C q[R];
+#pragma unroll
+            for (int r = 0; r < R; ++r)
+                if (pend & (1U << r)) q[r] = C::load_keys(t.slots + b[r] * W, li);
+            uint32_t hit = 0, to_empty = 0;
+            uint64_t tgt[R];
+#pragma unroll
+            for (int r = 0; r < R; ++r)
+            {
+                tgt[r] = 0;
+                if (pend & (1U << r))
                 {
-                    const int ml = ctz64(bm);
-                    uint64_t ok = 0;
-                    if (lane == ml)
+                    const K k = kr[r];
+                    const uint32_t m = q[r].mask([k](K x) { return x == k; });
+                    const uint32_t e = q[r].mask([](K x) { return x == empty_key<K>(); });
+                    const uint64_t bm = ballot(m != 0) & gmask;
+                    const uint64_t be = ballot(e != 0) & gmask;
+                    if (bm)
                     {
-                        const uint64_t slot = hb[r] * W + static_cast<uint64_t>(li * SPL + __builtin_ctz(m));
-                        const K repl = be ? empty_key<K>() : tomb_key<K>();
-                        ok = (cas_agent(t.key_ptr(slot), k, repl) == k) ? 1 : 0;
+                        const int ml = ctz64(bm);
+                        const uint64_t cand = m ? b[r] * W + static_cast<uint64_t>(li * SPL + __builtin_ctz(m)) : 0;
+                        tgt[r] = shfl64(cand, ml);
+                        hit |= 1U << r;
+                        if (be) to_empty |= 1U << r;
+                        pend &= ~(1U << r);
                     }
-                    ok = shfl64(ok, ml);
-                    if (ok && li == rb + r) mydel = true;
+                    else if (be)
+                    {
+                        pend &= ~(1U << r); // absent
+                    }
+                    else
+                    {
+                        b[r] = next_bucket(b[r], t.nbuckets);
+                    }
                 }
-                else if (!be)
+            }
+            K got[R];
+#pragma unroll
+            for (int r = 0; r < R; ++r)
+            {
+                got[r] = 0;
+                if ((hit & (1U << r)) && li == 0)
                 {
-                    pend |= 1U << (rb + r);
+                    const K repl = (to_empty & (1U << r)) ? empty_key<K>() : tomb_key<K>();
+                    got[r] = cas_agent(t.key_ptr(tgt[r]), kr[r], repl);
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < R; ++r)
+            {
+                if (hit & (1U << r))
+                {
+                    const K seen = shfl(got[r], gbase);
+                    if (seen == kr[r] && li == rb + r) mydel = true;
                 }
             }
         }
@@ -646,13 +759,13 @@ k_erase(table_ref<K, M> t, const K* __restrict__ keys, int64_t n, uint8_t* __res
     uint64_t b = 0, steps = 0;
     while (true)
     {
-        if (cur < 0 && pend)
+        if (cur < 0 && slow)
         {
-            cur = __builtin_ctz(pend);
-            pend &= pend - 1;
+            cur = __builtin_ctz(slow);
+            slow &= slow - 1;
             ck = shfl(mykey, gbase + cur);
-            b = next_bucket(home_bucket(static_cast<uint64_t>(ck), t.nbuckets), t.nbuckets);
-            steps = 1;
+            b = home_bucket(static_cast<uint64_t>(ck), t.nbuckets);
+            steps = 0;
         }
         const bool active = cur >= 0;
         if (ballot(active) == 0) break;
@@ -692,6 +805,131 @@ k_erase(table_ref<K, M> t, const K* __restrict__ keys, int64_t n, uint8_t* __res
     int64_t wave_prefix = 0;
     block_count_add(t.stripes, t.nstripes, mydel ? -1 : 0, s_wave, &s_old, &wave_prefix);
     if (valid && erased) erased[i] = mydel ? 1 : 0;
+}
+
+
+// ================================================================== plain bulk kernels
+template <typename K, typename M, int G>
+__global__ __launch_bounds__(1024) void
+k_find(table_ref<K, M> t, const K* __restrict__ keys, int64_t n, uint8_t* __restrict__ hits,
+       void* __restrict__ vals_out)
+{
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    find_body<K, M, G>(t, keys, i, i < n, hits, vals_out);
+}
+
+template <typename K, typename M, int G>
+__global__ __launch_bounds__(1024) void
+k_insert(table_ref<K, M> t, const void* __restrict__ values, const void* __restrict__ keys,
+         const void* __restrict__ vals, int64_t n, uint8_t* __restrict__ inserted)
+{
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    insert_body<K, M, G>(t, values, keys, vals, i, i < n, inserted);
+}
+
+template <typename K, typename M, int G>
+__global__ __launch_bounds__(1024) void
+k_erase(table_ref<K, M> t, const K* __restrict__ keys, int64_t n, uint8_t* __restrict__ erased)
+{
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    erase_body<K, M, G>(t, keys, i, i < n, erased);
+}
+
+
+// ================================================================== batched: many tables, one launch
+// Table t owns elements [offsets[t], offsets[t+1]) of the flattened key / value / flag arrays and
+// runs operation ops[t] (or `op` for every table when ops is null).  Work is cut into chunks of
+// blockDim elements that never straddle two tables; chunk_start[t] counts the chunks of tables
+// before t (k_batch_plan).  A persistent grid pulls chunks from one counter until all are done, so
+// every workgroup exits once the counter passes the last chunk.
+constexpr int kOpInsert = 0;
+constexpr int kOpFind = 1;
+constexpr int kOpErase = 2;
+constexpr int kBatchBlock = 256;
+
+__global__ __launch_bounds__(1024) void
+k_batch_plan(const int64_t* __restrict__ offsets, int32_t ntables, int64_t* __restrict__ chunk_start,
+             unsigned long long* __restrict__ counter)
+{
+    __shared__ int64_t s_tmp[17];
+    __shared__ int64_t s_carry;
+    if (threadIdx.x == 0)
+    {
+        s_carry = 0;
+        *counter = 0;
+    }
+    __syncthreads();
+    for (int64_t base = 0; base < ntables; base += blockDim.x)
+    {
+        const int64_t t = base + threadIdx.x;
+        int64_t c = 0;
+        if (t < ntables)
+        {
+            const int64_t cnt = offsets[t + 1] - offsets[t];
+            c = cnt > 0 ? (cnt + kBatchBlock - 1) / kBatchBlock : 0;
+        }
+        int64_t total = 0;
+        const int64_t ex = block_exclusive_scan(c, s_tmp, &total);
+        if (t < ntables) chunk_start[t] = s_carry + ex;
+        __syncthreads();
+        if (threadIdx.x == 0) s_carry += total;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) chunk_start[ntables] = s_carry;
+}
+
+template <typename K, typename M, int G>
+__global__ __launch_bounds__(kBatchBlock) void
+k_batch(const table_ref<K, M>* __restrict__ tables, int32_t ntables, const int64_t* __restrict__ offsets,
+        const int8_t* __restrict__ ops, int op, const int64_t* __restrict__ chunk_start,
+        unsigned long long* __restrict__ counter, const void* __restrict__ keys, const void* __restrict__ vals,
+        uint8_t* __restrict__ flags, void* __restrict__ vals_out)
+{
+    __shared__ int s_table;
+    __shared__ int64_t s_begin, s_end;
+    const int64_t total_chunks = chunk_start[ntables];
+    while (true)
+    {
+        if (threadIdx.x == 0)
+        {
+            const int64_t c = static_cast<int64_t>(atomicAdd(counter, 1ULL));
+            int t = -1;
+            int64_t b = 0, e = 0;
+            if (c < total_chunks)
+            {
+                int lo = 0, hi = ntables; // last t with chunk_start[t] <= c
+                while (lo < hi)
+                {
+                    const int mid = (lo + hi + 1) >> 1;
+                    if (chunk_start[mid] <= c)
+                        lo = mid;
+                    else
+                        hi = mid - 1;
+                }
+                t = lo;
+                b = offsets[t] + (c - chunk_start[t]) * kBatchBlock;
+                e = b + kBatchBlock < offsets[t + 1] ? b + kBatchBlock : offsets[t + 1];
+            }
+            s_table = t;
+            s_begin = b;
+            s_end = e;
+        }
+        __syncthreads();
+        const int t = s_table;
+        const int64_t b = s_begin, e = s_end;
+        __syncthreads();
+        if (t < 0) break;
+        const table_ref<K, M> tr = tables[t];
+        const int o = ops ? static_cast<int>(ops[t]) : op;
+        const int64_t i = b + threadIdx.x;
+        const bool valid = i < e;
+        if (o == kOpInsert)
+            insert_body<K, M, G>(tr, nullptr, keys, vals, i, valid, flags);
+        else if (o == kOpFind)
+            find_body<K, M, G>(tr, static_cast<const K*>(keys), i, valid, flags, vals_out);
+        else if (o == kOpErase)
+            erase_body<K, M, G>(tr, static_cast<const K*>(keys), i, valid, flags);
+    }
 }
 
 // ================================================================== clear / fill
@@ -738,39 +976,6 @@ constexpr int kRangeThreads = 256;
 constexpr int kRangePerThread = 16;
 constexpr uint64_t kRangeTile = static_cast<uint64_t>(kRangeThreads) * kRangePerThread;
 
-__device__ inline int64_t
-block_exclusive_scan(int64_t v, int64_t* s_tmp, int64_t* total)
-{
-    // LDS-staged scan: wave-level inclusive scan with shuffles, then wave totals in LDS
-    const int lane = lane_id();
-    const int wid = static_cast<int>(threadIdx.x) >> 6;
-    const int nw = static_cast<int>(blockDim.x) >> 6;
-    int64_t x = v;
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1)
-    {
-        const int64_t y = static_cast<int64_t>(shfl64(static_cast<uint64_t>(x), lane >= off ? lane - off : lane));
-        if (lane >= off) x += y;
-    }
-    if (lane == 63) s_tmp[wid] = x;
-    __syncthreads();
-    if (threadIdx.x == 0)
-    {
-        int64_t run = 0;
-        for (int w = 0; w < nw; ++w)
-        {
-            const int64_t c = s_tmp[w];
-            s_tmp[w] = run;
-            run += c;
-        }
-        s_tmp[nw] = run;
-    }
-    __syncthreads();
-    const int64_t res = s_tmp[wid] + x - v;
-    *total = s_tmp[nw];
-    __syncthreads();
-    return res;
-}
 
 template <typename K, typename M>
 __global__ __launch_bounds__(kRangeThreads) void

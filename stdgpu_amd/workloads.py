@@ -156,3 +156,46 @@ def map_u64_zipf(n: int = 1 << 26, load: float = 0.8, s: float = 1.1, hit: float
     ne = int(len(distinct) * erase_frac)
     erase = distinct[rng.permutation(len(distinct))[:ne]]
     return MapU64Workload(_capacity_for(n, load), keys, vals, queries, erase, erase.copy())
+
+
+# ---------------------------------------------------------------------------------------------
+@dataclass
+class BatchedWorkload:
+    """Many independent map<uint64_t,uint32_t> tables; flattened per-phase arrays with offsets."""
+
+    sizes: np.ndarray          # keys per table
+    capacities: np.ndarray     # capacity per table
+    ins_offsets: np.ndarray    # int64[ntables + 1]
+    ins_keys: np.ndarray       # uint64, unique within each table
+    ins_vals: np.ndarray       # uint32, value_of(key)
+    find_offsets: np.ndarray
+    find_keys: np.ndarray
+    erase_offsets: np.ndarray
+    erase_keys: np.ndarray
+
+
+def batched_maps(ntables: int = 4096, log2_lo: float = 8, log2_hi: float = 18, load: float = 0.8,
+                 hit: float = 0.7, erase_frac: float = 0.3, seed: int = 4) -> BatchedWorkload:
+    """Config: ntables maps whose key counts are log-uniform in [2^lo, 2^hi]; uniform keys with
+    value_of(key); all inserts, then one find per inserted key position (each a key of the same
+    table with probability `hit`, else a fresh key), then erases of the first erase_frac of each
+    table's keys (keys are uniform, so that is a random subset)."""
+    rng = np.random.default_rng(seed)
+    sizes = np.floor(2.0 ** rng.uniform(log2_lo, log2_hi, size=ntables)).astype(np.int64)
+    total = int(sizes.sum())
+    keys = stream(seed, total)
+    keys[keys == U64_MAX] = np.uint64(1)
+    ins_off = np.zeros(ntables + 1, np.int64)
+    ins_off[1:] = np.cumsum(sizes)
+    tid = np.repeat(np.arange(ntables), sizes)
+    pos = np.arange(total, dtype=np.int64) - ins_off[tid]
+    u = (stream(seed + 5, total) >> np.uint64(11)).astype(np.float64) * (1.0 / (1 << 53))
+    v = (stream(seed + 6, total) >> np.uint64(11)).astype(np.float64) * (1.0 / (1 << 53))
+    own = keys[ins_off[tid] + np.minimum((v * sizes[tid]).astype(np.int64), sizes[tid] - 1)]
+    f_keys = np.where(u < hit, own, stream(seed + 7, total))
+    e_sizes = np.floor(sizes * erase_frac).astype(np.int64)
+    e_keys = keys[pos < e_sizes[tid]]
+    e_off = np.zeros(ntables + 1, np.int64)
+    e_off[1:] = np.cumsum(e_sizes)
+    caps = np.ceil(sizes / load).astype(np.int64)
+    return BatchedWorkload(sizes, caps, ins_off, keys, value_of(keys), ins_off.copy(), f_keys, e_off, e_keys)

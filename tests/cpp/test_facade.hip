@@ -1,0 +1,540 @@
+// Source-compatibility tests of the stdgpu::unordered_map / unordered_set facade, written as the
+// reference's own test cases are (tests/stdgpu/unordered_datastructure.inc): user kernels call the
+// native device API (insert / emplace / find / contains / count / erase / bucket_size) on a table
+// passed by value; host code uses createDeviceObject, bulk insert / erase, device_range, size,
+// clear and valid.  Prints one line per case; exits non-zero if any case fails.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+#include <cstdio>
+#include <numeric>
+#include <random>
+#include <set>
+#include <vector>
+
+#include <stdgpu/unordered_map.cuh>
+#include <stdgpu/unordered_set.cuh>
+
+using map_t = stdgpu::unordered_map<std::uint64_t, std::uint32_t>;
+using set_t = stdgpu::unordered_set<std::uint32_t>;
+using imap_t = stdgpu::unordered_map<int, int>;
+using iset_t = stdgpu::unordered_set<int>;
+
+static int g_failures = 0;
+static const char* g_case = "";
+
+#define CHECK(cond)                                                                                                    \
+    do                                                                                                                 \
+    {                                                                                                                  \
+        if (!(cond))                                                                                                   \
+        {                                                                                                              \
+            std::printf("FAIL %s: %s (line %d)\n", g_case, #cond, __LINE__);                                          \
+            ++g_failures;                                                                                              \
+            return;                                                                                                    \
+        }                                                                                                              \
+    } while (0)
+
+#define HIP_OK(x) CHECK((x) == hipSuccess)
+
+template <typename T>
+T*
+dev_alloc(std::size_t n)
+{
+    T* p = nullptr;
+    if (hipMalloc(&p, n * sizeof(T) + 1) != hipSuccess) return nullptr;
+    return p;
+}
+
+template <typename T>
+std::vector<T>
+to_host(const T* d, std::size_t n)
+{
+    std::vector<T> h(n);
+    if (n) (void)hipMemcpy(h.data(), d, n * sizeof(T), hipMemcpyDeviceToHost);
+    return h;
+}
+
+template <typename T>
+T*
+to_dev(const std::vector<T>& h)
+{
+    T* d = dev_alloc<T>(h.size());
+    if (!h.empty()) (void)hipMemcpy(d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice);
+    return d;
+}
+
+static unsigned
+blocks(std::size_t n)
+{
+    return static_cast<unsigned>((n + 255) / 256);
+}
+
+// ------------------------------------------------------------------ kernels (user code)
+template <typename Table, typename Key>
+__global__ void
+k_insert_keys(Table t, const Key* keys, std::size_t n, int* ok)
+{
+    const std::size_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if constexpr (std::is_same_v<typename Table::value_type, Key>)
+        ok[i] = t.insert(keys[i]).second ? 1 : 0;
+    else
+        ok[i] = t.insert(typename Table::value_type(keys[i], static_cast<typename Table::mapped_type>(keys[i] * 3 + 1)))
+                                .second
+                        ? 1
+                        : 0;
+}
+
+template <typename Table, typename Key>
+__global__ void
+k_emplace_keys(Table t, const Key* keys, std::size_t n, int* ok)
+{
+    const std::size_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    ok[i] = t.emplace(keys[i], static_cast<typename Table::mapped_type>(keys[i] * 3 + 1)).second ? 1 : 0;
+}
+
+template <typename Table, typename Key>
+__global__ void
+k_erase_keys(Table t, const Key* keys, std::size_t n, int* ok)
+{
+    const std::size_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    ok[i] = static_cast<int>(t.erase(keys[i]));
+}
+
+template <typename Table, typename Key>
+__global__ void
+k_find_keys(Table t, const Key* keys, std::size_t n, int* found, std::uint64_t* val)
+{
+    const std::size_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    auto it = t.find(keys[i]);
+    found[i] = (it != t.end()) ? 1 : 0;
+    if (t.contains(keys[i]) != (it != t.end())) found[i] = 99;
+    if (t.count(keys[i]) != found[i]) found[i] = 98;
+    if constexpr (!std::is_same_v<typename Table::value_type, Key>)
+    {
+        if (it != t.end())
+        {
+            val[i] = static_cast<std::uint64_t>(it->second);
+            if (it->first != keys[i]) found[i] = 97;
+        }
+    }
+}
+
+template <typename Table>
+__global__ void
+k_bucket_sizes(Table t, int* sizes)
+{
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= t.bucket_count()) return;
+    sizes[n] = t.bucket_size(n);
+}
+
+template <typename Table, typename Key>
+__global__ void
+k_insert_then_erase(Table t, const Key* keys, std::size_t n, int* ok)
+{
+    const std::size_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const bool a = t.insert(typename Table::value_type(keys[i], 1)).second;
+    const bool b = t.erase(keys[i]) == 1;
+    ok[i] = (a && b) ? 1 : 0;
+}
+
+template <typename Range, typename T>
+__global__ void
+k_range_copy(Range r, T* out)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= r.size()) return;
+    new (&out[i]) T(*(r.begin() + i));
+}
+
+template <typename Table>
+__global__ void
+k_device_size(Table t, int* out)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) out[0] = t.size();
+}
+
+// ------------------------------------------------------------------ helpers
+template <typename Table, typename Key>
+std::vector<int>
+run_insert(Table& t, const std::vector<Key>& keys, bool emplace = false)
+{
+    Key* dk = to_dev(keys);
+    int* ok = dev_alloc<int>(keys.size());
+    if constexpr (!std::is_same_v<typename Table::value_type, Key>)
+    {
+        if (emplace)
+        {
+            hipLaunchKernelGGL((k_emplace_keys<Table, Key>), dim3(blocks(keys.size())), dim3(256), 0, 0, t, dk,
+                               keys.size(), ok);
+        }
+        else
+        {
+            hipLaunchKernelGGL((k_insert_keys<Table, Key>), dim3(blocks(keys.size())), dim3(256), 0, 0, t, dk,
+                               keys.size(), ok);
+        }
+    }
+    else
+    {
+        hipLaunchKernelGGL((k_insert_keys<Table, Key>), dim3(blocks(keys.size())), dim3(256), 0, 0, t, dk, keys.size(),
+                           ok);
+    }
+    std::vector<int> h = to_host(ok, keys.size());
+    (void)hipFree(dk);
+    (void)hipFree(ok);
+    return h;
+}
+
+template <typename Table, typename Key>
+std::vector<int>
+run_erase(Table& t, const std::vector<Key>& keys)
+{
+    Key* dk = to_dev(keys);
+    int* ok = dev_alloc<int>(keys.size());
+    hipLaunchKernelGGL((k_erase_keys<Table, Key>), dim3(blocks(keys.size())), dim3(256), 0, 0, t, dk, keys.size(), ok);
+    std::vector<int> h = to_host(ok, keys.size());
+    (void)hipFree(dk);
+    (void)hipFree(ok);
+    return h;
+}
+
+template <typename Table, typename Key>
+std::vector<int>
+run_find(Table& t, const std::vector<Key>& keys, std::vector<std::uint64_t>* vals = nullptr)
+{
+    Key* dk = to_dev(keys);
+    int* f = dev_alloc<int>(keys.size());
+    std::uint64_t* v = dev_alloc<std::uint64_t>(keys.size());
+    hipLaunchKernelGGL((k_find_keys<Table, Key>), dim3(blocks(keys.size())), dim3(256), 0, 0, t, dk, keys.size(), f, v);
+    std::vector<int> h = to_host(f, keys.size());
+    if (vals) *vals = to_host(v, keys.size());
+    (void)hipFree(dk);
+    (void)hipFree(f);
+    (void)hipFree(v);
+    return h;
+}
+
+static std::vector<std::uint64_t>
+unique_keys(std::size_t n, std::uint64_t seed)
+{
+    std::mt19937_64 rng(seed);
+    std::set<std::uint64_t> s;
+    std::vector<std::uint64_t> out;
+    while (out.size() < n)
+    {
+        const std::uint64_t k = rng() >> 8;
+        if (s.insert(k).second) out.push_back(k);
+    }
+    return out;
+}
+
+static int
+sum(const std::vector<int>& v)
+{
+    return std::accumulate(v.begin(), v.end(), 0);
+}
+
+// ------------------------------------------------------------------ cases
+static void
+empty_container()
+{
+    g_case = "empty_container";
+    map_t m;
+    CHECK(m.empty());
+    CHECK(m.full());
+    CHECK(m.size() == 0);
+    CHECK(m.valid());
+}
+
+static void
+size_limits_and_bucket()
+{
+    g_case = "size_limits_and_bucket";
+    map_t m = map_t::createDeviceObject(100000);
+    CHECK(m.bucket_count() == 131072);
+    CHECK(m.max_size() == 184444);
+    CHECK(m.bucket(1) == 81006 && m.bucket(2) == 30941 && m.bucket(3) == 111948);
+    CHECK(m.bucket(0xFFFFFFFEULL) == 34231 && m.bucket(0x0123456789ABCDEFULL) == 28855);
+    CHECK(m.max_load_factor() == 1.0F && m.load_factor() == 0.0F);
+    map_t::destroyDeviceObject(m);
+    map_t t = map_t::createDeviceObject(1);
+    CHECK(t.bucket_count() == 1 && t.max_size() == 2);
+    map_t::destroyDeviceObject(t);
+}
+
+static void
+insert_find_erase_single()
+{
+    g_case = "insert_find_erase_single";
+    map_t m = map_t::createDeviceObject(100000);
+    const std::vector<std::uint64_t> k{ 7, 42 };
+    CHECK(sum(run_insert(m, k)) == 2);
+    CHECK(m.size() == 2 && m.valid());
+    std::vector<std::uint64_t> vals;
+    CHECK(run_find(m, std::vector<std::uint64_t>{ 7, 42, 9 }, &vals) == (std::vector<int>{ 1, 1, 0 }));
+    CHECK(vals[0] == 22 && vals[1] == 127);
+    // insert_double: a second insert of the same key reports false and keeps the value
+    CHECK(sum(run_insert(m, std::vector<std::uint64_t>{ 7 })) == 0);
+    CHECK(m.size() == 2);
+    // erase_double
+    CHECK(sum(run_erase(m, std::vector<std::uint64_t>{ 7 })) == 1);
+    CHECK(sum(run_erase(m, std::vector<std::uint64_t>{ 7 })) == 0);
+    CHECK(m.size() == 1 && m.valid());
+    CHECK(run_find(m, std::vector<std::uint64_t>{ 7, 42 }) == (std::vector<int>{ 0, 1 }));
+    map_t::destroyDeviceObject(m);
+}
+
+static void
+insert_erase_multiple_same_key()
+{
+    g_case = "insert_erase_multiple_same_key";
+    map_t m = map_t::createDeviceObject(100000);
+    const std::size_t N = 100000;
+    CHECK(sum(run_insert(m, std::vector<std::uint64_t>(N, 1234))) == 1);
+    CHECK(m.size() == 1 && m.valid());
+    CHECK(sum(run_erase(m, std::vector<std::uint64_t>(N, 1234))) == 1);
+    CHECK(m.size() == 0 && m.valid());
+    map_t::destroyDeviceObject(m);
+}
+
+static void
+tiny_table_cases()
+{
+    g_case = "tiny_table_cases";
+    map_t m = map_t::createDeviceObject(1);
+    CHECK(sum(run_insert(m, std::vector<std::uint64_t>{ 1 })) == 1);
+    CHECK(!m.full() && m.size() == 1);
+    // insert_parallel_while_one_free: 100000 random keys, exactly one wins
+    std::vector<std::uint64_t> r = unique_keys(100000, 77);
+    CHECK(sum(run_insert(m, r)) == 1);
+    CHECK(m.full() && m.size() == 2 && m.valid());
+    // insert_while_full / insert_multiple_while_full
+    CHECK(sum(run_insert(m, std::vector<std::uint64_t>(100000, 999999))) == 0);
+    CHECK(m.size() == 2);
+    map_t::destroyDeviceObject(m);
+    // emplace_parallel_while_one_free
+    map_t e = map_t::createDeviceObject(1);
+    CHECK(sum(run_insert(e, std::vector<std::uint64_t>{ 1 })) == 1);
+    CHECK(sum(run_insert(e, r, true)) == 1);
+    CHECK(e.size() == 2 && e.full());
+    map_t::destroyDeviceObject(e);
+}
+
+static void
+unique_parallel_native()
+{
+    g_case = "unique_parallel_native";
+    const std::size_t N = 100000;
+    map_t m = map_t::createDeviceObject(static_cast<stdgpu::index_t>(N));
+    const std::vector<std::uint64_t> k = unique_keys(N, 5);
+    CHECK(sum(run_insert(m, k, true)) == static_cast<int>(N));
+    CHECK(m.size() == static_cast<stdgpu::index_t>(N) && m.valid());
+    std::vector<std::uint64_t> vals;
+    const std::vector<int> f = run_find(m, k, &vals);
+    CHECK(sum(f) == static_cast<int>(N));
+    for (std::size_t i = 0; i < N; ++i) CHECK(vals[i] == static_cast<std::uint32_t>(k[i] * 3 + 1));
+    // count_sum / bucket_size_sum (unordered_datastructure.inc:2229-2300)
+    int* sizes = dev_alloc<int>(m.bucket_count());
+    hipLaunchKernelGGL((k_bucket_sizes<map_t>), dim3(blocks(m.bucket_count())), dim3(256), 0, 0, m, sizes);
+    const std::vector<int> bs = to_host(sizes, m.bucket_count());
+    (void)hipFree(sizes);
+    CHECK(sum(bs) == static_cast<int>(N));
+    int* ds = dev_alloc<int>(1);
+    hipLaunchKernelGGL((k_device_size<map_t>), dim3(1), dim3(64), 0, 0, m, ds);
+    CHECK(to_host(ds, 1)[0] == static_cast<int>(N));
+    (void)hipFree(ds);
+    CHECK(sum(run_erase(m, k)) == static_cast<int>(N));
+    CHECK(m.size() == 0 && m.valid());
+    map_t::destroyDeviceObject(m);
+}
+
+static void
+range_insert_erase_and_device_range()
+{
+    g_case = "range_insert_erase_and_device_range";
+    const std::size_t N = 100000;
+    map_t m = map_t::createDeviceObject(static_cast<stdgpu::index_t>(N));
+    const std::vector<std::uint64_t> k = unique_keys(N, 9);
+    std::vector<map_t::value_type> host_values;
+    for (auto x : k) host_values.emplace_back(x, static_cast<std::uint32_t>(x ^ 0x5555));
+    map_t::value_type* dv = dev_alloc<map_t::value_type>(N);
+    HIP_OK(hipMemcpy(dv, host_values.data(), N * sizeof(map_t::value_type), hipMemcpyHostToDevice));
+    m.insert(stdgpu::make_device(dv), stdgpu::make_device(dv + N));
+    CHECK(m.size() == static_cast<stdgpu::index_t>(N) && m.valid());
+    auto range = m.device_range();
+    CHECK(range.size() == static_cast<stdgpu::index_t>(N));
+    map_t::value_type* out = dev_alloc<map_t::value_type>(N);
+    hipLaunchKernelGGL((k_range_copy<decltype(range), map_t::value_type>), dim3(blocks(N)), dim3(256), 0, 0, range, out);
+    std::vector<std::pair<std::uint64_t, std::uint32_t>> got, want;
+    std::vector<map_t::value_type> hv(N);
+    HIP_OK(hipMemcpy(hv.data(), out, N * sizeof(map_t::value_type), hipMemcpyDeviceToHost));
+    for (auto& p : hv) got.emplace_back(p.first, p.second);
+    for (auto& p : host_values) want.emplace_back(p.first, p.second);
+    std::sort(got.begin(), got.end());
+    std::sort(want.begin(), want.end());
+    CHECK(got == want);
+    // erase(begin, end) of the first half through a custom policy with its own stream
+    hipStream_t s;
+    HIP_OK(hipStreamCreate(&s));
+    stdgpu::execution::device_policy pol{ s };
+    std::uint64_t* dk = to_dev(k);
+    m.erase(pol, stdgpu::make_device(dk), stdgpu::make_device(dk + N / 2));
+    CHECK(m.size(pol) == static_cast<stdgpu::index_t>(N - N / 2));
+    const std::vector<int> f = run_find(m, k);
+    for (std::size_t i = 0; i < N; ++i) CHECK(f[i] == (i < N / 2 ? 0 : 1));
+    CHECK(m.valid(pol));
+    m.clear(pol);
+    CHECK(m.size() == 0 && m.empty() && m.valid());
+    CHECK(m.device_range().size() == 0);
+    HIP_OK(hipStreamDestroy(s));
+    (void)hipFree(dk);
+    (void)hipFree(dv);
+    (void)hipFree(out);
+    map_t::destroyDeviceObject(m);
+}
+
+static void
+insert_and_erase_unique_parallel()
+{
+    g_case = "insert_and_erase_unique_parallel";
+    const std::size_t N = 100000;
+    map_t m = map_t::createDeviceObject(static_cast<stdgpu::index_t>(N));
+    const std::vector<std::uint64_t> k = unique_keys(N, 13);
+    std::uint64_t* dk = to_dev(k);
+    int* ok = dev_alloc<int>(N);
+    hipLaunchKernelGGL((k_insert_then_erase<map_t, std::uint64_t>), dim3(blocks(N)), dim3(256), 0, 0, m, dk, N, ok);
+    CHECK(sum(to_host(ok, N)) == static_cast<int>(N));
+    CHECK(m.size() == 0 && m.valid());
+    (void)hipFree(dk);
+    (void)hipFree(ok);
+    map_t::destroyDeviceObject(m);
+}
+
+static void
+set_u32_native_and_bulk()
+{
+    g_case = "set_u32_native_and_bulk";
+    const std::size_t N = 50000;
+    set_t s = set_t::createDeviceObject(static_cast<stdgpu::index_t>(2 * N));
+    std::vector<std::uint32_t> k;
+    for (auto x : unique_keys(N, 21)) k.push_back(static_cast<std::uint32_t>(x));
+    std::sort(k.begin(), k.end());
+    k.erase(std::unique(k.begin(), k.end()), k.end());
+    CHECK(sum(run_insert(s, k)) == static_cast<int>(k.size()));
+    CHECK(sum(run_insert(s, k)) == 0);
+    CHECK(s.size() == static_cast<stdgpu::index_t>(k.size()) && s.valid());
+    std::uint32_t* dk = to_dev(k);
+    s.erase(dk, dk + k.size());
+    CHECK(s.size() == 0);
+    s.insert(dk, dk + k.size());
+    CHECK(s.size() == static_cast<stdgpu::index_t>(k.size()));
+    CHECK(sum(run_find(s, k)) == static_cast<int>(k.size()));
+    (void)hipFree(dk);
+    set_t::destroyDeviceObject(s);
+}
+
+// examples/openmp/unordered_set.cpp and unordered_map.cpp: duplicate-free neighbour sets checked
+// against closed-form sums
+template <typename Range, typename T>
+__global__ void
+k_sum_range(Range r, long long* out)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= r.size()) return;
+    const auto& v = *(r.begin() + i);
+    if constexpr (std::is_same_v<std::remove_cv_t<std::remove_reference_t<decltype(v)>>, int>)
+        atomicAdd(reinterpret_cast<unsigned long long*>(out), static_cast<unsigned long long>(v));
+    else
+    {
+        atomicAdd(reinterpret_cast<unsigned long long*>(out), static_cast<unsigned long long>(v.first));
+        atomicAdd(reinterpret_cast<unsigned long long*>(out + 1), static_cast<unsigned long long>(v.second));
+    }
+}
+
+template <typename Table>
+__global__ void
+k_insert_neighbors(Table t, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int num = 2 * i + 1;
+    for (int x : { num - 1, num, num + 1 })
+    {
+        if constexpr (std::is_same_v<typename Table::value_type, int>)
+            t.insert(x);
+        else
+            t.emplace(x, x * x);
+    }
+}
+
+static void
+examples_closed_form()
+{
+    g_case = "examples_closed_form";
+    const int n = 100;
+    iset_t s = iset_t::createDeviceObject(n);
+    hipLaunchKernelGGL((k_insert_neighbors<iset_t>), dim3(1), dim3(64), 0, 0, s, n / 2);
+    CHECK(s.size() == n + 1);
+    long long* acc = dev_alloc<long long>(2);
+    HIP_OK(hipMemset(acc, 0, 2 * sizeof(long long)));
+    auto rs = s.device_range();
+    hipLaunchKernelGGL((k_sum_range<decltype(rs), int>), dim3(1), dim3(128), 0, 0, rs, acc);
+    CHECK(to_host(acc, 1)[0] == n * (n + 1) / 2);
+    iset_t::destroyDeviceObject(s);
+
+    imap_t m = imap_t::createDeviceObject(n);
+    hipLaunchKernelGGL((k_insert_neighbors<imap_t>), dim3(1), dim3(64), 0, 0, m, n / 2);
+    CHECK(m.size() == n + 1);
+    HIP_OK(hipMemset(acc, 0, 2 * sizeof(long long)));
+    auto rm = m.device_range();
+    hipLaunchKernelGGL((k_sum_range<decltype(rm), int>), dim3(1), dim3(128), 0, 0, rm, acc);
+    const std::vector<long long> h = to_host(acc, 2);
+    CHECK(h[0] == n * (n + 1) / 2);
+    CHECK(h[1] == static_cast<long long>(n) * (n + 1) * (2 * n + 1) / 6);
+    (void)hipFree(acc);
+    imap_t::destroyDeviceObject(m);
+}
+
+int
+main()
+{
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+    {
+        std::printf("FAIL no GPU\n");
+        return 2;
+    }
+    void (*cases[])() = { empty_container,
+                          size_limits_and_bucket,
+                          insert_find_erase_single,
+                          insert_erase_multiple_same_key,
+                          tiny_table_cases,
+                          unique_parallel_native,
+                          range_insert_erase_and_device_range,
+                          insert_and_erase_unique_parallel,
+                          set_u32_native_and_bulk,
+                          examples_closed_form };
+    for (auto c : cases)
+    {
+        const int before = g_failures;
+        try
+        {
+            c();
+        }
+        catch (const std::exception& e)
+        {
+            std::printf("FAIL %s: exception %s\n", g_case, e.what());
+            ++g_failures;
+        }
+        if (g_failures == before) std::printf("PASS %s\n", g_case);
+        std::fflush(stdout);
+    }
+    std::printf("%s: %d failure(s)\n", g_failures ? "FAILED" : "OK", g_failures);
+    return g_failures ? 1 : 0;
+}

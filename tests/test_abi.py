@@ -61,3 +61,10 @@ def test_missing_library_fails_loudly(tmp_path):
             _lib.load(str(tmp_path / "nope.so"))
     finally:
         _lib._lib = saved
+
+
+def test_device_workload_generator_matches_numpy():
+    from stdgpu_amd import workloads_device as wd
+
+    assert wd.check_against_numpy()
+    assert wd.check_against_numpy(ntables=40, log2_lo=5, log2_hi=11)

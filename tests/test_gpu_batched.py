@@ -1,0 +1,177 @@
+"""Batched form: many tables in one persistent launch (and as a replayed HIP graph) must give every
+table exactly what a plain per-table call gives it (contents, flags, hits, values, size), and what
+the oracle gives it."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+import stdgpu_amd as sa  # noqa: E402
+from stdgpu_amd import workloads as wl  # noqa: E402
+from stdgpu_amd.batched import TableBatch  # noqa: E402
+from oracle import pyoracle as po  # noqa: E402
+
+
+def d64(a):
+    return torch.from_numpy(np.ascontiguousarray(a).view(np.int64)).cuda()
+
+
+def d32(a):
+    return torch.from_numpy(np.ascontiguousarray(a).view(np.int32)).cuda()
+
+
+def sorted_contents(t):
+    k, v = t.contents()
+    k = k.cpu().numpy().view(np.uint64)
+    v = v.cpu().numpy().view(np.uint32)
+    o = np.argsort(k)
+    return k[o], v[o]
+
+
+def make_tables(w, lanes):
+    return [sa.unordered_map_u64_u32.createDeviceObject(int(c), sa.Tuning(bucket_lanes=lanes)) for c in w.capacities]
+
+
+@pytest.mark.parametrize("lanes", [4, 8, 16])
+def test_batched_phases_equal_plain_calls(lanes):
+    w = wl.batched_maps(ntables=64, log2_lo=6, log2_hi=12, seed=11)
+    bt = make_tables(w, lanes)
+    pt = make_tables(w, lanes)
+    batch = TableBatch(bt)
+    n = len(w.ins_keys)
+    keys, vals = d64(w.ins_keys), d32(w.ins_vals)
+    flags = torch.zeros(n, dtype=torch.uint8, device="cuda")
+    batch.run("insert", d64(w.ins_offsets), keys, vals, flags=flags)
+    bflags = flags.cpu().numpy()
+    for t, tab in enumerate(pt):
+        a, b = w.ins_offsets[t], w.ins_offsets[t + 1]
+        f = torch.zeros(b - a, dtype=torch.uint8, device="cuda")
+        tab.insert(keys[a:b], vals[a:b], inserted=f)
+        np.testing.assert_array_equal(bflags[a:b], f.cpu().numpy())
+        assert bt[t].size() == tab.size() == b - a
+    # find
+    fk = d64(w.find_keys)
+    hits = torch.zeros(len(w.find_keys), dtype=torch.uint8, device="cuda")
+    got = torch.zeros(len(w.find_keys), dtype=torch.int32, device="cuda")
+    batch.run("find", d64(w.find_offsets), fk, flags=hits, vals_out=got)
+    bh, bv = hits.cpu().numpy(), got.cpu().numpy().view(np.uint32)
+    for t, tab in enumerate(pt):
+        a, b = w.find_offsets[t], w.find_offsets[t + 1]
+        h, v = tab.find(fk[a:b])
+        np.testing.assert_array_equal(bh[a:b], h.cpu().numpy())
+        hv = h.cpu().numpy() == 1
+        np.testing.assert_array_equal(bv[a:b][hv], v.cpu().numpy().view(np.uint32)[hv])
+    # erase
+    ek = d64(w.erase_keys)
+    ef = torch.zeros(len(w.erase_keys), dtype=torch.uint8, device="cuda")
+    batch.run("erase", d64(w.erase_offsets), ek, flags=ef)
+    be = ef.cpu().numpy()
+    for t, tab in enumerate(pt):
+        a, b = w.erase_offsets[t], w.erase_offsets[t + 1]
+        f = torch.zeros(b - a, dtype=torch.uint8, device="cuda")
+        tab.erase(ek[a:b], erased=f)
+        np.testing.assert_array_equal(be[a:b], f.cpu().numpy())
+        ka, va = sorted_contents(bt[t])
+        kb, vb = sorted_contents(tab)
+        np.testing.assert_array_equal(ka, kb)
+        np.testing.assert_array_equal(va, vb)
+        assert bt[t].valid()
+    # oracle on a few tables
+    for t in [0, 17, 63]:
+        ref = po.Table("u64", "u32", int(w.capacities[t]))
+        a, b = w.ins_offsets[t], w.ins_offsets[t + 1]
+        ref.insert(w.ins_keys[a:b], w.ins_vals[a:b])
+        a, b = w.erase_offsets[t], w.erase_offsets[t + 1]
+        ref.erase(w.erase_keys[a:b])
+        rk, rv = ref.sorted_contents()
+        ka, va = sorted_contents(bt[t])
+        np.testing.assert_array_equal(ka, rk)
+        np.testing.assert_array_equal(va, rv)
+    batch.destroy()
+
+
+def test_batched_graph_replays_match_eager():
+    w = wl.batched_maps(ntables=128, log2_lo=6, log2_hi=11, seed=12)
+    tabs = make_tables(w, 8)
+    batch = TableBatch(tabs)
+    batch.run("insert", d64(w.ins_offsets), d64(w.ins_keys), d32(w.ins_vals))
+    torch.cuda.synchronize()
+    fk, fo = d64(w.find_keys), d64(w.find_offsets)
+    eager_h = torch.zeros(len(w.find_keys), dtype=torch.uint8, device="cuda")
+    eager_v = torch.zeros(len(w.find_keys), dtype=torch.int32, device="cuda")
+    batch.run("find", fo, fk, flags=eager_h, vals_out=eager_v)
+    torch.cuda.synchronize()
+    gh = torch.zeros_like(eager_h)
+    gv = torch.zeros_like(eager_v)
+    s = torch.cuda.Stream()
+    g = batch.capture("find", fo, fk, flags=gh, vals_out=gv, stream=s)
+    other = sa.unordered_set_u64.createDeviceObject(1000)
+    for rep in range(3):
+        gh.zero_()
+        gv.zero_()
+        torch.cuda.synchronize()
+        g.launch(s)
+        s.synchronize()
+        assert torch.equal(gh, eager_h), rep
+        assert torch.equal(gv[eager_h == 1], eager_v[eager_h == 1]), rep
+        other.insert(torch.arange(1, 100, dtype=torch.int64, device="cuda") + rep)  # eager work between replays
+        torch.cuda.synchronize()
+    g.destroy()
+    batch.destroy()
+
+
+def test_batched_mixed_ops_one_launch():
+    """one launch: tables 0,3,6.. insert, 1,4,7.. find, 2,5,8.. erase (the serving form's launch)"""
+    w = wl.batched_maps(ntables=48, log2_lo=6, log2_hi=11, seed=13)
+    tabs = make_tables(w, 8)
+    plain = make_tables(w, 8)
+    batch = TableBatch(tabs)
+    # pre-load every table so finds and erases have something to hit
+    batch.run("insert", d64(w.ins_offsets), d64(w.ins_keys), d32(w.ins_vals))
+    for t, tab in enumerate(plain):
+        a, b = w.ins_offsets[t], w.ins_offsets[t + 1]
+        tab.insert(d64(w.ins_keys[a:b]), d32(w.ins_vals[a:b]))
+    ntab = len(tabs)
+    ops = np.array([t % 3 for t in range(ntab)], np.int8)
+    # per table: inserts use fresh keys, finds use the find keys, erases use the erase keys
+    parts_k, parts_v, offs = [], [], [0]
+    fresh = wl.stream(99, int(w.sizes.sum()))
+    for t in range(ntab):
+        if ops[t] == 0:
+            k = fresh[w.ins_offsets[t]: w.ins_offsets[t + 1]][: max(1, w.sizes[t] // 4)]
+        elif ops[t] == 1:
+            k = w.find_keys[w.find_offsets[t]: w.find_offsets[t + 1]]
+        else:
+            k = w.erase_keys[w.erase_offsets[t]: w.erase_offsets[t + 1]]
+        parts_k.append(k)
+        parts_v.append(wl.value_of(k))
+        offs.append(offs[-1] + len(k))
+    keys = np.concatenate(parts_k)
+    vals = np.concatenate(parts_v)
+    offs = np.array(offs, np.int64)
+    flags = torch.zeros(len(keys), dtype=torch.uint8, device="cuda")
+    vout = torch.zeros(len(keys), dtype=torch.int32, device="cuda")
+    dk = d64(keys)
+    batch.run(torch.from_numpy(ops).cuda(), d64(offs), dk, d32(vals), flags=flags, vals_out=vout)
+    fl = flags.cpu().numpy()
+    for t in range(ntab):
+        a, b = offs[t], offs[t + 1]
+        tab = plain[t]
+        if ops[t] == 0:
+            f = torch.zeros(b - a, dtype=torch.uint8, device="cuda")
+            tab.insert(dk[a:b], d32(vals[a:b]), inserted=f)
+            np.testing.assert_array_equal(fl[a:b], f.cpu().numpy())
+        elif ops[t] == 1:
+            h, _ = tab.find(dk[a:b])
+            np.testing.assert_array_equal(fl[a:b], h.cpu().numpy())
+        else:
+            f = torch.zeros(b - a, dtype=torch.uint8, device="cuda")
+            tab.erase(dk[a:b], erased=f)
+            np.testing.assert_array_equal(fl[a:b], f.cpu().numpy())
+        ka, va = sorted_contents(tabs[t])
+        kb, vb = sorted_contents(tab)
+        np.testing.assert_array_equal(ka, kb)
+        np.testing.assert_array_equal(va, vb)
+    batch.destroy()

@@ -168,7 +168,9 @@ KINDS = list(CLS)
 def test_all_kinds_random_rounds(kind, lanes):
     key, mapped = kind
     rng = np.random.default_rng(1234 + lanes)
-    cap = 20000
+    # sized so neither table overflows (parity is defined where neither refuses an insert):
+    # at most 40000 distinct keys in 60000 slots
+    cap = 60000
     ref = po.Table(key, mapped, cap)
     tab = CLS[kind].createDeviceObject(cap, sa.Tuning(bucket_lanes=lanes))
     uni = 40000
