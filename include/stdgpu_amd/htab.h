@@ -45,7 +45,7 @@ typedef struct htab_tuning
     int32_t bucket_lanes;      /* lanes that probe one bucket together: 4, 8 or 16 (bucket = 16*lanes bytes); default 8 */
     int32_t block_size;        /* threads per workgroup of the bulk kernels: 256, 512 or 1024; default 256 */
     float slots_per_capacity;  /* physical slots = ceil(capacity * this); default 1.0 (load 0.8 at 0.8*capacity keys) */
-    int32_t reserved;
+    int32_t probe_batch;       /* elements a lane group probes together: 2, 4 or 8 (at most bucket_lanes); default 8 */
 } htab_tuning;
 
 /* Sizes a caller can query.  bucket_count / max_size follow the reference formulas so
@@ -151,8 +151,27 @@ int htab_get_view(const htab* table, htab_view* out);
 typedef struct htab_batch htab_batch;
 typedef struct htab_graph htab_graph;
 
+/* One unit of work of a batched launch: `op` on `n` elements of table `table` (index into the
+ * batch's tables), with its own device arrays. */
+typedef struct htab_segment
+{
+    const void* keys;
+    const void* vals;     /* mapped values to insert (maps), else NULL */
+    uint8_t* flags;       /* inserted / hit / erased per element, or NULL */
+    void* vals_out;       /* mapped values of find hits, or NULL */
+    int64_t n;
+    int32_t table;
+    int32_t op;
+} htab_segment;
+
 int htab_batch_create(htab* const* tables, int32_t ntables, htab_batch** out);
 int htab_batch_destroy(htab_batch* batch);
+/* One persistent launch over nsegs segments given in host memory (copied to the device on `stream`). */
+int htab_batch_run_segments(htab_batch* batch, const htab_segment* segs, int32_t nsegs, void* stream);
+/* Plain per-table calls for the same work (one htab_insert_kv / htab_find / htab_erase per table),
+ * the baseline the batched launch is compared with.  offsets: host int64[ntables + 1]. */
+int htab_batch_run_per_table(htab_batch* batch, int op, const int64_t* host_offsets, const void* keys,
+                             const void* vals, uint8_t* flags, void* vals_out, void* stream);
 int htab_batch_run(htab_batch* batch, const int8_t* ops, int op, const int64_t* offsets, const void* keys,
                    const void* vals, uint8_t* flags, void* vals_out, void* stream);
 /* Records htab_batch_run with these arguments into a HIP graph (captured on `stream`, which must not

@@ -34,7 +34,7 @@ class htab_tuning(ctypes.Structure):
         ("bucket_lanes", ctypes.c_int32),
         ("block_size", ctypes.c_int32),
         ("slots_per_capacity", ctypes.c_float),
-        ("reserved", ctypes.c_int32),
+        ("probe_batch", ctypes.c_int32),
     ]
 
 
@@ -50,6 +50,18 @@ class htab_info(ctypes.Structure):
         ("limit", ctypes.c_int64),
         ("value_bytes", ctypes.c_int64),
         ("device_bytes", ctypes.c_int64),
+    ]
+
+
+class htab_segment(ctypes.Structure):
+    _fields_ = [
+        ("keys", ctypes.c_void_p),
+        ("vals", ctypes.c_void_p),
+        ("flags", ctypes.c_void_p),
+        ("vals_out", ctypes.c_void_p),
+        ("n", ctypes.c_int64),
+        ("table", ctypes.c_int32),
+        ("op", ctypes.c_int32),
     ]
 
 
@@ -92,6 +104,8 @@ SIGNATURES = {
     "htab_batch_create": (_I32, [ctypes.POINTER(_P), ctypes.c_int32, ctypes.POINTER(_P)]),
     "htab_batch_destroy": (_I32, [_P]),
     "htab_batch_run": (_I32, [_P, _P, _I32, _P, _P, _P, _P, _P, _P]),
+    "htab_batch_run_segments": (_I32, [_P, _P, ctypes.c_int32, _P]),
+    "htab_batch_run_per_table": (_I32, [_P, _I32, ctypes.POINTER(_I64), _P, _P, _P, _P, _P]),
     "htab_batch_capture": (_I32, [_P, _P, _I32, _P, _P, _P, _P, _P, _P, ctypes.POINTER(_P)]),
     "htab_graph_launch": (_I32, [_P, _P]),
     "htab_graph_destroy": (_I32, [_P]),

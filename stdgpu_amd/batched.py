@@ -56,6 +56,23 @@ class TableBatch:
             "htab_batch_run",
         )
 
+    def run_segments(self, segments, stream=None):
+        """segments: list of (table, op, keys, vals, flags, vals_out) with device tensors."""
+        arr = (_lib.htab_segment * max(len(segments), 1))()
+        for j, (t, op, k, v, f, vo) in enumerate(segments):
+            arr[j] = _lib.htab_segment(_ptr(k), _ptr(v), _ptr(f), _ptr(vo), k.numel(), t, OPS[op])
+        check(_lib.load().htab_batch_run_segments(self._b, arr, len(segments), _stream_ptr(stream)),
+              "htab_batch_run_segments")
+
+    def run_per_table(self, op: str, host_offsets, keys, vals=None, flags=None, vals_out=None, stream=None):
+        """The plain baseline: one htab_* call per table for the same work."""
+        off = (ctypes.c_int64 * len(host_offsets))(*[int(x) for x in host_offsets])
+        check(
+            _lib.load().htab_batch_run_per_table(self._b, OPS[op], off, _ptr(keys), _ptr(vals), _ptr(flags),
+                                                 _ptr(vals_out), _stream_ptr(stream)),
+            "htab_batch_run_per_table",
+        )
+
     def capture(self, op, offsets, keys, vals=None, flags=None, vals_out=None, stream=None) -> Graph:
         """Record run(...) into a HIP graph on `stream` (a created torch stream)."""
         import torch

@@ -14,12 +14,15 @@ PKG = os.path.join(ROOT, "stdgpu_amd")
 CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(PKG, "libstdgpu_amd.so")
 ROOFLINE_LIB = os.path.join(PKG, "libstdgpu_amd_roofline.so")
-SOURCES = ["htab.hip", "deque.hip"]
+SOURCES = ["htab.hip", "deque.hip", "kind_set_u32.hip", "kind_set_u64.hip", "kind_map_u64_u32.hip",
+           "kind_map_u32_u32.hip", "kind_map_u64_u64.hip"]
 HEADERS = [
     os.path.join(ROOT, "include", "stdgpu_amd", "htab.h"),
     os.path.join(ROOT, "include", "stdgpu_amd", "htab_device.hpp"),
     os.path.join(CSRC, "htab_kernels.hpp"),
     os.path.join(CSRC, "wave.hpp"),
+    os.path.join(CSRC, "htab_internal.hpp"),
+    os.path.join(CSRC, "htab_ops.hpp"),
 ]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -45,13 +48,18 @@ def _run(cmd: list[str], cwd: str | None = None) -> None:
 
 
 def build_library(out: str = LIB, sources=SOURCES) -> str:
-    objs = []
+    """Compiles each translation unit (in parallel, one process per unit) and links the library."""
+    from concurrent.futures import ThreadPoolExecutor
+
     build_dir = os.path.join(PKG, "_build")
     os.makedirs(build_dir, exist_ok=True)
-    for src in sources:
-        obj = os.path.join(build_dir, src.replace(".hip", ".o"))
-        _run([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj])
-        objs.append(obj)
+    objs = [os.path.join(build_dir, src.replace(".hip", ".o")) for src in sources]
+    jobs = max(1, min(len(sources), int(os.environ.get("MAX_JOBS", os.cpu_count() or 4))))
+    with ThreadPoolExecutor(max_workers=jobs) as ex:
+        futs = [ex.submit(_run, [HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj])
+                for src, obj in zip(sources, objs)]
+        for f in futs:
+            f.result()
     tmp = out + ".tmp"
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs])
     os.replace(tmp, out)

@@ -54,9 +54,10 @@ class Tuning:
     bucket_lanes: int = 8
     block_size: int = 256
     slots_per_capacity: float = 1.0
+    probe_batch: int = 0  # 0 = library default
 
     def to_c(self) -> _lib.htab_tuning:
-        return _lib.htab_tuning(self.bucket_lanes, self.block_size, self.slots_per_capacity, 0)
+        return _lib.htab_tuning(self.bucket_lanes, self.block_size, self.slots_per_capacity, self.probe_batch)
 
 
 class _unordered_base:

@@ -1,5 +1,6 @@
 // C ABI of the MI355X hashed containers: table lifetime, bulk operations, size, range,
-// validity check and the free-list deque.  Kernels live in htab_kernels.hpp.
+// validity check and the batched form.  Kernels live in htab_kernels.hpp; their typed dispatch is
+// compiled per key/value kind in kind_*.hip.
 //
 // Build: hipcc --offload-arch=gfx950 -O3 -shared -fPIC (see stdgpu_amd/build.py).
 #include <hip/hip_runtime.h>
@@ -9,23 +10,32 @@
 #include <cstring>
 #include <new>
 #include <string>
+#include <type_traits>
 
-#include <stdgpu_amd/htab.h>
+#include "htab_internal.hpp"
 
-#include "htab_kernels.hpp"
-
+using namespace htab_host;
 using namespace htab_dev;
+
+namespace htab_host
+{
 
 namespace
 {
-
 thread_local std::string g_last_error;
+}
 
 int
 fail(int code, const char* what)
 {
     g_last_error = what;
     return code;
+}
+
+const char*
+last_error()
+{
+    return g_last_error.c_str();
 }
 
 int
@@ -35,12 +45,6 @@ fail_hip(hipError_t e, const char* what)
     return HTAB_E_HIP;
 }
 
-#define HTAB_HIP_TRY(expr, what)                                                                                       \
-    do                                                                                                                 \
-    {                                                                                                                  \
-        hipError_t e_ = (expr);                                                                                        \
-        if (e_ != hipSuccess) return fail_hip(e_, what);                                                               \
-    } while (0)
 
 int
 launch_status(const char* what)
@@ -60,6 +64,18 @@ value_bytes_of(int kind)
         case HTAB_MAP_U64_U32: return 16;
         case HTAB_MAP_U32_U32: return 8;
         case HTAB_MAP_U64_U64: return 16;
+        default: return 0;
+    }
+}
+
+int
+mapped_bytes_of(int kind)
+{
+    switch (kind)
+    {
+        case HTAB_MAP_U64_U32: return 4;
+        case HTAB_MAP_U32_U32: return 4;
+        case HTAB_MAP_U64_U64: return 8;
         default: return 0;
     }
 }
@@ -85,237 +101,9 @@ reference_sizing(int64_t capacity, int64_t* bucket_count, int64_t* max_size)
     *max_size = static_cast<int64_t>(bc) + excess;
 }
 
-constexpr size_t kCtlBytes = 8192; // stripes (32 x 128 B) + flags + scratch words
-constexpr size_t kFlagsOffset = 4096;
 
-} // namespace
+} // namespace htab_host
 
-struct htab
-{
-    int kind = 0;
-    int lanes = 8;
-    int block = 256;
-    int64_t capacity = 0;
-    htab_view view{};
-    htab_info info{};
-    void* mem = nullptr;
-    size_t bytes = 0;
-    int64_t* scratch = nullptr; // range/valid scratch, grown on demand
-    size_t scratch_bytes = 0;
-    int64_t* range_idx = nullptr; // table-owned device_range() indices (limit + 2 entries)
-};
-
-struct htab_batch
-{
-    int kind = 0;
-    int lanes = 8;
-    int32_t ntables = 0;
-    void* tables = nullptr;            // device array of table_ref<K, M>
-    int64_t* chunk_start = nullptr;    // ntables + 1
-    unsigned long long* counter = nullptr;
-    uint32_t grid = 0;                 // persistent workgroups
-};
-
-struct htab_graph
-{
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
-};
-
-namespace
-{
-
-// ------------------------------------------------------------------ typed dispatch
-template <typename K, typename M, int G>
-struct ops
-{
-    using T = table_ref<K, M>;
-
-    static int insert(htab* h, const void* values, const void* keys, const void* vals, int64_t n, uint8_t* ins,
-                      hipStream_t s)
-    {
-        const int64_t grid = (n + h->block - 1) / h->block;
-        hipLaunchKernelGGL((k_insert<K, M, G>), dim3(static_cast<uint32_t>(grid)), dim3(h->block), 0, s,
-                           T::from_view(h->view), values, keys, vals, n, ins);
-        return launch_status("insert launch");
-    }
-
-    static int find(htab* h, const void* keys, int64_t n, uint8_t* hits, void* vals, hipStream_t s)
-    {
-        const int64_t grid = (n + h->block - 1) / h->block;
-        hipLaunchKernelGGL((k_find<K, M, G>), dim3(static_cast<uint32_t>(grid)), dim3(h->block), 0, s,
-                           T::from_view(h->view), static_cast<const K*>(keys), n, hits, vals);
-        return launch_status("find launch");
-    }
-
-    static int erase(htab* h, const void* keys, int64_t n, uint8_t* del, hipStream_t s)
-    {
-        const int64_t grid = (n + h->block - 1) / h->block;
-        hipLaunchKernelGGL((k_erase<K, M, G>), dim3(static_cast<uint32_t>(grid)), dim3(h->block), 0, s,
-                           T::from_view(h->view), static_cast<const K*>(keys), n, del);
-        return launch_status("erase launch");
-    }
-};
-
-template <typename K, typename M, int G>
-struct batch_ops
-{
-    static int build(htab_batch* b, htab* const* tables)
-    {
-        using T = table_ref<K, M>;
-        T* host = new (std::nothrow) T[b->ntables];
-        if (!host) return fail(HTAB_E_OOM, "htab_batch_create: host allocation");
-        for (int32_t t = 0; t < b->ntables; ++t) host[t] = T::from_view(tables[t]->view);
-        hipError_t e = hipMemcpy(b->tables, host, sizeof(T) * b->ntables, hipMemcpyHostToDevice);
-        delete[] host;
-        if (e != hipSuccess) return fail_hip(e, "htab_batch_create: descriptor copy");
-        return HTAB_OK;
-    }
-
-    static int run(htab_batch* b, const int8_t* ops, int op, const int64_t* offsets, const void* keys,
-                   const void* vals, uint8_t* flags, void* vals_out, hipStream_t s)
-    {
-        hipLaunchKernelGGL(k_batch_plan, dim3(1), dim3(1024), 0, s, offsets, b->ntables, b->chunk_start, b->counter);
-        int rc = launch_status("batch plan");
-        if (rc) return rc;
-        hipLaunchKernelGGL((k_batch<K, M, G>), dim3(b->grid), dim3(kBatchBlock), 0, s,
-                           static_cast<const table_ref<K, M>*>(b->tables), b->ntables, offsets, ops, op,
-                           b->chunk_start, b->counter, keys, vals, flags, vals_out);
-        return launch_status("batch launch");
-    }
-};
-
-template <typename K, typename M>
-struct table_ops
-{
-    using T = table_ref<K, M>;
-    using value_type = typename slot_traits<K, M>::value_type;
-
-    static int clear(htab* h, hipStream_t s)
-    {
-        const T t = T::from_view(h->view);
-        const uint64_t area = (t.nslots() + 2) * sizeof(value_type);
-        const uint64_t nchunks = (area + 15) / 16;
-        uint64_t grid = (nchunks + 255) / 256;
-        if (grid > 65536) grid = 65536;
-        if (grid == 0) grid = 1;
-        hipLaunchKernelGGL((k_clear<K, M>), dim3(static_cast<uint32_t>(grid)), dim3(256), 0, s, t, nchunks);
-        int rc = launch_status("clear launch");
-        if (rc) return rc;
-        hipLaunchKernelGGL((k_clear_tail<K, M>), dim3(1), dim3(1), 0, s, t);
-        return launch_status("clear launch");
-    }
-
-    static int range(htab* h, int64_t* idx_out, void* keys_out, void* vals_out, int64_t cap, int64_t* n_out,
-                     hipStream_t s)
-    {
-        const T t = T::from_view(h->view);
-        const uint64_t total = t.nslots() + 2;
-        const uint64_t ntiles = (total + kRangeTile - 1) / kRangeTile;
-        const size_t need = (ntiles + 1) * sizeof(int64_t);
-        if (h->scratch_bytes < need)
-        {
-            if (h->scratch) (void)hipFree(h->scratch);
-            h->scratch = nullptr;
-            h->scratch_bytes = 0;
-            HTAB_HIP_TRY(hipMalloc(&h->scratch, need), "range scratch");
-            h->scratch_bytes = need;
-        }
-        hipLaunchKernelGGL((k_range_count<K, M>), dim3(static_cast<uint32_t>(ntiles)), dim3(kRangeThreads), 0, s, t,
-                           total, h->scratch);
-        int rc = launch_status("range count");
-        if (rc) return rc;
-        hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(1024), 0, s, h->scratch, static_cast<int64_t>(ntiles));
-        rc = launch_status("range scan");
-        if (rc) return rc;
-        hipLaunchKernelGGL((k_range_scatter<K, M>), dim3(static_cast<uint32_t>(ntiles)), dim3(kRangeThreads), 0, s, t,
-                           total, h->scratch, idx_out, static_cast<K*>(keys_out), vals_out, cap);
-        rc = launch_status("range scatter");
-        if (rc) return rc;
-        int64_t n = 0;
-        HTAB_HIP_TRY(hipMemcpyAsync(&n, h->scratch + ntiles, sizeof(int64_t), hipMemcpyDeviceToHost, s), "range count copy");
-        HTAB_HIP_TRY(hipStreamSynchronize(s), "range sync");
-        *n_out = n;
-        return HTAB_OK;
-    }
-
-    static int valid(htab* h, int32_t* out, hipStream_t s)
-    {
-        const T t = T::from_view(h->view);
-        const size_t need = 2 * sizeof(unsigned long long);
-        if (h->scratch_bytes < need)
-        {
-            if (h->scratch) (void)hipFree(h->scratch);
-            h->scratch = nullptr;
-            h->scratch_bytes = 0;
-            HTAB_HIP_TRY(hipMalloc(&h->scratch, need), "valid scratch");
-            h->scratch_bytes = need;
-        }
-        hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, s, reinterpret_cast<unsigned long long*>(h->scratch), 2);
-        int rc = launch_status("valid zero");
-        if (rc) return rc;
-        const uint64_t total = t.nslots() + 2;
-        uint64_t grid = (total + 255) / 256;
-        if (grid > 16384) grid = 16384;
-        hipLaunchKernelGGL((k_valid<K, M>), dim3(static_cast<uint32_t>(grid)), dim3(256), 0, s, t, total,
-                           reinterpret_cast<unsigned long long*>(h->scratch));
-        rc = launch_status("valid launch");
-        if (rc) return rc;
-        unsigned long long res[2] = {0, 0};
-        int64_t stripes[kMaxStripes * kStripeStride];
-        HTAB_HIP_TRY(hipMemcpyAsync(res, h->scratch, sizeof(res), hipMemcpyDeviceToHost, s), "valid copy");
-        HTAB_HIP_TRY(hipMemcpyAsync(stripes, h->view.stripes, sizeof(stripes), hipMemcpyDeviceToHost, s), "valid copy");
-        HTAB_HIP_TRY(hipStreamSynchronize(s), "valid sync");
-        int64_t size = 0;
-        for (uint32_t i = 0; i < kMaxStripes; ++i) size += stripes[i * kStripeStride];
-        *out = (res[0] == 0 && static_cast<int64_t>(res[1]) == size && size >= 0 &&
-                size <= static_cast<int64_t>(h->view.limit))
-                       ? 1
-                       : 0;
-        return HTAB_OK;
-    }
-};
-
-template <template <typename, typename, int> class Op, typename F>
-int
-with_ops(int kind, int lanes, F&& f)
-{
-#define HTAB_G(K, M)                                                                                                   \
-    switch (lanes)                                                                                                     \
-    {                                                                                                                  \
-        case 4: return f(Op<K, M, 4>{});                                                                               \
-        case 8: return f(Op<K, M, 8>{});                                                                               \
-        case 16: return f(Op<K, M, 16>{});                                                                             \
-        default: return fail(HTAB_E_ARG, "bad bucket_lanes");                                                          \
-    }
-    switch (kind)
-    {
-        case HTAB_SET_U32: HTAB_G(uint32_t, void)
-        case HTAB_SET_U64: HTAB_G(uint64_t, void)
-        case HTAB_MAP_U64_U32: HTAB_G(uint64_t, uint32_t)
-        case HTAB_MAP_U32_U32: HTAB_G(uint32_t, uint32_t)
-        case HTAB_MAP_U64_U64: HTAB_G(uint64_t, uint64_t)
-        default: return fail(HTAB_E_ARG, "bad kind");
-    }
-#undef HTAB_G
-}
-
-template <typename F>
-int
-with_table_ops(int kind, F&& f)
-{
-    switch (kind)
-    {
-        case HTAB_SET_U32: return f(table_ops<uint32_t, void>{});
-        case HTAB_SET_U64: return f(table_ops<uint64_t, void>{});
-        case HTAB_MAP_U64_U32: return f(table_ops<uint64_t, uint32_t>{});
-        case HTAB_MAP_U32_U32: return f(table_ops<uint32_t, uint32_t>{});
-        case HTAB_MAP_U64_U64: return f(table_ops<uint64_t, uint64_t>{});
-        default: return fail(HTAB_E_ARG, "bad kind");
-    }
-}
-
-} // namespace
 
 // ====================================================================== C ABI
 extern "C" {
@@ -323,7 +111,7 @@ extern "C" {
 const char*
 htab_last_error(void)
 {
-    return g_last_error.c_str();
+    return htab_host::last_error();
 }
 
 int
@@ -335,15 +123,19 @@ htab_create(int kind, int64_t capacity, const htab_tuning* tuning, htab** out)
     if (vb == 0) return fail(HTAB_E_ARG, "htab_create: unknown kind");
     if (capacity <= 0) return fail(HTAB_E_ARG, "htab_create: capacity must be positive");
     if (capacity > (int64_t(1) << 34)) return fail(HTAB_E_LIMIT, "htab_create: capacity above 2^34");
-    int lanes = 8, block = 256;
+    int lanes = 8, block = 256, rb = 0;
     float spc = 1.0F;
     if (tuning)
     {
         if (tuning->bucket_lanes) lanes = tuning->bucket_lanes;
         if (tuning->block_size) block = tuning->block_size;
         if (tuning->slots_per_capacity > 0.0F) spc = tuning->slots_per_capacity;
+        rb = tuning->probe_batch;
     }
     if (lanes != 4 && lanes != 8 && lanes != 16) return fail(HTAB_E_ARG, "htab_create: bucket_lanes must be 4, 8 or 16");
+    if (rb == 0) rb = lanes < 8 ? lanes : 8;
+    if ((rb != 2 && rb != 4 && rb != 8) || rb > lanes)
+        return fail(HTAB_E_ARG, "htab_create: probe_batch must be 2, 4 or 8 and at most bucket_lanes");
     if (block != 256 && block != 512 && block != 1024) return fail(HTAB_E_ARG, "htab_create: block_size must be 256, 512 or 1024");
     if (!(spc >= 0.25F && spc <= 16.0F)) return fail(HTAB_E_ARG, "htab_create: slots_per_capacity out of [0.25, 16]");
 
@@ -354,6 +146,7 @@ htab_create(int kind, int64_t capacity, const htab_tuning* tuning, htab** out)
     if (!h) return fail(HTAB_E_OOM, "htab_create: host allocation");
     h->kind = kind;
     h->lanes = lanes;
+    h->rb = rb;
     h->block = block;
     h->capacity = capacity;
 
@@ -404,7 +197,10 @@ htab_create(int kind, int64_t capacity, const htab_tuning* tuning, htab** out)
         delete h;
         return fail_hip(e, "htab_create: control block init");
     }
-    int rc = with_table_ops(kind, [&](auto o) { return decltype(o)::clear(h, nullptr); });
+    htab_call cc;
+    cc.op = call_op::clear;
+    cc.h = h;
+    int rc = dispatch(kind, cc);
     if (rc == HTAB_OK)
     {
         e = hipStreamSynchronize(nullptr);
@@ -443,7 +239,16 @@ insert_common(htab* h, const void* values, const void* keys, const void* vals, i
     if (!values && !keys) return fail(HTAB_E_ARG, "insert: null input");
     if ((n + h->block - 1) / h->block > 0x7FFFFFFF) return fail(HTAB_E_LIMIT, "insert: too many elements");
     hipStream_t s = static_cast<hipStream_t>(stream);
-    return with_ops<ops>(h->kind, h->lanes, [&](auto o) { return decltype(o)::insert(h, values, keys, vals, n, ins, s); });
+    htab_call c;
+    c.op = call_op::insert;
+    c.h = h;
+    c.values = values;
+    c.keys = keys;
+    c.vals = vals;
+    c.n = n;
+    c.flags = ins;
+    c.s = s;
+    return dispatch(h->kind, c);
 }
 
 int
@@ -469,7 +274,15 @@ htab_find(htab* h, const void* keys, int64_t n, uint8_t* hits, void* vals_out, v
     if (!keys || !hits) return fail(HTAB_E_ARG, "htab_find: null keys or hits");
     if ((n + h->block - 1) / h->block > 0x7FFFFFFF) return fail(HTAB_E_LIMIT, "htab_find: too many elements");
     hipStream_t s = static_cast<hipStream_t>(stream);
-    return with_ops<ops>(h->kind, h->lanes, [&](auto o) { return decltype(o)::find(h, keys, n, hits, vals_out, s); });
+    htab_call c;
+    c.op = call_op::find;
+    c.h = h;
+    c.keys = keys;
+    c.n = n;
+    c.flags = hits;
+    c.vals_out = vals_out;
+    c.s = s;
+    return dispatch(h->kind, c);
 }
 
 int
@@ -481,7 +294,14 @@ htab_erase(htab* h, const void* keys, int64_t n, uint8_t* erased, void* stream)
     if (!keys) return fail(HTAB_E_ARG, "htab_erase: null keys");
     if ((n + h->block - 1) / h->block > 0x7FFFFFFF) return fail(HTAB_E_LIMIT, "htab_erase: too many elements");
     hipStream_t s = static_cast<hipStream_t>(stream);
-    return with_ops<ops>(h->kind, h->lanes, [&](auto o) { return decltype(o)::erase(h, keys, n, erased, s); });
+    htab_call c;
+    c.op = call_op::erase;
+    c.h = h;
+    c.keys = keys;
+    c.n = n;
+    c.flags = erased;
+    c.s = s;
+    return dispatch(h->kind, c);
 }
 
 int
@@ -503,7 +323,11 @@ htab_clear(htab* h, void* stream)
 {
     if (!h) return fail(HTAB_E_ARG, "htab_clear: null handle");
     hipStream_t s = static_cast<hipStream_t>(stream);
-    return with_table_ops(h->kind, [&](auto o) { return decltype(o)::clear(h, s); });
+    htab_call c;
+    c.op = call_op::clear;
+    c.h = h;
+    c.s = s;
+    return dispatch(h->kind, c);
 }
 
 int
@@ -512,7 +336,14 @@ htab_range(htab* h, int64_t* idx_out, int64_t cap, int64_t* n_out, void* stream)
     if (!h || !n_out) return fail(HTAB_E_ARG, "htab_range: null argument");
     if (cap < 0) return fail(HTAB_E_ARG, "htab_range: negative capacity");
     hipStream_t s = static_cast<hipStream_t>(stream);
-    return with_table_ops(h->kind, [&](auto o) { return decltype(o)::range(h, idx_out, nullptr, nullptr, cap, n_out, s); });
+    htab_call c;
+    c.op = call_op::range;
+    c.h = h;
+    c.idx_out = idx_out;
+    c.cap = cap;
+    c.n_out = n_out;
+    c.s = s;
+    return dispatch(h->kind, c);
 }
 
 int
@@ -534,8 +365,15 @@ htab_contents(htab* h, void* keys_out, void* vals_out, int64_t cap, int64_t* n_o
     if (!h || !n_out) return fail(HTAB_E_ARG, "htab_contents: null argument");
     if (cap < 0) return fail(HTAB_E_ARG, "htab_contents: negative capacity");
     hipStream_t s = static_cast<hipStream_t>(stream);
-    return with_table_ops(h->kind,
-                          [&](auto o) { return decltype(o)::range(h, nullptr, keys_out, vals_out, cap, n_out, s); });
+    htab_call c;
+    c.op = call_op::range;
+    c.h = h;
+    c.keys = keys_out;
+    c.vals_out = vals_out;
+    c.cap = cap;
+    c.n_out = n_out;
+    c.s = s;
+    return dispatch(h->kind, c);
 }
 
 int
@@ -543,7 +381,12 @@ htab_valid(htab* h, int32_t* out, void* stream)
 {
     if (!h || !out) return fail(HTAB_E_ARG, "htab_valid: null argument");
     hipStream_t s = static_cast<hipStream_t>(stream);
-    return with_table_ops(h->kind, [&](auto o) { return decltype(o)::valid(h, out, s); });
+    htab_call c;
+    c.op = call_op::valid;
+    c.h = h;
+    c.valid_out = out;
+    c.s = s;
+    return dispatch(h->kind, c);
 }
 
 int
@@ -563,31 +406,45 @@ htab_batch_create(htab* const* tables, int32_t ntables, htab_batch** out)
     for (int32_t t = 0; t < ntables; ++t)
     {
         if (!tables[t]) return fail(HTAB_E_ARG, "htab_batch_create: null table");
-        if (tables[t]->kind != tables[0]->kind || tables[t]->lanes != tables[0]->lanes)
-            return fail(HTAB_E_ARG, "htab_batch_create: tables differ in kind or bucket_lanes");
+        if (tables[t]->kind != tables[0]->kind || tables[t]->lanes != tables[0]->lanes ||
+            tables[t]->block != tables[0]->block)
+            return fail(HTAB_E_ARG, "htab_batch_create: tables differ in kind, bucket_lanes or block_size");
     }
     htab_batch* b = new (std::nothrow) htab_batch();
     if (!b) return fail(HTAB_E_OOM, "htab_batch_create: host allocation");
     b->kind = tables[0]->kind;
     b->lanes = tables[0]->lanes;
+    b->block = tables[0]->block;
     b->ntables = ntables;
     int dev = 0, cus = 0;
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
         cus = 256;
-    b->grid = static_cast<uint32_t>(cus) * 8U;
+    b->grid = static_cast<uint32_t>(cus) * static_cast<uint32_t>(2048 / b->block);
+    b->seg_cap = ntables < 4096 ? 4096 : ntables;
+    b->handles = new (std::nothrow) htab*[ntables];
+    if (!b->handles)
+    {
+        delete b;
+        return fail(HTAB_E_OOM, "htab_batch_create: host allocation");
+    }
+    for (int32_t t = 0; t < ntables; ++t) b->handles[t] = tables[t];
     const size_t desc = 64 * static_cast<size_t>(ntables); // >= sizeof(table_ref<K, M>)
     static_assert(sizeof(table_ref<uint64_t, uint64_t>) <= 64, "descriptor size");
+    static_assert(sizeof(htab_dev::batch_segment) == sizeof(htab_segment), "segment layout");
     if (hipMalloc(&b->tables, desc) != hipSuccess ||
-        hipMalloc(&b->chunk_start, sizeof(int64_t) * (static_cast<size_t>(ntables) + 1)) != hipSuccess ||
+        hipMalloc(&b->segs, sizeof(htab_dev::batch_segment) * static_cast<size_t>(b->seg_cap)) != hipSuccess ||
+        hipMalloc(&b->chunk_start, sizeof(int64_t) * (static_cast<size_t>(b->seg_cap) + 1)) != hipSuccess ||
         hipMalloc(&b->counter, 256) != hipSuccess)
     {
-        if (b->tables) (void)hipFree(b->tables);
-        if (b->chunk_start) (void)hipFree(b->chunk_start);
-        delete b;
+        htab_batch_destroy(b);
         return fail(HTAB_E_OOM, "htab_batch_create: device allocation");
     }
-    const int rc = with_ops<batch_ops>(b->kind, b->lanes, [&](auto o) { return decltype(o)::build(b, tables); });
+    htab_call c;
+    c.op = call_op::batch_build;
+    c.b = b;
+    c.tables = tables;
+    const int rc = dispatch(b->kind, c);
     if (rc != HTAB_OK)
     {
         htab_batch_destroy(b);
@@ -602,9 +459,62 @@ htab_batch_destroy(htab_batch* b)
 {
     if (!b) return fail(HTAB_E_ARG, "htab_batch_destroy: null handle");
     if (b->tables) (void)hipFree(b->tables);
+    if (b->segs) (void)hipFree(b->segs);
     if (b->chunk_start) (void)hipFree(b->chunk_start);
     if (b->counter) (void)hipFree(b->counter);
+    delete[] b->handles;
     delete b;
+    return HTAB_OK;
+}
+
+int
+htab_batch_run_segments(htab_batch* b, const htab_segment* segs, int32_t nsegs, void* stream)
+{
+    if (!b || (nsegs > 0 && !segs) || nsegs < 0) return fail(HTAB_E_ARG, "htab_batch_run_segments: bad argument");
+    if (nsegs > b->seg_cap) return fail(HTAB_E_LIMIT, "htab_batch_run_segments: more segments than capacity");
+    for (int32_t g = 0; g < nsegs; ++g)
+    {
+        if (segs[g].table < 0 || segs[g].table >= b->ntables || segs[g].n < 0 || segs[g].op < HTAB_OP_INSERT ||
+            segs[g].op > HTAB_OP_ERASE || (segs[g].n > 0 && !segs[g].keys))
+            return fail(HTAB_E_ARG, "htab_batch_run_segments: bad segment");
+    }
+    if (nsegs == 0) return HTAB_OK;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    HTAB_HIP_TRY(hipMemcpyAsync(b->segs, segs, sizeof(htab_segment) * static_cast<size_t>(nsegs), hipMemcpyHostToDevice, s),
+                 "htab_batch_run_segments copy");
+    htab_call c;
+    c.op = call_op::batch_run_segments;
+    c.b = b;
+    c.nsegs = nsegs;
+    c.s = s;
+    return dispatch(b->kind, c);
+}
+
+int
+htab_batch_run_per_table(htab_batch* b, int op, const int64_t* off, const void* keys, const void* vals, uint8_t* flags,
+                         void* vals_out, void* stream)
+{
+    if (!b || !off || !keys) return fail(HTAB_E_ARG, "htab_batch_run_per_table: null argument");
+    const int kb = key_bytes_of(b->kind);
+    const int vb = mapped_bytes_of(b->kind);
+    for (int32_t t = 0; t < b->ntables; ++t)
+    {
+        const int64_t a = off[t], n = off[t + 1] - off[t];
+        if (n <= 0) continue;
+        const char* k = static_cast<const char*>(keys) + a * kb;
+        int rc = HTAB_OK;
+        if (op == HTAB_OP_INSERT)
+            rc = htab_insert_kv(b->handles[t], k, vals ? static_cast<const char*>(vals) + a * vb : nullptr, n,
+                                flags ? flags + a : nullptr, stream);
+        else if (op == HTAB_OP_FIND)
+            rc = htab_find(b->handles[t], k, n, flags + a, vals_out ? static_cast<char*>(vals_out) + a * vb : nullptr,
+                           stream);
+        else if (op == HTAB_OP_ERASE)
+            rc = htab_erase(b->handles[t], k, n, flags ? flags + a : nullptr, stream);
+        else
+            rc = fail(HTAB_E_ARG, "htab_batch_run_per_table: bad op");
+        if (rc != HTAB_OK) return rc;
+    }
     return HTAB_OK;
 }
 
@@ -615,9 +525,18 @@ htab_batch_run(htab_batch* b, const int8_t* ops, int op, const int64_t* offsets,
     if (!b || !offsets || !keys) return fail(HTAB_E_ARG, "htab_batch_run: null argument");
     if (!ops && (op < HTAB_OP_INSERT || op > HTAB_OP_ERASE)) return fail(HTAB_E_ARG, "htab_batch_run: bad op");
     hipStream_t s = static_cast<hipStream_t>(stream);
-    return with_ops<batch_ops>(b->kind, b->lanes, [&](auto o) {
-        return decltype(o)::run(b, ops, op, offsets, keys, vals, flags, vals_out, s);
-    });
+    htab_call c;
+    c.op = call_op::batch_run_offsets;
+    c.b = b;
+    c.ops = ops;
+    c.bop = op;
+    c.offsets = offsets;
+    c.keys = keys;
+    c.vals = vals;
+    c.flags = flags;
+    c.vals_out = vals_out;
+    c.s = s;
+    return dispatch(b->kind, c);
 }
 
 int

@@ -186,13 +186,12 @@ block_exclusive_scan(int64_t v, int64_t* s_tmp, int64_t* total)
 // kBatchRounds buckets finish on the serial path.
 constexpr int kBatchRounds = 8;
 
-template <typename K, typename M, int G>
+template <typename K, typename M, int G, int R>
 __device__ __forceinline__ void
 find_body(const table_ref<K, M>& t, const K* __restrict__ keys, const int64_t i, const bool valid,
           uint8_t* __restrict__ hits, void* __restrict__ vals_out)
 {
     using C = chunk<K, M>;
-    constexpr int R = G < 4 ? G : 4; // elements probed together per sub-batch
     const int lane = lane_id();
     const int gbase = lane & ~(G - 1);
     const int li = lane & (G - 1);
@@ -321,7 +320,7 @@ find_body(const table_ref<K, M>& t, const K* __restrict__ keys, const int64_t i,
 
     if (valid)
     {
-        hits[i] = myhit ? 1 : 0;
+        if (hits) hits[i] = myhit ? 1 : 0;
         if constexpr (slot_traits<K, M>::is_map)
         {
             if (vals_out && myhit) static_cast<M*>(vals_out)[i] = static_cast<M>(myval);
@@ -365,14 +364,13 @@ constexpr uint64_t kNoSlot = ~0ULL;
 // Uniqueness: free -> key is the only transition in an insert-only launch, so every key
 // inserted here sits at the first free slot of its probe sequence as of its CAS, and a racing
 // inserter of the same key either sees it there or loses the CAS on that very slot.
-template <typename K, typename M, int G>
+template <typename K, typename M, int G, int R>
 __device__ __forceinline__ void
 insert_body(const table_ref<K, M>& t, const void* __restrict__ values, const void* __restrict__ keys,
             const void* __restrict__ vals, const int64_t i, const bool valid, uint8_t* __restrict__ inserted)
 {
     using C = chunk<K, M>;
     constexpr int SPL = C::SPL;
-    constexpr int R = G < 4 ? G : 4;
     __shared__ int64_t s_wave[16];
     __shared__ int64_t s_old;
     const int lane = lane_id();
@@ -646,14 +644,13 @@ insert_body(const table_ref<K, M>& t, const void* __restrict__ values, const voi
 // probe sequence runs through such a bucket, and none can start to while only erases run),
 // otherwise to TOMB so probe sequences through the bucket stay intact.  Exactly one of several
 // erasers of one key wins the CAS.
-template <typename K, typename M, int G>
+template <typename K, typename M, int G, int R>
 __device__ __forceinline__ void
 erase_body(const table_ref<K, M>& t, const K* __restrict__ keys, const int64_t i, const bool valid,
            uint8_t* __restrict__ erased)
 {
     using C = chunk<K, M>;
     constexpr int SPL = C::SPL;
-    constexpr int R = G < 4 ? G : 4;
     __shared__ int64_t s_wave[16];
     __shared__ int64_t s_old;
     const int lane = lane_id();
@@ -809,47 +806,60 @@ erase_body(const table_ref<K, M>& t, const K* __restrict__ keys, const int64_t i
 
 
 // ================================================================== plain bulk kernels
-template <typename K, typename M, int G>
+template <typename K, typename M, int G, int R>
 __global__ __launch_bounds__(1024) void
 k_find(table_ref<K, M> t, const K* __restrict__ keys, int64_t n, uint8_t* __restrict__ hits,
        void* __restrict__ vals_out)
 {
     const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    find_body<K, M, G>(t, keys, i, i < n, hits, vals_out);
+    find_body<K, M, G, R>(t, keys, i, i < n, hits, vals_out);
 }
 
-template <typename K, typename M, int G>
+template <typename K, typename M, int G, int R>
 __global__ __launch_bounds__(1024) void
 k_insert(table_ref<K, M> t, const void* __restrict__ values, const void* __restrict__ keys,
          const void* __restrict__ vals, int64_t n, uint8_t* __restrict__ inserted)
 {
     const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    insert_body<K, M, G>(t, values, keys, vals, i, i < n, inserted);
+    insert_body<K, M, G, R>(t, values, keys, vals, i, i < n, inserted);
 }
 
-template <typename K, typename M, int G>
+template <typename K, typename M, int G, int R>
 __global__ __launch_bounds__(1024) void
 k_erase(table_ref<K, M> t, const K* __restrict__ keys, int64_t n, uint8_t* __restrict__ erased)
 {
     const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    erase_body<K, M, G>(t, keys, i, i < n, erased);
+    erase_body<K, M, G, R>(t, keys, i, i < n, erased);
 }
 
 
 // ================================================================== batched: many tables, one launch
-// Table t owns elements [offsets[t], offsets[t+1]) of the flattened key / value / flag arrays and
-// runs operation ops[t] (or `op` for every table when ops is null).  Work is cut into chunks of
-// blockDim elements that never straddle two tables; chunk_start[t] counts the chunks of tables
-// before t (k_batch_plan).  A persistent grid pulls chunks from one counter until all are done, so
-// every workgroup exits once the counter passes the last chunk.
+// A launch works through segments; segment s applies operation op to n elements of table `table`,
+// read from / written to its own device arrays (keys, vals, flags = inserted / hit / erased,
+// vals_out = mapped values of find hits).  Work is cut into chunks of kBatchBlock elements that
+// never straddle two segments; chunk_start[s] counts the chunks of segments before s.  A persistent
+// grid of B-thread workgroups pulls chunks of B elements from one counter until all are done, so
+// every workgroup exits once the counter passes the last chunk.  Segments of one table in one launch
+// must share their operation.
 constexpr int kOpInsert = 0;
 constexpr int kOpFind = 1;
 constexpr int kOpErase = 2;
-constexpr int kBatchBlock = 256;
 
-__global__ __launch_bounds__(1024) void
-k_batch_plan(const int64_t* __restrict__ offsets, int32_t ntables, int64_t* __restrict__ chunk_start,
-             unsigned long long* __restrict__ counter)
+struct batch_segment
+{
+    const void* keys;
+    const void* vals;
+    uint8_t* flags;
+    void* vals_out;
+    int64_t n;
+    int32_t table;
+    int32_t op;
+};
+
+// chunk prefix over segments (one workgroup); also resets the work counter
+__device__ inline void
+plan_chunks(const batch_segment* __restrict__ segs, int32_t nsegs, int chunk, int64_t* __restrict__ chunk_start,
+            unsigned long long* __restrict__ counter)
 {
     __shared__ int64_t s_tmp[17];
     __shared__ int64_t s_carry;
@@ -859,45 +869,79 @@ k_batch_plan(const int64_t* __restrict__ offsets, int32_t ntables, int64_t* __re
         *counter = 0;
     }
     __syncthreads();
-    for (int64_t base = 0; base < ntables; base += blockDim.x)
+    for (int64_t base = 0; base < nsegs; base += blockDim.x)
     {
         const int64_t t = base + threadIdx.x;
         int64_t c = 0;
-        if (t < ntables)
+        if (t < nsegs)
         {
-            const int64_t cnt = offsets[t + 1] - offsets[t];
-            c = cnt > 0 ? (cnt + kBatchBlock - 1) / kBatchBlock : 0;
+            const int64_t cnt = segs[t].n;
+            c = cnt > 0 ? (cnt + chunk - 1) / chunk : 0;
         }
         int64_t total = 0;
         const int64_t ex = block_exclusive_scan(c, s_tmp, &total);
-        if (t < ntables) chunk_start[t] = s_carry + ex;
+        if (t < nsegs) chunk_start[t] = s_carry + ex;
         __syncthreads();
         if (threadIdx.x == 0) s_carry += total;
         __syncthreads();
     }
-    if (threadIdx.x == 0) chunk_start[ntables] = s_carry;
+    if (threadIdx.x == 0) chunk_start[nsegs] = s_carry;
 }
 
-template <typename K, typename M, int G>
-__global__ __launch_bounds__(kBatchBlock) void
-k_batch(const table_ref<K, M>* __restrict__ tables, int32_t ntables, const int64_t* __restrict__ offsets,
-        const int8_t* __restrict__ ops, int op, const int64_t* __restrict__ chunk_start,
-        unsigned long long* __restrict__ counter, const void* __restrict__ keys, const void* __restrict__ vals,
-        uint8_t* __restrict__ flags, void* __restrict__ vals_out)
+// a template so every translation unit may instantiate it (no duplicate symbols)
+template <int Unused = 0>
+__global__ __launch_bounds__(1024) void
+k_batch_plan(const batch_segment* __restrict__ segs, int32_t nsegs, int chunk, int64_t* __restrict__ chunk_start,
+             unsigned long long* __restrict__ counter)
 {
-    __shared__ int s_table;
-    __shared__ int64_t s_begin, s_end;
-    const int64_t total_chunks = chunk_start[ntables];
+    plan_chunks(segs, nsegs, chunk, chunk_start, counter);
+}
+
+// segments from a per-table work-offset array: segment t = table t, elements [offsets[t], offsets[t+1])
+// of the flattened arrays, operation ops[t] (or op)
+// a template so every translation unit may instantiate it (no duplicate symbols)
+template <int Unused = 0>
+__global__ __launch_bounds__(1024) void
+k_batch_plan_offsets(const int64_t* __restrict__ offsets, int32_t ntables, const int8_t* __restrict__ ops, int op,
+                     const char* keys, const char* vals, uint8_t* flags, char* vals_out, int key_bytes,
+                     int val_bytes, int chunk, batch_segment* __restrict__ segs,
+                     int64_t* __restrict__ chunk_start, unsigned long long* __restrict__ counter)
+{
+    for (int32_t t = threadIdx.x; t < ntables; t += blockDim.x)
+    {
+        const int64_t a = offsets[t];
+        batch_segment g;
+        g.keys = keys + a * key_bytes;
+        g.vals = vals ? vals + a * val_bytes : nullptr;
+        g.flags = flags ? flags + a : nullptr;
+        g.vals_out = vals_out ? vals_out + a * val_bytes : nullptr;
+        g.n = offsets[t + 1] - a;
+        g.table = t;
+        g.op = ops ? static_cast<int32_t>(ops[t]) : op;
+        segs[t] = g;
+    }
+    __syncthreads();
+    plan_chunks(segs, ntables, chunk, chunk_start, counter);
+}
+
+template <typename K, typename M, int G, int R, int B>
+__global__ __launch_bounds__(B) void
+k_batch(const table_ref<K, M>* __restrict__ tables, const batch_segment* __restrict__ segs, int32_t nsegs,
+        const int64_t* __restrict__ chunk_start, unsigned long long* __restrict__ counter)
+{
+    __shared__ int s_seg;
+    __shared__ int64_t s_begin;
+    const int64_t total_chunks = chunk_start[nsegs];
     while (true)
     {
         if (threadIdx.x == 0)
         {
             const int64_t c = static_cast<int64_t>(atomicAdd(counter, 1ULL));
-            int t = -1;
-            int64_t b = 0, e = 0;
+            int g = -1;
+            int64_t b = 0;
             if (c < total_chunks)
             {
-                int lo = 0, hi = ntables; // last t with chunk_start[t] <= c
+                int lo = 0, hi = nsegs; // last s with chunk_start[s] <= c
                 while (lo < hi)
                 {
                     const int mid = (lo + hi + 1) >> 1;
@@ -906,29 +950,27 @@ k_batch(const table_ref<K, M>* __restrict__ tables, int32_t ntables, const int64
                     else
                         hi = mid - 1;
                 }
-                t = lo;
-                b = offsets[t] + (c - chunk_start[t]) * kBatchBlock;
-                e = b + kBatchBlock < offsets[t + 1] ? b + kBatchBlock : offsets[t + 1];
+                g = lo;
+                b = (c - chunk_start[g]) * B;
             }
-            s_table = t;
+            s_seg = g;
             s_begin = b;
-            s_end = e;
         }
         __syncthreads();
-        const int t = s_table;
-        const int64_t b = s_begin, e = s_end;
+        const int g = s_seg;
+        const int64_t b = s_begin;
         __syncthreads();
-        if (t < 0) break;
-        const table_ref<K, M> tr = tables[t];
-        const int o = ops ? static_cast<int>(ops[t]) : op;
+        if (g < 0) break;
+        const batch_segment seg = segs[g];
+        const table_ref<K, M> tr = tables[seg.table];
         const int64_t i = b + threadIdx.x;
-        const bool valid = i < e;
-        if (o == kOpInsert)
-            insert_body<K, M, G>(tr, nullptr, keys, vals, i, valid, flags);
-        else if (o == kOpFind)
-            find_body<K, M, G>(tr, static_cast<const K*>(keys), i, valid, flags, vals_out);
-        else if (o == kOpErase)
-            erase_body<K, M, G>(tr, static_cast<const K*>(keys), i, valid, flags);
+        const bool valid = i < seg.n;
+        if (seg.op == kOpInsert)
+            insert_body<K, M, G, R>(tr, nullptr, seg.keys, seg.vals, i, valid, seg.flags);
+        else if (seg.op == kOpFind)
+            find_body<K, M, G, R>(tr, static_cast<const K*>(seg.keys), i, valid, seg.flags, seg.vals_out);
+        else if (seg.op == kOpErase)
+            erase_body<K, M, G, R>(tr, static_cast<const K*>(seg.keys), i, valid, seg.flags);
     }
 }
 
@@ -996,6 +1038,8 @@ k_range_count(table_ref<K, M> t, uint64_t total_slots, int64_t* __restrict__ til
 }
 
 // single-workgroup exclusive scan of the tile counts (in place); grand total at counts[ntiles]
+// a template so every translation unit may instantiate it (no duplicate symbols)
+template <int Unused = 0>
 __global__ __launch_bounds__(1024) void
 k_scan_tiles(int64_t* counts, int64_t ntiles)
 {
@@ -1055,6 +1099,8 @@ k_range_scatter(table_ref<K, M> t, uint64_t total_slots, const int64_t* __restri
     }
 }
 
+// a template so every translation unit may instantiate it (no duplicate symbols)
+template <int Unused = 0>
 __global__ void
 k_zero_u64(unsigned long long* p, int n)
 {

@@ -1,0 +1,269 @@
+// Typed dispatch of the C ABI onto the kernels: one instantiation set per key/value kind, each
+// compiled in its own translation unit (kind_*.hip).
+#ifndef STDGPU_AMD_HTAB_OPS_HPP
+#define STDGPU_AMD_HTAB_OPS_HPP
+
+#include <new>
+#include <type_traits>
+
+#include "htab_internal.hpp"
+
+namespace htab_host
+{
+
+constexpr int
+default_rb(int g)
+{
+    return g < 8 ? g : 8;
+}
+
+// Block sizes of the persistent batched kernel: all three for the benchmarked map kind, 256 elsewhere
+// (keeps the build small).
+template <typename K, typename M>
+constexpr bool batch_all_blocks = std::is_same_v<K, uint64_t> && std::is_same_v<M, uint32_t>;
+
+} // namespace htab_host
+
+namespace htab_host
+{
+
+using namespace htab_dev;
+
+// ------------------------------------------------------------------ typed dispatch
+template <typename K, typename M, int G, int R>
+struct ops
+{
+    using T = table_ref<K, M>;
+
+    static int insert(htab* h, const void* values, const void* keys, const void* vals, int64_t n, uint8_t* ins,
+                      hipStream_t s)
+    {
+        const int64_t grid = (n + h->block - 1) / h->block;
+        hipLaunchKernelGGL((k_insert<K, M, G, R>), dim3(static_cast<uint32_t>(grid)), dim3(h->block), 0, s,
+                           T::from_view(h->view), values, keys, vals, n, ins);
+        return launch_status("insert launch");
+    }
+
+    static int find(htab* h, const void* keys, int64_t n, uint8_t* hits, void* vals, hipStream_t s)
+    {
+        const int64_t grid = (n + h->block - 1) / h->block;
+        hipLaunchKernelGGL((k_find<K, M, G, R>), dim3(static_cast<uint32_t>(grid)), dim3(h->block), 0, s,
+                           T::from_view(h->view), static_cast<const K*>(keys), n, hits, vals);
+        return launch_status("find launch");
+    }
+
+    static int erase(htab* h, const void* keys, int64_t n, uint8_t* del, hipStream_t s)
+    {
+        const int64_t grid = (n + h->block - 1) / h->block;
+        hipLaunchKernelGGL((k_erase<K, M, G, R>), dim3(static_cast<uint32_t>(grid)), dim3(h->block), 0, s,
+                           T::from_view(h->view), static_cast<const K*>(keys), n, del);
+        return launch_status("erase launch");
+    }
+};
+
+template <typename K, typename M, int G>
+struct batch_ops
+{
+    template <int B>
+    static int launch_b(htab_batch* b, int32_t nsegs, hipStream_t s)
+    {
+        hipLaunchKernelGGL((k_batch<K, M, G, default_rb(G), B>), dim3(b->grid), dim3(B), 0, s,
+                           static_cast<const table_ref<K, M>*>(b->tables), b->segs, nsegs, b->chunk_start,
+                           b->counter);
+        return launch_status("batch launch");
+    }
+
+    static int build(htab_batch* b, htab* const* tables)
+    {
+        using T = table_ref<K, M>;
+        T* host = new (std::nothrow) T[b->ntables];
+        if (!host) return fail(HTAB_E_OOM, "htab_batch_create: host allocation");
+        for (int32_t t = 0; t < b->ntables; ++t) host[t] = T::from_view(tables[t]->view);
+        hipError_t e = hipMemcpy(b->tables, host, sizeof(T) * b->ntables, hipMemcpyHostToDevice);
+        delete[] host;
+        if (e != hipSuccess) return fail_hip(e, "htab_batch_create: descriptor copy");
+        return HTAB_OK;
+    }
+
+    static int launch(htab_batch* b, int32_t nsegs, hipStream_t s)
+    {
+        if constexpr (batch_all_blocks<K, M>)
+        {
+            if (b->block == 512) return launch_b<512>(b, nsegs, s);
+            if (b->block == 1024) return launch_b<1024>(b, nsegs, s);
+        }
+        return launch_b<256>(b, nsegs, s);
+    }
+
+    static int run(htab_batch* b, const int8_t* ops, int op, const int64_t* offsets, const void* keys,
+                   const void* vals, uint8_t* flags, void* vals_out, hipStream_t s)
+    {
+        const int vb = std::is_void_v<M> ? 0 : static_cast<int>(sizeof(std::conditional_t<std::is_void_v<M>, char, M>));
+        hipLaunchKernelGGL(k_batch_plan_offsets<>, dim3(1), dim3(1024), 0, s, offsets, b->ntables, ops, op,
+                           static_cast<const char*>(keys), static_cast<const char*>(vals), flags,
+                           static_cast<char*>(vals_out), static_cast<int>(sizeof(K)), vb, b->block, b->segs,
+                           b->chunk_start, b->counter);
+        int rc = launch_status("batch plan");
+        if (rc) return rc;
+        return launch(b, b->ntables, s);
+    }
+
+    static int run_segments(htab_batch* b, int32_t nsegs, hipStream_t s)
+    {
+        hipLaunchKernelGGL(k_batch_plan<>, dim3(1), dim3(1024), 0, s, b->segs, nsegs, b->block, b->chunk_start,
+                           b->counter);
+        int rc = launch_status("batch plan");
+        if (rc) return rc;
+        return launch(b, nsegs, s);
+    }
+};
+
+template <typename K, typename M>
+struct table_ops
+{
+    using T = table_ref<K, M>;
+    using value_type = typename slot_traits<K, M>::value_type;
+
+    static int clear(htab* h, hipStream_t s)
+    {
+        const T t = T::from_view(h->view);
+        const uint64_t area = (t.nslots() + 2) * sizeof(value_type);
+        const uint64_t nchunks = (area + 15) / 16;
+        uint64_t grid = (nchunks + 255) / 256;
+        if (grid > 65536) grid = 65536;
+        if (grid == 0) grid = 1;
+        hipLaunchKernelGGL((k_clear<K, M>), dim3(static_cast<uint32_t>(grid)), dim3(256), 0, s, t, nchunks);
+        int rc = launch_status("clear launch");
+        if (rc) return rc;
+        hipLaunchKernelGGL((k_clear_tail<K, M>), dim3(1), dim3(1), 0, s, t);
+        return launch_status("clear launch");
+    }
+
+    static int range(htab* h, int64_t* idx_out, void* keys_out, void* vals_out, int64_t cap, int64_t* n_out,
+                     hipStream_t s)
+    {
+        const T t = T::from_view(h->view);
+        const uint64_t total = t.nslots() + 2;
+        const uint64_t ntiles = (total + kRangeTile - 1) / kRangeTile;
+        const size_t need = (ntiles + 1) * sizeof(int64_t);
+        if (h->scratch_bytes < need)
+        {
+            if (h->scratch) (void)hipFree(h->scratch);
+            h->scratch = nullptr;
+            h->scratch_bytes = 0;
+            HTAB_HIP_TRY(hipMalloc(&h->scratch, need), "range scratch");
+            h->scratch_bytes = need;
+        }
+        hipLaunchKernelGGL((k_range_count<K, M>), dim3(static_cast<uint32_t>(ntiles)), dim3(kRangeThreads), 0, s, t,
+                           total, h->scratch);
+        int rc = launch_status("range count");
+        if (rc) return rc;
+        hipLaunchKernelGGL(k_scan_tiles<>, dim3(1), dim3(1024), 0, s, h->scratch, static_cast<int64_t>(ntiles));
+        rc = launch_status("range scan");
+        if (rc) return rc;
+        hipLaunchKernelGGL((k_range_scatter<K, M>), dim3(static_cast<uint32_t>(ntiles)), dim3(kRangeThreads), 0, s, t,
+                           total, h->scratch, idx_out, static_cast<K*>(keys_out), vals_out, cap);
+        rc = launch_status("range scatter");
+        if (rc) return rc;
+        int64_t n = 0;
+        HTAB_HIP_TRY(hipMemcpyAsync(&n, h->scratch + ntiles, sizeof(int64_t), hipMemcpyDeviceToHost, s), "range count copy");
+        HTAB_HIP_TRY(hipStreamSynchronize(s), "range sync");
+        *n_out = n;
+        return HTAB_OK;
+    }
+
+    static int valid(htab* h, int32_t* out, hipStream_t s)
+    {
+        const T t = T::from_view(h->view);
+        const size_t need = 2 * sizeof(unsigned long long);
+        if (h->scratch_bytes < need)
+        {
+            if (h->scratch) (void)hipFree(h->scratch);
+            h->scratch = nullptr;
+            h->scratch_bytes = 0;
+            HTAB_HIP_TRY(hipMalloc(&h->scratch, need), "valid scratch");
+            h->scratch_bytes = need;
+        }
+        hipLaunchKernelGGL(k_zero_u64<>, dim3(1), dim3(64), 0, s, reinterpret_cast<unsigned long long*>(h->scratch), 2);
+        int rc = launch_status("valid zero");
+        if (rc) return rc;
+        const uint64_t total = t.nslots() + 2;
+        uint64_t grid = (total + 255) / 256;
+        if (grid > 16384) grid = 16384;
+        hipLaunchKernelGGL((k_valid<K, M>), dim3(static_cast<uint32_t>(grid)), dim3(256), 0, s, t, total,
+                           reinterpret_cast<unsigned long long*>(h->scratch));
+        rc = launch_status("valid launch");
+        if (rc) return rc;
+        unsigned long long res[2] = {0, 0};
+        int64_t stripes[kMaxStripes * kStripeStride];
+        HTAB_HIP_TRY(hipMemcpyAsync(res, h->scratch, sizeof(res), hipMemcpyDeviceToHost, s), "valid copy");
+        HTAB_HIP_TRY(hipMemcpyAsync(stripes, h->view.stripes, sizeof(stripes), hipMemcpyDeviceToHost, s), "valid copy");
+        HTAB_HIP_TRY(hipStreamSynchronize(s), "valid sync");
+        int64_t size = 0;
+        for (uint32_t i = 0; i < kMaxStripes; ++i) size += stripes[i * kStripeStride];
+        *out = (res[0] == 0 && static_cast<int64_t>(res[1]) == size && size >= 0 &&
+                size <= static_cast<int64_t>(h->view.limit))
+                       ? 1
+                       : 0;
+        return HTAB_OK;
+    }
+};
+
+template <typename K, typename M, int G>
+int
+dispatch_g(htab_call& c)
+{
+    switch (c.op)
+    {
+        case call_op::insert:
+        case call_op::find:
+        case call_op::erase:
+        {
+            const int rb = c.h->rb;
+            auto go = [&](auto o) -> int {
+                using O = decltype(o);
+                if (c.op == call_op::insert) return O::insert(c.h, c.values, c.keys, c.vals, c.n, c.flags, c.s);
+                if (c.op == call_op::find) return O::find(c.h, c.keys, c.n, c.flags, c.vals_out, c.s);
+                return O::erase(c.h, c.keys, c.n, c.flags, c.s);
+            };
+            if constexpr (G >= 8)
+            {
+                if (rb == 8) return go(ops<K, M, G, 8>{});
+            }
+            if (rb == 4) return go(ops<K, M, G, 4>{});
+            if (rb == 2) return go(ops<K, M, G, 2>{});
+            return fail(HTAB_E_ARG, "bad probe batch");
+        }
+        case call_op::batch_build: return batch_ops<K, M, G>::build(c.b, c.tables);
+        case call_op::batch_run_offsets:
+            return batch_ops<K, M, G>::run(c.b, c.ops, c.bop, c.offsets, c.keys, c.vals, c.flags, c.vals_out, c.s);
+        case call_op::batch_run_segments: return batch_ops<K, M, G>::run_segments(c.b, c.nsegs, c.s);
+        default: return fail(HTAB_E_ARG, "bad call");
+    }
+}
+
+template <typename K, typename M>
+int
+dispatch_kind(htab_call& c)
+{
+    switch (c.op)
+    {
+        case call_op::clear: return table_ops<K, M>::clear(c.h, c.s);
+        case call_op::range: return table_ops<K, M>::range(c.h, c.idx_out, const_cast<void*>(c.keys), c.vals_out, c.cap,
+                                                           c.n_out, c.s);
+        case call_op::valid: return table_ops<K, M>::valid(c.h, c.valid_out, c.s);
+        default: break;
+    }
+    const int lanes = c.h ? c.h->lanes : c.b->lanes;
+    switch (lanes)
+    {
+        case 4: return dispatch_g<K, M, 4>(c);
+        case 8: return dispatch_g<K, M, 8>(c);
+        case 16: return dispatch_g<K, M, 16>(c);
+        default: return fail(HTAB_E_ARG, "bad bucket_lanes");
+    }
+}
+
+} // namespace htab_host
+
+#endif // STDGPU_AMD_HTAB_OPS_HPP

@@ -175,3 +175,62 @@ def test_batched_mixed_ops_one_launch():
         np.testing.assert_array_equal(ka, kb)
         np.testing.assert_array_equal(va, vb)
     batch.destroy()
+
+
+def test_segments_coalesce_requests_like_plain_calls():
+    """Several requests per table in one launch (same operation within a table), each with its own
+    buffers, against the same requests issued as plain calls in order."""
+    rng = np.random.default_rng(21)
+    ntab = 12
+    caps = [4000 + 500 * t for t in range(ntab)]
+    tabs = [sa.unordered_map_u64_u32.createDeviceObject(c) for c in caps]
+    plain = [sa.unordered_map_u64_u32.createDeviceObject(c) for c in caps]
+    batch = TableBatch(tabs)
+    # round 1: two insert requests per table (overlapping keys); round 2: finds; round 3: erases
+    for rnd, op in enumerate(["insert", "find", "erase", "find"]):
+        segs, expect = [], []
+        for t in range(ntab):
+            for r in range(2):
+                k = rng.integers(0, 3000, size=int(rng.integers(1, 900)), dtype=np.uint64) + np.uint64(t << 32)
+                v = wl.value_of(k)
+                dk, dv = d64(k), d32(v)
+                f = torch.zeros(len(k), dtype=torch.uint8, device="cuda")
+                vo = torch.zeros(len(k), dtype=torch.int32, device="cuda")
+                segs.append((t, op, dk, dv, f, vo))
+                expect.append((t, dk, dv))
+        batch.run_segments(segs)
+        torch.cuda.synchronize()
+        for (t, op_, dk, dv, f, vo), (_, ek, ev) in zip(segs, expect):
+            if op == "find":
+                h, v = plain[t].find(ek)
+                np.testing.assert_array_equal(f.cpu().numpy(), h.cpu().numpy())
+                hv = h.cpu().numpy() == 1
+                np.testing.assert_array_equal(vo.cpu().numpy()[hv], v.cpu().numpy()[hv])
+        # inserts / erases: compare contents after applying the same requests in order
+        if op in ("insert", "erase"):
+            for (t, op_, dk, dv, f, vo) in segs:
+                if op == "insert":
+                    plain[t].insert(dk, dv)
+                else:
+                    plain[t].erase(dk)
+            for t in range(ntab):
+                ka, va = sorted_contents(tabs[t])
+                kb, vb = sorted_contents(plain[t])
+                np.testing.assert_array_equal(ka, kb)
+                np.testing.assert_array_equal(va, vb)
+    batch.destroy()
+
+
+def test_per_table_baseline_matches_batched():
+    w = wl.batched_maps(ntables=32, log2_lo=6, log2_hi=11, seed=14)
+    bt = make_tables(w, 8)
+    pt = make_tables(w, 8)
+    b1, b2 = TableBatch(bt), TableBatch(pt)
+    n = len(w.ins_keys)
+    f1 = torch.zeros(n, dtype=torch.uint8, device="cuda")
+    f2 = torch.zeros(n, dtype=torch.uint8, device="cuda")
+    b1.run("insert", d64(w.ins_offsets), d64(w.ins_keys), d32(w.ins_vals), flags=f1)
+    b2.run_per_table("insert", w.ins_offsets, d64(w.ins_keys), d32(w.ins_vals), flags=f2)
+    assert torch.equal(f1, f2)
+    for t in range(len(bt)):
+        assert bt[t].size() == pt[t].size()
