@@ -35,7 +35,14 @@ def parse():
     ap.add_argument("--lanes", type=int, default=8)
     ap.add_argument("--block", type=int, default=256)
     ap.add_argument("--cpu-baseline", type=int, default=1, help="time the oracle on a bounded sample")
-    ap.add_argument("--cpu-log2n", type=int, default=20)
+    ap.add_argument("--cpu-log2n", type=int, default=22)
+    ap.add_argument("--probe-batch", type=int, default=0)
+    ap.add_argument("--batched", type=int, default=1, help="also measure the batched form")
+    ap.add_argument("--batched-tables", type=int, default=4096)
+    ap.add_argument("--tune-tables", type=int, default=1024)
+    ap.add_argument("--served", type=int, default=1, help="also measure the served form")
+    ap.add_argument("--served-requests", type=int, default=3000)
+    ap.add_argument("--served-seconds", type=float, default=0.25)
     return ap.parse_args()
 
 
@@ -71,6 +78,125 @@ def cpu_baseline(log2n: int) -> dict:
     }
 
 
+def bench_batched(args) -> dict:
+    """Batched form: 4096 maps (log-uniform 2^8..2^18 keys) served by one persistent,
+    graph-captured launch per phase after autotuning, against one plain call per table."""
+    import torch
+
+    from stdgpu_amd import autotune as at
+    from stdgpu_amd import containers as C
+    from stdgpu_amd import workloads_device as wd
+    from stdgpu_amd.batched import TableBatch
+
+    w = wd.batched_maps("cuda", ntables=args.batched_tables)
+    t0 = time.perf_counter()
+    tuned = at.tune(w, sample_tables=min(args.batched_tables, args.tune_tables))
+    tune_s = time.perf_counter() - t0
+    best = tuned["best"]
+    ops = w["ins_keys"].numel() + w["find_keys"].numel() + w["erase_keys"].numel()
+
+    def fresh():
+        tabs = at.build_tables(w["sizes"], best)
+        return tabs, TableBatch(tabs)
+
+    tabs, batch = fresh()
+    at.run_phases(batch, w)  # warm-up on fresh tables
+    batch.destroy()
+    for t in tabs:
+        C.unordered_map_u64_u32.destroyDeviceObject(t)
+    tabs, batch = fresh()
+    ms, out = at.run_phases(batch, w)
+    # plain per-table calls on fresh tables: the same work, one htab_* call per table
+    ptabs, pbatch = fresh()
+    s = torch.cuda.Stream()
+    pout = {k: torch.zeros_like(v) for k, v in out.items()}
+    off = {k: w[f"{k}_offsets"].cpu().numpy() for k in ("ins", "find", "erase")}
+    pms = {}
+    for op, o, k, v, f, vo in [("insert", off["ins"], w["ins_keys"], w["ins_vals"], pout["inserted"], None),
+                               ("find", off["find"], w["find_keys"], None, pout["hits"], pout["values"]),
+                               ("erase", off["erase"], w["erase_keys"], None, pout["erased"], None)]:
+        torch.cuda.synchronize()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(s)
+        pbatch.run_per_table(op, o, k, v, f, vo, stream=s)
+        b.record(s)
+        b.synchronize()
+        pms[op] = a.elapsed_time(b)
+    hit = out["hits"] == 1
+    identical = (torch.equal(out["inserted"], pout["inserted"]) and torch.equal(out["hits"], pout["hits"])
+                 and torch.equal(out["values"][hit], pout["values"][hit]) and torch.equal(out["erased"], pout["erased"])
+                 and all(tabs[i].size() == ptabs[i].size() for i in range(0, len(tabs), 97)))
+    total_ms = sum(ms.values())
+    plain_ms = sum(pms.values())
+    res = {
+        "workload": f"{args.batched_tables} unordered_map<u64,u32>, keys log-uniform in [2^8, 2^18] per table; "
+                    "insert all, find (70% hit), erase 30%; one graph-captured persistent launch per phase",
+        "ops": ops,
+        "gops": round(ops / (total_ms / 1e3) / 1e9, 3),
+        "phase_ms": {k: round(v, 3) for k, v in ms.items()},
+        "plain_per_call_gops": round(ops / (plain_ms / 1e3) / 1e9, 3),
+        "plain_phase_ms": {k: round(v, 3) for k, v in pms.items()},
+        "speedup_vs_plain": round(plain_ms / total_ms, 2),
+        "identical_to_plain": bool(identical),
+        "autotune": {"best": at.describe(best), "candidates": len(tuned["results"]),
+                     "sample_tables": min(args.batched_tables, args.tune_tables), "seconds": round(tune_s, 1),
+                     "top3": [{k: r[k] for k in ("bucket_lanes", "probe_group", "block_size", "load", "total_ms")}
+                              for r in tuned["results"][:3]]},
+    }
+    for b_ in (batch, pbatch):
+        b_.destroy()
+    for t in tabs + ptabs:
+        C.unordered_map_u64_u32.destroyDeviceObject(t)
+    return res
+
+
+def bench_served(args) -> dict:
+    """Served form: 64 resident maps (capacities log-uniform in [2^16, 2^26] keys), open-loop Poisson
+    requests (table ~ Zipf(1.0), insert/find/erase 0.3/0.6/0.1, batch log-uniform in [2^6, 2^20] keys)
+    at rates up to saturation; p50/p95/p99 latency and throughput, against the same log as plain
+    calls; every find's hits and the final contents checked against the in-order replay."""
+    import torch
+
+    from stdgpu_amd import serving as sv
+
+    caps = sv.capacities(ntables=64)
+    pools = sv.Pools(caps)
+    nreq = args.served_requests
+    sat_log = sv.make_log(pools.sizes, nreq=nreq, rate_per_s=None, seed=8)
+    # warm-up, then saturation (every request arrives at once)
+    tabs, _, _ = sv.served_run(caps, pools, sv.make_log(pools.sizes, nreq=200, rate_per_s=None, seed=3))
+    sv.destroy_tables(tabs)
+    tabs, (flags, vals, st), sat = sv.served_run(caps, pools, sat_log)
+    chk = sv.verify_against_replay(caps, pools, sat_log, tabs, st, flags, vals)
+    sv.destroy_tables(tabs)
+    del flags, vals
+    torch.cuda.empty_cache()
+    plain_block = sv.plain_run(caps, pools, sat_log, blocking=True)
+    plain_async = sv.plain_run(caps, pools, sat_log, blocking=False)
+    sat_rps = sat["req_per_s"]
+    rates = []
+    for frac in (0.2, 0.4, 0.6, 0.8, 1.0):
+        rate = frac * sat_rps
+        k = int(min(nreq, max(200, rate * args.served_seconds)))
+        log = sv.make_log(pools.sizes, nreq=k, rate_per_s=rate, seed=20 + int(frac * 10))
+        tabs, outs, r = sv.served_run(caps, pools, log)
+        sv.destroy_tables(tabs)
+        del outs
+        rates.append({"rate_req_per_s": round(rate, 1), "offered_frac_of_saturation": frac, **r})
+    return {
+        "workload": "64 unordered_map<u64,u32>, capacities log-uniform in [2^16, 2^26]; requests: table ~ Zipf(1.0), "
+                    "insert/find/erase 0.3/0.6/0.1, batch log-uniform in [2^6, 2^20] keys, Poisson arrivals",
+        "saturation": sat,
+        "saturation_req_per_s": round(sat_rps, 1),
+        "plain_blocking_gops": plain_block["gops"],
+        "plain_async_gops": plain_async["gops"],
+        "served_vs_plain_blocking": round(sat["gops"] / max(plain_block["gops"], 1e-9), 2),
+        "served_vs_plain_async": round(sat["gops"] / max(plain_async["gops"], 1e-9), 2),
+        "identical_to_in_order_replay": bool(chk["finds_identical"] and chk["contents_identical"]),
+        "rates": rates,
+    }
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -103,7 +229,7 @@ def main():
     er_flags = torch.empty(len(w.erase), dtype=torch.uint8, device=dev)
 
     tab = sa.unordered_map_u64_u32.createDeviceObject(
-        w.capacity, sa.Tuning(bucket_lanes=args.lanes, block_size=args.block))
+        w.capacity, sa.Tuning(bucket_lanes=args.lanes, block_size=args.block, probe_batch=args.probe_batch))
     stream = torch.cuda.current_stream()
     names = ["clear", "insert", "find", "erase", "reinsert"]
 
@@ -198,6 +324,10 @@ def main():
     }
     sa.unordered_map_u64_u32.destroyDeviceObject(tab)
     del keys, vals, queries, erase, reins, reins_v, hits, got, ins_flags, er_flags
+    if args.batched:
+        out["batched"] = bench_batched(args)
+    if args.served:
+        out["served"] = bench_served(args)
     if args.cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.cpu_log2n)
     print(json.dumps(out), flush=True)

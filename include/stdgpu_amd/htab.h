@@ -45,7 +45,8 @@ typedef struct htab_tuning
     int32_t bucket_lanes;      /* lanes that probe one bucket together: 4, 8 or 16 (bucket = 16*lanes bytes); default 8 */
     int32_t block_size;        /* threads per workgroup of the bulk kernels: 256, 512 or 1024; default 256 */
     float slots_per_capacity;  /* physical slots = ceil(capacity * this); default 1.0 (load 0.8 at 0.8*capacity keys) */
-    int32_t probe_batch;       /* elements a lane group probes together: 2, 4 or 8 (at most bucket_lanes); default 8 */
+    int32_t probe_batch;       /* elements a lane group probes together: 2, 4 or 8 (at most bucket_lanes);
+                                * default: 8 for find, 4 for insert and erase */
 } htab_tuning;
 
 /* Sizes a caller can query.  bucket_count / max_size follow the reference formulas so
@@ -180,6 +181,39 @@ int htab_batch_capture(htab_batch* batch, const int8_t* ops, int op, const int64
                        const void* vals, uint8_t* flags, void* vals_out, void* stream, htab_graph** out);
 int htab_graph_launch(htab_graph* graph, void* stream);
 int htab_graph_destroy(htab_graph* graph);
+
+/* ------------------------------------------------------------------------------------------
+ * Served form: a request loop over resident tables with continuous batching (no reference
+ * counterpart).  Requests are htab_segment records (table index, op, n, device buffers).  While
+ * requests arrive, a scheduler coalesces the same-operation prefix of every idle table's queue into
+ * one persistent launch per free stream slot; a table is in one launch at a time, so per table the
+ * requests take effect in arrival order, and launches of different tables overlap.  An insert that
+ * might not fit the table's free slots is refused (status HTAB_E_LIMIT, no effect).
+ * ------------------------------------------------------------------------------------------ */
+typedef struct htab_server htab_server;
+
+typedef struct htab_server_stats
+{
+    int64_t launches;
+    int64_t segments;
+    int64_t max_concurrent_launches;
+    int64_t wall_ns;
+} htab_server_stats;
+
+/* nstreams launch slots (1..16); max_inflight_elems bounds the elements in flight over all slots. */
+int htab_server_create(htab* const* tables, int32_t ntables, int32_t nstreams, int64_t max_inflight_elems,
+                       htab_server** out);
+int htab_server_destroy(htab_server* server);
+/* Open-loop replay of a request log: request i is submitted at arrival_ns[i] after the start.
+ * Returns per request its submit and completion times (ns since the start) and status (0 or
+ * HTAB_E_LIMIT when refused).  Blocks until every request has completed. */
+int htab_server_run(htab_server* server, const htab_segment* requests, const int64_t* arrival_ns, int64_t nreq,
+                    int64_t* submit_ns, int64_t* done_ns, int32_t* status, htab_server_stats* stats);
+/* The same requests as plain calls, one per request in log order on `stream` (skipping requests whose
+ * status is non-zero when status is given); blocking = 1 synchronises after every call, as the
+ * reference's bulk calls do. */
+int htab_server_replay_plain(htab_server* server, const htab_segment* requests, const int32_t* status, int64_t nreq,
+                             int blocking, void* stream);
 
 /* Message for the last non-zero status returned to the calling thread. */
 const char* htab_last_error(void);

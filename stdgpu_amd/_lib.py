@@ -65,6 +65,15 @@ class htab_segment(ctypes.Structure):
     ]
 
 
+class htab_server_stats(ctypes.Structure):
+    _fields_ = [
+        ("launches", ctypes.c_int64),
+        ("segments", ctypes.c_int64),
+        ("max_concurrent_launches", ctypes.c_int64),
+        ("wall_ns", ctypes.c_int64),
+    ]
+
+
 class htab_view(ctypes.Structure):
     _fields_ = [
         ("slots", ctypes.c_void_p),
@@ -109,6 +118,10 @@ SIGNATURES = {
     "htab_batch_capture": (_I32, [_P, _P, _I32, _P, _P, _P, _P, _P, _P, ctypes.POINTER(_P)]),
     "htab_graph_launch": (_I32, [_P, _P]),
     "htab_graph_destroy": (_I32, [_P]),
+    "htab_server_create": (_I32, [ctypes.POINTER(_P), ctypes.c_int32, ctypes.c_int32, _I64, ctypes.POINTER(_P)]),
+    "htab_server_destroy": (_I32, [_P]),
+    "htab_server_run": (_I32, [_P, _P, _P, _I64, _P, _P, _P, _P]),
+    "htab_server_replay_plain": (_I32, [_P, _P, _P, _I64, _I32, _P]),
     "htab_deque_create": (_I32, [_I64, ctypes.POINTER(_P)]),
     "htab_deque_destroy": (_I32, [_P]),
     "htab_deque_push_back": (_I32, [_P, _P, _I64, _P, _P]),

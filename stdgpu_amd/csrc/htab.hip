@@ -133,8 +133,7 @@ htab_create(int kind, int64_t capacity, const htab_tuning* tuning, htab** out)
         rb = tuning->probe_batch;
     }
     if (lanes != 4 && lanes != 8 && lanes != 16) return fail(HTAB_E_ARG, "htab_create: bucket_lanes must be 4, 8 or 16");
-    if (rb == 0) rb = lanes < 8 ? lanes : 8;
-    if ((rb != 2 && rb != 4 && rb != 8) || rb > lanes)
+    if (rb != 0 && ((rb != 2 && rb != 4 && rb != 8) || rb > lanes))
         return fail(HTAB_E_ARG, "htab_create: probe_batch must be 2, 4 or 8 and at most bucket_lanes");
     if (block != 256 && block != 512 && block != 1024) return fail(HTAB_E_ARG, "htab_create: block_size must be 256, 512 or 1024");
     if (!(spc >= 0.25F && spc <= 16.0F)) return fail(HTAB_E_ARG, "htab_create: slots_per_capacity out of [0.25, 16]");
@@ -407,14 +406,15 @@ htab_batch_create(htab* const* tables, int32_t ntables, htab_batch** out)
     {
         if (!tables[t]) return fail(HTAB_E_ARG, "htab_batch_create: null table");
         if (tables[t]->kind != tables[0]->kind || tables[t]->lanes != tables[0]->lanes ||
-            tables[t]->block != tables[0]->block)
-            return fail(HTAB_E_ARG, "htab_batch_create: tables differ in kind, bucket_lanes or block_size");
+            tables[t]->block != tables[0]->block || tables[t]->rb != tables[0]->rb)
+            return fail(HTAB_E_ARG, "htab_batch_create: tables differ in kind, bucket_lanes, block_size or probe_batch");
     }
     htab_batch* b = new (std::nothrow) htab_batch();
     if (!b) return fail(HTAB_E_OOM, "htab_batch_create: host allocation");
     b->kind = tables[0]->kind;
     b->lanes = tables[0]->lanes;
     b->block = tables[0]->block;
+    b->rb = tables[0]->rb;
     b->ntables = ntables;
     int dev = 0, cus = 0;
     if (hipGetDevice(&dev) != hipSuccess ||

@@ -42,7 +42,7 @@ struct htab
 {
     int kind = 0;
     int lanes = 8;
-    int rb = 8;                   // elements a lane group probes together (sub-batch)
+    int rb = 0;                   // elements a lane group probes together; 0 = per-operation default
     int block = 256;
     int64_t capacity = 0;
     htab_view view{};
@@ -59,6 +59,7 @@ struct htab_batch
     int kind = 0;
     int lanes = 8;
     int block = 256;                   // workgroup size = chunk size of the persistent kernel
+    int rb = 0;                        // probe batch of the tables (0 = default)
     int32_t ntables = 0;
     htab** handles = nullptr;          // host copies of the table handles (per-table baseline)
     void* tables = nullptr;            // device array of table_ref<K, M>

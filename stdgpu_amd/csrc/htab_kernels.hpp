@@ -4,8 +4,8 @@
 // groups of G lanes; a group probes one bucket (16*G bytes) with a single 16-byte load per
 // lane, so one vector-memory instruction probes 64/G buckets.  A group works through the G
 // elements owned by its own lanes:
-//   1. first probes: the home buckets of all of the group's elements are loaded back to
-//      back (up to 8 loads in flight per lane), so a wave keeps 64 bucket lines in flight;
+//   1. first probes: the home buckets of R of the group's elements (R = probe batch, a tuning
+//      knob) are loaded back to back, so a wave keeps 64*R/G bucket lines in flight;
 //   2. each element is resolved against its home bucket with __ballot votes: a key match,
 //      an EMPTY slot (end of the probe sequence) or, for insert, a CAS on the first free
 //      slot elected with ctz over the ballot mask;
@@ -180,23 +180,19 @@ block_exclusive_scan(int64_t v, int64_t* s_tmp, int64_t* total)
 }
 
 // ================================================================== find
-// Rounds: every pending element of the group's sub-batch loads its current bucket (all loads of
-// a round are in flight together), then each is resolved by ballot: a key match (hit), an EMPTY
-// slot (miss) or neither (continue in the next bucket).  Elements still pending after
-// kBatchRounds buckets finish on the serial path.
-constexpr int kBatchRounds = 8;
-
 template <typename K, typename M, int G, int R>
 __device__ __forceinline__ void
 find_body(const table_ref<K, M>& t, const K* __restrict__ keys, const int64_t i, const bool valid,
           uint8_t* __restrict__ hits, void* __restrict__ vals_out)
 {
     using C = chunk<K, M>;
+    using value_type = typename C::value_type;
+    constexpr int SPL = C::SPL;
     const int lane = lane_id();
     const int gbase = lane & ~(G - 1);
     const int li = lane & (G - 1);
     const uint64_t gmask = ((G == 64) ? ~0ULL : ((1ULL << G) - 1)) << gbase;
-    const uint32_t W = static_cast<uint32_t>(G * C::SPL);
+    const uint32_t W = static_cast<uint32_t>(G * SPL);
 
     const K mykey = valid ? keys[i] : empty_key<K>();
     bool myhit = false;
@@ -210,81 +206,68 @@ find_body(const table_ref<K, M>& t, const K* __restrict__ keys, const int64_t i,
             if constexpr (slot_traits<K, M>::is_map) myval = static_cast<uint64_t>(t.slots[t.side_index(mykey)].second);
         }
     }
-    const uint32_t probe = (valid && !reserved) ? 1U : 0U;
-    uint32_t slow = 0; // group-uniform: elements left to the serial path
+    const bool probe = valid && !reserved;
+    uint32_t pend = 0; // group-uniform: elements not resolved by their home bucket
 
 #pragma unroll
     for (int rb = 0; rb < G; rb += R)
     {
+        C q[R];
         K kr[R];
-        uint64_t b[R];
-        uint32_t pend = 0;
+        bool act[R];
 #pragma unroll
         for (int r = 0; r < R; ++r)
         {
             kr[r] = shfl(mykey, gbase + rb + r);
-            b[r] = home_bucket(static_cast<uint64_t>(kr[r]), t.nbuckets);
-            pend |= shfl32(probe, gbase + rb + r) << r;
-        }
-        for (int round = 0;; ++round)
-        {
-            if (ballot(pend != 0) == 0) break;
-            if (round == kBatchRounds)
+            act[r] = shfl32(probe ? 1U : 0U, gbase + rb + r) != 0;
+            if (act[r])
             {
-                slow |= pend << rb;
-                break;
+                const uint64_t b = home_bucket(static_cast<uint64_t>(kr[r]), t.nbuckets);
+                q[r] = C::load_plain(t.slots + b * W, li);
             }
-            C q[R];
+        }
 #pragma unroll
-            for (int r = 0; r < R; ++r)
-                if (pend & (1U << r)) q[r] = C::load_plain(t.slots + b[r] * W, li);
-#pragma unroll
-            for (int r = 0; r < R; ++r)
+        for (int r = 0; r < R; ++r)
+        {
+            if (act[r])
             {
-                if (pend & (1U << r))
+                const K k = kr[r];
+                const uint32_t m = q[r].mask([k](K x) { return x == k; });
+                const uint32_t e = q[r].mask([](K x) { return x == empty_key<K>(); });
+                const uint64_t bm = ballot(m != 0) & gmask;
+                const uint64_t be = ballot(e != 0) & gmask;
+                if (bm)
                 {
-                    const K k = kr[r];
-                    const uint32_t m = q[r].mask([k](K x) { return x == k; });
-                    const uint32_t e = q[r].mask([](K x) { return x == empty_key<K>(); });
-                    const uint64_t bm = ballot(m != 0) & gmask;
-                    const uint64_t be = ballot(e != 0) & gmask;
-                    if (bm)
+                    const int src = ctz64(bm);
+                    const uint64_t cand = m ? q[r].mapped(__builtin_ctz(m)) : 0;
+                    const uint64_t v = shfl64(cand, src);
+                    if (li == rb + r)
                     {
-                        const uint64_t cand = m ? q[r].mapped(__builtin_ctz(m)) : 0;
-                        const uint64_t v = shfl64(cand, ctz64(bm));
-                        if (li == rb + r)
-                        {
-                            myhit = true;
-                            myval = v;
-                        }
-                        pend &= ~(1U << r);
+                        myhit = true;
+                        myval = v;
                     }
-                    else if (be)
-                    {
-                        pend &= ~(1U << r);
-                    }
-                    else
-                    {
-                        b[r] = next_bucket(b[r], t.nbuckets);
-                    }
+                }
+                else if (!be)
+                {
+                    pend |= 1U << (rb + r);
                 }
             }
         }
     }
 
-    // serial path: one element of the group at a time, from its home bucket, with wrap detection
+    // overflowed elements: walk the probe sequence one group element at a time
     int cur = -1;
     K ck = 0;
     uint64_t b = 0, steps = 0;
     while (true)
     {
-        if (cur < 0 && slow)
+        if (cur < 0 && pend)
         {
-            cur = __builtin_ctz(slow);
-            slow &= slow - 1;
+            cur = __builtin_ctz(pend);
+            pend &= pend - 1;
             ck = shfl(mykey, gbase + cur);
-            b = home_bucket(static_cast<uint64_t>(ck), t.nbuckets);
-            steps = 0;
+            b = next_bucket(home_bucket(static_cast<uint64_t>(ck), t.nbuckets), t.nbuckets);
+            steps = 1;
         }
         const bool active = cur >= 0;
         if (ballot(active) == 0) break;
@@ -298,8 +281,9 @@ find_body(const table_ref<K, M>& t, const K* __restrict__ keys, const int64_t i,
             const uint64_t be = ballot(e != 0) & gmask;
             if (bm)
             {
+                const int src = ctz64(bm);
                 const uint64_t cand = m ? q.mapped(__builtin_ctz(m)) : 0;
-                const uint64_t v = shfl64(cand, ctz64(bm));
+                const uint64_t v = shfl64(cand, src);
                 if (li == cur)
                 {
                     myhit = true;
@@ -356,20 +340,13 @@ fetch_element(const void* values, const void* keys, const void* vals, int64_t i,
 
 constexpr uint64_t kNoSlot = ~0ULL;
 
-// Insert-only launch.  Each pending element scans its probe sequence up to the first bucket
-// with an EMPTY slot (a key found on the way means "already present") while remembering the
-// first free slot (EMPTY or TOMB) it passed; it then claims that slot with one CAS.  All CASes
-// of a round are issued before any result is consumed.  A lost CAS (another key took the slot)
-// sends the element to the serial path, which rescans and resumes after the lost slot.
-// Uniqueness: free -> key is the only transition in an insert-only launch, so every key
-// inserted here sits at the first free slot of its probe sequence as of its CAS, and a racing
-// inserter of the same key either sees it there or loses the CAS on that very slot.
 template <typename K, typename M, int G, int R>
 __device__ __forceinline__ void
 insert_body(const table_ref<K, M>& t, const void* __restrict__ values, const void* __restrict__ keys,
             const void* __restrict__ vals, const int64_t i, const bool valid, uint8_t* __restrict__ inserted)
 {
     using C = chunk<K, M>;
+    using value_type = typename C::value_type;
     constexpr int SPL = C::SPL;
     __shared__ int64_t s_wave[16];
     __shared__ int64_t s_old;
@@ -400,115 +377,85 @@ insert_body(const table_ref<K, M>& t, const void* __restrict__ values, const voi
             myins = true;
         }
     }
-    const uint32_t probe = (valid && !reserved && !at_limit) ? 1U : 0U;
-    uint32_t slow = 0;
+    const bool probe = valid && !reserved && !at_limit;
+    uint32_t pend = 0;
 
 #pragma unroll
     for (int rb = 0; rb < G; rb += R)
     {
+        C q[R];
         K kr[R];
-        uint64_t b[R], ff[R];
-        uint32_t pend = 0, ff_tomb = 0;
+        bool act[R];
+        uint64_t hb[R];
 #pragma unroll
         for (int r = 0; r < R; ++r)
         {
             kr[r] = shfl(mykey, gbase + rb + r);
-            b[r] = home_bucket(static_cast<uint64_t>(kr[r]), t.nbuckets);
-            ff[r] = kNoSlot;
-            pend |= shfl32(probe, gbase + rb + r) << r;
+            act[r] = shfl32(probe ? 1U : 0U, gbase + rb + r) != 0;
+            hb[r] = 0;
+            if (act[r])
+            {
+                hb[r] = home_bucket(static_cast<uint64_t>(kr[r]), t.nbuckets);
+                q[r] = C::load_keys(t.slots + hb[r] * W, li);
+            }
         }
-        for (int round = 0;; ++round)
+#pragma unroll
+        for (int r = 0; r < R; ++r)
         {
-            if (ballot(pend != 0) == 0) break;
-            if (round == kBatchRounds)
+            if (act[r])
             {
-                slow |= pend << rb;
-                break;
-            }
-            C q[R];
-#pragma unroll
-            for (int r = 0; r < R; ++r)
-                if (pend & (1U << r)) q[r] = C::load_keys(t.slots + b[r] * W, li);
-            uint32_t claim = 0; // elements whose probe sequence ended this round
-#pragma unroll
-            for (int r = 0; r < R; ++r)
-            {
-                if (pend & (1U << r))
+                const K k = kr[r];
+                const uint32_t m = q[r].mask([k](K x) { return x == k; });
+                const uint32_t e = q[r].mask([](K x) { return x == empty_key<K>(); });
+                const uint32_t f = q[r].mask([](K x) { return x == empty_key<K>() || x == tomb_key<K>(); });
+                const uint64_t bm = ballot(m != 0) & gmask;
+                const uint64_t be = ballot(e != 0) & gmask;
+                const uint64_t bf = ballot(f != 0) & gmask;
+                if (bm)
                 {
-                    const K k = kr[r];
-                    const uint32_t m = q[r].mask([k](K x) { return x == k; });
-                    const uint32_t e = q[r].mask([](K x) { return x == empty_key<K>(); });
-                    const uint32_t f = q[r].mask([](K x) { return x == empty_key<K>() || x == tomb_key<K>(); });
-                    const uint64_t bm = ballot(m != 0) & gmask;
-                    const uint64_t be = ballot(e != 0) & gmask;
-                    const uint64_t bf = ballot(f != 0) & gmask;
-                    if (bm)
-                    {
-                        pend &= ~(1U << r); // present: insert never overwrites
-                    }
-                    else
-                    {
-                        if (ff[r] == kNoSlot && bf)
-                        {
-                            const int fl = ctz64(bf);
-                            uint64_t cand = 0;
-                            uint32_t tomb = 0;
-                            if (lane == fl)
-                            {
-                                const int j = __builtin_ctz(f);
-                                cand = b[r] * W + static_cast<uint64_t>(li * SPL + j);
-                                tomb = (q[r].key(j) == tomb_key<K>()) ? 1U : 0U;
-                            }
-                            ff[r] = shfl64(cand, fl);
-                            ff_tomb |= shfl32(tomb, fl) << r;
-                        }
-                        if (be)
-                            claim |= 1U << r;
-                        else
-                            b[r] = next_bucket(b[r], t.nbuckets);
-                    }
+                    // key present: nothing to do (insert never overwrites)
                 }
-            }
-            // claims: the group's first lane issues every CAS of the round back to back
-            K got[R];
-#pragma unroll
-            for (int r = 0; r < R; ++r)
-            {
-                got[r] = 0;
-                if ((claim & (1U << r)) && li == 0)
+                else if (be)
                 {
-                    const K expect = (ff_tomb & (1U << r)) ? tomb_key<K>() : empty_key<K>();
-                    got[r] = cas_agent(t.key_ptr(ff[r]), expect, kr[r]);
-                }
-            }
-#pragma unroll
-            for (int r = 0; r < R; ++r)
-            {
-                if (claim & (1U << r))
-                {
-                    const K seen = shfl(got[r], gbase);
-                    const K expect = (ff_tomb & (1U << r)) ? tomb_key<K>() : empty_key<K>();
-                    pend &= ~(1U << r);
-                    if (seen == expect)
+                    // probe sequence ends in the home bucket: claim its first free slot
+                    const int fl = ctz64(bf); // lane holding the first free slot
+                    // the value travels from the owner lane; the claiming lane stores it
+                    const uint64_t v = shfl64(myv, gbase + rb + r);
+                    uint64_t seen = 0;
+                    uint64_t slot = 0;
+                    if (lane == fl)
                     {
-                        const uint64_t v = shfl64(myv, gbase + rb + r);
-                        if (li == 0) store_mapped<K, M>(t.slots + ff[r], v);
+                        const int j = __builtin_ctz(f);
+                        const K obs = q[r].key(j);
+                        slot = hb[r] * W + static_cast<uint64_t>(li * SPL + j);
+                        const K got = cas_agent(t.key_ptr(slot), obs, k);
+                        seen = (got == obs) ? 1 : ((got == k) ? 2 : 0);
+                        if (seen == 1) store_mapped<K, M>(t.slots + slot, v);
+                    }
+                    seen = shfl64(seen, fl);
+                    slot = shfl64(slot, fl);
+                    if (seen == 1)
+                    {
                         if (li == rb + r)
                         {
                             myins = true;
-                            myslot = ff[r];
+                            myslot = slot;
                         }
                     }
-                    else if (seen != kr[r])
+                    else if (seen == 0)
                     {
-                        slow |= 1U << (rb + r); // lost the slot to another key
+                        pend |= 1U << (rb + r);
                     }
+                }
+                else
+                {
+                    pend |= 1U << (rb + r);
                 }
             }
         }
     }
 
-    // serial path: full probe sequence with first-free tracking, CAS, and resume after a lost CAS
+    // slow path: full probe sequence with first-free tracking, CAS, and resume after a lost CAS
     int cur = -1;
     K ck = 0;
     uint64_t cv = 0;
@@ -517,10 +464,10 @@ insert_body(const table_ref<K, M>& t, const void* __restrict__ values, const voi
     uint32_t jmin = 0;
     while (true)
     {
-        if (cur < 0 && slow)
+        if (cur < 0 && pend)
         {
-            cur = __builtin_ctz(slow);
-            slow &= slow - 1;
+            cur = __builtin_ctz(pend);
+            pend &= pend - 1;
             ck = shfl(mykey, gbase + cur);
             cv = shfl64(myv, gbase + cur);
             hb = home_bucket(static_cast<uint64_t>(ck), t.nbuckets);
@@ -642,8 +589,7 @@ insert_body(const table_ref<K, M>& t, const void* __restrict__ values, const voi
 // ================================================================== erase
 // Erase-only launch: a slot is reset to EMPTY when its bucket already holds an EMPTY slot (no
 // probe sequence runs through such a bucket, and none can start to while only erases run),
-// otherwise to TOMB so probe sequences through the bucket stay intact.  Exactly one of several
-// erasers of one key wins the CAS.
+// otherwise to TOMB so probe sequences through the bucket stay intact.
 template <typename K, typename M, int G, int R>
 __device__ __forceinline__ void
 erase_body(const table_ref<K, M>& t, const K* __restrict__ keys, const int64_t i, const bool valid,
@@ -668,84 +614,54 @@ erase_body(const table_ref<K, M>& t, const K* __restrict__ keys, const int64_t i
         const uint32_t old = __hip_atomic_fetch_and(t.flags, ~bit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         mydel = (old & bit) != 0;
     }
-    const uint32_t probe = (valid && !reserved) ? 1U : 0U;
-    uint32_t slow = 0;
+    const bool probe = valid && !reserved;
+    uint32_t pend = 0;
 
 #pragma unroll
     for (int rb = 0; rb < G; rb += R)
     {
+        C q[R];
         K kr[R];
-        uint64_t b[R];
-        uint32_t pend = 0;
+        bool act[R];
+        uint64_t hb[R];
 #pragma unroll
         for (int r = 0; r < R; ++r)
         {
             kr[r] = shfl(mykey, gbase + rb + r);
-            b[r] = home_bucket(static_cast<uint64_t>(kr[r]), t.nbuckets);
-            pend |= shfl32(probe, gbase + rb + r) << r;
+            act[r] = shfl32(probe ? 1U : 0U, gbase + rb + r) != 0;
+            hb[r] = 0;
+            if (act[r])
+            {
+                hb[r] = home_bucket(static_cast<uint64_t>(kr[r]), t.nbuckets);
+                q[r] = C::load_keys(t.slots + hb[r] * W, li);
+            }
         }
-        for (int round = 0;; ++round)
+#pragma unroll
+        for (int r = 0; r < R; ++r)
         {
-            if (ballot(pend != 0) == 0) break;
-            if (round == kBatchRounds)
+            if (act[r])
             {
-                slow |= pend << rb;
-                break;
-            }
-            C q[R];
-#pragma unroll
-            for (int r = 0; r < R; ++r)
-                if (pend & (1U << r)) q[r] = C::load_keys(t.slots + b[r] * W, li);
-            uint32_t hit = 0, to_empty = 0;
-            uint64_t tgt[R];
-#pragma unroll
-            for (int r = 0; r < R; ++r)
-            {
-                tgt[r] = 0;
-                if (pend & (1U << r))
+                const K k = kr[r];
+                const uint32_t m = q[r].mask([k](K x) { return x == k; });
+                const uint32_t e = q[r].mask([](K x) { return x == empty_key<K>(); });
+                const uint64_t bm = ballot(m != 0) & gmask;
+                const uint64_t be = ballot(e != 0) & gmask;
+                if (bm)
                 {
-                    const K k = kr[r];
-                    const uint32_t m = q[r].mask([k](K x) { return x == k; });
-                    const uint32_t e = q[r].mask([](K x) { return x == empty_key<K>(); });
-                    const uint64_t bm = ballot(m != 0) & gmask;
-                    const uint64_t be = ballot(e != 0) & gmask;
-                    if (bm)
+                    const int ml = ctz64(bm);
+                    uint64_t ok = 0;
+                    if (lane == ml)
                     {
-                        const int ml = ctz64(bm);
-                        const uint64_t cand = m ? b[r] * W + static_cast<uint64_t>(li * SPL + __builtin_ctz(m)) : 0;
-                        tgt[r] = shfl64(cand, ml);
-                        hit |= 1U << r;
-                        if (be) to_empty |= 1U << r;
-                        pend &= ~(1U << r);
+                        const uint64_t slot = hb[r] * W + static_cast<uint64_t>(li * SPL + __builtin_ctz(m));
+                        const K repl = be ? empty_key<K>() : tomb_key<K>();
+                        ok = (cas_agent(t.key_ptr(slot), k, repl) == k) ? 1 : 0;
                     }
-                    else if (be)
-                    {
-                        pend &= ~(1U << r); // absent
-                    }
-                    else
-                    {
-                        b[r] = next_bucket(b[r], t.nbuckets);
-                    }
+                    ok = shfl64(ok, ml);
+                    if (ok && li == rb + r) mydel = true;
                 }
-            }
-            K got[R];
-#pragma unroll
-            for (int r = 0; r < R; ++r)
-            {
-                got[r] = 0;
-                if ((hit & (1U << r)) && li == 0)
+                else if (!be)
                 {
-                    const K repl = (to_empty & (1U << r)) ? empty_key<K>() : tomb_key<K>();
-                    got[r] = cas_agent(t.key_ptr(tgt[r]), kr[r], repl);
-                }
-            }
-#pragma unroll
-            for (int r = 0; r < R; ++r)
-            {
-                if (hit & (1U << r))
-                {
-                    const K seen = shfl(got[r], gbase);
-                    if (seen == kr[r] && li == rb + r) mydel = true;
+                    pend |= 1U << (rb + r);
                 }
             }
         }
@@ -756,13 +672,13 @@ erase_body(const table_ref<K, M>& t, const K* __restrict__ keys, const int64_t i
     uint64_t b = 0, steps = 0;
     while (true)
     {
-        if (cur < 0 && slow)
+        if (cur < 0 && pend)
         {
-            cur = __builtin_ctz(slow);
-            slow &= slow - 1;
+            cur = __builtin_ctz(pend);
+            pend &= pend - 1;
             ck = shfl(mykey, gbase + cur);
-            b = home_bucket(static_cast<uint64_t>(ck), t.nbuckets);
-            steps = 0;
+            b = next_bucket(home_bucket(static_cast<uint64_t>(ck), t.nbuckets), t.nbuckets);
+            steps = 1;
         }
         const bool active = cur >= 0;
         if (ballot(active) == 0) break;
@@ -803,7 +719,6 @@ erase_body(const table_ref<K, M>& t, const K* __restrict__ keys, const int64_t i
     block_count_add(t.stripes, t.nstripes, mydel ? -1 : 0, s_wave, &s_old, &wave_prefix);
     if (valid && erased) erased[i] = mydel ? 1 : 0;
 }
-
 
 // ================================================================== plain bulk kernels
 template <typename K, typename M, int G, int R>

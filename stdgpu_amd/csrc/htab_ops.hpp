@@ -64,13 +64,32 @@ struct ops
 template <typename K, typename M, int G>
 struct batch_ops
 {
-    template <int B>
-    static int launch_b(htab_batch* b, int32_t nsegs, hipStream_t s)
+    template <int R, int B>
+    static int launch_rb(htab_batch* b, int32_t nsegs, hipStream_t s)
     {
-        hipLaunchKernelGGL((k_batch<K, M, G, default_rb(G), B>), dim3(b->grid), dim3(B), 0, s,
+        hipLaunchKernelGGL((k_batch<K, M, G, R, B>), dim3(b->grid), dim3(B), 0, s,
                            static_cast<const table_ref<K, M>*>(b->tables), b->segs, nsegs, b->chunk_start,
                            b->counter);
         return launch_status("batch launch");
+    }
+
+    template <int B>
+    static int launch_b(htab_batch* b, int32_t nsegs, hipStream_t s)
+    {
+        // every probe batch for the benchmarked kind; the default one elsewhere
+        if constexpr (batch_all_blocks<K, M>)
+        {
+            if constexpr (G >= 8)
+            {
+                if (b->rb == 8) return launch_rb<8, B>(b, nsegs, s);
+            }
+            if constexpr (G >= 4)
+            {
+                if (b->rb == 4) return launch_rb<4, B>(b, nsegs, s);
+            }
+            if (b->rb == 2) return launch_rb<2, B>(b, nsegs, s);
+        }
+        return launch_rb<default_rb(G), B>(b, nsegs, s);
     }
 
     static int build(htab_batch* b, htab* const* tables)
@@ -219,7 +238,10 @@ dispatch_g(htab_call& c)
         case call_op::find:
         case call_op::erase:
         {
-            const int rb = c.h->rb;
+            // probe batch: the table's setting, or per operation by default (find keeps 8 home
+            // buckets per lane group in flight; insert and erase 4, which measured faster on gfx950)
+            int rb = c.h->rb;
+            if (rb == 0) rb = (c.op == call_op::find) ? default_rb(G) : (G < 4 ? G : 4);
             auto go = [&](auto o) -> int {
                 using O = decltype(o);
                 if (c.op == call_op::insert) return O::insert(c.h, c.values, c.keys, c.vals, c.n, c.flags, c.s);
