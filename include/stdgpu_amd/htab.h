@@ -103,6 +103,11 @@ int htab_find(htab* table, const void* keys, int64_t n, uint8_t* hits, void* val
  * erased[i] (optional) = 1 iff keys[i] was removed by this call (device erase() == 1). */
 int htab_erase(htab* table, const void* keys, int64_t n, uint8_t* erased, void* stream);
 
+/* Overwrites the mapped value of every present key: what user code does in the reference with
+ * find(key)->second = value (unordered_map.cuh:288-317) over a range.  Maps only; each key at most
+ * once per call.  found[i] (optional) = 1 iff keys[i] was present. */
+int htab_assign(htab* table, const void* keys, const void* vals, int64_t n, uint8_t* found, void* stream);
+
 /* size() (unordered_base_detail.cuh:1110-1132): synchronises the stream, then reads the count. */
 int htab_size(htab* table, int64_t* out, void* stream);
 

@@ -802,6 +802,29 @@ orc_deque_pop_front(void* d, std::uint32_t* out, std::int64_t n, std::uint8_t* o
     }
 }
 
+// The voxel caller pattern (a user loop in the reference's applications): per point, pop a slot
+// from the free list, insert (key, slot), and push the slot back when the key was already present.
+// Sequential, so the result is deterministic.  Returns the number of new keys.
+std::int64_t
+orc_voxel_insert(void* t, void* d, const std::uint64_t* keys, std::int64_t n)
+{
+    auto* tab = dynamic_cast<table<std::uint64_t, std::uint32_t>*>(static_cast<table_base*>(t));
+    u32_deque* q = static_cast<u32_deque*>(d);
+    if (!tab) return -1;
+    std::int64_t added = 0;
+    for (std::int64_t i = 0; i < n; ++i)
+    {
+        unsigned int slot = 0;
+        if (!q->pop_front(&slot)) continue;
+        kv<std::uint64_t, std::uint32_t> v{keys[i], slot};
+        if (tab->insert_one(v))
+            ++added;
+        else
+            q->push_back(slot);
+    }
+    return added;
+}
+
 // contents of the deque in front-to-back order; returns the count
 std::int64_t
 orc_deque_contents(void* d, std::uint32_t* out)

@@ -49,6 +49,7 @@ def load():
         "orc_deque_push_back": (None, [P, P, I64, P, I32]),
         "orc_deque_pop_front": (None, [P, P, I64, P, I32]),
         "orc_deque_contents": (I64, [P, P]),
+        "orc_voxel_insert": (I64, [P, P, P, I64]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -146,6 +147,12 @@ class Table:
 
     def bucket(self, key: int) -> int:
         return load().orc_bucket(self._t, ctypes.c_uint64(key & 0xFFFFFFFFFFFFFFFF))
+
+
+def voxel_insert(table: "Table", deque: "Deque", keys) -> int:
+    """The reference caller pattern: per key pop a slot, insert (key, slot), push back if present."""
+    k = np.ascontiguousarray(keys, dtype=np.uint64)
+    return load().orc_voxel_insert(table._t, deque._d, _p(k), len(k))
 
 
 class Deque:

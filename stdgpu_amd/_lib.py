@@ -100,6 +100,7 @@ SIGNATURES = {
     "htab_insert_kv": (_I32, [_P, _P, _P, _I64, _P, _P]),
     "htab_find": (_I32, [_P, _P, _I64, _P, _P, _P]),
     "htab_erase": (_I32, [_P, _P, _I64, _P, _P]),
+    "htab_assign": (_I32, [_P, _P, _P, _I64, _P, _P]),
     "htab_size": (_I32, [_P, ctypes.POINTER(_I64), _P]),
     "htab_clear": (_I32, [_P, _P]),
     "htab_range": (_I32, [_P, _P, _I64, ctypes.POINTER(_I64), _P]),

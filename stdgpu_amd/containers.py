@@ -149,6 +149,13 @@ class _unordered_base:
         )
         return erased
 
+    def assign(self, keys, values, found=None, stream=None):
+        """find(key)->second = value for every present key (maps; each key once per call)."""
+        self._check_keys(keys)
+        check(_lib.load().htab_assign(self._h, _ptr(keys), _ptr(values), keys.numel(), _ptr(found),
+                                      _stream_ptr(stream)), "htab_assign")
+        return found
+
     def clear(self, stream=None):
         check(_lib.load().htab_clear(self._h, _stream_ptr(stream)), "htab_clear")
 

@@ -304,6 +304,25 @@ htab_erase(htab* h, const void* keys, int64_t n, uint8_t* erased, void* stream)
 }
 
 int
+htab_assign(htab* h, const void* keys, const void* vals, int64_t n, uint8_t* found, void* stream)
+{
+    if (!h) return fail(HTAB_E_ARG, "htab_assign: null handle");
+    if (n < 0) return fail(HTAB_E_ARG, "htab_assign: negative count");
+    if (n == 0) return HTAB_OK;
+    if (!keys || !vals) return fail(HTAB_E_ARG, "htab_assign: null keys or values");
+    if ((n + 255) / 256 > 0x7FFFFFFF) return fail(HTAB_E_LIMIT, "htab_assign: too many elements");
+    htab_call c;
+    c.op = call_op::assign;
+    c.h = h;
+    c.keys = keys;
+    c.vals = vals;
+    c.n = n;
+    c.flags = found;
+    c.s = static_cast<hipStream_t>(stream);
+    return dispatch(h->kind, c);
+}
+
+int
 htab_size(htab* h, int64_t* out, void* stream)
 {
     if (!h || !out) return fail(HTAB_E_ARG, "htab_size: null argument");

@@ -85,6 +85,7 @@ enum class call_op
     insert,
     find,
     erase,
+    assign,
     clear,
     range,
     valid,

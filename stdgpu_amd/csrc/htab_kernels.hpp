@@ -889,6 +889,22 @@ k_batch(const table_ref<K, M>* __restrict__ tables, const batch_segment* __restr
     }
 }
 
+// ================================================================== assign
+// find(key)->second = value for every present key (one thread per element, native find); a key
+// may appear once per launch.  found[i] (optional) = 1 iff keys[i] was present.
+template <typename K, typename M>
+__global__ __launch_bounds__(256) void
+k_assign(table_ref<K, M> t, const K* __restrict__ keys, const M* __restrict__ vals, int64_t n,
+         uint8_t* __restrict__ found)
+{
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    typename slot_traits<K, M>::value_type* p = t.find(keys[i]);
+    const bool hit = p != t.end_ptr();
+    if (hit) p->second = vals[i];
+    if (found) found[i] = hit ? 1 : 0;
+}
+
 // ================================================================== clear / fill
 template <typename K, typename M>
 __global__ __launch_bounds__(256) void

@@ -270,6 +270,18 @@ dispatch_kind(htab_call& c)
 {
     switch (c.op)
     {
+        case call_op::assign:
+            if constexpr (std::is_void_v<M>)
+            {
+                return fail(HTAB_E_ARG, "htab_assign: sets have no mapped values");
+            }
+            else
+            {
+                hipLaunchKernelGGL((k_assign<K, M>), dim3(static_cast<uint32_t>((c.n + 255) / 256)), dim3(256), 0,
+                                   c.s, table_ref<K, M>::from_view(c.h->view), static_cast<const K*>(c.keys),
+                                   static_cast<const M*>(c.vals), c.n, c.flags);
+                return launch_status("assign launch");
+            }
         case call_op::clear: return table_ops<K, M>::clear(c.h, c.s);
         case call_op::range: return table_ops<K, M>::range(c.h, c.idx_out, const_cast<void*>(c.keys), c.vals_out, c.cap,
                                                            c.n_out, c.s);

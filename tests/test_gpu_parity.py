@@ -372,3 +372,35 @@ def test_deque_free_list_multiset():
     vals = out2.cpu().numpy().view(np.uint32)[u8(ok2) == 1]
     assert sorted(vals.tolist()) == list(range(cap))
     q.destroy()
+
+
+def test_against_committed_golden_fixtures():
+    """the GPU path against the committed fixture vectors (tests/golden/small_workloads.npz)"""
+    import os
+
+    g = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "small_workloads.npz"))
+    s = sa.unordered_set_u32.createDeviceObject(int(g["set_capacity"][0]))
+    ins = torch.empty(len(g["set_keys"]), dtype=torch.uint8, device="cuda")
+    s.insert(dev(g["set_keys"], "u32"), inserted=ins)
+    np.testing.assert_array_equal(u8(ins), g["set_ins"])
+    np.testing.assert_array_equal(u8(s.find(dev(g["set_queries"], "u32"))[0]), g["set_hits1"])
+    er = torch.empty(len(g["set_erase"]), dtype=torch.uint8, device="cuda")
+    s.erase(dev(g["set_erase"], "u32"), erased=er)
+    np.testing.assert_array_equal(u8(er), g["set_erased"])
+    np.testing.assert_array_equal(u8(s.find(dev(g["set_queries"], "u32"))[0]), g["set_hits2"])
+    k, _ = s.contents()
+    np.testing.assert_array_equal(np.sort(host(k, "u32")), g["set_contents"])
+    m = sa.unordered_map_u64_u32.createDeviceObject(int(g["map_capacity"][0]))
+    ins = torch.empty(len(g["map_keys"]), dtype=torch.uint8, device="cuda")
+    m.insert(dev(g["map_keys"], "u64"), dev(g["map_vals"], "u32"), inserted=ins)
+    np.testing.assert_array_equal(u8(ins), g["map_ins"])
+    h, v = m.find(dev(g["map_queries"], "u64"))
+    np.testing.assert_array_equal(u8(h), g["map_hits"])
+    np.testing.assert_array_equal(np.where(u8(h) == 1, host(v, "u32"), 0), g["map_found_vals"])
+    er = torch.empty(len(g["map_erase"]), dtype=torch.uint8, device="cuda")
+    m.erase(dev(g["map_erase"], "u64"), erased=er)
+    np.testing.assert_array_equal(u8(er), g["map_erased"])
+    k, v = m.contents()
+    o = np.argsort(host(k, "u64"))
+    np.testing.assert_array_equal(host(k, "u64")[o], g["map_contents_keys"])
+    np.testing.assert_array_equal(host(v, "u32")[o], g["map_contents_vals"])

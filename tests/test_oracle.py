@@ -248,3 +248,34 @@ def test_deque_multiset_conservation():
     got, ok = q.pop_front(cap)
     assert ok.sum() == cap - 10
     assert sorted(np.concatenate([got[ok == 1], out2]).tolist()) == list(range(cap))
+
+
+# ---------------------------------------------------------------------------------------------
+# committed golden fixtures (tests/golden/make_golden.py)
+import json  # noqa: E402
+import os  # noqa: E402
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def test_golden_reference_kats_match_oracle():
+    kats = json.load(open(os.path.join(GOLDEN, "reference_kats.json")))
+    for name, k in kats.items():
+        kind = ("u32", None) if name.startswith("set") else ("u64", "u32")
+        t = po.Table(kind[0], kind[1], k["capacity"])
+        assert t.bucket_count() == k["bucket_count"] and t.max_size() == k["max_size"]
+        for key, b in k["bucket"].items():
+            assert t.bucket(int(key)) == b
+
+
+def test_golden_small_workloads_regenerate_identically():
+    import importlib.util
+
+    spec = importlib.util.spec_from_file_location("make_golden", os.path.join(GOLDEN, "make_golden.py"))
+    mg = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mg)
+    fresh = mg.small_workloads()
+    stored = np.load(os.path.join(GOLDEN, "small_workloads.npz"))
+    assert sorted(stored.files) == sorted(fresh)
+    for k in stored.files:
+        np.testing.assert_array_equal(stored[k], fresh[k], err_msg=k)
