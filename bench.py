@@ -24,6 +24,11 @@ HBM_PEAK_GBS = 8000.0
 # algorithmic bytes per operation (u64 -> u32 map): streamed input + streamed output + one
 # 64 B random sector per bucket probe (+ the 16 B slot write for insert)
 BYTES_PER_OP = {"insert": 16 + 64 + 16 + 1, "find": 8 + 1 + 4 + 64, "erase": 8 + 64 + 8 + 1, "reinsert": 16 + 64 + 16 + 1}
+# HBM bytes per launch measured with rocprofv3 on MI355X for the default workload (2^26 keys,
+# 8-slot buckets, 256-thread workgroups): FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000 KiB, the scales
+# that make both counters read a 2 GiB copy kernel's bytes exactly (profiles/map_u64_u32_2e26_bulk_kernels.md)
+MEASURED_TRAFFIC = {(26, 8, 256): {"insert": (2 * 5537617 + 6397355) * 1024, "find": (2 * 5549580 + 336876) * 1024,
+                                   "erase": (2 * 1270539 + 1067007) * 1024, "reinsert": (2 * 3560311 + 1580010) * 1024}}
 
 
 def parse():
@@ -366,7 +371,8 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": MEASURED_TRAFFIC.get((args.log2n, args.lanes, args.block), {}).get(dom)
+            if args.probe_batch == 0 else None,
             "bytes_per_op": BYTES_PER_OP[dom],
             "avg_launch_ms": round(dom_ms, 4),
         },
