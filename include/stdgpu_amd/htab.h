@@ -77,6 +77,7 @@ typedef struct htab_view
     uint32_t bucket_lanes;
     int64_t ref_bucket_count;  /* reference bucket_count(), for bucket() */
     int64_t ref_max_size;      /* reference max_size() */
+    uint32_t* ovf;             /* per bucket: stored keys whose probe sequence runs through it */
 } htab_view;
 
 /* unordered_map::createDeviceObject / unordered_set::createDeviceObject
@@ -189,11 +190,13 @@ int htab_graph_destroy(htab_graph* graph);
 
 /* ------------------------------------------------------------------------------------------
  * Served form: a request loop over resident tables with continuous batching (no reference
- * counterpart).  Requests are htab_segment records (table index, op, n, device buffers).  While
- * requests arrive, a scheduler coalesces the same-operation prefix of every idle table's queue into
- * one persistent launch per free stream slot; a table is in one launch at a time, so per table the
- * requests take effect in arrival order, and launches of different tables overlap.  An insert that
- * might not fit the table's free slots is refused (status HTAB_E_LIMIT, no effect).
+ * counterpart).  Requests are htab_segment records (table index, op, n <= 2^32-1, device buffers).
+ * While requests arrive, a scheduler keeps up to two launches per stream in flight; each launch
+ * takes the longest same-operation run at the head of every table's queue.  A launch waits on the
+ * device for the previous launch of each of its tables, so per table the requests take effect in
+ * arrival order while launches run back to back.  An insert that might not fit the table's free
+ * slots waits for the table's launches in flight and is refused (status HTAB_E_LIMIT, no effect)
+ * when it still cannot be guaranteed to fit.
  * ------------------------------------------------------------------------------------------ */
 typedef struct htab_server htab_server;
 
@@ -205,7 +208,9 @@ typedef struct htab_server_stats
     int64_t wall_ns;
 } htab_server_stats;
 
-/* nstreams launch slots (1..16); max_inflight_elems bounds the elements in flight over all slots. */
+/* nstreams streams (1..16) with two launches in flight each; max_inflight_elems bounds the elements
+ * in flight over all of them (a launch takes at most max_inflight_elems / (2 * nstreams) elements,
+ * and at least one request).  ntables <= 32767. */
 int htab_server_create(htab* const* tables, int32_t ntables, int32_t nstreams, int64_t max_inflight_elems,
                        htab_server** out);
 int htab_server_destroy(htab_server* server);

@@ -11,7 +11,12 @@
 //     = TOMB (erased).  Those two key values are still storable: they live in two side
 //     slots after the buckets, with a presence bit each in `flags`.
 //   * Collisions overflow into the next bucket (linear probing over buckets, wrapping).
-//     A probe sequence ends at the first bucket that holds an EMPTY slot.
+//   * ovf[b] (one uint32 per bucket) counts the stored keys whose probe sequence runs through
+//     bucket b to a later bucket (home bucket <= b < stored bucket, cyclically).  A probe
+//     sequence ends at the first bucket that holds the key, holds an EMPTY slot, or has
+//     ovf == 0.  A bucket with ovf > 0 never holds an EMPTY slot (an erase there leaves TOMB),
+//     so the EMPTY test needs no extra load, and TOMBs never lengthen a probe sequence through
+//     buckets that no key overflows.
 //   * Home bucket = (hi32(mix64(key)) * nbuckets) >> 32, mix64 = MurmurHash3's fmix64.
 #ifndef STDGPU_AMD_HTAB_DEVICE_HPP
 #define STDGPU_AMD_HTAB_DEVICE_HPP
@@ -148,6 +153,7 @@ struct table_ref
     value_type* slots;
     int64_t* stripes;
     uint32_t* flags;
+    uint32_t* ovf; // per bucket: stored keys whose probe sequence runs through it
     uint64_t nbuckets;
     uint64_t limit;
     uint32_t nstripes;
@@ -159,6 +165,7 @@ struct table_ref
         t.slots = static_cast<value_type*>(v.slots);
         t.stripes = v.stripes;
         t.flags = v.flags;
+        t.ovf = v.ovf;
         t.nbuckets = v.nbuckets;
         t.limit = v.limit;
         t.nstripes = v.nstripes;
@@ -174,6 +181,22 @@ struct table_ref
     __device__ K* key_ptr(uint64_t slot) const { return reinterpret_cast<K*>(slots + slot); }
 
     __device__ int64_t size() const { return stripe_sum(stripes, nstripes); }
+
+    __device__ uint32_t overflow(uint64_t b) const { return load_agent(ovf + b); }
+
+    // adds d to ovf of the `count` buckets starting at h (cyclic)
+    __device__ void overflow_add(uint64_t h, uint64_t count, int32_t d) const
+    {
+        uint64_t b = h;
+        for (uint64_t c = 0; c < count; ++c)
+        {
+            __hip_atomic_fetch_add(ovf + b, static_cast<uint32_t>(d), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            b = next_bucket(b, nbuckets);
+        }
+    }
+
+    // buckets from h up to (not including) f, cyclically
+    __device__ uint64_t distance(uint64_t h, uint64_t f) const { return f >= h ? f - h : f + nbuckets - h; }
 
     // ---- native per-thread operations (user kernels; the facade's find/insert/erase) ----
 
@@ -195,7 +218,7 @@ struct table_ref
                 if (k == key) return base + j;
                 saw_empty |= (k == empty_key<K>());
             }
-            if (saw_empty) break;
+            if (saw_empty || overflow(b) == 0) break;
             b = next_bucket(b, nbuckets);
         }
         return end_ptr();
@@ -217,6 +240,9 @@ struct table_ref
 
     // insert (unordered_base_detail.cuh:708-809, 967-994): never overwrites; claims only EMPTY
     // slots, so it stays duplicate-free when it runs concurrently with erase() in one kernel.
+    // Every bucket it walks past without an EMPTY slot gets its ovf raised before the key can
+    // appear further on (and lowered again if the key is not inserted), so a concurrent find()
+    // that sees the key's slot also sees the way there.
     // Returns {slot pointer, true} on insertion, {end_ptr(), false} otherwise.
     template <typename WriteValue>
     __device__ value_type* insert(K key, const WriteValue& write_value, bool* inserted) const
@@ -238,16 +264,19 @@ struct table_ref
             *inserted = true;
             return p;
         }
-        uint64_t b = home_bucket(static_cast<uint64_t>(key), nbuckets);
-        uint32_t j0 = 0;
-        for (uint64_t step = 0; step <= nbuckets;)
+        const uint64_t h = home_bucket(static_cast<uint64_t>(key), nbuckets);
+        uint64_t b = h;
+        for (uint64_t step = 0; step < nbuckets; ++step)
         {
-            bool advanced = true;
-            for (uint32_t j = j0; j < W; ++j)
+            for (uint32_t j = 0; j < W; ++j)
             {
                 const uint64_t s = b * W + j;
                 K k = load_agent(key_ptr(s));
-                if (k == key) return end_ptr();
+                if (k == key)
+                {
+                    overflow_add(h, step, -1);
+                    return end_ptr();
+                }
                 if (k == empty_key<K>())
                 {
                     const K seen = cas_agent(key_ptr(s), empty_key<K>(), key);
@@ -256,23 +285,26 @@ struct table_ref
                         if (!count_add(1))
                         {
                             __hip_atomic_store(key_ptr(s), tomb_key<K>(), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                            overflow_add(h, step, -1);
                             return end_ptr();
                         }
                         write_value(slots + s);
                         *inserted = true;
                         return slots + s;
                     }
-                    if (seen == key) return end_ptr();
+                    if (seen == key)
+                    {
+                        overflow_add(h, step, -1);
+                        return end_ptr();
+                    }
                     // lost the slot to another key: keep scanning from the next slot
                 }
             }
-            if (advanced)
-            {
-                b = next_bucket(b, nbuckets);
-                j0 = 0;
-                ++step;
-            }
+            // no EMPTY slot left here: the key's probe sequence runs through this bucket
+            __hip_atomic_fetch_add(ovf + b, 1U, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            b = next_bucket(b, nbuckets);
         }
+        overflow_add(h, nbuckets, -1);
         return end_ptr();
     }
 
@@ -288,7 +320,8 @@ struct table_ref
             count_add(-1);
             return true;
         }
-        uint64_t b = home_bucket(static_cast<uint64_t>(key), nbuckets);
+        const uint64_t h = home_bucket(static_cast<uint64_t>(key), nbuckets);
+        uint64_t b = h;
         for (uint64_t step = 0; step < nbuckets; ++step)
         {
             bool saw_empty = false;
@@ -300,11 +333,12 @@ struct table_ref
                 {
                     if (cas_agent(key_ptr(s), key, tomb_key<K>()) != key) return false;
                     count_add(-1);
+                    overflow_add(h, step, -1);
                     return true;
                 }
                 saw_empty |= (k == empty_key<K>());
             }
-            if (saw_empty) break;
+            if (saw_empty || overflow(b) == 0) break;
             b = next_bucket(b, nbuckets);
         }
         return false;

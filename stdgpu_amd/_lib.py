@@ -85,6 +85,7 @@ class htab_view(ctypes.Structure):
         ("bucket_lanes", ctypes.c_uint32),
         ("ref_bucket_count", ctypes.c_int64),
         ("ref_max_size", ctypes.c_int64),
+        ("ovf", ctypes.c_void_p),
     ]
 
 

@@ -160,7 +160,9 @@ htab_create(int kind, int64_t capacity, const htab_tuning* tuning, htab** out)
     const int64_t limit = ref_max < nslots ? ref_max : nslots;
 
     const size_t area = static_cast<size_t>((nslots + 2) * vb);
-    h->bytes = kCtlBytes + ((area + 255) / 256) * 256;
+    const size_t area_padded = ((area + 255) / 256) * 256;
+    const size_t ovf_bytes = ((static_cast<size_t>(nbuckets) * sizeof(uint32_t) + 255) / 256) * 256;
+    h->bytes = kCtlBytes + area_padded + ovf_bytes;
     hipError_t e = hipMalloc(&h->mem, h->bytes);
     if (e != hipSuccess)
     {
@@ -171,6 +173,7 @@ htab_create(int kind, int64_t capacity, const htab_tuning* tuning, htab** out)
     h->view.slots = base + kCtlBytes;
     h->view.stripes = reinterpret_cast<int64_t*>(base);
     h->view.flags = reinterpret_cast<uint32_t*>(base + kFlagsOffset);
+    h->view.ovf = reinterpret_cast<uint32_t*>(base + kCtlBytes + area_padded);
     h->view.nbuckets = static_cast<uint64_t>(nbuckets);
     h->view.limit = static_cast<uint64_t>(limit);
     h->view.nstripes = (limit < nslots) ? 1U : kMaxStripes;

@@ -91,7 +91,9 @@ enum class call_op
     valid,
     batch_build,
     batch_run_offsets,
-    batch_run_segments
+    batch_run_segments,
+    serve,
+    serve_sizes
 };
 
 // One request to a kind dispatcher; fields unused by an operation stay zero.
@@ -115,6 +117,9 @@ struct htab_call
     int bop = 0;
     const int64_t* offsets = nullptr;
     int32_t nsegs = 0;
+    const htab_dev::serve_args* serve = nullptr;       // served launch: segments and chunk plan
+    const htab_dev::serve_tables* serve_tabs = nullptr; // served launch: its tables (size readback)
+    int64_t* sizes_out = nullptr;                      // served launch: host-mapped table sizes
     hipStream_t s = nullptr;
 };
 

@@ -255,10 +255,12 @@ find_body(const table_ref<K, M>& t, const K* __restrict__ keys, const int64_t i,
         }
     }
 
-    // overflowed elements: walk the probe sequence one group element at a time
+    // overflowed elements: walk the probe sequence one group element at a time.  Bucket b is
+    // loaded together with ovf of the bucket before it (p): the walk goes on only while p has
+    // keys running through it, so both loads are in flight at once.
     int cur = -1;
     K ck = 0;
-    uint64_t b = 0, steps = 0;
+    uint64_t b = 0, p = 0, steps = 0;
     while (true)
     {
         if (cur < 0 && pend)
@@ -266,7 +268,8 @@ find_body(const table_ref<K, M>& t, const K* __restrict__ keys, const int64_t i,
             cur = __builtin_ctz(pend);
             pend &= pend - 1;
             ck = shfl(mykey, gbase + cur);
-            b = next_bucket(home_bucket(static_cast<uint64_t>(ck), t.nbuckets), t.nbuckets);
+            p = home_bucket(static_cast<uint64_t>(ck), t.nbuckets);
+            b = next_bucket(p, t.nbuckets);
             steps = 1;
         }
         const bool active = cur >= 0;
@@ -274,12 +277,17 @@ find_body(const table_ref<K, M>& t, const K* __restrict__ keys, const int64_t i,
         if (active)
         {
             const C q = C::load_plain(t.slots + b * W, li);
+            const uint32_t ov = shfl32(li == 0 ? t.ovf[p] : 0U, gbase);
             const K k = ck;
             const uint32_t m = q.mask([k](K x) { return x == k; });
             const uint32_t e = q.mask([](K x) { return x == empty_key<K>(); });
             const uint64_t bm = ballot(m != 0) & gmask;
             const uint64_t be = ballot(e != 0) & gmask;
-            if (bm)
+            if (ov == 0)
+            {
+                cur = -1; // no key runs past p: absent
+            }
+            else if (bm)
             {
                 const int src = ctz64(bm);
                 const uint64_t cand = m ? q.mapped(__builtin_ctz(m)) : 0;
@@ -297,6 +305,7 @@ find_body(const table_ref<K, M>& t, const K* __restrict__ keys, const int64_t i,
             }
             else
             {
+                p = b;
                 b = next_bucket(b, t.nbuckets);
             }
         }
@@ -455,13 +464,17 @@ insert_body(const table_ref<K, M>& t, const void* __restrict__ values, const voi
         }
     }
 
-    // slow path: full probe sequence with first-free tracking, CAS, and resume after a lost CAS
+    // slow path: the probe sequence with first-free tracking, CAS, and resume after a lost CAS.
+    // The key is absent once a bucket without it holds an EMPTY slot or has ovf == 0; it then
+    // takes the first free slot (EMPTY or TOMB) seen, or the first one further on.  A key stored
+    // past its home bucket raises ovf of every bucket it runs through.
     int cur = -1;
     K ck = 0;
     uint64_t cv = 0;
     uint64_t b = 0, hb = 0, walked = 0, ff = kNoSlot;
     K ffobs = 0;
     uint32_t jmin = 0;
+    bool absent = false;
     while (true)
     {
         if (cur < 0 && pend)
@@ -475,12 +488,14 @@ insert_body(const table_ref<K, M>& t, const void* __restrict__ values, const voi
             walked = 0;
             ff = kNoSlot;
             jmin = 0;
+            absent = false;
         }
         const bool active = cur >= 0;
         if (ballot(active) == 0) break;
         if (active)
         {
             const C q = C::load_keys(t.slots + b * W, li);
+            const uint32_t ov = shfl32(li == 0 ? t.overflow(b) : 0U, gbase);
             const K k = ck;
             const int jloc = static_cast<int>(jmin) - li * SPL; // first slot of this chunk still in play
             const uint32_t m = q.mask([k](K x) { return x == k; }, jloc);
@@ -508,9 +523,11 @@ insert_body(const table_ref<K, M>& t, const void* __restrict__ values, const voi
                     ff = shfl64(cand, fl);
                     ffobs = static_cast<K>(shfl64(obs, fl));
                 }
-                if (be)
+                // no key runs past a bucket with ovf == 0, and a bucket holding an EMPTY slot has ovf == 0
+                if (be || ov == 0) absent = true;
+                if (absent && ff != kNoSlot)
                 {
-                    // end of the probe sequence: claim ff (the group's first lane issues the CAS)
+                    // claim ff (the group's first lane issues the CAS)
                     uint64_t res = 0;
                     if (li == 0)
                     {
@@ -521,6 +538,15 @@ insert_body(const table_ref<K, M>& t, const void* __restrict__ values, const voi
                     res = shfl64(res, gbase);
                     if (res == 1)
                     {
+                        // the key runs through every bucket from its home bucket to ff's
+                        const uint64_t fb = ff / W;
+                        const uint64_t dist = t.distance(hb, fb);
+                        for (uint64_t d = static_cast<uint64_t>(li); d < dist; d += G)
+                        {
+                            uint64_t x = hb + d;
+                            if (x >= t.nbuckets) x -= t.nbuckets;
+                            __hip_atomic_fetch_add(t.ovf + x, 1U, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        }
                         if (li == cur)
                         {
                             myins = true;
@@ -575,10 +601,16 @@ insert_body(const table_ref<K, M>& t, const void* __restrict__ values, const voi
         if (myins && rank >= static_cast<int64_t>(t.limit))
         {
             if (myslot >= t.nslots())
+            {
                 __hip_atomic_fetch_and(t.flags, ~table_ref<K, M>::side_bit(mykey), __ATOMIC_RELAXED,
                                        __HIP_MEMORY_SCOPE_AGENT);
+            }
             else
+            {
                 __hip_atomic_store(t.key_ptr(myslot), tomb_key<K>(), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                const uint64_t h = home_bucket(static_cast<uint64_t>(mykey), t.nbuckets);
+                t.overflow_add(h, t.distance(h, myslot / W), -1);
+            }
             __hip_atomic_fetch_add(t.stripes, static_cast<int64_t>(-1), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             myins = false;
         }
@@ -589,7 +621,8 @@ insert_body(const table_ref<K, M>& t, const void* __restrict__ values, const voi
 // ================================================================== erase
 // Erase-only launch: a slot is reset to EMPTY when its bucket already holds an EMPTY slot (no
 // probe sequence runs through such a bucket, and none can start to while only erases run),
-// otherwise to TOMB so probe sequences through the bucket stay intact.
+// otherwise to TOMB so probe sequences through the bucket stay intact.  An erased key that was
+// stored past its home bucket lowers ovf of the buckets it ran through.
 template <typename K, typename M, int G, int R>
 __device__ __forceinline__ void
 erase_body(const table_ref<K, M>& t, const K* __restrict__ keys, const int64_t i, const bool valid,
@@ -667,9 +700,11 @@ erase_body(const table_ref<K, M>& t, const K* __restrict__ keys, const int64_t i
         }
     }
 
+    // overflowed elements: as in find, bucket b is loaded with ovf of the bucket before it; an
+    // erased key stops running through the buckets between its home bucket and its own
     int cur = -1;
     K ck = 0;
-    uint64_t b = 0, steps = 0;
+    uint64_t b = 0, p = 0, hb = 0, steps = 0;
     while (true)
     {
         if (cur < 0 && pend)
@@ -677,7 +712,9 @@ erase_body(const table_ref<K, M>& t, const K* __restrict__ keys, const int64_t i
             cur = __builtin_ctz(pend);
             pend &= pend - 1;
             ck = shfl(mykey, gbase + cur);
-            b = next_bucket(home_bucket(static_cast<uint64_t>(ck), t.nbuckets), t.nbuckets);
+            hb = home_bucket(static_cast<uint64_t>(ck), t.nbuckets);
+            p = hb;
+            b = next_bucket(hb, t.nbuckets);
             steps = 1;
         }
         const bool active = cur >= 0;
@@ -685,12 +722,17 @@ erase_body(const table_ref<K, M>& t, const K* __restrict__ keys, const int64_t i
         if (active)
         {
             const C q = C::load_keys(t.slots + b * W, li);
+            const uint32_t ov = shfl32(li == 0 ? t.overflow(p) : 0U, gbase);
             const K k = ck;
             const uint32_t m = q.mask([k](K x) { return x == k; });
             const uint32_t e = q.mask([](K x) { return x == empty_key<K>(); });
             const uint64_t bm = ballot(m != 0) & gmask;
             const uint64_t be = ballot(e != 0) & gmask;
-            if (bm)
+            if (ov == 0)
+            {
+                cur = -1; // no key runs past p: absent
+            }
+            else if (bm)
             {
                 const int ml = ctz64(bm);
                 uint64_t ok = 0;
@@ -701,7 +743,18 @@ erase_body(const table_ref<K, M>& t, const K* __restrict__ keys, const int64_t i
                     ok = (cas_agent(t.key_ptr(slot), k, repl) == k) ? 1 : 0;
                 }
                 ok = shfl64(ok, ml);
-                if (ok && li == cur) mydel = true;
+                if (ok)
+                {
+                    const uint64_t dist = t.distance(hb, b);
+                    for (uint64_t d = static_cast<uint64_t>(li); d < dist; d += G)
+                    {
+                        uint64_t x = hb + d;
+                        if (x >= t.nbuckets) x -= t.nbuckets;
+                        __hip_atomic_fetch_add(t.ovf + x, static_cast<uint32_t>(-1), __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_AGENT);
+                    }
+                    if (li == cur) mydel = true;
+                }
                 cur = -1;
             }
             else if (be || ++steps >= t.nbuckets)
@@ -710,6 +763,7 @@ erase_body(const table_ref<K, M>& t, const K* __restrict__ keys, const int64_t i
             }
             else
             {
+                p = b;
                 b = next_bucket(b, t.nbuckets);
             }
         }
@@ -889,6 +943,79 @@ k_batch(const table_ref<K, M>* __restrict__ tables, const batch_segment* __restr
     }
 }
 
+// ================================================================== served launch
+// One launch of the request server.  Its segments travel by value in the kernel arguments (no
+// upload, no planning kernel), one 256-thread workgroup takes one chunk of 256 elements and the
+// grid is exactly the launch's chunk count, so no work counter is needed.  k_serve_sizes, queued
+// right after on the same stream, stores the sizes of the launch's tables into host-mapped memory
+// for the scheduler's admission control (one workgroup; the kernel boundary orders it after every
+// size-stripe update of the launch).
+constexpr int kServeSegs = 96;
+constexpr int kServeBlock = 256;
+constexpr int kServeTables = 1024;
+
+struct serve_seg
+{
+    const void* keys;
+    void* vals; // insert: mapped values in (optional); find: mapped values out (optional)
+    uint8_t* flags;
+    uint32_t n;
+    int16_t table;
+    int8_t op;
+    int8_t pad;
+};
+
+struct serve_args
+{
+    int32_t nsegs;
+    uint32_t chunk_start[kServeSegs + 1]; // chunk_start[nsegs] = grid size
+    serve_seg segs[kServeSegs];
+};
+
+struct serve_tables
+{
+    int32_t n;
+    int16_t table[kServeTables];
+};
+
+template <typename K, typename M, int G, int RF, int RI>
+__global__ __launch_bounds__(kServeBlock) void
+k_serve(const table_ref<K, M>* __restrict__ tables, const serve_args a)
+{
+    const uint32_t c = blockIdx.x;
+    int lo = 0, hi = a.nsegs - 1; // last segment whose first chunk is <= c
+    while (lo < hi)
+    {
+        const int mid = (lo + hi + 1) >> 1;
+        if (a.chunk_start[mid] <= c)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    const serve_seg seg = a.segs[lo];
+    const table_ref<K, M> tr = tables[seg.table];
+    const int64_t i = static_cast<int64_t>(c - a.chunk_start[lo]) * kServeBlock + threadIdx.x;
+    const bool valid = i < static_cast<int64_t>(seg.n);
+    if (seg.op == kOpInsert)
+        insert_body<K, M, G, RI>(tr, nullptr, seg.keys, seg.vals, i, valid, seg.flags);
+    else if (seg.op == kOpFind)
+        find_body<K, M, G, RF>(tr, static_cast<const K*>(seg.keys), i, valid, seg.flags, seg.vals);
+    else
+        erase_body<K, M, G, RI>(tr, static_cast<const K*>(seg.keys), i, valid, seg.flags);
+}
+
+template <typename K, typename M>
+__global__ __launch_bounds__(256) void
+k_serve_sizes(const table_ref<K, M>* __restrict__ tables, const serve_tables a, int64_t* __restrict__ sizes_out)
+{
+    for (int s = static_cast<int>(threadIdx.x); s < a.n; s += static_cast<int>(blockDim.x))
+    {
+        const table_ref<K, M> t = tables[a.table[s]];
+        __hip_atomic_store(sizes_out + a.table[s], stripe_sum(t.stripes, t.nstripes), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_SYSTEM);
+    }
+}
+
 // ================================================================== assign
 // find(key)->second = value for every present key (one thread per element, native find); a key
 // may appear once per launch.  found[i] (optional) = 1 iff keys[i] was present.
@@ -915,6 +1042,8 @@ k_clear(table_ref<K, M> t, uint64_t nchunks)
     const uint4 ones = make_uint4(~0U, ~0U, ~0U, ~0U);
     const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
     for (uint64_t c = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; c < nchunks; c += stride) p[c] = ones;
+    for (uint64_t b = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; b < t.nbuckets; b += stride)
+        t.ovf[b] = 0;
     if (blockIdx.x == 0 && threadIdx.x < kMaxStripes) t.stripes[static_cast<uint64_t>(threadIdx.x) * kStripeStride] = 0;
     if (blockIdx.x == 0 && threadIdx.x == 0) *t.flags = 0;
 }
@@ -1040,13 +1169,16 @@ k_zero_u64(unsigned long long* p, int n)
 
 // ================================================================== valid()
 // Every stored key must be found at its own slot (reachable and unique); bad[0] counts violations,
-// bad[1] counts stored entries (compared with the size counter on the host).
+// bad[1] counts stored entries (compared with the size counter on the host).  Each stored key also
+// adds one to want[b] for every bucket b its probe sequence runs through; k_valid_overflow then
+// checks ovf against those counts, and that no bucket with ovf > 0 holds an EMPTY slot.
 template <typename K, typename M>
 __global__ __launch_bounds__(256) void
-k_valid(table_ref<K, M> t, uint64_t total_slots, unsigned long long* bad)
+k_valid(table_ref<K, M> t, uint64_t total_slots, unsigned long long* bad, uint32_t* want)
 {
     const uint32_t flags = *t.flags;
     const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+    const uint64_t ns = t.nslots();
     unsigned long long nbad = 0, nstored = 0;
     for (uint64_t s = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; s < total_slots; s += stride)
     {
@@ -1054,9 +1186,39 @@ k_valid(table_ref<K, M> t, uint64_t total_slots, unsigned long long* bad)
         ++nstored;
         const K k = slot_traits<K, M>::key(t.slots[s]);
         if (t.find(k) != t.slots + s) ++nbad;
+        if (s < ns)
+        {
+            const uint64_t h = home_bucket(static_cast<uint64_t>(k), t.nbuckets);
+            const uint64_t dist = t.distance(h, s / t.W);
+            uint64_t b = h;
+            for (uint64_t d = 0; d < dist; ++d)
+            {
+                atomicAdd(want + b, 1U);
+                b = next_bucket(b, t.nbuckets);
+            }
+        }
     }
     if (nbad) atomicAdd(bad, nbad);
     if (nstored) atomicAdd(bad + 1, nstored);
+}
+
+template <typename K, typename M>
+__global__ __launch_bounds__(256) void
+k_valid_overflow(table_ref<K, M> t, const uint32_t* __restrict__ want, unsigned long long* bad)
+{
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+    unsigned long long nbad = 0;
+    for (uint64_t b = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; b < t.nbuckets; b += stride)
+    {
+        const uint32_t ov = t.ovf[b];
+        if (ov != want[b]) ++nbad;
+        if (ov != 0)
+        {
+            for (uint32_t j = 0; j < t.W; ++j)
+                if (*t.key_ptr(b * t.W + j) == empty_key<K>()) ++nbad;
+        }
+    }
+    if (nbad) atomicAdd(bad, nbad);
 }
 
 } // namespace htab_dev

@@ -194,7 +194,8 @@ struct table_ops
     static int valid(htab* h, int32_t* out, hipStream_t s)
     {
         const T t = T::from_view(h->view);
-        const size_t need = 2 * sizeof(unsigned long long);
+        // scratch: two counters, then one expected ovf count per bucket
+        const size_t need = 2 * sizeof(unsigned long long) + t.nbuckets * sizeof(uint32_t);
         if (h->scratch_bytes < need)
         {
             if (h->scratch) (void)hipFree(h->scratch);
@@ -203,19 +204,23 @@ struct table_ops
             HTAB_HIP_TRY(hipMalloc(&h->scratch, need), "valid scratch");
             h->scratch_bytes = need;
         }
-        hipLaunchKernelGGL(k_zero_u64<>, dim3(1), dim3(64), 0, s, reinterpret_cast<unsigned long long*>(h->scratch), 2);
-        int rc = launch_status("valid zero");
-        if (rc) return rc;
+        unsigned long long* bad = reinterpret_cast<unsigned long long*>(h->scratch);
+        uint32_t* want = reinterpret_cast<uint32_t*>(bad + 2);
+        HTAB_HIP_TRY(hipMemsetAsync(h->scratch, 0, need, s), "valid scratch clear");
         const uint64_t total = t.nslots() + 2;
         uint64_t grid = (total + 255) / 256;
         if (grid > 16384) grid = 16384;
-        hipLaunchKernelGGL((k_valid<K, M>), dim3(static_cast<uint32_t>(grid)), dim3(256), 0, s, t, total,
-                           reinterpret_cast<unsigned long long*>(h->scratch));
-        rc = launch_status("valid launch");
+        hipLaunchKernelGGL((k_valid<K, M>), dim3(static_cast<uint32_t>(grid)), dim3(256), 0, s, t, total, bad, want);
+        int rc = launch_status("valid launch");
+        if (rc) return rc;
+        uint64_t bgrid = (t.nbuckets + 255) / 256;
+        if (bgrid > 16384) bgrid = 16384;
+        hipLaunchKernelGGL((k_valid_overflow<K, M>), dim3(static_cast<uint32_t>(bgrid)), dim3(256), 0, s, t, want, bad);
+        rc = launch_status("valid overflow launch");
         if (rc) return rc;
         unsigned long long res[2] = {0, 0};
         int64_t stripes[kMaxStripes * kStripeStride];
-        HTAB_HIP_TRY(hipMemcpyAsync(res, h->scratch, sizeof(res), hipMemcpyDeviceToHost, s), "valid copy");
+        HTAB_HIP_TRY(hipMemcpyAsync(res, bad, sizeof(res), hipMemcpyDeviceToHost, s), "valid copy");
         HTAB_HIP_TRY(hipMemcpyAsync(stripes, h->view.stripes, sizeof(stripes), hipMemcpyDeviceToHost, s), "valid copy");
         HTAB_HIP_TRY(hipStreamSynchronize(s), "valid sync");
         int64_t size = 0;
@@ -260,6 +265,14 @@ dispatch_g(htab_call& c)
         case call_op::batch_run_offsets:
             return batch_ops<K, M, G>::run(c.b, c.ops, c.bop, c.offsets, c.keys, c.vals, c.flags, c.vals_out, c.s);
         case call_op::batch_run_segments: return batch_ops<K, M, G>::run_segments(c.b, c.nsegs, c.s);
+        case call_op::serve:
+        {
+            // per-operation probe batches of the plain calls
+            const uint32_t grid = c.serve->chunk_start[c.serve->nsegs];
+            hipLaunchKernelGGL((k_serve<K, M, G, default_rb(G), (G < 4 ? G : 4)>), dim3(grid), dim3(kServeBlock), 0,
+                               c.s, static_cast<const table_ref<K, M>*>(c.b->tables), *c.serve);
+            return launch_status("served launch");
+        }
         default: return fail(HTAB_E_ARG, "bad call");
     }
 }
@@ -282,6 +295,10 @@ dispatch_kind(htab_call& c)
                                    static_cast<const M*>(c.vals), c.n, c.flags);
                 return launch_status("assign launch");
             }
+        case call_op::serve_sizes:
+            hipLaunchKernelGGL((k_serve_sizes<K, M>), dim3(1), dim3(256), 0, c.s,
+                               static_cast<const table_ref<K, M>*>(c.b->tables), *c.serve_tabs, c.sizes_out);
+            return launch_status("served sizes");
         case call_op::clear: return table_ops<K, M>::clear(c.h, c.s);
         case call_op::range: return table_ops<K, M>::range(c.h, c.idx_out, const_cast<void*>(c.keys), c.vals_out, c.cap,
                                                            c.n_out, c.s);

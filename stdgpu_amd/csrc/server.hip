@@ -1,15 +1,23 @@
 // Served form: a request loop over resident tables with continuous batching.
 //
 // Requests (table, op, n keys with their own device buffers) arrive open-loop.  A scheduler thread
-// keeps one FIFO per table and, whenever one of its stream slots is free, forms ONE persistent
-// launch (htab_batch_run_segments) from the longest same-operation prefix of every idle table's
-// FIFO, oldest head first, up to an element budget per launch.  A table is in at most one launch at
-// a time, so per table the requests take effect in arrival order (finds see exactly the inserts and
-// erases that arrived before them) while launches on different tables overlap on the device.
-// Admission: an insert is taken only if the table's known size plus in-flight inserts plus its n
-// fits the table's limit (otherwise it is refused with HTAB_E_LIMIT and has no effect); the element
-// budget bounds the work in flight.  After each launch a small kernel reads the sizes of the tables
-// it touched, which the scheduler uses for admission once the launch's event completes.
+// keeps one FIFO per table.  While fewer than `depth` launches are in flight it forms the next
+// launch from the longest same-operation run at the head of every table's FIFO (oldest head first,
+// up to an element budget) and enqueues it at once: launches are not held back until the tables
+// they touch are idle.  Per table the order is kept on the device instead: a launch goes onto the
+// stream with the fewest launches in flight and waits (hipStreamWaitEvent) for the last launch of
+// each of its tables that sits on another stream, so a table's requests take effect in arrival order
+// while launches of unrelated tables run side by side.  Finds therefore see exactly the inserts and
+// erases that arrived before them.
+//
+// A launch is one kernel per 96 segments (k_serve), whose segments travel in the kernel arguments,
+// then one single-workgroup kernel that stores the sizes of the launch's tables into host-mapped
+// memory, and an event record: no copies and no planning kernel.
+//
+// Admission: an insert is taken only if the table's known size plus the inserts in flight plus its
+// n fits the table's limit.  If it does not fit while launches on the table are in flight, the
+// table waits for them (its size is then exact); if it does not fit with nothing in flight it is
+// refused with HTAB_E_LIMIT and has no effect.
 //
 // No reference counterpart: the reference serves one blocking bulk call at a time.
 #include <hip/hip_runtime.h>
@@ -17,7 +25,6 @@
 #include <algorithm>
 #include <atomic>
 #include <chrono>
-#include <condition_variable>
 #include <deque>
 #include <mutex>
 #include <thread>
@@ -30,36 +37,23 @@ using namespace htab_host;
 namespace
 {
 
-template <int Unused = 0>
-__global__ void
-k_table_sizes(const int64_t* const* stripes, int32_t ntables, uint32_t nstripes_max, int64_t* out)
-{
-    const int t = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
-    if (t >= ntables) return;
-    int64_t s = 0;
-    for (uint32_t i = 0; i < nstripes_max; ++i) s += stripes[t][static_cast<uint64_t>(i) * htab_dev::kStripeStride];
-    out[t] = s;
-}
-
 int64_t
 now_ns(std::chrono::steady_clock::time_point t0)
 {
     return std::chrono::duration_cast<std::chrono::nanoseconds>(std::chrono::steady_clock::now() - t0).count();
 }
 
-struct slot
+struct launch_rec
 {
-    hipStream_t stream = nullptr;
+    int stream = -1;
     hipEvent_t done = nullptr;
-    htab_batch* batch = nullptr;
-    htab_segment* segs = nullptr; // pinned
-    int64_t* sizes = nullptr;     // pinned, ntables
-    int64_t* sizes_dev = nullptr;
-    std::vector<int64_t> reqs;     // request indices in flight
-    std::vector<int32_t> tables;   // tables in flight
-    int64_t elems = 0;
-    bool busy = false;
+    std::vector<int64_t> reqs;   // request indices
+    std::vector<int32_t> tables; // distinct tables
+    bool live = false;
 };
+
+constexpr int32_t kMaxLaunchSegs = 1024;
+constexpr int kLaunchesPerStream = 2;
 
 } // namespace
 
@@ -67,10 +61,12 @@ struct htab_server
 {
     std::vector<htab*> tables;
     std::vector<int64_t> limit;
-    std::vector<slot> slots;
-    const int64_t** stripes_dev = nullptr; // device array of each table's stripes pointer
+    std::vector<hipStream_t> streams;
+    int64_t* sizes_host = nullptr;         // pinned, host-mapped, one entry per table
+    int64_t* sizes_dev = nullptr;          // device address of sizes_host
+    htab_batch* batch = nullptr;           // device descriptors of the tables
+    std::vector<launch_rec> launches;      // depth records
     int64_t max_launch_elems = 0;
-    int32_t max_segs = 0;
 };
 
 extern "C" {
@@ -79,39 +75,38 @@ int
 htab_server_create(htab* const* tables, int32_t ntables, int32_t nstreams, int64_t max_inflight_elems,
                    htab_server** out)
 {
-    if (!out || !tables || ntables <= 0 || nstreams <= 0 || nstreams > 16 || max_inflight_elems <= 0)
+    if (!out || !tables || ntables <= 0 || ntables > 32767 || nstreams <= 0 || nstreams > 16 || max_inflight_elems <= 0)
         return fail(HTAB_E_ARG, "htab_server_create: bad argument");
     *out = nullptr;
     htab_server* s = new (std::nothrow) htab_server();
     if (!s) return fail(HTAB_E_OOM, "htab_server_create: host allocation");
-    s->tables.assign(tables, tables + ntables);
-    for (int32_t t = 0; t < ntables; ++t) s->limit.push_back(tables[t]->info.limit);
-    s->max_launch_elems = std::max<int64_t>(1, max_inflight_elems / nstreams);
-    s->max_segs = 4096;
-    s->slots.resize(static_cast<size_t>(nstreams));
     auto cleanup = [&](int rc) {
         htab_server_destroy(s);
         return rc;
     };
-    std::vector<const int64_t*> sp(static_cast<size_t>(ntables));
-    for (int32_t t = 0; t < ntables; ++t) sp[static_cast<size_t>(t)] = tables[t]->view.stripes;
-    if (hipMalloc(&s->stripes_dev, sizeof(int64_t*) * static_cast<size_t>(ntables)) != hipSuccess)
-        return cleanup(fail(HTAB_E_OOM, "htab_server_create: device allocation"));
-    if (hipMemcpy(s->stripes_dev, sp.data(), sizeof(int64_t*) * sp.size(), hipMemcpyHostToDevice) != hipSuccess)
-        return cleanup(fail(HTAB_E_HIP, "htab_server_create: copy"));
-    for (auto& sl : s->slots)
+    s->tables.assign(tables, tables + ntables);
+    for (int32_t t = 0; t < ntables; ++t)
     {
-        if (hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking) != hipSuccess ||
-            hipEventCreateWithFlags(&sl.done, hipEventDisableTiming) != hipSuccess ||
-            hipHostMalloc(reinterpret_cast<void**>(&sl.segs), sizeof(htab_segment) * static_cast<size_t>(s->max_segs),
-                          hipHostMallocDefault) != hipSuccess ||
-            hipHostMalloc(reinterpret_cast<void**>(&sl.sizes), sizeof(int64_t) * static_cast<size_t>(ntables),
-                          hipHostMallocDefault) != hipSuccess ||
-            hipMalloc(&sl.sizes_dev, sizeof(int64_t) * static_cast<size_t>(ntables)) != hipSuccess)
-            return cleanup(fail(HTAB_E_OOM, "htab_server_create: stream slot"));
-        const int rc = htab_batch_create(tables, ntables, &sl.batch);
-        if (rc != HTAB_OK) return cleanup(rc);
+        if (!tables[t]) return cleanup(fail(HTAB_E_ARG, "htab_server_create: null table"));
+        s->limit.push_back(tables[t]->info.limit);
     }
+    const int depth = kLaunchesPerStream * nstreams;
+    s->max_launch_elems = std::max<int64_t>(1, max_inflight_elems / depth);
+    int rc = htab_batch_create(tables, ntables, &s->batch);
+    if (rc != HTAB_OK) return cleanup(rc);
+    if (hipHostMalloc(reinterpret_cast<void**>(&s->sizes_host), sizeof(int64_t) * static_cast<size_t>(ntables),
+                      hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess)
+        return cleanup(fail(HTAB_E_OOM, "htab_server_create: pinned allocation"));
+    if (hipHostGetDevicePointer(reinterpret_cast<void**>(&s->sizes_dev), s->sizes_host, 0) != hipSuccess)
+        return cleanup(fail(HTAB_E_HIP, "htab_server_create: mapped size array"));
+    s->streams.assign(static_cast<size_t>(nstreams), nullptr);
+    for (auto& st : s->streams)
+        if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess)
+            return cleanup(fail(HTAB_E_HIP, "htab_server_create: stream"));
+    s->launches.resize(static_cast<size_t>(depth));
+    for (auto& l : s->launches)
+        if (hipEventCreateWithFlags(&l.done, hipEventDisableTiming) != hipSuccess)
+            return cleanup(fail(HTAB_E_HIP, "htab_server_create: event"));
     *out = s;
     return HTAB_OK;
 }
@@ -120,17 +115,14 @@ int
 htab_server_destroy(htab_server* s)
 {
     if (!s) return fail(HTAB_E_ARG, "htab_server_destroy: null handle");
-    for (auto& sl : s->slots)
-    {
-        if (sl.stream) (void)hipStreamSynchronize(sl.stream);
-        if (sl.batch) htab_batch_destroy(sl.batch);
-        if (sl.segs) (void)hipHostFree(sl.segs);
-        if (sl.sizes) (void)hipHostFree(sl.sizes);
-        if (sl.sizes_dev) (void)hipFree(sl.sizes_dev);
-        if (sl.done) (void)hipEventDestroy(sl.done);
-        if (sl.stream) (void)hipStreamDestroy(sl.stream);
-    }
-    if (s->stripes_dev) (void)hipFree(s->stripes_dev);
+    for (auto st : s->streams)
+        if (st) (void)hipStreamSynchronize(st);
+    for (auto& l : s->launches)
+        if (l.done) (void)hipEventDestroy(l.done);
+    for (auto st : s->streams)
+        if (st) (void)hipStreamDestroy(st);
+    if (s->batch) htab_batch_destroy(s->batch);
+    if (s->sizes_host) (void)hipHostFree(s->sizes_host);
     delete s;
     return HTAB_OK;
 }
@@ -145,7 +137,7 @@ htab_server_run(htab_server* s, const htab_segment* reqs, const int64_t* arrival
     for (int64_t i = 0; i < nreq; ++i)
     {
         if (reqs[i].table < 0 || reqs[i].table >= ntables || reqs[i].op < HTAB_OP_INSERT || reqs[i].op > HTAB_OP_ERASE ||
-            reqs[i].n < 0 || (reqs[i].n > 0 && !reqs[i].keys))
+            reqs[i].n < 0 || reqs[i].n > 0xFFFFFFFFLL || (reqs[i].n > 0 && !reqs[i].keys))
             return fail(HTAB_E_ARG, "htab_server_run: bad request");
         status[i] = 1; // pending
     }
@@ -155,13 +147,16 @@ htab_server_run(htab_server* s, const htab_segment* reqs, const int64_t* arrival
     {
         const int rc = htab_size(s->tables[static_cast<size_t>(t)], &known[static_cast<size_t>(t)], nullptr);
         if (rc != HTAB_OK) return rc;
+        s->sizes_host[t] = known[static_cast<size_t>(t)];
     }
+    std::vector<int32_t> table_launches(static_cast<size_t>(ntables), 0); // launches in flight per table
+    std::vector<int32_t> last_launch(static_cast<size_t>(ntables), -1);  // latest launch per table
+    std::vector<int32_t> stream_launches(s->streams.size(), 0);
     std::vector<std::deque<int64_t>> fifo(static_cast<size_t>(ntables));
-    std::vector<char> busy(static_cast<size_t>(ntables), 0);
     std::mutex mu;
-    std::atomic<int64_t> submitted{0};
-    std::atomic<int> error{HTAB_OK};
-    int64_t launches = 0, segments = 0, max_inflight = 0;
+    int error = HTAB_OK;
+    int64_t launches = 0, segments = 0, max_inflight = 0, live = 0;
+    const int32_t depth = static_cast<int32_t>(s->launches.size());
     const auto t0 = std::chrono::steady_clock::now();
 
     // submitter: open-loop arrivals at their scheduled times
@@ -172,124 +167,193 @@ htab_server_run(htab_server* s, const htab_segment* reqs, const int64_t* arrival
             std::lock_guard<std::mutex> lk(mu);
             submit_ns[i] = now_ns(t0);
             fifo[static_cast<size_t>(reqs[i].table)].push_back(i);
-            submitted.store(i + 1, std::memory_order_release);
         }
     });
 
     int64_t finished = 0;
     std::vector<int32_t> order;
-    while (finished < nreq && error.load() == HTAB_OK)
+    std::vector<htab_segment> segs;
+    std::vector<int32_t> waits;
+    htab_dev::serve_args args{};
+    htab_dev::serve_tables tabs{};
+    while (finished < nreq && error == HTAB_OK)
     {
         bool progressed = false;
         // 1. retire completed launches
-        for (auto& sl : s->slots)
+        for (int32_t li = 0; li < depth && error == HTAB_OK; ++li)
         {
-            if (!sl.busy) continue;
-            const hipError_t q = hipEventQuery(sl.done);
+            launch_rec& l = s->launches[static_cast<size_t>(li)];
+            if (!l.live) continue;
+            const hipError_t q = hipEventQuery(l.done);
             if (q == hipErrorNotReady) continue;
             if (q != hipSuccess)
             {
-                error.store(fail_hip(q, "htab_server_run: launch"));
+                error = fail_hip(q, "htab_server_run: launch");
                 break;
             }
             const int64_t t = now_ns(t0);
             std::lock_guard<std::mutex> lk(mu);
-            for (int64_t r : sl.reqs)
+            for (int64_t r : l.reqs)
             {
                 done_ns[r] = t;
                 status[r] = HTAB_OK;
                 if (reqs[r].op == HTAB_OP_INSERT) inflight_ins[static_cast<size_t>(reqs[r].table)] -= reqs[r].n;
             }
-            for (int32_t tb : sl.tables)
+            for (int32_t tb : l.tables)
             {
-                known[static_cast<size_t>(tb)] = sl.sizes[tb];
-                busy[static_cast<size_t>(tb)] = 0;
+                // the latest completed launch on this table wrote its size
+                known[static_cast<size_t>(tb)] = __atomic_load_n(s->sizes_host + tb, __ATOMIC_ACQUIRE);
+                --table_launches[static_cast<size_t>(tb)];
+                if (last_launch[static_cast<size_t>(tb)] == li) last_launch[static_cast<size_t>(tb)] = -1;
             }
-            finished += static_cast<int64_t>(sl.reqs.size());
-            sl.reqs.clear();
-            sl.tables.clear();
-            sl.busy = false;
+            finished += static_cast<int64_t>(l.reqs.size());
+            --stream_launches[static_cast<size_t>(l.stream)];
+            l.reqs.clear();
+            l.tables.clear();
+            l.live = false;
+            --live;
             progressed = true;
         }
-        // 2. form launches on free slots
-        for (auto& sl : s->slots)
+        // 2. form launches while the pipeline has room
+        while (live < depth && error == HTAB_OK)
         {
-            if (sl.busy) continue;
-            std::lock_guard<std::mutex> lk(mu);
-            order.clear();
-            for (int32_t t = 0; t < ntables; ++t)
-                if (!busy[static_cast<size_t>(t)] && !fifo[static_cast<size_t>(t)].empty()) order.push_back(t);
-            if (order.empty()) break;
-            std::sort(order.begin(), order.end(), [&](int32_t a, int32_t b) {
-                return submit_ns[fifo[static_cast<size_t>(a)].front()] < submit_ns[fifo[static_cast<size_t>(b)].front()];
-            });
-            int32_t nseg = 0;
+            int32_t li = 0;
+            while (s->launches[static_cast<size_t>(li)].live) ++li;
+            launch_rec& l = s->launches[static_cast<size_t>(li)];
+            int32_t sid = 0;
+            for (int32_t k = 1; k < static_cast<int32_t>(s->streams.size()); ++k)
+                if (stream_launches[static_cast<size_t>(k)] < stream_launches[static_cast<size_t>(sid)]) sid = k;
+            segs.clear();
             int64_t elems = 0;
-            for (int32_t t : order)
             {
-                auto& q = fifo[static_cast<size_t>(t)];
-                const int op = reqs[q.front()].op;
-                bool took = false;
-                while (!q.empty() && reqs[q.front()].op == op && nseg < s->max_segs)
+                std::lock_guard<std::mutex> lk(mu);
+                order.clear();
+                for (int32_t t = 0; t < ntables; ++t)
+                    if (!fifo[static_cast<size_t>(t)].empty()) order.push_back(t);
+                std::sort(order.begin(), order.end(), [&](int32_t a, int32_t b) {
+                    return fifo[static_cast<size_t>(a)].front() < fifo[static_cast<size_t>(b)].front();
+                });
+                for (int32_t t : order)
                 {
-                    const int64_t r = q.front();
-                    const int64_t n = reqs[r].n;
-                    if (took && elems + n > s->max_launch_elems) break;
-                    if (op == HTAB_OP_INSERT &&
-                        known[static_cast<size_t>(t)] + inflight_ins[static_cast<size_t>(t)] + n > s->limit[static_cast<size_t>(t)])
+                    auto& q = fifo[static_cast<size_t>(t)];
+                    const int op = reqs[q.front()].op;
+                    bool took = false;
+                    while (!q.empty() && reqs[q.front()].op == op && static_cast<int32_t>(segs.size()) < kMaxLaunchSegs)
                     {
-                        // cannot be guaranteed to fit: refused, no effect on the table
+                        const int64_t r = q.front();
+                        const int64_t n = reqs[r].n;
+                        if (took && elems + n > s->max_launch_elems) break;
+                        if (op == HTAB_OP_INSERT &&
+                            known[static_cast<size_t>(t)] + inflight_ins[static_cast<size_t>(t)] + n >
+                                    s->limit[static_cast<size_t>(t)])
+                        {
+                            // wait for the table's launches in flight (its size is exact afterwards) ...
+                            if (took || table_launches[static_cast<size_t>(t)] > 0) break;
+                            // ... or refuse: cannot fit, no effect on the table
+                            q.pop_front();
+                            status[r] = HTAB_E_LIMIT;
+                            done_ns[r] = now_ns(t0);
+                            ++finished;
+                            continue;
+                        }
                         q.pop_front();
-                        status[r] = HTAB_E_LIMIT;
-                        done_ns[r] = now_ns(t0);
-                        ++finished;
-                        continue;
+                        segs.push_back(reqs[r]);
+                        l.reqs.push_back(r);
+                        if (op == HTAB_OP_INSERT) inflight_ins[static_cast<size_t>(t)] += n;
+                        elems += n;
+                        took = true;
                     }
-                    q.pop_front();
-                    sl.segs[nseg++] = reqs[r];
-                    sl.reqs.push_back(r);
-                    if (op == HTAB_OP_INSERT) inflight_ins[static_cast<size_t>(t)] += n;
-                    elems += n;
-                    took = true;
+                    if (took) l.tables.push_back(t);
+                    if (elems >= s->max_launch_elems || static_cast<int32_t>(segs.size()) >= kMaxLaunchSegs) break;
                 }
-                if (took)
-                {
-                    busy[static_cast<size_t>(t)] = 1;
-                    sl.tables.push_back(t);
-                }
-                if (elems >= s->max_launch_elems || nseg >= s->max_segs) break;
             }
-            if (nseg == 0) continue;
-            int rc = htab_batch_run_segments(sl.batch, sl.segs, nseg, sl.stream);
-            if (rc == HTAB_OK)
+            if (segs.empty()) break;
+            hipStream_t st = s->streams[static_cast<size_t>(sid)];
+            // per-table order: wait for the latest launch of each table that sits on another stream
+            waits.clear();
+            for (int32_t t : l.tables)
             {
-                hipLaunchKernelGGL(k_table_sizes<>, dim3((ntables + 255) / 256), dim3(256), 0, sl.stream, s->stripes_dev,
-                                   ntables, htab_dev::kMaxStripes, sl.sizes_dev);
-                rc = launch_status("htab_server_run: sizes");
+                const int32_t p = last_launch[static_cast<size_t>(t)];
+                if (p >= 0 && s->launches[static_cast<size_t>(p)].stream != sid &&
+                    std::find(waits.begin(), waits.end(), p) == waits.end())
+                    waits.push_back(p);
             }
-            if (rc == HTAB_OK && hipMemcpyAsync(sl.sizes, sl.sizes_dev, sizeof(int64_t) * static_cast<size_t>(ntables),
-                                                hipMemcpyDeviceToHost, sl.stream) != hipSuccess)
-                rc = fail(HTAB_E_HIP, "htab_server_run: size copy");
-            if (rc == HTAB_OK && hipEventRecord(sl.done, sl.stream) != hipSuccess)
+            int rc = HTAB_OK;
+            for (int32_t p : waits)
+                if (hipStreamWaitEvent(st, s->launches[static_cast<size_t>(p)].done, 0) != hipSuccess)
+                    rc = fail(HTAB_E_HIP, "htab_server_run: stream wait");
+            for (size_t g = 0; g < segs.size() && rc == HTAB_OK; g += htab_dev::kServeSegs)
+            {
+                const size_t m = std::min(segs.size() - g, static_cast<size_t>(htab_dev::kServeSegs));
+                uint32_t chunks = 0;
+                for (size_t j = 0; j < m; ++j)
+                {
+                    const htab_segment& r = segs[g + j];
+                    htab_dev::serve_seg& d = args.segs[j];
+                    d.keys = r.keys;
+                    d.vals = r.op == HTAB_OP_FIND ? r.vals_out : const_cast<void*>(r.vals);
+                    d.flags = r.flags;
+                    d.n = static_cast<uint32_t>(r.n);
+                    d.table = static_cast<int16_t>(r.table);
+                    d.op = static_cast<int8_t>(r.op);
+                    d.pad = 0;
+                    args.chunk_start[j] = chunks;
+                    chunks += static_cast<uint32_t>((r.n + htab_dev::kServeBlock - 1) / htab_dev::kServeBlock);
+                }
+                args.chunk_start[m] = chunks;
+                args.nsegs = static_cast<int32_t>(m);
+                if (chunks == 0) continue;
+                htab_call c;
+                c.op = call_op::serve;
+                c.b = s->batch;
+                c.serve = &args;
+                c.s = st;
+                rc = dispatch(s->batch->kind, c);
+            }
+            for (size_t g = 0; g < l.tables.size() && rc == HTAB_OK; g += htab_dev::kServeTables)
+            {
+                const size_t m = std::min(l.tables.size() - g, static_cast<size_t>(htab_dev::kServeTables));
+                tabs.n = static_cast<int32_t>(m);
+                for (size_t j = 0; j < m; ++j) tabs.table[j] = static_cast<int16_t>(l.tables[g + j]);
+                htab_call c;
+                c.op = call_op::serve_sizes;
+                c.b = s->batch;
+                c.serve_tabs = &tabs;
+                c.sizes_out = s->sizes_dev;
+                c.s = st;
+                rc = dispatch(s->batch->kind, c);
+            }
+            if (rc == HTAB_OK && hipEventRecord(l.done, st) != hipSuccess)
                 rc = fail(HTAB_E_HIP, "htab_server_run: event");
             if (rc != HTAB_OK)
             {
-                error.store(rc);
+                error = rc;
                 break;
             }
-            sl.busy = true;
-            sl.elems = elems;
+            for (int32_t t : l.tables)
+            {
+                ++table_launches[static_cast<size_t>(t)];
+                last_launch[static_cast<size_t>(t)] = li;
+            }
+            l.stream = sid;
+            l.live = true;
+            ++stream_launches[static_cast<size_t>(sid)];
+            ++live;
             ++launches;
-            segments += nseg;
-            int64_t inflight = 0;
-            for (auto& x : s->slots) inflight += x.busy ? 1 : 0;
-            max_inflight = std::max(max_inflight, inflight);
+            segments += static_cast<int64_t>(segs.size());
+            max_inflight = std::max(max_inflight, live);
             progressed = true;
         }
         if (!progressed) std::this_thread::yield();
     }
     submitter.join();
-    for (auto& sl : s->slots) (void)hipStreamSynchronize(sl.stream);
+    for (auto st : s->streams) (void)hipStreamSynchronize(st);
+    for (auto& l : s->launches)
+    {
+        l.reqs.clear();
+        l.tables.clear();
+        l.live = false;
+    }
     if (stats)
     {
         stats->launches = launches;
@@ -297,7 +361,7 @@ htab_server_run(htab_server* s, const htab_segment* reqs, const int64_t* arrival
         stats->max_concurrent_launches = max_inflight;
         stats->wall_ns = now_ns(t0);
     }
-    return error.load();
+    return error;
 }
 
 int

@@ -85,7 +85,7 @@ def build_requests(log: RequestLog, pools: Pools, out_flags, out_vals):
 
 
 class Server:
-    def __init__(self, tables, nstreams: int = 4, max_inflight_elems: int = 1 << 24):
+    def __init__(self, tables, nstreams: int = 1, max_inflight_elems: int = 1 << 24):
         arr = (ctypes.c_void_p * len(tables))(*[t._h.value for t in tables])
         h = ctypes.c_void_p()
         check(_lib.load().htab_server_create(arr, len(tables), nstreams, max_inflight_elems, ctypes.byref(h)),
@@ -182,7 +182,7 @@ def verify_against_replay(caps, pools: Pools, log: RequestLog, served_tables, st
     return {"finds_identical": finds_ok, "contents_identical": contents_ok}
 
 
-def served_run(caps, pools, log, nstreams=4, max_inflight_elems=1 << 24, keep_outputs=True):
+def served_run(caps, pools, log, nstreams=1, max_inflight_elems=1 << 24, keep_outputs=True):
     """Runs the log through a fresh server over fresh tables; returns (tables, outputs, results)."""
     import torch
 

@@ -39,3 +39,18 @@ def test_admission_refuses_inserts_that_cannot_fit():
     chk = sv.verify_against_replay(caps, pools, log, tables, status, flags, vals)
     sv.destroy_tables(tables)
     assert chk["finds_identical"] and chk["contents_identical"], chk
+
+
+def test_served_long_runs_of_small_requests():
+    """long same-operation runs of tiny requests (some empty): launches carry more segments than one
+    kernel's argument block holds, so a launch is split over several kernels on its stream"""
+    caps = sv.capacities(ntables=24, log2_lo=9, log2_hi=13, seed=3)
+    pools = sv.Pools(caps, seed=31)
+    log = sv.make_log(pools.sizes, nreq=4000, rate_per_s=None, seed=4, log2_lo=0, log2_hi=5, mix=(0.1, 0.85, 0.05))
+    log.n[::37] = 0
+    tables, (flags, vals, status), res = sv.served_run(caps, pools, log, nstreams=2, max_inflight_elems=1 << 22)
+    print(res, flush=True)
+    assert (status == 0).all()
+    chk = sv.verify_against_replay(caps, pools, log, tables, status, flags, vals)
+    sv.destroy_tables(tables)
+    assert chk["finds_identical"] and chk["contents_identical"], chk

@@ -15,7 +15,8 @@ from oracle import pyoracle as po  # noqa: E402
 
 
 def test_voxel_frames_match_reference_caller_pattern():
-    frames, npts, cap, fl_cap = 6, 1 << 16, 1 << 17, 1 << 17
+    # sparse frames: nearly every point is a block of its own, so 6 frames need up to 6 x 2^16 slots
+    frames, npts, cap, fl_cap = 6, 1 << 16, 1 << 19, 1 << 19
     scene = vx.make_scene(frames=frames, seed=3)
     tab = sa.unordered_map_u64_u32.createDeviceObject(cap)
     fl = sa.deque_u32(fl_cap)
