@@ -77,7 +77,8 @@ typedef struct htab_view
     uint32_t bucket_lanes;
     int64_t ref_bucket_count;  /* reference bucket_count(), for bucket() */
     int64_t ref_max_size;      /* reference max_size() */
-    uint32_t* ovf;             /* per bucket: stored keys whose probe sequence runs through it */
+    uint32_t* ovf;             /* per bucket: stored keys whose probe sequence runs through it; NULL
+                                  when the count lives in the padding of each bucket's first slot */
 } htab_view;
 
 /* unordered_map::createDeviceObject / unordered_set::createDeviceObject

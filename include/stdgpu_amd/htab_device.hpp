@@ -12,7 +12,8 @@
 //     slots after the buckets, with a presence bit each in `flags`.
 //   * Collisions overflow into the next bucket (linear probing over buckets, wrapping).
 //   * ovf[b] (one uint32 per bucket) counts the stored keys whose probe sequence runs through
-//     bucket b to a later bucket (home bucket <= b < stored bucket, cyclically).  A probe
+//     bucket b to a later bucket (home bucket <= b < stored bucket, cyclically).  It lives in the
+//     padding of the bucket's first slot when value_type has 4 spare bytes, else in an array.  A probe
 //     sequence ends at the first bucket that holds the key, holds an EMPTY slot, or has
 //     ovf == 0.  A bucket with ovf > 0 never holds an EMPTY slot (an erase there leaves TOMB),
 //     so the EMPTY test needs no extra load, and TOMBs never lengthen a probe sequence through
@@ -49,6 +50,12 @@ struct slot_traits
     using mapped_type = M;
     using value_type = kv_slot<K, M>;
     static constexpr bool is_map = true;
+    // a bucket's ovf count lives in the padding of its first slot when the slot has 4 spare
+    // bytes (unordered_map<u64, u32>: key at 0, value at 8, padding at 12): a probe then reads
+    // the count with the bucket, in the same line, and no separate array exists
+    static constexpr bool ovf_in_bucket =
+            sizeof(value_type) >= sizeof(K) + sizeof(M) + 4 && (sizeof(K) + sizeof(M)) % 4 == 0;
+    static constexpr int ovf_offset = static_cast<int>(sizeof(K) + sizeof(M));
     __host__ __device__ static K key(const value_type& v) { return v.first; }
 };
 
@@ -59,6 +66,8 @@ struct slot_traits<K, void>
     using mapped_type = void;
     using value_type = K;
     static constexpr bool is_map = false;
+    static constexpr bool ovf_in_bucket = false;
+    static constexpr int ovf_offset = 0;
     __host__ __device__ static K key(const value_type& v) { return v; }
 };
 
@@ -153,7 +162,7 @@ struct table_ref
     value_type* slots;
     int64_t* stripes;
     uint32_t* flags;
-    uint32_t* ovf; // per bucket: stored keys whose probe sequence runs through it
+    uint32_t* ovf; // per bucket: stored keys whose probe sequence runs through it (null when in-bucket)
     uint64_t nbuckets;
     uint64_t limit;
     uint32_t nstripes;
@@ -182,7 +191,15 @@ struct table_ref
 
     __device__ int64_t size() const { return stripe_sum(stripes, nstripes); }
 
-    __device__ uint32_t overflow(uint64_t b) const { return load_agent(ovf + b); }
+    __host__ __device__ uint32_t* ovf_ptr(uint64_t b) const
+    {
+        if constexpr (traits::ovf_in_bucket)
+            return reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(slots + b * W) + traits::ovf_offset);
+        else
+            return ovf + b;
+    }
+
+    __device__ uint32_t overflow(uint64_t b) const { return load_agent(ovf_ptr(b)); }
 
     // adds d to ovf of the `count` buckets starting at h (cyclic)
     __device__ void overflow_add(uint64_t h, uint64_t count, int32_t d) const
@@ -190,7 +207,7 @@ struct table_ref
         uint64_t b = h;
         for (uint64_t c = 0; c < count; ++c)
         {
-            __hip_atomic_fetch_add(ovf + b, static_cast<uint32_t>(d), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_fetch_add(ovf_ptr(b), static_cast<uint32_t>(d), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             b = next_bucket(b, nbuckets);
         }
     }
@@ -301,7 +318,7 @@ struct table_ref
                 }
             }
             // no EMPTY slot left here: the key's probe sequence runs through this bucket
-            __hip_atomic_fetch_add(ovf + b, 1U, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_fetch_add(ovf_ptr(b), 1U, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             b = next_bucket(b, nbuckets);
         }
         overflow_add(h, nbuckets, -1);

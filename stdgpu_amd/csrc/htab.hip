@@ -68,6 +68,20 @@ value_bytes_of(int kind)
     }
 }
 
+bool
+ovf_in_bucket_of(int kind)
+{
+    switch (kind)
+    {
+        case HTAB_SET_U32: return htab_dev::slot_traits<uint32_t, void>::ovf_in_bucket;
+        case HTAB_SET_U64: return htab_dev::slot_traits<uint64_t, void>::ovf_in_bucket;
+        case HTAB_MAP_U64_U32: return htab_dev::slot_traits<uint64_t, uint32_t>::ovf_in_bucket;
+        case HTAB_MAP_U32_U32: return htab_dev::slot_traits<uint32_t, uint32_t>::ovf_in_bucket;
+        case HTAB_MAP_U64_U64: return htab_dev::slot_traits<uint64_t, uint64_t>::ovf_in_bucket;
+        default: return false;
+    }
+}
+
 int
 mapped_bytes_of(int kind)
 {
@@ -161,7 +175,9 @@ htab_create(int kind, int64_t capacity, const htab_tuning* tuning, htab** out)
 
     const size_t area = static_cast<size_t>((nslots + 2) * vb);
     const size_t area_padded = ((area + 255) / 256) * 256;
-    const size_t ovf_bytes = ((static_cast<size_t>(nbuckets) * sizeof(uint32_t) + 255) / 256) * 256;
+    // the ovf count array (none when the count lives in the first slot's padding)
+    const size_t ovf_bytes =
+            ovf_in_bucket_of(kind) ? 0 : ((static_cast<size_t>(nbuckets) * sizeof(uint32_t) + 255) / 256) * 256;
     h->bytes = kCtlBytes + area_padded + ovf_bytes;
     hipError_t e = hipMalloc(&h->mem, h->bytes);
     if (e != hipSuccess)
@@ -173,7 +189,7 @@ htab_create(int kind, int64_t capacity, const htab_tuning* tuning, htab** out)
     h->view.slots = base + kCtlBytes;
     h->view.stripes = reinterpret_cast<int64_t*>(base);
     h->view.flags = reinterpret_cast<uint32_t*>(base + kFlagsOffset);
-    h->view.ovf = reinterpret_cast<uint32_t*>(base + kCtlBytes + area_padded);
+    h->view.ovf = ovf_bytes ? reinterpret_cast<uint32_t*>(base + kCtlBytes + area_padded) : nullptr;
     h->view.nbuckets = static_cast<uint64_t>(nbuckets);
     h->view.limit = static_cast<uint64_t>(limit);
     h->view.nstripes = (limit < nslots) ? 1U : kMaxStripes;

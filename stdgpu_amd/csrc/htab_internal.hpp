@@ -23,6 +23,7 @@ int fail_hip(hipError_t e, const char* what);
 int launch_status(const char* what);
 int value_bytes_of(int kind);
 int mapped_bytes_of(int kind);
+bool ovf_in_bucket_of(int kind);
 int key_bytes_of(int kind);
 void reference_sizing(int64_t capacity, int64_t* bucket_count, int64_t* max_size);
 

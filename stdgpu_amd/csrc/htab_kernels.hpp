@@ -88,6 +88,13 @@ struct chunk
         return c;
     }
 
+    // in-bucket ovf count (first slot's padding), valid on the lane holding slot 0 of a plain load
+    __device__ uint32_t ovf_word() const
+    {
+        static_assert(SB == 16, "the in-bucket count lives in 16-byte slots");
+        return static_cast<uint32_t>(hi >> 32);
+    }
+
     // bitmask over the chunk's slots j >= jmin_local for which pred(key) holds
     template <typename Pred>
     __device__ uint32_t mask(Pred pred, int jmin_local = 0) const
@@ -193,6 +200,7 @@ find_body(const table_ref<K, M>& t, const K* __restrict__ keys, const int64_t i,
     const int li = lane & (G - 1);
     const uint64_t gmask = ((G == 64) ? ~0ULL : ((1ULL << G) - 1)) << gbase;
     const uint32_t W = static_cast<uint32_t>(G * SPL);
+    constexpr bool kInline = slot_traits<K, M>::ovf_in_bucket;
 
     const K mykey = valid ? keys[i] : empty_key<K>();
     bool myhit = false;
@@ -249,15 +257,23 @@ find_body(const table_ref<K, M>& t, const K* __restrict__ keys, const int64_t i,
                 }
                 else if (!be)
                 {
-                    pend |= 1U << (rb + r);
+                    // no key runs past a home bucket with ovf == 0 (read with the bucket when in-bucket)
+                    if constexpr (kInline)
+                    {
+                        if (shfl32(li == 0 ? q[r].ovf_word() : 0U, gbase) != 0) pend |= 1U << (rb + r);
+                    }
+                    else
+                    {
+                        pend |= 1U << (rb + r);
+                    }
                 }
             }
         }
     }
 
-    // overflowed elements: walk the probe sequence one group element at a time.  Bucket b is
-    // loaded together with ovf of the bucket before it (p): the walk goes on only while p has
-    // keys running through it, so both loads are in flight at once.
+    // overflowed elements: walk the probe sequence one group element at a time.  With the count
+    // in the bucket, bucket b's own count decides whether the walk goes on past it; with the count
+    // array, bucket b is loaded together with ovf of the bucket before it (p), both in flight at once.
     int cur = -1;
     K ck = 0;
     uint64_t b = 0, p = 0, steps = 0;
@@ -277,13 +293,17 @@ find_body(const table_ref<K, M>& t, const K* __restrict__ keys, const int64_t i,
         if (active)
         {
             const C q = C::load_plain(t.slots + b * W, li);
-            const uint32_t ov = shfl32(li == 0 ? t.ovf[p] : 0U, gbase);
+            uint32_t ov = 0;
+            if constexpr (kInline)
+                ov = shfl32(li == 0 ? q.ovf_word() : 0U, gbase); // bucket b's own count
+            else
+                ov = shfl32(li == 0 ? t.ovf[p] : 0U, gbase); // count of the bucket before b
             const K k = ck;
             const uint32_t m = q.mask([k](K x) { return x == k; });
             const uint32_t e = q.mask([](K x) { return x == empty_key<K>(); });
             const uint64_t bm = ballot(m != 0) & gmask;
             const uint64_t be = ballot(e != 0) & gmask;
-            if (ov == 0)
+            if (!kInline && ov == 0)
             {
                 cur = -1; // no key runs past p: absent
             }
@@ -299,7 +319,7 @@ find_body(const table_ref<K, M>& t, const K* __restrict__ keys, const int64_t i,
                 }
                 cur = -1;
             }
-            else if (be || ++steps >= t.nbuckets)
+            else if (be || (kInline && ov == 0) || ++steps >= t.nbuckets)
             {
                 cur = -1;
             }
@@ -364,6 +384,7 @@ insert_body(const table_ref<K, M>& t, const void* __restrict__ values, const voi
     const int li = lane & (G - 1);
     const uint64_t gmask = ((G == 64) ? ~0ULL : ((1ULL << G) - 1)) << gbase;
     const uint32_t W = static_cast<uint32_t>(G * SPL);
+    constexpr bool kInline = slot_traits<K, M>::ovf_in_bucket;
 
     K mykey = empty_key<K>();
     uint64_t myv = 0;
@@ -405,7 +426,12 @@ insert_body(const table_ref<K, M>& t, const void* __restrict__ values, const voi
             if (act[r])
             {
                 hb[r] = home_bucket(static_cast<uint64_t>(kr[r]), t.nbuckets);
-                q[r] = C::load_keys(t.slots + hb[r] * W, li);
+                // a stale view is harmless here: a slot seen free but taken fails its CAS, and an
+                // insert-only launch never frees a slot (the in-bucket count comes with a plain load)
+                if constexpr (kInline)
+                    q[r] = C::load_plain(t.slots + hb[r] * W, li);
+                else
+                    q[r] = C::load_keys(t.slots + hb[r] * W, li);
             }
         }
 #pragma unroll
@@ -420,11 +446,18 @@ insert_body(const table_ref<K, M>& t, const void* __restrict__ values, const voi
                 const uint64_t bm = ballot(m != 0) & gmask;
                 const uint64_t be = ballot(e != 0) & gmask;
                 const uint64_t bf = ballot(f != 0) & gmask;
+                bool ends_home = be != 0;
+                if constexpr (kInline)
+                {
+                    // no key runs past a home bucket with ovf == 0: a TOMB there is free to take
+                    const uint32_t ovh = shfl32(li == 0 ? q[r].ovf_word() : 0U, gbase);
+                    ends_home = ends_home || (bf != 0 && ovh == 0);
+                }
                 if (bm)
                 {
                     // key present: nothing to do (insert never overwrites)
                 }
-                else if (be)
+                else if (ends_home)
                 {
                     // probe sequence ends in the home bucket: claim its first free slot
                     const int fl = ctz64(bf); // lane holding the first free slot
@@ -545,7 +578,7 @@ insert_body(const table_ref<K, M>& t, const void* __restrict__ values, const voi
                         {
                             uint64_t x = hb + d;
                             if (x >= t.nbuckets) x -= t.nbuckets;
-                            __hip_atomic_fetch_add(t.ovf + x, 1U, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                            __hip_atomic_fetch_add(t.ovf_ptr(x), 1U, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                         }
                         if (li == cur)
                         {
@@ -637,6 +670,7 @@ erase_body(const table_ref<K, M>& t, const K* __restrict__ keys, const int64_t i
     const int li = lane & (G - 1);
     const uint64_t gmask = ((G == 64) ? ~0ULL : ((1ULL << G) - 1)) << gbase;
     const uint32_t W = static_cast<uint32_t>(G * SPL);
+    constexpr bool kInline = slot_traits<K, M>::ovf_in_bucket;
 
     const K mykey = valid ? keys[i] : empty_key<K>();
     bool mydel = false;
@@ -666,7 +700,12 @@ erase_body(const table_ref<K, M>& t, const K* __restrict__ keys, const int64_t i
             if (act[r])
             {
                 hb[r] = home_bucket(static_cast<uint64_t>(kr[r]), t.nbuckets);
-                q[r] = C::load_keys(t.slots + hb[r] * W, li);
+                // a stale view is harmless in an erase-only launch: keys and counts only go away,
+                // a key seen but already erased fails its CAS, and a stale count is never too low
+                if constexpr (kInline)
+                    q[r] = C::load_plain(t.slots + hb[r] * W, li);
+                else
+                    q[r] = C::load_keys(t.slots + hb[r] * W, li);
             }
         }
 #pragma unroll
@@ -679,6 +718,10 @@ erase_body(const table_ref<K, M>& t, const K* __restrict__ keys, const int64_t i
                 const uint32_t e = q[r].mask([](K x) { return x == empty_key<K>(); });
                 const uint64_t bm = ballot(m != 0) & gmask;
                 const uint64_t be = ballot(e != 0) & gmask;
+                // no probe sequence runs through the home bucket: it ends here, and the erased
+                // slot may become EMPTY
+                bool ends_home = be != 0;
+                if constexpr (kInline) ends_home = ends_home || shfl32(li == 0 ? q[r].ovf_word() : 0U, gbase) == 0;
                 if (bm)
                 {
                     const int ml = ctz64(bm);
@@ -686,13 +729,13 @@ erase_body(const table_ref<K, M>& t, const K* __restrict__ keys, const int64_t i
                     if (lane == ml)
                     {
                         const uint64_t slot = hb[r] * W + static_cast<uint64_t>(li * SPL + __builtin_ctz(m));
-                        const K repl = be ? empty_key<K>() : tomb_key<K>();
+                        const K repl = ends_home ? empty_key<K>() : tomb_key<K>();
                         ok = (cas_agent(t.key_ptr(slot), k, repl) == k) ? 1 : 0;
                     }
                     ok = shfl64(ok, ml);
                     if (ok && li == rb + r) mydel = true;
                 }
-                else if (!be)
+                else if (!ends_home)
                 {
                     pend |= 1U << (rb + r);
                 }
@@ -722,13 +765,14 @@ erase_body(const table_ref<K, M>& t, const K* __restrict__ keys, const int64_t i
         if (active)
         {
             const C q = C::load_keys(t.slots + b * W, li);
-            const uint32_t ov = shfl32(li == 0 ? t.overflow(p) : 0U, gbase);
+            // in-bucket layout: bucket b's own count; count array: the count of the bucket before b
+            const uint32_t ov = shfl32(li == 0 ? t.overflow(kInline ? b : p) : 0U, gbase);
             const K k = ck;
             const uint32_t m = q.mask([k](K x) { return x == k; });
             const uint32_t e = q.mask([](K x) { return x == empty_key<K>(); });
             const uint64_t bm = ballot(m != 0) & gmask;
             const uint64_t be = ballot(e != 0) & gmask;
-            if (ov == 0)
+            if (!kInline && ov == 0)
             {
                 cur = -1; // no key runs past p: absent
             }
@@ -739,7 +783,7 @@ erase_body(const table_ref<K, M>& t, const K* __restrict__ keys, const int64_t i
                 if (lane == ml)
                 {
                     const uint64_t slot = b * W + static_cast<uint64_t>(li * SPL + __builtin_ctz(m));
-                    const K repl = be ? empty_key<K>() : tomb_key<K>();
+                    const K repl = (be || (kInline && ov == 0)) ? empty_key<K>() : tomb_key<K>();
                     ok = (cas_agent(t.key_ptr(slot), k, repl) == k) ? 1 : 0;
                 }
                 ok = shfl64(ok, ml);
@@ -750,14 +794,14 @@ erase_body(const table_ref<K, M>& t, const K* __restrict__ keys, const int64_t i
                     {
                         uint64_t x = hb + d;
                         if (x >= t.nbuckets) x -= t.nbuckets;
-                        __hip_atomic_fetch_add(t.ovf + x, static_cast<uint32_t>(-1), __ATOMIC_RELAXED,
+                        __hip_atomic_fetch_add(t.ovf_ptr(x), static_cast<uint32_t>(-1), __ATOMIC_RELAXED,
                                                __HIP_MEMORY_SCOPE_AGENT);
                     }
                     if (li == cur) mydel = true;
                 }
                 cur = -1;
             }
-            else if (be || ++steps >= t.nbuckets)
+            else if (be || (kInline && ov == 0) || ++steps >= t.nbuckets)
             {
                 cur = -1;
             }
@@ -1004,15 +1048,26 @@ k_serve(const table_ref<K, M>* __restrict__ tables, const serve_args a)
         erase_body<K, M, G, RI>(tr, static_cast<const K*>(seg.keys), i, valid, seg.flags);
 }
 
+// 32 lanes per table, one size stripe each, summed with shuffles inside the 32-lane half-wave
 template <typename K, typename M>
 __global__ __launch_bounds__(256) void
 k_serve_sizes(const table_ref<K, M>* __restrict__ tables, const serve_tables a, int64_t* __restrict__ sizes_out)
 {
-    for (int s = static_cast<int>(threadIdx.x); s < a.n; s += static_cast<int>(blockDim.x))
+    const int sub = static_cast<int>(threadIdx.x) & 31;
+    const int lane = lane_id();
+    for (int base = 0; base < a.n; base += 8)
     {
-        const table_ref<K, M> t = tables[a.table[s]];
-        __hip_atomic_store(sizes_out + a.table[s], stripe_sum(t.stripes, t.nstripes), __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_SYSTEM);
+        const int s = base + (static_cast<int>(threadIdx.x) >> 5);
+        int64_t v = 0;
+        if (s < a.n)
+        {
+            const table_ref<K, M> t = tables[a.table[s]];
+            if (static_cast<uint32_t>(sub) < t.nstripes) v = t.stripes[static_cast<uint64_t>(sub) * kStripeStride];
+        }
+#pragma unroll
+        for (int off = 16; off >= 1; off >>= 1) v += static_cast<int64_t>(shfl64(static_cast<uint64_t>(v), lane ^ off));
+        if (s < a.n && sub == 0)
+            __hip_atomic_store(sizes_out + a.table[s], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     }
 }
 
@@ -1037,13 +1092,18 @@ template <typename K, typename M>
 __global__ __launch_bounds__(256) void
 k_clear(table_ref<K, M> t, uint64_t nchunks)
 {
-    // every key field becomes EMPTY: the whole slot area is filled with 0xFF bytes
+    // every key field becomes EMPTY: the slot area is filled with 0xFF bytes, except the padding
+    // word of 16-byte slots that carries the in-bucket ovf count (0); otherwise ovf[] is zeroed
+    constexpr bool kInline = slot_traits<K, M>::ovf_in_bucket;
     uint4* p = reinterpret_cast<uint4*>(t.slots);
-    const uint4 ones = make_uint4(~0U, ~0U, ~0U, ~0U);
+    const uint4 fill = make_uint4(~0U, ~0U, ~0U, kInline ? 0U : ~0U);
     const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
-    for (uint64_t c = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; c < nchunks; c += stride) p[c] = ones;
-    for (uint64_t b = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; b < t.nbuckets; b += stride)
-        t.ovf[b] = 0;
+    for (uint64_t c = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; c < nchunks; c += stride) p[c] = fill;
+    if constexpr (!kInline)
+    {
+        for (uint64_t b = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; b < t.nbuckets; b += stride)
+            t.ovf[b] = 0;
+    }
     if (blockIdx.x == 0 && threadIdx.x < kMaxStripes) t.stripes[static_cast<uint64_t>(threadIdx.x) * kStripeStride] = 0;
     if (blockIdx.x == 0 && threadIdx.x == 0) *t.flags = 0;
 }
@@ -1210,7 +1270,7 @@ k_valid_overflow(table_ref<K, M> t, const uint32_t* __restrict__ want, unsigned 
     unsigned long long nbad = 0;
     for (uint64_t b = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; b < t.nbuckets; b += stride)
     {
-        const uint32_t ov = t.ovf[b];
+        const uint32_t ov = *t.ovf_ptr(b);
         if (ov != want[b]) ++nbad;
         if (ov != 0)
         {

@@ -176,6 +176,8 @@ htab_server_run(htab_server* s, const htab_segment* reqs, const int64_t* arrival
     std::vector<int32_t> waits;
     htab_dev::serve_args args{};
     htab_dev::serve_tables tabs{};
+    std::vector<int32_t> sized;
+    std::vector<int64_t> run_first(static_cast<size_t>(ntables), -1); // first request of a table's run
     while (finished < nreq && error == HTAB_OK)
     {
         bool progressed = false;
@@ -257,6 +259,7 @@ htab_server_run(htab_server* s, const htab_segment* reqs, const int64_t* arrival
                             continue;
                         }
                         q.pop_front();
+                        if (!took) run_first[static_cast<size_t>(t)] = r;
                         segs.push_back(reqs[r]);
                         l.reqs.push_back(r);
                         if (op == HTAB_OP_INSERT) inflight_ins[static_cast<size_t>(t)] += n;
@@ -310,11 +313,15 @@ htab_server_run(htab_server* s, const htab_segment* reqs, const int64_t* arrival
                 c.s = st;
                 rc = dispatch(s->batch->kind, c);
             }
-            for (size_t g = 0; g < l.tables.size() && rc == HTAB_OK; g += htab_dev::kServeTables)
+            // sizes of the tables whose run inserts or erases (a find leaves the size alone)
+            sized.clear();
+            for (int32_t t : l.tables)
+                if (reqs[run_first[static_cast<size_t>(t)]].op != HTAB_OP_FIND) sized.push_back(t);
+            for (size_t g = 0; g < sized.size() && rc == HTAB_OK; g += htab_dev::kServeTables)
             {
-                const size_t m = std::min(l.tables.size() - g, static_cast<size_t>(htab_dev::kServeTables));
+                const size_t m = std::min(sized.size() - g, static_cast<size_t>(htab_dev::kServeTables));
                 tabs.n = static_cast<int32_t>(m);
-                for (size_t j = 0; j < m; ++j) tabs.table[j] = static_cast<int16_t>(l.tables[g + j]);
+                for (size_t j = 0; j < m; ++j) tabs.table[j] = static_cast<int16_t>(sized[g + j]);
                 htab_call c;
                 c.op = call_op::serve_sizes;
                 c.b = s->batch;

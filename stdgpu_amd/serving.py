@@ -231,3 +231,33 @@ def plain_run(caps, pools, log, blocking: bool):
 def destroy_tables(tables):
     for t in tables:
         C.unordered_map_u64_u32.destroyDeviceObject(t)
+
+
+def main(argv=None) -> None:
+    """`python -m stdgpu_amd.serving served|plain [nreq]`: the 64-table saturation log (every request
+    at once) through the server, or through one plain call per request; prints one JSON line."""
+    import json
+    import sys
+
+    import torch
+
+    args = sys.argv[1:] if argv is None else argv
+    mode = args[0] if args else "served"
+    nreq = int(args[1]) if len(args) > 1 else 3000
+    caps = capacities(ntables=64)
+    pools = Pools(caps)
+    log = make_log(pools.sizes, nreq=nreq, rate_per_s=None, seed=8)
+    if mode == "served":
+        tabs, outs, res = served_run(caps, pools, log)
+        destroy_tables(tabs)
+        del outs
+    elif mode == "plain":
+        res = plain_run(caps, pools, log, blocking=False)
+    else:
+        raise SystemExit("mode must be served or plain")
+    torch.cuda.synchronize()
+    print(json.dumps({"mode": mode, "requests": nreq, "keys": int(log.n.sum()), **res}), flush=True)
+
+
+if __name__ == "__main__":
+    main()
