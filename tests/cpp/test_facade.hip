@@ -439,6 +439,58 @@ set_u32_native_and_bulk()
     set_t::destroyDeviceObject(s);
 }
 
+// insert_multiple_collision / erase_multiple_collision_{head,tail}_first
+// (unordered_datastructure.inc:1295-1373) on this layout: keys sharing the last home bucket, so their
+// probe sequence runs over the table's end; native inserts and erases, valid() after every step
+// (valid() recomputes each bucket's overflow count)
+static std::vector<std::uint64_t>
+keys_homed_at(std::uint64_t bucket, std::uint64_t nbuckets, std::size_t count, std::uint64_t seed)
+{
+    std::mt19937_64 rng(seed);
+    std::vector<std::uint64_t> out;
+    std::set<std::uint64_t> seen;
+    while (out.size() < count)
+    {
+        const std::uint64_t k = rng() >> 4;
+        if (htab_dev::home_bucket(k, nbuckets) == bucket && seen.insert(k).second) out.push_back(k);
+    }
+    return out;
+}
+
+static void
+collision_chain_native()
+{
+    g_case = "collision_chain_native";
+    const stdgpu::index_t cap = 4096; // 512 buckets of 8 pairs
+    const std::uint64_t nb = 512;
+    for (int order = 0; order < 2; ++order)
+    {
+        map_t m = map_t::createDeviceObject(cap);
+        const std::vector<std::uint64_t> chain = keys_homed_at(nb - 1, nb, 40, 31 + order);
+        const std::vector<std::uint64_t> other = unique_keys(1000, 41);
+        CHECK(sum(run_insert(m, other)) == 1000);
+        CHECK(sum(run_insert(m, chain, true)) == 40);
+        CHECK(m.size() == 1040 && m.valid());
+        std::vector<std::uint64_t> vals;
+        CHECK(sum(run_find(m, chain, &vals)) == 40);
+        for (std::size_t i = 0; i < chain.size(); ++i) CHECK(vals[i] == static_cast<std::uint32_t>(chain[i] * 3 + 1));
+        std::vector<std::uint64_t> seq = chain;
+        if (order == 1) std::reverse(seq.begin(), seq.end());
+        for (std::size_t part = 0; part < 4; ++part)
+        {
+            const std::vector<std::uint64_t> go(seq.begin() + part * 10, seq.begin() + (part + 1) * 10);
+            CHECK(sum(run_erase(m, go)) == 10);
+            CHECK(m.valid());
+            const std::vector<int> f = run_find(m, seq);
+            for (std::size_t i = 0; i < seq.size(); ++i) CHECK(f[i] == (i < (part + 1) * 10 ? 0 : 1));
+        }
+        CHECK(m.size() == 1000);
+        CHECK(sum(run_insert(m, chain)) == 40);
+        CHECK(m.size() == 1040 && m.valid());
+        map_t::destroyDeviceObject(m);
+    }
+}
+
 // examples/openmp/unordered_set.cpp and unordered_map.cpp: duplicate-free neighbour sets checked
 // against closed-form sums
 template <typename Range, typename T>
@@ -519,6 +571,7 @@ main()
                           range_insert_erase_and_device_range,
                           insert_and_erase_unique_parallel,
                           set_u32_native_and_bulk,
+                          collision_chain_native,
                           examples_closed_form };
     for (auto c : cases)
     {
