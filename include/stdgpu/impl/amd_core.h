@@ -90,8 +90,9 @@ inline constexpr device_policy device{};
 template <typename T>
 inline constexpr bool is_execution_policy_v = std::is_base_of_v<execution::device_policy, std::decay_t<T>>;
 
-// Allocation goes through the C ABI (hipMalloc inside libstdgpu_amd.so); the allocator is
-// carried for API compatibility and returned by get_allocator().
+// The tables allocate through the C ABI (hipMalloc inside libstdgpu_amd.so); the allocator is
+// carried for API compatibility, returned by get_allocator(), and allocates device arrays itself
+// (memory.h:340-380: allocate(n) / deallocate(p, n)).
 template <typename T>
 struct safe_device_allocator
 {
@@ -100,6 +101,20 @@ struct safe_device_allocator
     template <typename U>
     constexpr __host__ __device__ safe_device_allocator(const safe_device_allocator<U>&) noexcept
     {
+    }
+
+    [[nodiscard]] T* allocate(std::int64_t n)
+    {
+        if (n < 0) throw std::invalid_argument("stdgpu: negative allocation size");
+        void* p = nullptr;
+        if (hipMalloc(&p, static_cast<std::size_t>(n) * sizeof(T) + (n == 0 ? 1 : 0)) != hipSuccess)
+            throw std::runtime_error("stdgpu: device allocation failed");
+        return static_cast<T*>(p);
+    }
+
+    void deallocate(T* p, std::int64_t /*n*/)
+    {
+        if (p) (void)hipFree(p);
     }
 };
 

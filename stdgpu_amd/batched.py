@@ -86,3 +86,56 @@ class TableBatch:
             "htab_batch_capture",
         )
         return Graph(g, (ops, offsets, keys, vals, flags, vals_out, s))
+
+
+def main(argv=None) -> None:
+    """`python -m stdgpu_amd.batched [bucket_lanes probe_batch block_size load]`: the 4096-table
+    workload (sizes log-uniform in [2^8, 2^18]) through one graph-captured launch per phase, then
+    through one plain call per table; prints one JSON line with the phase times."""
+    import json
+    import sys
+
+    import torch
+
+    from . import autotune as at
+    from . import containers as C
+    from . import workloads_device as wd
+
+    args = sys.argv[1:] if argv is None else argv
+    g, rb, blk, load = (int(args[0]), int(args[1]), int(args[2]), float(args[3])) if len(args) >= 4 else (4, 4, 1024, 0.5)
+    cfg = {"bucket_lanes": g, "probe_batch": rb, "probe_group": g * rb, "block_size": blk, "load": load}
+    w = wd.batched_maps("cuda", ntables=4096)
+    out = {"config": cfg}
+    for label in ("batched_warmup", "batched"):
+        tabs = at.build_tables(w["sizes"], cfg)
+        batch = TableBatch(tabs)
+        ms, _ = at.run_phases(batch, w)
+        out[label + "_ms"] = {k: round(v, 3) for k, v in ms.items()}
+        batch.destroy()
+        for t in tabs:
+            C.unordered_map_u64_u32.destroyDeviceObject(t)
+    tabs = at.build_tables(w["sizes"], cfg)
+    batch = TableBatch(tabs)
+    s = torch.cuda.Stream()
+    pms = {}
+    for op, name in (("insert", "ins"), ("find", "find"), ("erase", "erase")):
+        n = w[f"{name}_keys"].numel()
+        flags = torch.zeros(n, dtype=torch.uint8, device="cuda")
+        vals_out = torch.zeros(n, dtype=torch.int32, device="cuda") if op == "find" else None
+        torch.cuda.synchronize()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(s)
+        batch.run_per_table(op, w[f"{name}_offsets"].cpu().numpy(), w[f"{name}_keys"],
+                            w["ins_vals"] if op == "insert" else None, flags, vals_out, stream=s)
+        b.record(s)
+        b.synchronize()
+        pms[op] = round(a.elapsed_time(b), 3)
+    out["plain_per_table_ms"] = pms
+    batch.destroy()
+    for t in tabs:
+        C.unordered_map_u64_u32.destroyDeviceObject(t)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -491,6 +491,37 @@ collision_chain_native()
     }
 }
 
+// clear_empty, get_allocator, hash_objects, custom_execution_policy
+// (unordered_datastructure.inc:2450-2560, 267-290)
+static void
+accessors_and_policies()
+{
+    g_case = "accessors_and_policies";
+    hipStream_t st;
+    HIP_OK(hipStreamCreate(&st));
+    stdgpu::execution::device_policy pol{ st };
+    map_t m = map_t::createDeviceObject(pol, 10000);
+    CHECK(m.size(pol) == 0 && m.valid(pol));
+    m.clear(pol); // clear_empty
+    CHECK(m.size(pol) == 0 && m.empty() && m.valid(pol));
+    map_t::allocator_type a = m.get_allocator();
+    map_t::value_type* arr = a.allocate(10000);
+    CHECK(arr != nullptr);
+    HIP_OK(hipMemset(arr, 0, 10000 * sizeof(map_t::value_type)));
+    a.deallocate(arr, 10000);
+    const map_t::hasher h = m.hash_function();
+    CHECK(h(std::uint64_t{ 42 }) == 42); // the reference's identity hash on integers
+    const map_t::key_equal eq = m.key_eq();
+    CHECK(eq(std::uint64_t{ 7 }, std::uint64_t{ 7 }) && !eq(std::uint64_t{ 7 }, std::uint64_t{ 8 }));
+    map_t::destroyDeviceObject(pol, m);
+    set_t s = set_t::createDeviceObject(pol, 100);
+    set_t::allocator_type sa = s.get_allocator();
+    std::uint32_t* sarr = sa.allocate(100);
+    sa.deallocate(sarr, 100);
+    set_t::destroyDeviceObject(pol, s);
+    HIP_OK(hipStreamDestroy(st));
+}
+
 // examples/openmp/unordered_set.cpp and unordered_map.cpp: duplicate-free neighbour sets checked
 // against closed-form sums
 template <typename Range, typename T>
@@ -572,6 +603,7 @@ main()
                           insert_and_erase_unique_parallel,
                           set_u32_native_and_bulk,
                           collision_chain_native,
+                          accessors_and_policies,
                           examples_closed_form };
     for (auto c : cases)
     {
