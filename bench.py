@@ -48,6 +48,8 @@ def parse():
     ap.add_argument("--voxel", type=int, default=1, help="also measure the voxel-hashing pipeline")
     ap.add_argument("--voxel-frames", type=int, default=30)
     ap.add_argument("--voxel-log2-points", type=int, default=24)
+    ap.add_argument("--zipf", type=int, default=1, help="also run the sequence with Zipf(1.1) keys")
+    ap.add_argument("--set-u32", type=int, default=1, help="also run the small unordered_set<u32> sequence")
     ap.add_argument("--served", type=int, default=1, help="also measure the served form")
     ap.add_argument("--served-requests", type=int, default=3000)
     ap.add_argument("--served-seconds", type=float, default=0.25)
@@ -276,6 +278,116 @@ def bench_voxel(args) -> dict:
     return res
 
 
+def _timed_sequence(step, names, steps: int, warmup: int):
+    """Runs step(events) warmup + steps times; returns per-phase ms lists (HIP events)."""
+    import torch
+
+    for _ in range(warmup):
+        step(None)
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    per = {k: [] for k in names}
+    for _ in range(steps):
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(len(names) + 1)]
+        step((ev, stream))
+        ev[-1].synchronize()
+        for j, k in enumerate(names):
+            per[k].append(ev[j].elapsed_time(ev[j + 1]))
+    return per
+
+
+def bench_zipf(args) -> dict:
+    """The large map sequence again with Zipf(1.1) key draws (heavy duplicates: many threads race
+    on the same key and bucket): clear, insert 2^n draws, find 2^n queries (90% present), erase 25%
+    of the distinct keys, re-insert them."""
+    import torch
+
+    import stdgpu_amd as sa
+    from stdgpu_amd import workloads_device as wd
+
+    n = 1 << args.log2n
+    w = wd.map_u64_zipf("cuda", n=n)
+    tab = sa.unordered_map_u64_u32.createDeviceObject(w["capacity"])
+    hits = torch.empty(n, dtype=torch.uint8, device="cuda")
+    got = torch.empty(n, dtype=torch.int32, device="cuda")
+    names = ["clear", "insert", "find", "erase", "reinsert"]
+
+    def step(evs):
+        ev = evs[0] if evs else None
+        rec = (lambda j: ev[j].record(evs[1])) if ev else (lambda j: None)
+        rec(0)
+        tab.clear()
+        rec(1)
+        tab.insert(w["keys"], w["values"])
+        rec(2)
+        tab.find(w["queries"], hits=hits, values_out=got)
+        rec(3)
+        tab.erase(w["erase"])
+        rec(4)
+        tab.insert(w["reinsert"], w["reinsert_values"])
+        rec(5)
+
+    per = _timed_sequence(step, names, steps=3, warmup=1)
+    ok = tab.size() == w["distinct"] and tab.valid()
+    ops = {"insert": n, "find": n, "erase": int(w["erase"].numel()), "reinsert": int(w["reinsert"].numel())}
+    step_ms = float(np.mean([sum(per[k][i] for k in names) for i in range(3)]))
+    res = {
+        "workload": f"unordered_map<u64,u32>, 2^{args.log2n} Zipf(1.1) draws ({w['distinct']} distinct keys), "
+                    "capacity for load 0.8: clear, insert, find 2^n at 90% present, erase 25% of distinct, re-insert",
+        "gops": round(sum(ops.values()) / (step_ms / 1e3) / 1e9, 3),
+        "per_op_gops": {k: round(ops[k] / (np.mean(per[k]) / 1e3) / 1e9, 3) for k in ops},
+        "size_and_valid_ok": bool(ok),
+    }
+    sa.unordered_map_u64_u32.destroyDeviceObject(tab)
+    return res
+
+
+def bench_set_u32(args) -> dict:
+    """The small set sequence: unordered_set<u32>, capacity 2^21; insert 2^20 unique keys, find 2^20
+    queries (half present), erase a random half, find again (a table of 8 MB: Infinity-Cache resident)."""
+    import torch
+
+    import stdgpu_amd as sa
+    from stdgpu_amd import workloads as wl
+
+    w = wl.set_u32()
+    d = lambda a: torch.from_numpy(np.ascontiguousarray(a).view(np.int32)).cuda()  # noqa: E731
+    keys, queries, erase = d(w.keys), d(w.queries), d(w.erase)
+    tab = sa.unordered_set_u32.createDeviceObject(w.capacity)
+    hits = torch.empty(queries.numel(), dtype=torch.uint8, device="cuda")
+    names = ["clear", "insert", "find", "erase", "find_after_erase"]
+
+    def step(evs):
+        ev = evs[0] if evs else None
+        rec = (lambda j: ev[j].record(evs[1])) if ev else (lambda j: None)
+        rec(0)
+        tab.clear()
+        rec(1)
+        tab.insert(keys)
+        rec(2)
+        tab.find(queries, hits=hits)
+        rec(3)
+        tab.erase(erase)
+        rec(4)
+        tab.find(queries, hits=hits)
+        rec(5)
+
+    per = _timed_sequence(step, names, steps=10, warmup=2)
+    ok = tab.size() == len(w.keys) - len(np.unique(w.erase)) and tab.valid()
+    ops = {"insert": keys.numel(), "find": queries.numel(), "erase": erase.numel(), "find_after_erase": queries.numel()}
+    step_ms = float(np.mean([sum(per[k][i] for k in names) for i in range(10)]))
+    res = {
+        "workload": "unordered_set<u32>, capacity 2^21: insert 2^20 unique keys, find 2^20 (50% present), "
+                    "erase a random half, find again",
+        "gops": round(sum(ops.values()) / (step_ms / 1e3) / 1e9, 3),
+        "per_op_gops": {k: round(ops[k] / (np.mean(per[k]) / 1e3) / 1e9, 3) for k in ops},
+        "per_op_us": {k: round(float(np.mean(per[k])) * 1e3, 1) for k in names},
+        "size_and_valid_ok": bool(ok),
+    }
+    sa.unordered_set_u32.destroyDeviceObject(tab)
+    return res
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -404,6 +516,10 @@ def main():
     }
     sa.unordered_map_u64_u32.destroyDeviceObject(tab)
     del keys, vals, queries, erase, reins, reins_v, hits, got, ins_flags, er_flags
+    if args.zipf:
+        out["zipf"] = bench_zipf(args)
+    if args.set_u32:
+        out["set_u32"] = bench_set_u32(args)
     if args.batched:
         out["batched"] = bench_batched(args)
     if args.served:

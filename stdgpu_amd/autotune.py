@@ -1,9 +1,10 @@
 """Autotuner of the batched form (many tables, one persistent launch per phase).
 
 Sweeps, on the workload itself:
-  * bucket width: 4, 8 or 16 slots of 16 bytes (= lanes that load one bucket together);
+  * bucket width: 1, 4, 8 or 16 slots of 16 bytes (= lanes that load one bucket together);
   * probe group: lanes whose bucket loads are issued together = bucket lanes x probe batch,
-    16, 32 or 64 (probe batch 2, 4 or 8, at most the bucket lanes);
+    16, 32 or 64 (probe batch 2, 4 or 8, at most the bucket lanes); with one-slot buckets every
+    lane probes its own element (probe group 1);
   * workgroup size: 256, 512 or 1024 (also the chunk of elements a workgroup takes at a time);
   * load factor: 0.5, 0.7, 0.8 or 0.9 (physical slots = keys / load).
 Each candidate builds its tables, runs the insert, find and erase phases once as captured HIP
@@ -17,18 +18,23 @@ import math
 from . import containers as C
 from .batched import TableBatch
 
-BUCKET_LANES = (4, 8, 16)
+BUCKET_LANES = (1, 4, 8, 16)
 PROBE_GROUPS = (16, 32, 64)
 BLOCKS = (256, 512, 1024)
 LOADS = (0.5, 0.7, 0.8, 0.9)
 
 
 def candidates(lanes=BUCKET_LANES, groups=PROBE_GROUPS, blocks=BLOCKS, loads=LOADS):
-    for g, pg, blk, lf in itertools.product(lanes, groups, blocks, loads):
-        rb = pg // g
-        if rb not in (2, 4, 8) or rb > g:
+    for g, blk, lf in itertools.product(lanes, blocks, loads):
+        if g == 1:
+            # one lane per bucket: the probe batch is 1 (probe_batch 0 selects it)
+            yield {"bucket_lanes": 1, "probe_batch": 0, "probe_group": 1, "block_size": blk, "load": lf}
             continue
-        yield {"bucket_lanes": g, "probe_batch": rb, "probe_group": pg, "block_size": blk, "load": lf}
+        for pg in groups:
+            rb = pg // g
+            if rb not in (2, 4, 8) or rb > g:
+                continue
+            yield {"bucket_lanes": g, "probe_batch": rb, "probe_group": pg, "block_size": blk, "load": lf}
 
 
 def build_tables(sizes, cfg, cls=C.unordered_map_u64_u32):
@@ -115,4 +121,4 @@ def _subset(w: dict, k: int) -> dict:
 
 def describe(cfg: dict) -> str:
     return (f"bucket {cfg['bucket_lanes']} x 16 B, probe group {cfg['probe_group']} lanes "
-            f"(batch {cfg['probe_batch']}), workgroup {cfg['block_size']}, load {cfg['load']}")
+            f"(batch {cfg['probe_batch'] or 1}), workgroup {cfg['block_size']}, load {cfg['load']}")

@@ -146,7 +146,8 @@ htab_create(int kind, int64_t capacity, const htab_tuning* tuning, htab** out)
         if (tuning->slots_per_capacity > 0.0F) spc = tuning->slots_per_capacity;
         rb = tuning->probe_batch;
     }
-    if (lanes != 4 && lanes != 8 && lanes != 16) return fail(HTAB_E_ARG, "htab_create: bucket_lanes must be 4, 8 or 16");
+    if (lanes != 1 && lanes != 4 && lanes != 8 && lanes != 16)
+        return fail(HTAB_E_ARG, "htab_create: bucket_lanes must be 1, 4, 8 or 16");
     if (rb != 0 && ((rb != 2 && rb != 4 && rb != 8) || rb > lanes))
         return fail(HTAB_E_ARG, "htab_create: probe_batch must be 2, 4 or 8 and at most bucket_lanes");
     if (block != 256 && block != 512 && block != 1024) return fail(HTAB_E_ARG, "htab_create: block_size must be 256, 512 or 1024");
@@ -167,6 +168,9 @@ htab_create(int kind, int64_t capacity, const htab_tuning* tuning, htab** out)
     const double want = std::ceil(static_cast<double>(capacity) * static_cast<double>(spc));
     int64_t nbuckets = static_cast<int64_t>(std::ceil(want / static_cast<double>(W)));
     if (nbuckets < 1) nbuckets = 1;
+    // room for two entries at least, as the reference's capacity-1 table (one bucket head plus one
+    // excess entry); only one-slot buckets need the second bucket for it
+    if (nbuckets * W < 2) nbuckets = (2 + W - 1) / W;
     if (nbuckets >= (int64_t(1) << 32)) { delete h; return fail(HTAB_E_LIMIT, "htab_create: too many buckets"); }
     const int64_t nslots = nbuckets * W;
     int64_t ref_bc = 0, ref_max = 0;

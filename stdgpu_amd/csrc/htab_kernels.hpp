@@ -427,11 +427,8 @@ insert_body(const table_ref<K, M>& t, const void* __restrict__ values, const voi
             {
                 hb[r] = home_bucket(static_cast<uint64_t>(kr[r]), t.nbuckets);
                 // a stale view is harmless here: a slot seen free but taken fails its CAS, and an
-                // insert-only launch never frees a slot (the in-bucket count comes with a plain load)
-                if constexpr (kInline)
-                    q[r] = C::load_plain(t.slots + hb[r] * W, li);
-                else
-                    q[r] = C::load_keys(t.slots + hb[r] * W, li);
+                // insert-only launch never frees a slot (the in-bucket count comes with the load)
+                q[r] = C::load_plain(t.slots + hb[r] * W, li);
             }
         }
 #pragma unroll
@@ -507,7 +504,7 @@ insert_body(const table_ref<K, M>& t, const void* __restrict__ values, const voi
     uint64_t b = 0, hb = 0, walked = 0, ff = kNoSlot;
     K ffobs = 0;
     uint32_t jmin = 0;
-    bool absent = false;
+    bool absent = false, fresh = false;
     while (true)
     {
         if (cur < 0 && pend)
@@ -522,13 +519,31 @@ insert_body(const table_ref<K, M>& t, const void* __restrict__ values, const voi
             ff = kNoSlot;
             jmin = 0;
             absent = false;
+            fresh = false;
         }
         const bool active = cur >= 0;
         if (ballot(active) == 0) break;
         if (active)
         {
-            const C q = C::load_keys(t.slots + b * W, li);
-            const uint32_t ov = shfl32(li == 0 ? t.overflow(b) : 0U, gbase);
+            // plain (L2-cacheable) loads until the element loses a CAS, coherent loads after that: a
+            // stale view only shows taken slots as free or counts as too low, and every claim is
+            // checked by its CAS against the value it saw
+            C q;
+            uint32_t ovl = 0;
+            if (fresh)
+            {
+                q = C::load_keys(t.slots + b * W, li);
+                ovl = li == 0 ? t.overflow(b) : 0U;
+            }
+            else
+            {
+                q = C::load_plain(t.slots + b * W, li);
+                if constexpr (kInline)
+                    ovl = li == 0 ? q.ovf_word() : 0U;
+                else
+                    ovl = li == 0 ? t.ovf[b] : 0U;
+            }
+            const uint32_t ov = shfl32(ovl, gbase);
             const K k = ck;
             const int jloc = static_cast<int>(jmin) - li * SPL; // first slot of this chunk still in play
             const uint32_t m = q.mask([k](K x) { return x == k; }, jloc);
@@ -599,6 +614,7 @@ insert_body(const table_ref<K, M>& t, const void* __restrict__ values, const voi
                         b = ff / W;
                         jmin = static_cast<uint32_t>(ff % W) + 1;
                         ff = kNoSlot;
+                        fresh = true;
                         if (jmin == W)
                         {
                             b = next_bucket(b, t.nbuckets);
@@ -702,10 +718,7 @@ erase_body(const table_ref<K, M>& t, const K* __restrict__ keys, const int64_t i
                 hb[r] = home_bucket(static_cast<uint64_t>(kr[r]), t.nbuckets);
                 // a stale view is harmless in an erase-only launch: keys and counts only go away,
                 // a key seen but already erased fails its CAS, and a stale count is never too low
-                if constexpr (kInline)
-                    q[r] = C::load_plain(t.slots + hb[r] * W, li);
-                else
-                    q[r] = C::load_keys(t.slots + hb[r] * W, li);
+                q[r] = C::load_plain(t.slots + hb[r] * W, li);
             }
         }
 #pragma unroll
@@ -764,9 +777,16 @@ erase_body(const table_ref<K, M>& t, const K* __restrict__ keys, const int64_t i
         if (ballot(active) == 0) break;
         if (active)
         {
-            const C q = C::load_keys(t.slots + b * W, li);
+            // plain loads: in an erase-only launch a stale view only shows a key that is already gone
+            // (its CAS fails), too few EMPTY slots or too high a count (both keep TOMB and walk on)
+            const C q = C::load_plain(t.slots + b * W, li);
             // in-bucket layout: bucket b's own count; count array: the count of the bucket before b
-            const uint32_t ov = shfl32(li == 0 ? t.overflow(kInline ? b : p) : 0U, gbase);
+            uint32_t ovl = 0;
+            if constexpr (kInline)
+                ovl = li == 0 ? q.ovf_word() : 0U;
+            else
+                ovl = li == 0 ? t.ovf[p] : 0U;
+            const uint32_t ov = shfl32(ovl, gbase);
             const K k = ck;
             const uint32_t m = q.mask([k](K x) { return x == k; });
             const uint32_t e = q.mask([](K x) { return x == empty_key<K>(); });

@@ -87,7 +87,10 @@ struct batch_ops
             {
                 if (b->rb == 4) return launch_rb<4, B>(b, nsegs, s);
             }
-            if (b->rb == 2) return launch_rb<2, B>(b, nsegs, s);
+            if constexpr (G >= 2)
+            {
+                if (b->rb == 2) return launch_rb<2, B>(b, nsegs, s);
+            }
         }
         return launch_rb<default_rb(G), B>(b, nsegs, s);
     }
@@ -257,8 +260,16 @@ dispatch_g(htab_call& c)
             {
                 if (rb == 8) return go(ops<K, M, G, 8>{});
             }
-            if (rb == 4) return go(ops<K, M, G, 4>{});
-            if (rb == 2) return go(ops<K, M, G, 2>{});
+            if constexpr (G >= 4)
+            {
+                if (rb == 4) return go(ops<K, M, G, 4>{});
+                if (rb == 2) return go(ops<K, M, G, 2>{});
+            }
+            else
+            {
+                // one lane per bucket: each lane probes its own element
+                if (rb == 1) return go(ops<K, M, G, 1>{});
+            }
             return fail(HTAB_E_ARG, "bad probe batch");
         }
         case call_op::batch_build: return batch_ops<K, M, G>::build(c.b, c.tables);
@@ -308,6 +319,7 @@ dispatch_kind(htab_call& c)
     const int lanes = c.h ? c.h->lanes : c.b->lanes;
     switch (lanes)
     {
+        case 1: return dispatch_g<K, M, 1>(c);
         case 4: return dispatch_g<K, M, 4>(c);
         case 8: return dispatch_g<K, M, 8>(c);
         case 16: return dispatch_g<K, M, 16>(c);

@@ -106,3 +106,40 @@ def check_against_numpy(ntables: int = 16, log2_lo: float = 4, log2_hi: float = 
             and eq(a.ins_keys, b["ins_keys"]) and np.array_equal(a.ins_vals.view(np.int32), b["ins_vals"].numpy())
             and eq(a.find_keys, b["find_keys"]) and eq(a.erase_keys, b["erase_keys"])
             and np.array_equal(a.erase_offsets, b["erase_offsets"].numpy()) and bool(torch.is_tensor(b["ins_keys"])))
+
+
+def map_u64_zipf(device, n: int = 1 << 26, load: float = 0.8, s: float = 1.1, hit: float = 0.9,
+                 erase_frac: float = 0.25, seed: int = 2, universe: int | None = None) -> dict:
+    """Device twin of workloads.map_u64_zipf (the Zipf(1.1) run of the large map sequence): keys,
+    values, queries, erase and re-insert keys as int64/int32 tensors (uint64/uint32 bits)."""
+    import torch
+
+    universe = universe or n
+    r = torch.arange(1, universe + 1, dtype=torch.float64, device=device)
+    cdf = torch.cumsum(r ** (-s), 0)
+    cdf /= cdf[-1].clone()
+    ranks = torch.searchsorted(cdf, _unit(stream(seed, n, device)), right=True).to(torch.int64) + 1
+    del r, cdf
+    keys = mix(ranks * _GOLD)
+    keys = torch.where(keys == -1, torch.ones_like(keys), keys)
+    # ascending unsigned order, as numpy sorts uint64
+    distinct = torch.unique(keys ^ _i64(1 << 63)) ^ _i64(1 << 63)
+    rng = np.random.default_rng(seed + 100)
+    nhit = int(round(n * hit))
+    q_present = keys[torch.from_numpy(rng.integers(0, n, size=nhit)).to(device)]
+    q_absent = stream(seed + 7, n - nhit, device)
+    queries = torch.cat([q_present, q_absent])[torch.from_numpy(rng.permutation(n)).to(device)]
+    ne = int(distinct.numel() * erase_frac)
+    erase = distinct[torch.from_numpy(rng.permutation(distinct.numel())[:ne]).to(device)]
+    return {"capacity": wl._capacity_for(n, load), "keys": keys, "values": value_of(keys), "queries": queries,
+            "erase": erase, "reinsert": erase.clone(), "reinsert_values": value_of(erase),
+            "distinct": int(distinct.numel())}
+
+
+def check_zipf_against_numpy(n: int = 1 << 12) -> bool:
+    """Element-for-element equality of the Zipf twin with the numpy generator (CPU tests)."""
+    a = wl.map_u64_zipf(n=n)
+    b = map_u64_zipf("cpu", n=n)
+    eq = lambda x, y: np.array_equal(x.view(np.int64), y.numpy())  # noqa: E731
+    return (a.capacity == b["capacity"] and eq(a.keys, b["keys"]) and eq(a.queries, b["queries"])
+            and eq(a.erase, b["erase"]) and np.array_equal(a.values.view(np.int32), b["values"].numpy()))

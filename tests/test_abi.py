@@ -67,4 +67,5 @@ def test_device_workload_generator_matches_numpy():
     from stdgpu_amd import workloads_device as wd
 
     assert wd.check_against_numpy()
+    assert wd.check_zipf_against_numpy(1 << 12)
     assert wd.check_against_numpy(ntables=40, log2_lo=5, log2_hi=11)

@@ -34,7 +34,7 @@ def make_tables(w, lanes):
     return [sa.unordered_map_u64_u32.createDeviceObject(int(c), sa.Tuning(bucket_lanes=lanes)) for c in w.capacities]
 
 
-@pytest.mark.parametrize("lanes", [4, 8, 16])
+@pytest.mark.parametrize("lanes", [1, 4, 8, 16])
 def test_batched_phases_equal_plain_calls(lanes):
     w = wl.batched_maps(ntables=64, log2_lo=6, log2_hi=12, seed=11)
     bt = make_tables(w, lanes)

@@ -91,7 +91,8 @@ class Model:
 
 
 def dev(a, sdt):
-    return torch.from_numpy(np.ascontiguousarray(a).view(sdt)).cuda()
+    # a fresh contiguous copy (a reversed one-element view keeps its negative stride otherwise)
+    return torch.from_numpy(np.array(a, copy=True).view(sdt)).cuda()
 
 
 def g_insert(tab, keys, kind):
@@ -125,7 +126,7 @@ def check_table(tab, model, kind, probe):
     assert tab.valid()
 
 
-@pytest.mark.parametrize("lanes", [4, 8, 16])
+@pytest.mark.parametrize("lanes", [1, 4, 8, 16])
 @pytest.mark.parametrize("kind", list(KINDS))
 def test_collision_chain_wraps_and_unwinds(kind, lanes):
     cls, kdt, sdt, vdt = KINDS[kind]

@@ -148,13 +148,13 @@ def _map_sequence(w, lanes=8, check_ins_flags=True):
     sa.unordered_map_u64_u32.destroyDeviceObject(tab)
 
 
-@pytest.mark.parametrize("lanes", [4, 8, 16])
+@pytest.mark.parametrize("lanes", [1, 4, 8, 16])
 def test_config_map_u64_uniform_reduced(lanes):
     """map<uint64_t,uint32_t> at load 0.8: insert, find (90% hit), erase 25%, re-insert."""
     _map_sequence(wl.map_u64_uniform(n=1 << 18), lanes=lanes)
 
 
-@pytest.mark.parametrize("lanes", [4, 8, 16])
+@pytest.mark.parametrize("lanes", [1, 4, 8, 16])
 def test_config_map_u64_zipf_reduced(lanes):
     """the same sequence with Zipf(1.1) draws: many lanes race on the same key and bucket."""
     _map_sequence(wl.map_u64_zipf(n=1 << 18), lanes=lanes, check_ins_flags=False)
@@ -164,7 +164,7 @@ KINDS = list(CLS)
 
 
 @pytest.mark.parametrize("kind", KINDS)
-@pytest.mark.parametrize("lanes", [4, 8, 16])
+@pytest.mark.parametrize("lanes", [1, 4, 8, 16])
 def test_all_kinds_random_rounds(kind, lanes):
     key, mapped = kind
     rng = np.random.default_rng(1234 + lanes)
@@ -240,7 +240,7 @@ def test_tiny_table_holds_exactly_two():
     assert int(u8(ins).sum()) == 0 and tab.size() == 2
 
 
-@pytest.mark.parametrize("lanes", [4, 8, 16])
+@pytest.mark.parametrize("lanes", [1, 4, 8, 16])
 def test_tiny_table_parallel_one_free(lanes):
     # unordered_datastructure.inc:1549-1603: one free place, 100000 random keys -> exactly one wins
     tab = sa.unordered_map_u64_u32.createDeviceObject(1, sa.Tuning(bucket_lanes=lanes))
@@ -256,7 +256,7 @@ def test_tiny_table_parallel_one_free(lanes):
     assert u8(h).tolist() == [1, 1]
 
 
-@pytest.mark.parametrize("lanes", [4, 8, 16])
+@pytest.mark.parametrize("lanes", [1, 4, 8, 16])
 def test_contention_same_key(lanes):
     # insert_multiple_* / erase_multiple_* (unordered_datastructure.inc:1167-1373)
     tab = sa.unordered_map_u64_u32.createDeviceObject(100000, sa.Tuning(bucket_lanes=lanes))
@@ -404,3 +404,37 @@ def test_against_committed_golden_fixtures():
     o = np.argsort(host(k, "u64"))
     np.testing.assert_array_equal(host(k, "u64")[o], g["map_contents_keys"])
     np.testing.assert_array_equal(host(v, "u32")[o], g["map_contents_vals"])
+
+
+def test_config_map_u64_zipf_full_size_properties():
+    """The 2^26 Zipf(1.1) sequence (heavy duplicates, generated on the device): exactly one inserted
+    flag per distinct key, size = distinct keys, every present query hits with value_of(key), erase
+    and re-insert counts, valid() at the end."""
+    from stdgpu_amd import workloads_device as wd
+
+    n = 1 << 26
+    w = wd.map_u64_zipf("cuda", n=n)
+    tab = sa.unordered_map_u64_u32.createDeviceObject(w["capacity"])
+    ins = torch.empty(n, dtype=torch.uint8, device="cuda")
+    tab.insert(w["keys"], w["values"], inserted=ins)
+    assert int(ins.sum()) == w["distinct"] and tab.size() == w["distinct"]
+    # one winner per distinct key: the winners' keys are exactly the distinct keys
+    winners = torch.sort(w["keys"][ins.bool()])[0]
+    assert winners.numel() == w["distinct"] and bool((winners[1:] != winners[:-1]).all())
+    hits, got = tab.find(w["queries"])
+    nhit = int(round(n * 0.9))
+    present = torch.isin(w["queries"], w["keys"])
+    assert torch.equal(hits.bool(), present)
+    assert int(present.sum()) >= nhit  # 90% of the queries are drawn from the keys
+    assert torch.equal(got[present], wd.value_of(w["queries"][present]))
+    er = torch.empty(w["erase"].numel(), dtype=torch.uint8, device="cuda")
+    tab.erase(w["erase"], erased=er)
+    assert int(er.sum()) == w["erase"].numel()
+    assert tab.size() == w["distinct"] - w["erase"].numel()
+    h2, _ = tab.find(w["erase"])
+    assert int(h2.sum()) == 0
+    ins2 = torch.empty(w["reinsert"].numel(), dtype=torch.uint8, device="cuda")
+    tab.insert(w["reinsert"], w["reinsert_values"], inserted=ins2)
+    assert int(ins2.sum()) == w["reinsert"].numel() and tab.size() == w["distinct"]
+    assert tab.valid()
+    sa.unordered_map_u64_u32.destroyDeviceObject(tab)
