@@ -369,6 +369,62 @@ fetch_element(const void* values, const void* keys, const void* vals, int64_t i,
 
 constexpr uint64_t kNoSlot = ~0ULL;
 
+// Claim order of a key inside a bucket: slots o, o+1, ..., W-1, 0, ..., o-1, with o taken from the
+// key's hash.  Concurrent inserts of different keys into one bucket then aim at different slots,
+// while every inserter of one key follows one order over its whole probe sequence, which is what
+// the no-duplicate argument needs (a key's claims move forward along that order only).  Tables of
+// kOrderedClaimBuckets buckets or more keep o = 0 (plain first-free order): a launch's elements in
+// flight (about 2^19) rarely meet in one of so many buckets, and the offset then only costs ALU work.
+// The choice depends on the table alone, so all inserters of a key in a launch share it.
+constexpr uint64_t kOrderedClaimBuckets = 1ULL << 22;
+
+__device__ __forceinline__ uint32_t
+claim_offset(uint64_t key, uint32_t W)
+{
+    return static_cast<uint32_t>(mix64(key)) & (W - 1);
+}
+
+// bits j of a lane's chunk whose bucket slot li*SPL + j lies in [lo, hi)
+template <int SPL>
+__device__ __forceinline__ uint32_t
+slot_range_mask(int li, uint32_t lo, uint32_t hi)
+{
+    uint32_t m = 0;
+#pragma unroll
+    for (int j = 0; j < SPL; ++j)
+    {
+        const uint32_t x = static_cast<uint32_t>(li * SPL + j);
+        if (x >= lo && x < hi) m |= 1U << j;
+    }
+    return m;
+}
+
+// The group's first free slot in the order starting at o, from order position jmin on: returns the
+// lane that holds it (-1 if none) and, in *fsel, that lane's candidate bits (the lowest one is it).
+// Positions >= jmin are the slots [o + jmin, W) followed by [max(o + jmin - W, 0), o).
+template <int SPL>
+__device__ __forceinline__ int
+first_free_in_order(uint32_t f, uint32_t o, uint32_t jmin, uint32_t W, int li, uint64_t gmask, uint32_t* fsel)
+{
+    const uint32_t a1 = o + jmin;
+    const uint32_t fa = a1 < W ? (f & slot_range_mask<SPL>(li, a1, W)) : 0U;
+    const uint64_t ba = ballot(fa != 0) & gmask;
+    if (ba)
+    {
+        *fsel = fa;
+        return ctz64(ba);
+    }
+    if (o == 0)
+    {
+        *fsel = 0; // plain first-free order: nothing before the start
+        return -1;
+    }
+    const uint32_t fb = f & slot_range_mask<SPL>(li, a1 > W ? a1 - W : 0U, o);
+    const uint64_t bb = ballot(fb != 0) & gmask;
+    *fsel = fb;
+    return bb ? ctz64(bb) : -1;
+}
+
 template <typename K, typename M, int G, int R>
 __device__ __forceinline__ void
 insert_body(const table_ref<K, M>& t, const void* __restrict__ values, const void* __restrict__ keys,
@@ -389,6 +445,7 @@ insert_body(const table_ref<K, M>& t, const void* __restrict__ values, const voi
     K mykey = empty_key<K>();
     uint64_t myv = 0;
     if (valid) fetch_element<K, M>(values, keys, vals, i, &mykey, &myv);
+    const bool ordered = t.nbuckets < kOrderedClaimBuckets; // uniform over the launch
 
     // a table at its exact limit refuses every insert (reference: the !full() loop guard)
     const bool at_limit = (t.nstripes == 1) && (load_agent(t.stripes) >= static_cast<int64_t>(t.limit));
@@ -456,15 +513,21 @@ insert_body(const table_ref<K, M>& t, const void* __restrict__ values, const voi
                 }
                 else if (ends_home)
                 {
-                    // probe sequence ends in the home bucket: claim its first free slot
-                    const int fl = ctz64(bf); // lane holding the first free slot
+                    // probe sequence ends in the home bucket: claim its first free slot in the key's order
+                    uint32_t fsel = f;
+                    int fl = 0;
+                    if (ordered)
+                        fl = first_free_in_order<SPL>(f, claim_offset(static_cast<uint64_t>(k), W), 0, W, li, gmask,
+                                                      &fsel);
+                    else
+                        fl = ctz64(bf);
                     // the value travels from the owner lane; the claiming lane stores it
                     const uint64_t v = shfl64(myv, gbase + rb + r);
                     uint64_t seen = 0;
                     uint64_t slot = 0;
                     if (lane == fl)
                     {
-                        const int j = __builtin_ctz(f);
+                        const int j = __builtin_ctz(fsel);
                         const K obs = q[r].key(j);
                         slot = hb[r] * W + static_cast<uint64_t>(li * SPL + j);
                         const K got = cas_agent(t.key_ptr(slot), obs, k);
@@ -503,7 +566,7 @@ insert_body(const table_ref<K, M>& t, const void* __restrict__ values, const voi
     uint64_t cv = 0;
     uint64_t b = 0, hb = 0, walked = 0, ff = kNoSlot;
     K ffobs = 0;
-    uint32_t jmin = 0;
+    uint32_t jmin = 0, co = 0; // order position to resume from; the key's claim offset
     bool absent = false, fresh = false;
     while (true)
     {
@@ -514,6 +577,7 @@ insert_body(const table_ref<K, M>& t, const void* __restrict__ values, const voi
             ck = shfl(mykey, gbase + cur);
             cv = shfl64(myv, gbase + cur);
             hb = home_bucket(static_cast<uint64_t>(ck), t.nbuckets);
+            co = ordered ? claim_offset(static_cast<uint64_t>(ck), W) : 0U;
             b = hb;
             walked = 0;
             ff = kNoSlot;
@@ -545,26 +609,27 @@ insert_body(const table_ref<K, M>& t, const void* __restrict__ values, const voi
             }
             const uint32_t ov = shfl32(ovl, gbase);
             const K k = ck;
-            const int jloc = static_cast<int>(jmin) - li * SPL; // first slot of this chunk still in play
-            const uint32_t m = q.mask([k](K x) { return x == k; }, jloc);
-            const uint32_t e = q.mask([](K x) { return x == empty_key<K>(); }, jloc);
-            const uint32_t f = q.mask([](K x) { return x == empty_key<K>() || x == tomb_key<K>(); }, jloc);
+            // the whole bucket is checked for the key and for EMPTY slots (slots passed after a lost
+            // CAS hold other keys for good); free slots count from order position jmin on
+            const uint32_t m = q.mask([k](K x) { return x == k; });
+            const uint32_t e = q.mask([](K x) { return x == empty_key<K>(); });
+            const uint32_t f = q.mask([](K x) { return x == empty_key<K>() || x == tomb_key<K>(); });
             const uint64_t bm = ballot(m != 0) & gmask;
             const uint64_t be = ballot(e != 0) & gmask;
-            const uint64_t bf = ballot(f != 0) & gmask;
+            uint32_t fsel = 0;
+            const int fl = first_free_in_order<SPL>(f, co, jmin, W, li, gmask, &fsel);
             if (bm)
             {
                 cur = -1; // present
             }
             else
             {
-                if (ff == kNoSlot && bf)
+                if (ff == kNoSlot && fl >= 0)
                 {
-                    const int fl = ctz64(bf);
                     uint64_t cand = 0, obs = 0;
                     if (lane == fl)
                     {
-                        const int j = __builtin_ctz(f);
+                        const int j = __builtin_ctz(fsel);
                         cand = b * W + static_cast<uint64_t>(li * SPL + j);
                         obs = static_cast<uint64_t>(q.key(j));
                     }
@@ -608,11 +673,11 @@ insert_body(const table_ref<K, M>& t, const void* __restrict__ values, const voi
                     }
                     else
                     {
-                        // lost ff to another key: every slot before ff holds another key (an insert-only
-                        // launch never frees a slot), so the scan resumes right after ff; ff only moves
-                        // forward, so this terminates
+                        // lost ff to another key: every slot before ff in the key's order holds another
+                        // key (an insert-only launch never frees a slot), so the scan resumes right after
+                        // ff; ff only moves forward, so this terminates
                         b = ff / W;
-                        jmin = static_cast<uint32_t>(ff % W) + 1;
+                        jmin = ((static_cast<uint32_t>(ff % W) + W - co) & (W - 1)) + 1;
                         ff = kNoSlot;
                         fresh = true;
                         if (jmin == W)
