@@ -50,6 +50,7 @@ def parse():
     ap.add_argument("--voxel-log2-points", type=int, default=24)
     ap.add_argument("--zipf", type=int, default=1, help="also run the sequence with Zipf(1.1) keys")
     ap.add_argument("--set-u32", type=int, default=1, help="also run the small unordered_set<u32> sequence")
+    ap.add_argument("--free-lists", type=int, default=1, help="also time the vector / deque free lists")
     ap.add_argument("--served", type=int, default=1, help="also measure the served form")
     ap.add_argument("--served-requests", type=int, default=3000)
     ap.add_argument("--served-seconds", type=float, default=0.25)
@@ -243,6 +244,14 @@ def bench_voxel(args) -> dict:
 
     frames, npts, cap = args.voxel_frames, 1 << args.voxel_log2_points, 1 << 23
     scene = vx.make_scene(frames=frames)
+    # one untimed frame on throwaway objects: the allocator's first allocations of the frame's
+    # temporaries are not part of a frame
+    wtab, wfl = sa.unordered_map_u64_u32.createDeviceObject(cap), sa.deque_u32(cap)
+    wfl.fill_iota()
+    vx.run_frame(wtab, wfl, vx.frame_keys_device(scene, 0, npts, "cuda"), scene.cams[0], check=False)
+    torch.cuda.synchronize()
+    wfl.destroy()
+    sa.unordered_map_u64_u32.destroyDeviceObject(wtab)
     tab = sa.unordered_map_u64_u32.createDeviceObject(cap)
     fl = sa.deque_u32(cap)
     fl.fill_iota()
@@ -388,6 +397,42 @@ def bench_set_u32(args) -> dict:
     return res
 
 
+def bench_free_lists(args) -> dict:
+    """The free lists: vector<uint32_t> stack push_back / pop_back and deque<uint32_t> push_back /
+    pop_front of 2^24 values each (one atomic per wavefront; 4 B in + 4 B out + 1 B flag per op)."""
+    import torch
+
+    import stdgpu_amd as sa
+
+    n = 1 << 24
+    vals = torch.arange(n, dtype=torch.int32, device="cuda")
+    out = torch.empty(n, dtype=torch.int32, device="cuda")
+    flags = torch.empty(n, dtype=torch.uint8, device="cuda")
+    res = {"workload": "2^24 uint32 values pushed, then popped, per container", "ops_per_phase": n}
+    for name, cls, pop in (("vector", sa.vector_u32, "pop_back"), ("deque", sa.deque_u32, "pop_front")):
+        c = cls(n)
+        names = ["push", "pop"]
+
+        def step(evs):
+            ev = evs[0] if evs else None
+            rec = (lambda j: ev[j].record(evs[1])) if ev else (lambda j: None)
+            rec(0)
+            c.push_back(vals, pushed=flags)
+            rec(1)
+            getattr(c, pop)(n, out=out, popped=flags)
+            rec(2)
+
+        per = _timed_sequence(step, names, steps=5, warmup=1)
+        ok = c.size() == 0 and bool(flags.all())
+        res[name] = {"push_gops": round(n / (np.mean(per["push"]) / 1e3) / 1e9, 3),
+                     "pop_gops": round(n / (np.mean(per["pop"]) / 1e3) / 1e9, 3),
+                     "push_gbs": round(9 * n / (np.mean(per["push"]) / 1e3) / 1e9, 1),
+                     "pop_gbs": round(9 * n / (np.mean(per["pop"]) / 1e3) / 1e9, 1),
+                     "all_pushed_and_popped": ok}
+        c.destroy()
+    return res
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -520,6 +565,8 @@ def main():
         out["zipf"] = bench_zipf(args)
     if args.set_u32:
         out["set_u32"] = bench_set_u32(args)
+    if args.free_lists:
+        out["free_lists"] = bench_free_lists(args)
     if args.batched:
         out["batched"] = bench_batched(args)
     if args.served:

@@ -226,6 +226,27 @@ int htab_server_run(htab_server* server, const htab_segment* requests, const int
 int htab_server_replay_plain(htab_server* server, const htab_segment* requests, const int32_t* status, int64_t nreq,
                              int blocking, void* stream);
 
+/* ------------------------------------------------------------------------------------------
+ * vector<uint32_t> used as a stack, the container of the reference's excess free list
+ * (src/stdgpu/vector.cuh; push_back / pop_back vector_detail.cuh:181-302, clear_iota :361-368).
+ * One bulk call only pushes or only pops; positions are claimed with one atomic per wavefront.
+ * ------------------------------------------------------------------------------------------ */
+typedef struct htab_vector htab_vector;
+
+int htab_vector_create(int64_t capacity, htab_vector** out);
+int htab_vector_destroy(htab_vector* vec);
+/* push_back of n values; pushed[i] (optional) = 1 iff value i found room (vector_detail.cuh:181-240) */
+int htab_vector_push_back(htab_vector* vec, const uint32_t* values, int64_t n, uint8_t* pushed, void* stream);
+/* pop_back of up to n values from the top: out[i] = the value, or 0xFFFFFFFF when the stack ran empty;
+ * popped[i] (optional) = 1 iff element i got a value (vector_detail.cuh:242-302) */
+int htab_vector_pop_back(htab_vector* vec, uint32_t* out, int64_t n, uint8_t* popped, void* stream);
+/* vector_clear_iota: element i = first + i, size = capacity (vector_detail.cuh:361-368) */
+int htab_vector_fill_iota(htab_vector* vec, uint32_t first, void* stream);
+/* size() clamped to [0, capacity] (vector_detail.cuh:454-512) */
+int htab_vector_size(htab_vector* vec, int64_t* out, void* stream);
+/* the elements from bottom to top into out (device memory, capacity entries); *n_out = size */
+int htab_vector_contents(htab_vector* vec, uint32_t* out, int64_t* n_out, void* stream);
+
 /* Message for the last non-zero status returned to the calling thread. */
 const char* htab_last_error(void);
 

@@ -770,6 +770,68 @@ orc_sizing(std::int64_t capacity, std::int64_t* bucket_count, std::int64_t* exce
     *excess_count = ex;
 }
 
+// vector<uint32_t> as a stack: the excess-list container (vector_detail.cuh:181-302, 361-368, 454-512),
+// values carried as index_t bit patterns
+void*
+orc_vector_create(std::int64_t capacity)
+{
+    auto* v = new index_stack();
+    v->init(static_cast<index_t>(capacity));
+    v->size_ = 0;
+    std::memset(v->occ.w, 0, sizeof(unsigned int) * static_cast<std::size_t>((v->cap + 31) / 32));
+    return v;
+}
+
+void
+orc_vector_destroy(void* p)
+{
+    auto* v = static_cast<index_stack*>(p);
+    v->release();
+    delete v;
+}
+
+std::int64_t orc_vector_size(void* p) { return static_cast<index_stack*>(p)->size(); }
+
+void orc_vector_fill_iota(void* p, std::uint32_t first) { static_cast<index_stack*>(p)->clear_iota(static_cast<index_t>(first)); }
+
+// push n values; ok[i] = pushed
+void
+orc_vector_push_back(void* p, const std::uint32_t* v, std::int64_t n, std::uint8_t* ok, int threads)
+{
+    auto* st = static_cast<index_stack*>(p);
+#pragma omp parallel for num_threads(threads > 0 ? threads : 1) schedule(static)
+    for (std::int64_t i = 0; i < n; ++i)
+    {
+        const bool r = st->push_back(static_cast<index_t>(v[i]));
+        if (ok) ok[i] = r ? 1 : 0;
+    }
+}
+
+// pop n values from the top; ok[i] = popped, out[i] = value
+void
+orc_vector_pop_back(void* p, std::uint32_t* out, std::int64_t n, std::uint8_t* ok, int threads)
+{
+    auto* st = static_cast<index_stack*>(p);
+#pragma omp parallel for num_threads(threads > 0 ? threads : 1) schedule(static)
+    for (std::int64_t i = 0; i < n; ++i)
+    {
+        index_t v = 0;
+        const bool r = st->pop_back(&v);
+        if (ok) ok[i] = r ? 1 : 0;
+        out[i] = r ? static_cast<std::uint32_t>(v) : 0xFFFFFFFFU;
+    }
+}
+
+// elements bottom to top; returns the count
+std::int64_t
+orc_vector_contents(void* p, std::uint32_t* out)
+{
+    auto* st = static_cast<index_stack*>(p);
+    const index_t n = st->size();
+    for (index_t i = 0; i < n; ++i) out[i] = static_cast<std::uint32_t>(st->data[i]);
+    return n;
+}
+
 void* orc_deque_create(std::int64_t capacity) { return new u32_deque(static_cast<index_t>(capacity)); }
 void orc_deque_destroy(void* d) { delete static_cast<u32_deque*>(d); }
 std::int64_t orc_deque_size(void* d) { return static_cast<u32_deque*>(d)->size(); }

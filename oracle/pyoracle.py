@@ -43,6 +43,13 @@ def load():
         "orc_bucket": (I64, [P, U64]),
         "orc_fibonacci_hashing": (I64, [U64, I64]),
         "orc_sizing": (None, [I64, ctypes.POINTER(I64), ctypes.POINTER(I64)]),
+        "orc_vector_create": (P, [I64]),
+        "orc_vector_destroy": (None, [P]),
+        "orc_vector_size": (I64, [P]),
+        "orc_vector_fill_iota": (None, [P, ctypes.c_uint32]),
+        "orc_vector_push_back": (None, [P, P, I64, P, I32]),
+        "orc_vector_pop_back": (None, [P, P, I64, P, I32]),
+        "orc_vector_contents": (I64, [P, P]),
         "orc_deque_create": (P, [I64]),
         "orc_deque_destroy": (None, [P]),
         "orc_deque_size": (I64, [P]),
@@ -153,6 +160,48 @@ def voxel_insert(table: "Table", deque: "Deque", keys) -> int:
     """The reference caller pattern: per key pop a slot, insert (key, slot), push back if present."""
     k = np.ascontiguousarray(keys, dtype=np.uint64)
     return load().orc_voxel_insert(table._t, deque._d, _p(k), len(k))
+
+
+class Vector:
+    """vector<uint32_t> used as a stack on the CPU, reference algorithm (push_back / pop_back)."""
+
+    def __init__(self, capacity: int):
+        self._v = load().orc_vector_create(capacity)
+        self.capacity = capacity
+
+    def close(self):
+        if self._v:
+            load().orc_vector_destroy(self._v)
+            self._v = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def push_back(self, values, threads: int = 1):
+        v = np.ascontiguousarray(values, dtype=np.uint32)
+        ok = np.zeros(len(v), dtype=np.uint8)
+        load().orc_vector_push_back(self._v, _p(v), len(v), _p(ok), threads)
+        return ok
+
+    def pop_back(self, n: int, threads: int = 1):
+        out = np.zeros(n, dtype=np.uint32)
+        ok = np.zeros(n, dtype=np.uint8)
+        load().orc_vector_pop_back(self._v, _p(out), n, _p(ok), threads)
+        return out, ok
+
+    def fill_iota(self, first: int = 0):
+        load().orc_vector_fill_iota(self._v, first)
+
+    def size(self) -> int:
+        return load().orc_vector_size(self._v)
+
+    def contents(self):
+        out = np.zeros(max(self.capacity, 1), dtype=np.uint32)
+        n = load().orc_vector_contents(self._v, _p(out))
+        return out[:n]
 
 
 class Deque:

@@ -13,5 +13,6 @@ from .containers import (  # noqa: F401
     unordered_set,
     unordered_set_u32,
     unordered_set_u64,
+    vector_u32,
 )
 from ._lib import HtabError, load  # noqa: F401

@@ -131,6 +131,13 @@ SIGNATURES = {
     "htab_deque_fill_iota": (_I32, [_P, _P]),
     "htab_deque_size": (_I32, [_P, ctypes.POINTER(_I64), _P]),
     "htab_deque_contents": (_I32, [_P, _P, ctypes.POINTER(_I64), _P]),
+    "htab_vector_create": (_I32, [_I64, ctypes.POINTER(_P)]),
+    "htab_vector_destroy": (_I32, [_P]),
+    "htab_vector_push_back": (_I32, [_P, _P, _I64, _P, _P]),
+    "htab_vector_pop_back": (_I32, [_P, _P, _I64, _P, _P]),
+    "htab_vector_fill_iota": (_I32, [_P, ctypes.c_uint32, _P]),
+    "htab_vector_size": (_I32, [_P, ctypes.POINTER(_I64), _P]),
+    "htab_vector_contents": (_I32, [_P, _P, ctypes.POINTER(_I64), _P]),
 }
 
 _lib = None

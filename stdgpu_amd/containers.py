@@ -292,3 +292,55 @@ class deque_u32:
         check(_lib.load().htab_deque_contents(self._h, _ptr(out), ctypes.byref(n), _stream_ptr(stream)),
               "htab_deque_contents")
         return out[: n.value]
+
+
+class vector_u32:
+    """vector<uint32_t> used as a stack (vector.cuh; the excess free list's container) with bulk
+    push_back / pop_back, one atomic per wavefront."""
+
+    def __init__(self, capacity: int):
+        h = ctypes.c_void_p()
+        check(_lib.load().htab_vector_create(int(capacity), ctypes.byref(h)), "htab_vector_create")
+        self._h = h
+        self.capacity = capacity
+
+    def destroy(self):
+        if self._h:
+            check(_lib.load().htab_vector_destroy(self._h), "htab_vector_destroy")
+            self._h = ctypes.c_void_p()
+
+    def fill_iota(self, first: int = 0, stream=None):
+        check(_lib.load().htab_vector_fill_iota(self._h, int(first), _stream_ptr(stream)), "htab_vector_fill_iota")
+
+    def push_back(self, values, pushed=None, stream=None):
+        check(
+            _lib.load().htab_vector_push_back(self._h, _ptr(values), values.numel(), _ptr(pushed), _stream_ptr(stream)),
+            "htab_vector_push_back",
+        )
+        return pushed
+
+    def pop_back(self, n: int, out=None, popped=None, stream=None):
+        torch = _torch()
+        if out is None:
+            out = torch.empty(n, dtype=torch.int32, device="cuda")
+        if popped is None:
+            popped = torch.empty(n, dtype=torch.uint8, device="cuda")
+        check(
+            _lib.load().htab_vector_pop_back(self._h, _ptr(out), int(n), _ptr(popped), _stream_ptr(stream)),
+            "htab_vector_pop_back",
+        )
+        return out, popped
+
+    def size(self, stream=None) -> int:
+        out = ctypes.c_int64()
+        check(_lib.load().htab_vector_size(self._h, ctypes.byref(out), _stream_ptr(stream)), "htab_vector_size")
+        return out.value
+
+    def contents(self, stream=None):
+        """elements from bottom to top (int32 device tensor of uint32 bits)"""
+        torch = _torch()
+        out = torch.empty(max(self.capacity, 1), dtype=torch.int32, device="cuda")
+        n = ctypes.c_int64()
+        check(_lib.load().htab_vector_contents(self._h, _ptr(out), ctypes.byref(n), _stream_ptr(stream)),
+              "htab_vector_contents")
+        return out[: n.value]

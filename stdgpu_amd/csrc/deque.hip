@@ -1,20 +1,17 @@
-// Free-list deque<uint32_t> with wavefront-aggregated push_back / pop_front.
+// Free-list deque<uint32_t> with bulk push_back / pop_front.
 //
 // Replaces the reference's per-element path (src/stdgpu/impl/deque_detail.cuh:202-260 push_back,
 // 390-448 pop_front: atomic size +-1, atomicInc of begin/end modulo capacity, then a spin on a
-// per-slot lock bit and an occupancy-bit flip).  Here a wave reserves room for all of its lanes
-// with one atomic on the size word and one on the begin/end counter, and lanes then read or write
-// their ring positions directly; no lock or occupancy bits are needed because one bulk launch only
-// pushes or only pops.  The ring position of counter value c is c mod capacity.
+// per-slot lock bit and an occupancy-bit flip).  One bulk call only pushes or only pops, so it
+// reserves all of its ring positions at once: a one-thread kernel reads the size and the end (push)
+// or begin (pop) counter, takes min(n, room) positions and advances both, then a streaming kernel
+// moves element i to or from ring position (counter + i) mod capacity.  No per-element atomic, lock
+// or occupancy bit remains, and the result is the one of a sequential loop of calls.
 #include <hip/hip_runtime.h>
 
-#include <string>
+#include <new>
 
 #include <stdgpu_amd/htab.h>
-
-#include "wave.hpp"
-
-using namespace htab_dev;
 
 namespace
 {
@@ -22,73 +19,46 @@ namespace
 constexpr int kSize = 0;   // int64 size word
 constexpr int kBegin = 16; // uint64 front counter (128 B apart)
 constexpr int kEnd = 32;   // uint64 back counter
+constexpr int kResBase = 48;  // reservation of the current bulk call: first counter value
+constexpr int kResCount = 49; // ... and the number of elements it covers
 constexpr size_t kDequeCtlBytes = 512;
 
-int
-dq_fail(int code, const char* what)
+// the clamped size (deque_detail.cuh:482-512), then the call's reservation
+__global__ void
+k_deque_reserve(int64_t* ctl, int64_t cap, int64_t n, int push)
 {
-    (void)what;
-    return code;
+    int64_t s = ctl[kSize];
+    s = s < 0 ? 0 : (s > cap ? cap : s);
+    const int64_t room = push ? cap - s : s;
+    const int64_t k = n < room ? n : room;
+    int64_t* counter = ctl + (push ? kEnd : kBegin);
+    ctl[kResBase] = *counter;
+    ctl[kResCount] = k;
+    *counter += k;
+    ctl[kSize] = push ? s + k : s - k;
 }
 
 __global__ __launch_bounds__(256) void
-k_deque_push(uint32_t* __restrict__ data, int64_t* ctl, int64_t cap, const uint32_t* __restrict__ v, int64_t n,
-             uint8_t* __restrict__ pushed)
+k_deque_push(uint32_t* __restrict__ data, const int64_t* __restrict__ ctl, int64_t cap, const uint32_t* __restrict__ v,
+             int64_t n, uint8_t* __restrict__ pushed)
 {
-    const int lane = lane_id();
     const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    const bool valid = i < n;
-    const uint64_t bal = ballot(valid);
-    const int64_t k = __builtin_popcountll(bal);
-    const int64_t rank = __builtin_popcountll(bal & ((1ULL << lane) - 1));
-    int64_t room = 0;
-    uint64_t pos0 = 0;
-    if (k > 0 && lane == 0)
-    {
-        const int64_t old = __hip_atomic_fetch_add(ctl + kSize, k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        room = cap - old;
-        room = room < 0 ? 0 : (room > k ? k : room);
-        if (room < k) __hip_atomic_fetch_add(ctl + kSize, -(k - room), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (room > 0)
-            pos0 = __hip_atomic_fetch_add(reinterpret_cast<uint64_t*>(ctl + kEnd), static_cast<uint64_t>(room),
-                                          __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    room = static_cast<int64_t>(shfl64(static_cast<uint64_t>(room), 0));
-    pos0 = shfl64(pos0, 0);
-    const bool ok = valid && rank < room;
-    if (ok) data[(pos0 + static_cast<uint64_t>(rank)) % static_cast<uint64_t>(cap)] = v[i];
-    if (valid && pushed) pushed[i] = ok ? 1 : 0;
+    if (i >= n) return;
+    const bool ok = i < ctl[kResCount];
+    if (ok) data[(static_cast<uint64_t>(ctl[kResBase]) + static_cast<uint64_t>(i)) % static_cast<uint64_t>(cap)] = v[i];
+    if (pushed) pushed[i] = ok ? 1 : 0;
 }
 
 __global__ __launch_bounds__(256) void
-k_deque_pop(const uint32_t* __restrict__ data, int64_t* ctl, int64_t cap, uint32_t* __restrict__ out, int64_t n,
-            uint8_t* __restrict__ popped)
+k_deque_pop(const uint32_t* __restrict__ data, const int64_t* __restrict__ ctl, int64_t cap, uint32_t* __restrict__ out,
+            int64_t n, uint8_t* __restrict__ popped)
 {
-    const int lane = lane_id();
     const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    const bool valid = i < n;
-    const uint64_t bal = ballot(valid);
-    const int64_t k = __builtin_popcountll(bal);
-    const int64_t rank = __builtin_popcountll(bal & ((1ULL << lane) - 1));
-    int64_t avail = 0;
-    uint64_t pos0 = 0;
-    if (k > 0 && lane == 0)
-    {
-        const int64_t old = __hip_atomic_fetch_add(ctl + kSize, -k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        avail = old < 0 ? 0 : (old > k ? k : old);
-        if (avail < k) __hip_atomic_fetch_add(ctl + kSize, k - avail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (avail > 0)
-            pos0 = __hip_atomic_fetch_add(reinterpret_cast<uint64_t*>(ctl + kBegin), static_cast<uint64_t>(avail),
-                                          __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    avail = static_cast<int64_t>(shfl64(static_cast<uint64_t>(avail), 0));
-    pos0 = shfl64(pos0, 0);
-    const bool ok = valid && rank < avail;
-    if (valid)
-    {
-        out[i] = ok ? data[(pos0 + static_cast<uint64_t>(rank)) % static_cast<uint64_t>(cap)] : 0xFFFFFFFFU;
-        if (popped) popped[i] = ok ? 1 : 0;
-    }
+    if (i >= n) return;
+    const bool ok = i < ctl[kResCount];
+    out[i] = ok ? data[(static_cast<uint64_t>(ctl[kResBase]) + static_cast<uint64_t>(i)) % static_cast<uint64_t>(cap)]
+                : 0xFFFFFFFFU;
+    if (popped) popped[i] = ok ? 1 : 0;
 }
 
 __global__ __launch_bounds__(256) void
@@ -133,7 +103,7 @@ extern "C" {
 int
 htab_deque_create(int64_t capacity, htab_deque** out)
 {
-    if (!out || capacity <= 0 || capacity > (int64_t(1) << 32)) return dq_fail(HTAB_E_ARG, "deque create");
+    if (!out || capacity <= 0 || capacity > (int64_t(1) << 32)) return HTAB_E_ARG;
     *out = nullptr;
     htab_deque* q = new (std::nothrow) htab_deque();
     if (!q) return HTAB_E_OOM;
@@ -172,8 +142,10 @@ htab_deque_push_back(htab_deque* q, const uint32_t* values, int64_t n, uint8_t* 
     if (n == 0) return HTAB_OK;
     const int64_t grid = (n + 255) / 256;
     if (grid > 0x7FFFFFFF) return HTAB_E_LIMIT;
-    hipLaunchKernelGGL(k_deque_push, dim3(static_cast<uint32_t>(grid)), dim3(256), 0, static_cast<hipStream_t>(stream),
-                       q->data, q->ctl, q->capacity, values, n, pushed);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    hipLaunchKernelGGL(k_deque_reserve, dim3(1), dim3(1), 0, s, q->ctl, q->capacity, n, 1);
+    hipLaunchKernelGGL(k_deque_push, dim3(static_cast<uint32_t>(grid)), dim3(256), 0, s, q->data, q->ctl, q->capacity,
+                       values, n, pushed);
     return dq_status();
 }
 
@@ -184,8 +156,10 @@ htab_deque_pop_front(htab_deque* q, uint32_t* out, int64_t n, uint8_t* popped, v
     if (n == 0) return HTAB_OK;
     const int64_t grid = (n + 255) / 256;
     if (grid > 0x7FFFFFFF) return HTAB_E_LIMIT;
-    hipLaunchKernelGGL(k_deque_pop, dim3(static_cast<uint32_t>(grid)), dim3(256), 0, static_cast<hipStream_t>(stream),
-                       q->data, q->ctl, q->capacity, out, n, popped);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    hipLaunchKernelGGL(k_deque_reserve, dim3(1), dim3(1), 0, s, q->ctl, q->capacity, n, 0);
+    hipLaunchKernelGGL(k_deque_pop, dim3(static_cast<uint32_t>(grid)), dim3(256), 0, s, q->data, q->ctl, q->capacity,
+                       out, n, popped);
     return dq_status();
 }
 

@@ -250,6 +250,35 @@ def test_deque_multiset_conservation():
     assert sorted(np.concatenate([got[ok == 1], out2]).tolist()) == list(range(cap))
 
 
+def test_vector_stack_semantics():
+    """vector<uint32_t> as the excess stack (vector_detail.cuh:181-302, 361-368): LIFO order single-
+    threaded, refusal when full / empty, clamped size, iota fill, multiset conservation with threads."""
+    cap = 1 << 12
+    v = po.Vector(cap)
+    assert v.size() == 0
+    out, ok = v.pop_back(3)
+    assert ok.tolist() == [0, 0, 0] and out.tolist() == [0xFFFFFFFF] * 3  # empty
+    assert v.push_back(np.arange(10, dtype=np.uint32)).all()
+    assert v.contents().tolist() == list(range(10))
+    out, ok = v.pop_back(4)
+    assert ok.all() and out.tolist() == [9, 8, 7, 6]  # last in, first out
+    assert v.size() == 6
+    pushed = v.push_back(np.arange(100, 100 + cap, dtype=np.uint32))
+    assert pushed.sum() == cap - 6 and v.size() == cap  # full: the rest is refused
+    assert v.push_back(np.array([5], np.uint32)).tolist() == [0]
+    v.fill_iota(1000)
+    assert v.size() == cap and v.contents().tolist() == list(range(1000, 1000 + cap))
+    # threads: the popped values and the rest are the whole set, each once
+    out, ok = v.pop_back(1500, threads=4)
+    assert ok.all()
+    rest = v.contents()
+    assert sorted(np.concatenate([out, rest]).tolist()) == list(range(1000, 1000 + cap))
+    assert rest.tolist() == list(range(1000, 1000 + cap - 1500))  # the bottom stays in place
+    assert v.push_back(out, threads=4).all()
+    assert sorted(v.contents().tolist()) == list(range(1000, 1000 + cap))
+    v.close()
+
+
 # ---------------------------------------------------------------------------------------------
 # committed golden fixtures (tests/golden/make_golden.py)
 import json  # noqa: E402
