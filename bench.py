@@ -559,6 +559,18 @@ def main():
             "avg_launch_ms": round(dom_ms, 4),
         },
     }
+    # snapshot / restore of the table the step leaves (host memory over PCIe; not part of `value`)
+    t0 = time.perf_counter()
+    snap = tab.snapshot()
+    t1 = time.perf_counter()
+    restored = sa.unordered_map_u64_u32.restore(snap)
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    out["snapshot"] = {"bytes": int(snap.size), "snapshot_gbs": round(snap.size / (t1 - t0) / 1e9, 2),
+                       "restore_gbs": round(snap.size / (t2 - t1) / 1e9, 2),
+                       "restored_size_ok": restored.size() == tab.size()}
+    sa.unordered_map_u64_u32.destroyDeviceObject(restored)
+    del snap
     sa.unordered_map_u64_u32.destroyDeviceObject(tab)
     del keys, vals, queries, erase, reins, reins_v, hits, got, ins_flags, er_flags
     if args.zipf:

@@ -227,6 +227,36 @@ int htab_server_replay_plain(htab_server* server, const htab_segment* requests, 
                              int blocking, void* stream);
 
 /* ------------------------------------------------------------------------------------------
+ * Snapshot and restore of a table (no reference counterpart: restart / resume of resident tables).
+ * Byte layout, little-endian: htab_snapshot_header, then the slot area
+ * ((physical_buckets * slots per bucket + 2) * value_bytes bytes: buckets in order, the two side slots
+ * of the reserved key values last), then the overflow-count array (physical_buckets uint32, present
+ * when ovf_bytes > 0).  A restored table has the snapshot's geometry, contents and size.
+ * ------------------------------------------------------------------------------------------ */
+typedef struct htab_snapshot_header
+{
+    char magic[8];            /* "HTABSNP1" */
+    int32_t version;          /* 1 */
+    int32_t kind;
+    int32_t bucket_lanes;
+    int32_t block_size;
+    int32_t probe_batch;
+    uint32_t flags;           /* presence bits of the two reserved key values */
+    int64_t capacity;
+    int64_t physical_buckets;
+    int64_t size;
+    int64_t slot_bytes;
+    int64_t ovf_bytes;
+} htab_snapshot_header;
+
+/* bytes a snapshot of the table takes (header included) */
+int htab_snapshot_bytes(const htab* table, int64_t* bytes);
+/* writes the snapshot into host memory of at least htab_snapshot_bytes bytes; waits for the copies */
+int htab_snapshot(htab* table, void* host_buffer, int64_t bytes, void* stream);
+/* a new table from a snapshot in host memory (HTAB_E_ARG for a buffer that is not a whole snapshot) */
+int htab_restore(const void* host_buffer, int64_t bytes, htab** out);
+
+/* ------------------------------------------------------------------------------------------
  * vector<uint32_t> used as a stack, the container of the reference's excess free list
  * (src/stdgpu/vector.cuh; push_back / pop_back vector_detail.cuh:181-302, clear_iota :361-368).
  * One bulk call only pushes or only pops; positions are claimed with one atomic per wavefront.

@@ -74,6 +74,23 @@ class htab_server_stats(ctypes.Structure):
     ]
 
 
+class htab_snapshot_header(ctypes.Structure):
+    _fields_ = [
+        ("magic", ctypes.c_char * 8),
+        ("version", ctypes.c_int32),
+        ("kind", ctypes.c_int32),
+        ("bucket_lanes", ctypes.c_int32),
+        ("block_size", ctypes.c_int32),
+        ("probe_batch", ctypes.c_int32),
+        ("flags", ctypes.c_uint32),
+        ("capacity", ctypes.c_int64),
+        ("physical_buckets", ctypes.c_int64),
+        ("size", ctypes.c_int64),
+        ("slot_bytes", ctypes.c_int64),
+        ("ovf_bytes", ctypes.c_int64),
+    ]
+
+
 class htab_view(ctypes.Structure):
     _fields_ = [
         ("slots", ctypes.c_void_p),
@@ -131,6 +148,9 @@ SIGNATURES = {
     "htab_deque_fill_iota": (_I32, [_P, _P]),
     "htab_deque_size": (_I32, [_P, ctypes.POINTER(_I64), _P]),
     "htab_deque_contents": (_I32, [_P, _P, ctypes.POINTER(_I64), _P]),
+    "htab_snapshot_bytes": (_I32, [_P, ctypes.POINTER(_I64)]),
+    "htab_snapshot": (_I32, [_P, _P, _I64, _P]),
+    "htab_restore": (_I32, [_P, _I64, ctypes.POINTER(_P)]),
     "htab_vector_create": (_I32, [_I64, ctypes.POINTER(_P)]),
     "htab_vector_destroy": (_I32, [_P]),
     "htab_vector_push_back": (_I32, [_P, _P, _I64, _P, _P]),

@@ -87,6 +87,31 @@ class _unordered_base:
             check(_lib.load().htab_destroy(obj._h), "htab_destroy")
             obj._h = ctypes.c_void_p()
 
+    # ---- snapshot / restore (no reference counterpart) ----
+    def snapshot(self, stream=None):
+        """The table as bytes (numpy uint8, layout of htab_snapshot_header in htab.h): save it anywhere."""
+        import numpy as np
+
+        n = ctypes.c_int64()
+        check(_lib.load().htab_snapshot_bytes(self._h, ctypes.byref(n)), "htab_snapshot_bytes")
+        buf = np.empty(n.value, dtype=np.uint8)
+        check(_lib.load().htab_snapshot(self._h, buf.ctypes.data_as(ctypes.c_void_p), n.value, _stream_ptr(stream)),
+              "htab_snapshot")
+        return buf
+
+    @classmethod
+    def restore(cls, buf):
+        """A new table from snapshot() bytes (same geometry, contents and size)."""
+        import numpy as np
+
+        b = np.ascontiguousarray(buf, dtype=np.uint8)
+        hd = _lib.htab_snapshot_header.from_buffer_copy(b[: ctypes.sizeof(_lib.htab_snapshot_header)].tobytes())
+        if hd.kind != _KINDS[(cls.key, cls.mapped)]:
+            raise TypeError("snapshot holds a different key/value kind")
+        h = ctypes.c_void_p()
+        check(_lib.load().htab_restore(b.ctypes.data_as(ctypes.c_void_p), b.size, ctypes.byref(h)), "htab_restore")
+        return cls(h.value, hd.capacity)
+
     # ---- dtype helpers ----
     @classmethod
     def key_dtype(cls):

@@ -128,33 +128,15 @@ htab_last_error(void)
     return htab_host::last_error();
 }
 
-int
-htab_create(int kind, int64_t capacity, const htab_tuning* tuning, htab** out)
+// A table of a given geometry: allocation, view, sizes, and every slot EMPTY.  htab_create derives
+// nbuckets from the capacity; htab_restore takes it from a snapshot.
+static int
+build_table(int kind, int64_t capacity, int lanes, int block, int rb, int64_t nbuckets, htab** out)
 {
-    if (!out) return fail(HTAB_E_ARG, "htab_create: out is null");
-    *out = nullptr;
     const int vb = value_bytes_of(kind);
-    if (vb == 0) return fail(HTAB_E_ARG, "htab_create: unknown kind");
-    if (capacity <= 0) return fail(HTAB_E_ARG, "htab_create: capacity must be positive");
-    if (capacity > (int64_t(1) << 34)) return fail(HTAB_E_LIMIT, "htab_create: capacity above 2^34");
-    int lanes = 8, block = 256, rb = 0;
-    float spc = 1.0F;
-    if (tuning)
-    {
-        if (tuning->bucket_lanes) lanes = tuning->bucket_lanes;
-        if (tuning->block_size) block = tuning->block_size;
-        if (tuning->slots_per_capacity > 0.0F) spc = tuning->slots_per_capacity;
-        rb = tuning->probe_batch;
-    }
-    if (lanes != 1 && lanes != 4 && lanes != 8 && lanes != 16)
-        return fail(HTAB_E_ARG, "htab_create: bucket_lanes must be 1, 4, 8 or 16");
-    if (rb != 0 && ((rb != 2 && rb != 4 && rb != 8) || rb > lanes))
-        return fail(HTAB_E_ARG, "htab_create: probe_batch must be 2, 4 or 8 and at most bucket_lanes");
-    if (block != 256 && block != 512 && block != 1024) return fail(HTAB_E_ARG, "htab_create: block_size must be 256, 512 or 1024");
-    if (!(spc >= 0.25F && spc <= 16.0F)) return fail(HTAB_E_ARG, "htab_create: slots_per_capacity out of [0.25, 16]");
-
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(HTAB_E_NODEV, "htab_create: no GPU");
+    if (nbuckets < 1 || nbuckets >= (int64_t(1) << 32)) return fail(HTAB_E_LIMIT, "htab_create: too many buckets");
 
     htab* h = new (std::nothrow) htab();
     if (!h) return fail(HTAB_E_OOM, "htab_create: host allocation");
@@ -165,13 +147,6 @@ htab_create(int kind, int64_t capacity, const htab_tuning* tuning, htab** out)
     h->capacity = capacity;
 
     const int64_t W = 16LL * lanes / vb;
-    const double want = std::ceil(static_cast<double>(capacity) * static_cast<double>(spc));
-    int64_t nbuckets = static_cast<int64_t>(std::ceil(want / static_cast<double>(W)));
-    if (nbuckets < 1) nbuckets = 1;
-    // room for two entries at least, as the reference's capacity-1 table (one bucket head plus one
-    // excess entry); only one-slot buckets need the second bucket for it
-    if (nbuckets * W < 2) nbuckets = (2 + W - 1) / W;
-    if (nbuckets >= (int64_t(1) << 32)) { delete h; return fail(HTAB_E_LIMIT, "htab_create: too many buckets"); }
     const int64_t nslots = nbuckets * W;
     int64_t ref_bc = 0, ref_max = 0;
     reference_sizing(capacity, &ref_bc, &ref_max);
@@ -233,6 +208,146 @@ htab_create(int kind, int64_t capacity, const htab_tuning* tuning, htab** out)
         (void)hipFree(h->mem);
         delete h;
         return rc;
+    }
+    *out = h;
+    return HTAB_OK;
+}
+
+static int
+check_geometry(int kind, int64_t capacity, int lanes, int block, int rb, const char* who)
+{
+    (void)who;
+    if (value_bytes_of(kind) == 0) return fail(HTAB_E_ARG, "htab_create: unknown kind");
+    if (capacity <= 0) return fail(HTAB_E_ARG, "htab_create: capacity must be positive");
+    if (capacity > (int64_t(1) << 34)) return fail(HTAB_E_LIMIT, "htab_create: capacity above 2^34");
+    if (lanes != 1 && lanes != 4 && lanes != 8 && lanes != 16)
+        return fail(HTAB_E_ARG, "htab_create: bucket_lanes must be 1, 4, 8 or 16");
+    if (rb != 0 && ((rb != 2 && rb != 4 && rb != 8) || rb > lanes))
+        return fail(HTAB_E_ARG, "htab_create: probe_batch must be 2, 4 or 8 and at most bucket_lanes");
+    if (block != 256 && block != 512 && block != 1024)
+        return fail(HTAB_E_ARG, "htab_create: block_size must be 256, 512 or 1024");
+    return HTAB_OK;
+}
+
+int
+htab_create(int kind, int64_t capacity, const htab_tuning* tuning, htab** out)
+{
+    if (!out) return fail(HTAB_E_ARG, "htab_create: out is null");
+    *out = nullptr;
+    int lanes = 8, block = 256, rb = 0;
+    float spc = 1.0F;
+    if (tuning)
+    {
+        if (tuning->bucket_lanes) lanes = tuning->bucket_lanes;
+        if (tuning->block_size) block = tuning->block_size;
+        if (tuning->slots_per_capacity > 0.0F) spc = tuning->slots_per_capacity;
+        rb = tuning->probe_batch;
+    }
+    int rc = check_geometry(kind, capacity, lanes, block, rb, "htab_create");
+    if (rc != HTAB_OK) return rc;
+    if (!(spc >= 0.25F && spc <= 16.0F)) return fail(HTAB_E_ARG, "htab_create: slots_per_capacity out of [0.25, 16]");
+
+    const int64_t W = 16LL * lanes / value_bytes_of(kind);
+    const double want = std::ceil(static_cast<double>(capacity) * static_cast<double>(spc));
+    int64_t nbuckets = static_cast<int64_t>(std::ceil(want / static_cast<double>(W)));
+    if (nbuckets < 1) nbuckets = 1;
+    // room for two entries at least, as the reference's capacity-1 table (one bucket head plus one
+    // excess entry); only one-slot buckets need the second bucket for it
+    if (nbuckets * W < 2) nbuckets = (2 + W - 1) / W;
+    return build_table(kind, capacity, lanes, block, rb, nbuckets, out);
+}
+
+// ---------------------------------------------------------------- snapshot / restore
+static int64_t
+slot_area_bytes(const htab* h)
+{
+    return (h->info.physical_slots + 2) * h->info.value_bytes;
+}
+
+static int64_t
+ovf_array_bytes(const htab* h)
+{
+    return h->view.ovf ? h->info.physical_buckets * static_cast<int64_t>(sizeof(uint32_t)) : 0;
+}
+
+int
+htab_snapshot_bytes(const htab* h, int64_t* bytes)
+{
+    if (!h || !bytes) return fail(HTAB_E_ARG, "htab_snapshot_bytes: null argument");
+    *bytes = static_cast<int64_t>(sizeof(htab_snapshot_header)) + slot_area_bytes(h) + ovf_array_bytes(h);
+    return HTAB_OK;
+}
+
+int
+htab_snapshot(htab* h, void* host_buffer, int64_t bytes, void* stream)
+{
+    if (!h || !host_buffer) return fail(HTAB_E_ARG, "htab_snapshot: null argument");
+    int64_t need = 0;
+    (void)htab_snapshot_bytes(h, &need);
+    if (bytes < need) return fail(HTAB_E_ARG, "htab_snapshot: buffer too small");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    htab_snapshot_header hd{};
+    std::memcpy(hd.magic, "HTABSNP1", 8);
+    hd.version = 1;
+    hd.kind = h->kind;
+    hd.bucket_lanes = h->lanes;
+    hd.block_size = h->block;
+    hd.probe_batch = h->rb;
+    hd.capacity = h->capacity;
+    hd.physical_buckets = h->info.physical_buckets;
+    hd.slot_bytes = slot_area_bytes(h);
+    hd.ovf_bytes = ovf_array_bytes(h);
+    int rc = htab_size(h, &hd.size, stream);
+    if (rc != HTAB_OK) return rc;
+    char* out = static_cast<char*>(host_buffer);
+    HTAB_HIP_TRY(hipMemcpyAsync(&hd.flags, h->view.flags, sizeof(uint32_t), hipMemcpyDeviceToHost, s), "htab_snapshot flags");
+    HTAB_HIP_TRY(hipMemcpyAsync(out + sizeof(hd), h->view.slots, static_cast<size_t>(hd.slot_bytes),
+                                hipMemcpyDeviceToHost, s),
+                 "htab_snapshot slots");
+    if (hd.ovf_bytes)
+        HTAB_HIP_TRY(hipMemcpyAsync(out + sizeof(hd) + hd.slot_bytes, h->view.ovf, static_cast<size_t>(hd.ovf_bytes),
+                                    hipMemcpyDeviceToHost, s),
+                     "htab_snapshot counts");
+    HTAB_HIP_TRY(hipStreamSynchronize(s), "htab_snapshot sync");
+    std::memcpy(out, &hd, sizeof(hd));
+    return HTAB_OK;
+}
+
+int
+htab_restore(const void* host_buffer, int64_t bytes, htab** out)
+{
+    if (!host_buffer || !out) return fail(HTAB_E_ARG, "htab_restore: null argument");
+    *out = nullptr;
+    if (bytes < static_cast<int64_t>(sizeof(htab_snapshot_header))) return fail(HTAB_E_ARG, "htab_restore: truncated");
+    htab_snapshot_header hd{};
+    std::memcpy(&hd, host_buffer, sizeof(hd));
+    if (std::memcmp(hd.magic, "HTABSNP1", 8) != 0 || hd.version != 1)
+        return fail(HTAB_E_ARG, "htab_restore: not a table snapshot");
+    int rc = check_geometry(hd.kind, hd.capacity, hd.bucket_lanes, hd.block_size, hd.probe_batch, "htab_restore");
+    if (rc != HTAB_OK) return rc;
+    const int vb = value_bytes_of(hd.kind);
+    const int64_t W = 16LL * hd.bucket_lanes / vb;
+    if (hd.physical_buckets < 1 || hd.physical_buckets >= (int64_t(1) << 32) ||
+        hd.slot_bytes != (hd.physical_buckets * W + 2) * vb ||
+        hd.ovf_bytes != (ovf_in_bucket_of(hd.kind) ? 0 : hd.physical_buckets * static_cast<int64_t>(sizeof(uint32_t))) ||
+        hd.size < 0 || hd.size > hd.physical_buckets * W + 2 ||
+        bytes < static_cast<int64_t>(sizeof(hd)) + hd.slot_bytes + hd.ovf_bytes)
+        return fail(HTAB_E_ARG, "htab_restore: inconsistent snapshot");
+    htab* h = nullptr;
+    rc = build_table(hd.kind, hd.capacity, hd.bucket_lanes, hd.block_size, hd.probe_batch, hd.physical_buckets, &h);
+    if (rc != HTAB_OK) return rc;
+    const char* in = static_cast<const char*>(host_buffer);
+    hipError_t e = hipMemcpy(h->view.slots, in + sizeof(hd), static_cast<size_t>(hd.slot_bytes), hipMemcpyHostToDevice);
+    if (e == hipSuccess && hd.ovf_bytes)
+        e = hipMemcpy(h->view.ovf, in + sizeof(hd) + hd.slot_bytes, static_cast<size_t>(hd.ovf_bytes),
+                      hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(h->view.flags, &hd.flags, sizeof(uint32_t), hipMemcpyHostToDevice);
+    // the size goes into the first stripe (the others are zero after creation)
+    if (e == hipSuccess) e = hipMemcpy(h->view.stripes, &hd.size, sizeof(int64_t), hipMemcpyHostToDevice);
+    if (e != hipSuccess)
+    {
+        (void)htab_destroy(h);
+        return fail_hip(e, "htab_restore: copy");
     }
     *out = h;
     return HTAB_OK;

@@ -69,3 +69,33 @@ def test_device_workload_generator_matches_numpy():
     assert wd.check_against_numpy()
     assert wd.check_zipf_against_numpy(1 << 12)
     assert wd.check_against_numpy(ntables=40, log2_lo=5, log2_hi=11)
+
+
+def test_restore_rejects_what_is_not_a_snapshot():
+    # checked before any device work, so this runs without a GPU
+    import numpy as np
+
+    lib = _lib.load()
+    h = ctypes.c_void_p()
+    hs = ctypes.sizeof(_lib.htab_snapshot_header)
+    assert hs == 8 + 6 * 4 + 5 * 8
+    junk = np.zeros(4096, dtype=np.uint8)
+    assert lib.htab_restore(junk.ctypes.data_as(ctypes.c_void_p), junk.size, ctypes.byref(h)) == _lib.HTAB_E_ARG
+    assert lib.htab_restore(junk.ctypes.data_as(ctypes.c_void_p), 10, ctypes.byref(h)) == _lib.HTAB_E_ARG
+    hd = _lib.htab_snapshot_header(b"HTABSNP1", 1, _lib.HTAB_MAP_U64_U32, 8, 256, 0, 0, 1000, 125, 0,
+                                   (125 * 8 + 2) * 16, 0)
+    raw = np.frombuffer(bytes(hd), dtype=np.uint8)
+    # a header that promises more bytes than the buffer holds
+    assert lib.htab_restore(raw.ctypes.data_as(ctypes.c_void_p), raw.size, ctypes.byref(h)) == _lib.HTAB_E_ARG
+    assert b"inconsistent" in lib.htab_last_error()
+    # inconsistent slot bytes for the geometry
+    hd.slot_bytes += 16
+    buf = np.zeros(hs + hd.slot_bytes, dtype=np.uint8)
+    buf[:hs] = np.frombuffer(bytes(hd), dtype=np.uint8)
+    assert lib.htab_restore(buf.ctypes.data_as(ctypes.c_void_p), buf.size, ctypes.byref(h)) == _lib.HTAB_E_ARG
+    # wrong magic
+    hd.slot_bytes -= 16
+    hd.magic = b"HTABSNPX"
+    buf[:hs] = np.frombuffer(bytes(hd), dtype=np.uint8)
+    assert lib.htab_restore(buf.ctypes.data_as(ctypes.c_void_p), buf.size, ctypes.byref(h)) == _lib.HTAB_E_ARG
+    assert b"not a table snapshot" in lib.htab_last_error()
