@@ -437,7 +437,7 @@ def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (the hot path "
-              f"shards by table or by hash range, see DESIGN.md).", file=sys.stderr)
+              f"shards by table or by hash range; one GPU runs one shard at its real shape).", file=sys.stderr)
         sys.exit(2)
 
     import torch

@@ -55,7 +55,7 @@ struct pair
 template <typename T>
 struct hash
 {
-    static_assert(std::is_integral_v<T>, "stdgpu::hash: this build hashes integral keys");
+    static_assert(std::is_integral_v<T>, "stdgpu::hash: the MI355X library hashes integral keys");
     constexpr __host__ __device__ std::size_t operator()(const T& key) const { return static_cast<std::size_t>(key); }
 };
 

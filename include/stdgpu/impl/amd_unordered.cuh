@@ -64,7 +64,7 @@ public:
     using const_iterator = const_pointer;
 
     static_assert(std::is_integral_v<Key> && (sizeof(Key) == 4 || sizeof(Key) == 8),
-                  "stdgpu (MI355X build): keys are 4- or 8-byte integers");
+                  "stdgpu (MI355X library): keys are 4- or 8-byte integers");
     static constexpr bool mapped_ok()
     {
         if constexpr (std::is_void_v<Mapped>)
@@ -73,9 +73,9 @@ public:
             return std::is_trivially_copyable_v<Mapped> && (sizeof(Mapped) == 4 || sizeof(Mapped) == 8) &&
                    !(sizeof(Key) == 4 && sizeof(Mapped) == 8);
     }
-    static_assert(mapped_ok(), "stdgpu (MI355X build): mapped types are trivially copyable, 4 or 8 bytes "
+    static_assert(mapped_ok(), "stdgpu (MI355X library): mapped types are trivially copyable, 4 or 8 bytes "
                                "(uint32->uint64 maps are not provided)");
-    static_assert(std::is_same_v<Hash, hash<Key>>, "stdgpu (MI355X build): keys use stdgpu::hash<Key>");
+    static_assert(std::is_same_v<Hash, hash<Key>>, "stdgpu (MI355X library): keys use stdgpu::hash<Key>");
 
     using skey = storage_uint_t<sizeof(Key)>;
     using smapped = typename storage_mapped<Mapped>::type;

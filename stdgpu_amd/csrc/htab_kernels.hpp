@@ -95,14 +95,14 @@ struct chunk
         return static_cast<uint32_t>(hi >> 32);
     }
 
-    // bitmask over the chunk's slots j >= jmin_local for which pred(key) holds
+    // bitmask over the chunk's slots j for which pred(key) holds
     template <typename Pred>
-    __device__ uint32_t mask(Pred pred, int jmin_local = 0) const
+    __device__ uint32_t mask(Pred pred) const
     {
         uint32_t m = 0;
 #pragma unroll
         for (int j = 0; j < SPL; ++j)
-            if (j >= jmin_local && pred(key(j))) m |= 1U << j;
+            if (pred(key(j))) m |= 1U << j;
         return m;
     }
 };
@@ -1302,14 +1302,6 @@ k_range_scatter(table_ref<K, M> t, uint64_t total_slots, const int64_t* __restri
         }
         ++pos;
     }
-}
-
-// a template so every translation unit may instantiate it (no duplicate symbols)
-template <int Unused = 0>
-__global__ void
-k_zero_u64(unsigned long long* p, int n)
-{
-    if (static_cast<int>(threadIdx.x) < n) p[threadIdx.x] = 0;
 }
 
 // ================================================================== valid()

@@ -1,7 +1,7 @@
 """Writes the golden fixtures under tests/golden/ (run: python tests/golden/make_golden.py).
 
 * reference_kats.json: values printed by the compiled reference (stdgpu 1.3.0, OpenMP backend) for
-  sizing and bucket(), recorded in the survey of the reference; they are data, not generated here.
+  sizing and bucket(), recorded from a run of it; they are data, not generated here.
 * small_workloads.npz: seeded inputs and the oracle's outputs (sorted contents, hit vectors, values,
   per-element flags, sizes) for small instances of the set<u32> and map<u64,u32> workloads, so the
   GPU tests can be checked against fixed vectors as well as against a live oracle.

@@ -186,7 +186,7 @@ def test_segments_coalesce_requests_like_plain_calls():
     tabs = [sa.unordered_map_u64_u32.createDeviceObject(c) for c in caps]
     plain = [sa.unordered_map_u64_u32.createDeviceObject(c) for c in caps]
     batch = TableBatch(tabs)
-    # round 1: two insert requests per table (overlapping keys); round 2: finds; round 3: erases
+    # phase 1: two insert requests per table (overlapping keys); phase 2: finds; phase 3: erases
     for rnd, op in enumerate(["insert", "find", "erase", "find"]):
         segs, expect = [], []
         for t in range(ntab):

@@ -1,8 +1,8 @@
 """Pins the CPU oracle (oracle/htab_oracle.cpp) before it is trusted as the parity checker.
 
 Each check is chosen so that a plausible slip in the restatement fails it:
-  * sizing and bucket() values printed by the compiled reference (OpenMP backend) in the
-    survey session, and an independent Python restatement of the closed forms;
+  * sizing and bucket() values printed by a run of the compiled reference (OpenMP backend),
+    and an independent Python restatement of the closed forms;
   * the reference tests' pinned colliding / non-colliding key pairs
     (tests/stdgpu/unordered_datastructure.inc:899-934, vec_hash unordered_map.inc:181-206);
   * brute-force set semantics against Python sets, single- and multi-threaded;
@@ -46,7 +46,7 @@ def py_sizing(capacity: int):
     return bc, int(ec.to_integral_value(rounding="ROUND_CEILING")) + 1
 
 
-# values printed by the compiled reference (stdgpu OpenMP backend) in the survey session
+# values printed by a run of the compiled reference (stdgpu OpenMP backend)
 REFERENCE_KATS = [
     # kind, capacity, bucket_count, max_size, {key: bucket}
     (("u32", None), 1 << 21, 2097152, 3422806, {1: 1296111, 2: 495070, 3: 1791181, 0xFFFFFFFE: 547697}),
