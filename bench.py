@@ -408,7 +408,8 @@ def bench_free_lists(args) -> dict:
     vals = torch.arange(n, dtype=torch.int32, device="cuda")
     out = torch.empty(n, dtype=torch.int32, device="cuda")
     flags = torch.empty(n, dtype=torch.uint8, device="cuda")
-    res = {"workload": "2^24 uint32 values pushed, then popped, per container", "ops_per_phase": n}
+    res = {"workload": "2^24 uint32 values pushed, then popped, per container; bitset of 2^30 bits: "
+                       "set, then flip, of 2^24 random positions", "ops_per_phase": n}
     for name, cls, pop in (("vector", sa.vector_u32, "pop_back"), ("deque", sa.deque_u32, "pop_front")):
         c = cls(n)
         names = ["push", "pop"]
@@ -430,6 +431,22 @@ def bench_free_lists(args) -> dict:
                      "pop_gbs": round(9 * n / (np.mean(per["pop"]) / 1e3) / 1e9, 1),
                      "all_pushed_and_popped": ok}
         c.destroy()
+    bs = sa.bitset(1 << 30)
+    pos = torch.randint(0, 1 << 30, (n,), dtype=torch.int64, device="cuda")
+
+    def bstep(evs):
+        ev = evs[0] if evs else None
+        rec = (lambda j: ev[j].record(evs[1])) if ev else (lambda j: None)
+        rec(0)
+        bs.update("set", pos)
+        rec(1)
+        bs.update("flip", pos)
+        rec(2)
+
+    per = _timed_sequence(bstep, ["set", "flip"], steps=5, warmup=1)
+    res["bitset"] = {"set_gops": round(n / (np.mean(per["set"]) / 1e3) / 1e9, 3),
+                     "flip_gops": round(n / (np.mean(per["flip"]) / 1e3) / 1e9, 3)}
+    bs.destroy()
     return res
 
 

@@ -277,6 +277,27 @@ int htab_vector_size(htab_vector* vec, int64_t* out, void* stream);
 /* the elements from bottom to top into out (device memory, capacity entries); *n_out = size */
 int htab_vector_contents(htab_vector* vec, uint32_t* out, int64_t* n_out, void* stream);
 
+/* ------------------------------------------------------------------------------------------
+ * bitset over 32-bit blocks (src/stdgpu/bitset.cuh).  Bulk forms of the per-bit device calls:
+ * set / reset / flip of bit positions[i] return its previous value in old_out[i]
+ * (bitset_detail.cuh:47-112, 265-329; positions outside [0, size) are skipped, in_range[i] = 0);
+ * set() / reset() / flip() of every bit (:247-319); count() of the set bits below size (:131-160).
+ * all() = count == size, any() = count > 0, none() = count == 0.
+ * ------------------------------------------------------------------------------------------ */
+typedef struct htab_bitset htab_bitset;
+
+#define HTAB_BIT_SET 0
+#define HTAB_BIT_RESET 1
+#define HTAB_BIT_FLIP 2
+
+int htab_bitset_create(int64_t nbits, htab_bitset** out);
+int htab_bitset_destroy(htab_bitset* bits);
+int htab_bitset_update(htab_bitset* bits, int op, const int64_t* positions, int64_t n, uint8_t* old_out,
+                       uint8_t* in_range, void* stream);
+int htab_bitset_test(htab_bitset* bits, const int64_t* positions, int64_t n, uint8_t* out, void* stream);
+int htab_bitset_all_bits(htab_bitset* bits, int op, void* stream);
+int htab_bitset_count(htab_bitset* bits, int64_t* out, void* stream);
+
 /* Message for the last non-zero status returned to the calling thread. */
 const char* htab_last_error(void);
 

@@ -5,6 +5,7 @@ this package is its host-side mirror for Python callers, tests and the benchmark
 """
 from .containers import (  # noqa: F401
     Tuning,
+    bitset,
     deque_u32,
     unordered_map,
     unordered_map_u32_u32,

@@ -24,6 +24,10 @@ HTAB_MAP_U64_U32 = 2
 HTAB_MAP_U32_U32 = 3
 HTAB_MAP_U64_U64 = 4
 
+HTAB_BIT_SET = 0
+HTAB_BIT_RESET = 1
+HTAB_BIT_FLIP = 2
+
 HTAB_OP_INSERT = 0
 HTAB_OP_FIND = 1
 HTAB_OP_ERASE = 2
@@ -151,6 +155,12 @@ SIGNATURES = {
     "htab_snapshot_bytes": (_I32, [_P, ctypes.POINTER(_I64)]),
     "htab_snapshot": (_I32, [_P, _P, _I64, _P]),
     "htab_restore": (_I32, [_P, _I64, ctypes.POINTER(_P)]),
+    "htab_bitset_create": (_I32, [_I64, ctypes.POINTER(_P)]),
+    "htab_bitset_destroy": (_I32, [_P]),
+    "htab_bitset_update": (_I32, [_P, _I32, _P, _I64, _P, _P, _P]),
+    "htab_bitset_test": (_I32, [_P, _P, _I64, _P, _P]),
+    "htab_bitset_all_bits": (_I32, [_P, _I32, _P]),
+    "htab_bitset_count": (_I32, [_P, ctypes.POINTER(_I64), _P]),
     "htab_vector_create": (_I32, [_I64, ctypes.POINTER(_P)]),
     "htab_vector_destroy": (_I32, [_P]),
     "htab_vector_push_back": (_I32, [_P, _P, _I64, _P, _P]),

@@ -369,3 +369,72 @@ class vector_u32:
         check(_lib.load().htab_vector_contents(self._h, _ptr(out), ctypes.byref(n), _stream_ptr(stream)),
               "htab_vector_contents")
         return out[: n.value]
+
+
+class bitset:
+    """bitset over 32-bit blocks (bitset.cuh): bulk set / reset / flip / test of bit positions, and
+    set() / reset() / flip() / count() / all() / any() / none() of the whole set."""
+
+    _OPS = {"set": _lib.HTAB_BIT_SET, "reset": _lib.HTAB_BIT_RESET, "flip": _lib.HTAB_BIT_FLIP}
+
+    def __init__(self, nbits: int):
+        h = ctypes.c_void_p()
+        check(_lib.load().htab_bitset_create(int(nbits), ctypes.byref(h)), "htab_bitset_create")
+        self._h = h
+        self.nbits = nbits
+
+    @classmethod
+    def createDeviceObject(cls, size: int):
+        return cls(size)
+
+    @staticmethod
+    def destroyDeviceObject(obj) -> None:
+        obj.destroy()
+
+    def destroy(self):
+        if self._h:
+            check(_lib.load().htab_bitset_destroy(self._h), "htab_bitset_destroy")
+            self._h = ctypes.c_void_p()
+
+    def size(self) -> int:
+        return self.nbits
+
+    def update(self, op: str, positions, stream=None):
+        """op in set / reset / flip for each position (int64 device tensor); returns the previous bits."""
+        torch = _torch()
+        old = torch.empty(positions.numel(), dtype=torch.uint8, device=positions.device)
+        check(_lib.load().htab_bitset_update(self._h, self._OPS[op], _ptr(positions), positions.numel(), _ptr(old),
+                                             None, _stream_ptr(stream)), "htab_bitset_update")
+        return old
+
+    def test(self, positions, stream=None):
+        torch = _torch()
+        out = torch.empty(positions.numel(), dtype=torch.uint8, device=positions.device)
+        check(_lib.load().htab_bitset_test(self._h, _ptr(positions), positions.numel(), _ptr(out),
+                                           _stream_ptr(stream)), "htab_bitset_test")
+        return out
+
+    def set(self, stream=None):
+        check(_lib.load().htab_bitset_all_bits(self._h, _lib.HTAB_BIT_SET, _stream_ptr(stream)), "htab_bitset_all_bits")
+
+    def reset(self, stream=None):
+        check(_lib.load().htab_bitset_all_bits(self._h, _lib.HTAB_BIT_RESET, _stream_ptr(stream)),
+              "htab_bitset_all_bits")
+
+    def flip(self, stream=None):
+        check(_lib.load().htab_bitset_all_bits(self._h, _lib.HTAB_BIT_FLIP, _stream_ptr(stream)),
+              "htab_bitset_all_bits")
+
+    def count(self, stream=None) -> int:
+        out = ctypes.c_int64()
+        check(_lib.load().htab_bitset_count(self._h, ctypes.byref(out), _stream_ptr(stream)), "htab_bitset_count")
+        return out.value
+
+    def all(self) -> bool:
+        return self.count() == self.nbits
+
+    def any(self) -> bool:
+        return self.count() > 0
+
+    def none(self) -> bool:
+        return self.count() == 0
