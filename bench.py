@@ -409,7 +409,8 @@ def bench_free_lists(args) -> dict:
     out = torch.empty(n, dtype=torch.int32, device="cuda")
     flags = torch.empty(n, dtype=torch.uint8, device="cuda")
     res = {"workload": "2^24 uint32 values pushed, then popped, per container; bitset of 2^30 bits: "
-                       "set, then flip, of 2^24 random positions", "ops_per_phase": n}
+                       "set, then flip, of 2^24 random positions; 2^24 atomic fetch_add onto one "
+                       "unsigned int, then onto 2^20 at random", "ops_per_phase": n}
     for name, cls, pop in (("vector", sa.vector_u32, "pop_back"), ("deque", sa.deque_u32, "pop_front")):
         c = cls(n)
         names = ["push", "pop"]
@@ -447,6 +448,29 @@ def bench_free_lists(args) -> dict:
     res["bitset"] = {"set_gops": round(n / (np.mean(per["set"]) / 1e3) / 1e9, 3),
                      "flip_gops": round(n / (np.mean(per["flip"]) / 1e3) / 1e9, 3)}
     bs.destroy()
+    # atomic<unsigned int>: 2^24 fetch_add onto one counter (every wavefront names the one value: one
+    # hardware atomic per wavefront), and onto 2^20 counters at random
+    one = sa.atomic("unsigned int", 1)
+    many = sa.atomic("unsigned int", 1 << 20)
+    ones = torch.ones(n, dtype=torch.int32, device="cuda")
+    tg = torch.randint(0, 1 << 20, (n,), dtype=torch.int64, device="cuda")
+
+    def astep(evs):
+        ev = evs[0] if evs else None
+        rec = (lambda j: ev[j].record(evs[1])) if ev else (lambda j: None)
+        rec(0)
+        one.apply("fetch_add", ones)
+        rec(1)
+        many.apply("fetch_add", ones, tg)
+        rec(2)
+
+    per = _timed_sequence(astep, ["one", "random"], steps=5, warmup=1)
+    total = one.load()
+    res["atomic_u32"] = {"one_counter_gops": round(n / (np.mean(per["one"]) / 1e3) / 1e9, 3),
+                         "random_2e20_gops": round(n / (np.mean(per["random"]) / 1e3) / 1e9, 3),
+                         "one_counter_total_ok": total == (n * 6) % (1 << 32)}
+    one.destroy()
+    many.destroy()
     return res
 
 

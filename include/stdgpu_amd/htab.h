@@ -298,13 +298,62 @@ int htab_bitset_test(htab_bitset* bits, const int64_t* positions, int64_t n, uin
 int htab_bitset_all_bits(htab_bitset* bits, int op, void* stream);
 int htab_bitset_count(htab_bitset* bits, int64_t* out, void* stream);
 
+/* ------------------------------------------------------------------------------------------
+ * atomic<T> / atomic_ref<T> (src/stdgpu/atomic.cuh) for T = int, unsigned int,
+ * unsigned long long int, float (atomic.cuh:137-139).  A bulk call applies n operations, each as the
+ * device call of the same name would (atomic_detail.cuh:428-760; semantics of the OpenMP backend,
+ * openmp/impl/atomic_detail.h:40-217): operation i applies operands[i] to element targets[i]
+ * (element 0 for every i when targets is NULL) and writes the value it replaced into old_out[i]
+ * (may be NULL; nothing is written for STORE).  done_out[i] (may be NULL) is 1 when target i is in
+ * [0, count) and, for COMPARE_EXCHANGE, when the value equalled expected[i] and was replaced; out of
+ * range targets are skipped.  The n operations take effect in some order, as n concurrent device calls
+ * do.  FETCH_INC_MOD / FETCH_DEC_MOD take the modulus arg of fetch_inc_mod(arg) / fetch_dec_mod(arg)
+ * (unsigned int only, atomic_detail.cuh:679-702); AND / OR / XOR exist for the integer types only
+ * (HTAB_E_ARG otherwise).
+ * ------------------------------------------------------------------------------------------ */
+typedef struct htab_atomic htab_atomic;
+
+#define HTAB_ATOMIC_I32 0
+#define HTAB_ATOMIC_U32 1
+#define HTAB_ATOMIC_U64 2
+#define HTAB_ATOMIC_F32 3
+
+#define HTAB_ATOMIC_LOAD 0
+#define HTAB_ATOMIC_STORE 1
+#define HTAB_ATOMIC_EXCHANGE 2
+#define HTAB_ATOMIC_COMPARE_EXCHANGE 3
+#define HTAB_ATOMIC_FETCH_ADD 4
+#define HTAB_ATOMIC_FETCH_SUB 5
+#define HTAB_ATOMIC_FETCH_MIN 6
+#define HTAB_ATOMIC_FETCH_MAX 7
+#define HTAB_ATOMIC_FETCH_AND 8
+#define HTAB_ATOMIC_FETCH_OR 9
+#define HTAB_ATOMIC_FETCH_XOR 10
+#define HTAB_ATOMIC_FETCH_INC_MOD 11
+#define HTAB_ATOMIC_FETCH_DEC_MOD 12
+
+/* count device values of the given type, all 0 (createDeviceObject, atomic_detail.cuh:143-151) */
+int htab_atomic_create(int type, int64_t count, htab_atomic** out);
+int htab_atomic_destroy(htab_atomic* a);
+/* the device address of element 0, for device code that takes the values by atomic_ref */
+int htab_atomic_data(const htab_atomic* a, void** out);
+int htab_atomic_apply(htab_atomic* a, int op, const int64_t* targets, const void* operands, const void* expected,
+                      int64_t n, void* old_out, uint8_t* done_out, void* stream);
+/* the same over count values of the given type at data (device memory): atomic_ref<T> */
+int htab_atomic_ref_apply(int type, void* data, int64_t count, int op, const int64_t* targets, const void* operands,
+                          const void* expected, int64_t n, void* old_out, uint8_t* done_out, void* stream);
+/* host-side load() / store() of element index (atomic_detail.cuh:461-526); value in host or device memory */
+int htab_atomic_load(const htab_atomic* a, int64_t index, void* value_out, void* stream);
+int htab_atomic_store(htab_atomic* a, int64_t index, const void* value, void* stream);
+
 /* Message for the last non-zero status returned to the calling thread. */
 const char* htab_last_error(void);
 
 /* ------------------------------------------------------------------------------------------
  * Free-list deque<uint32_t> (deque.cuh; deque_detail.cuh:202-260 push_back, 390-448 pop_front),
- * used by voxel-block pipelines to hand out slot numbers.  Pushes and pops are aggregated to one
- * atomic per wavefront.  Bulk calls only; size follows the reference's clamped size().
+ * used by voxel-block pipelines to hand out slot numbers.  A bulk call only pushes or only pops and
+ * reserves its positions once, so its result is that of a loop of single calls.  Bulk calls only;
+ * size follows the reference's clamped size().
  * ------------------------------------------------------------------------------------------ */
 typedef struct htab_deque htab_deque;
 

@@ -241,3 +241,98 @@ class Deque:
         out = np.zeros(max(self.capacity, 1), dtype=np.uint32)
         n = load().orc_deque_contents(self._d, _p(out))
         return out[:n]
+
+
+# ------------------------------------------------------------------ atomic<T>
+# Sequential statement of the reference's atomic operations: the OpenMP backend writes each as a
+# critical section over the value (src/stdgpu/openmp/impl/atomic_detail.h:40-217), and the public
+# fetch_inc_mod(arg) / fetch_dec_mod(arg) pass arg - 1 as the backend's limit
+# (src/stdgpu/impl/atomic_detail.cuh:679-702).  Pure Python on small inputs.
+
+ATOMIC_BITS = {"int": 32, "unsigned int": 32, "unsigned long long int": 64}
+
+
+def _wrap(type_: str, v):
+    if type_ == "float":
+        return float(np.float32(v))
+    bits = ATOMIC_BITS[type_]
+    v &= (1 << bits) - 1
+    if type_ == "int" and v >= 1 << 31:
+        v -= 1 << 32
+    return v
+
+
+def atomic_step(type_: str, op: str, state, arg=0, expected=0):
+    """(old, new state, replaced) of one operation on a value holding state."""
+    old = state
+    if op == "load":
+        return old, state, True
+    if op in ("store", "exchange"):
+        return old, _wrap(type_, arg), True
+    if op == "compare_exchange":  # atomic_detail.h:71-81
+        hit = old == expected
+        return old, (_wrap(type_, arg) if hit else old), hit
+    if type_ == "float":
+        f = np.float32
+        new = {"fetch_add": lambda: f(f(old) + f(arg)), "fetch_sub": lambda: f(f(old) - f(arg)),
+               "fetch_min": lambda: f(arg) if f(arg) < f(old) else f(old),  # std::min(old, arg)
+               "fetch_max": lambda: f(arg) if f(old) < f(arg) else f(old)}[op]()
+        return old, float(new), True
+    if op == "fetch_add":
+        new = old + arg
+    elif op == "fetch_sub":
+        new = old - arg
+    elif op == "fetch_min":
+        new = arg if arg < old else old
+    elif op == "fetch_max":
+        new = arg if old < arg else old
+    elif op == "fetch_and":
+        new = old & arg
+    elif op == "fetch_or":
+        new = old | arg
+    elif op == "fetch_xor":
+        new = old ^ arg
+    elif op == "fetch_inc_mod":  # atomic_detail.h:195-204 with limit = arg - 1
+        limit = _wrap(type_, arg - 1)
+        new = 0 if old >= limit else old + 1
+    elif op == "fetch_dec_mod":  # atomic_detail.h:208-217 with limit = arg - 1
+        limit = _wrap(type_, arg - 1)
+        new = limit if (old == 0 or old > limit) else old - 1
+    else:
+        raise ValueError(op)
+    return old, _wrap(type_, new), True
+
+
+def atomic_sequence(type_: str, op: str, initial, operands, expected=None):
+    """final value and the old values of the operations applied one after another, in index order"""
+    s = initial
+    olds = []
+    for i, a in enumerate(operands):
+        old, s, _ = atomic_step(type_, op, s, a, 0 if expected is None else expected[i])
+        olds.append(old)
+    return s, olds
+
+
+def atomic_linearize(type_: str, op: str, initial, operands, olds, expected=None):
+    """The final value of some one-at-a-time order of the operations on one value in which operation i
+    sees olds[i]; raises AssertionError when no order is found.  Greedy: from the current state, take
+    an operation that saw it, preferring one that leaves the state unchanged.  Exact when at most one
+    operation that changes the state sees each state (distinct addends, distinct exchange operands,
+    monotone min / max / and / or), or when the operations that see one state are equal."""
+    by_old = {}
+    for i, o in enumerate(olds):
+        by_old.setdefault(o, []).append(i)
+    s = initial
+    for _ in range(len(olds)):
+        cand = by_old.get(s)
+        assert cand, f"no operation saw the value {s}"
+        pick = None
+        for j in range(len(cand) - 1, -1, -1):
+            i = cand[j]
+            _, ns, _ = atomic_step(type_, op, s, operands[i], 0 if expected is None else expected[i])
+            if ns == s:
+                pick = j
+                break
+        i = cand.pop(len(cand) - 1 if pick is None else pick)
+        _, s, _ = atomic_step(type_, op, s, operands[i], 0 if expected is None else expected[i])
+    return s

@@ -5,6 +5,8 @@ this package is its host-side mirror for Python callers, tests and the benchmark
 """
 from .containers import (  # noqa: F401
     Tuning,
+    atomic,
+    atomic_ref_apply,
     bitset,
     deque_u32,
     unordered_map,

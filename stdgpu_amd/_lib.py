@@ -28,6 +28,15 @@ HTAB_BIT_SET = 0
 HTAB_BIT_RESET = 1
 HTAB_BIT_FLIP = 2
 
+HTAB_ATOMIC_I32 = 0
+HTAB_ATOMIC_U32 = 1
+HTAB_ATOMIC_U64 = 2
+HTAB_ATOMIC_F32 = 3
+HTAB_ATOMIC_OPS = {
+    "load": 0, "store": 1, "exchange": 2, "compare_exchange": 3, "fetch_add": 4, "fetch_sub": 5, "fetch_min": 6,
+    "fetch_max": 7, "fetch_and": 8, "fetch_or": 9, "fetch_xor": 10, "fetch_inc_mod": 11, "fetch_dec_mod": 12,
+}
+
 HTAB_OP_INSERT = 0
 HTAB_OP_FIND = 1
 HTAB_OP_ERASE = 2
@@ -161,6 +170,13 @@ SIGNATURES = {
     "htab_bitset_test": (_I32, [_P, _P, _I64, _P, _P]),
     "htab_bitset_all_bits": (_I32, [_P, _I32, _P]),
     "htab_bitset_count": (_I32, [_P, ctypes.POINTER(_I64), _P]),
+    "htab_atomic_create": (_I32, [_I32, _I64, ctypes.POINTER(_P)]),
+    "htab_atomic_destroy": (_I32, [_P]),
+    "htab_atomic_data": (_I32, [_P, ctypes.POINTER(_P)]),
+    "htab_atomic_apply": (_I32, [_P, _I32, _P, _P, _P, _I64, _P, _P, _P]),
+    "htab_atomic_ref_apply": (_I32, [_I32, _P, _I64, _I32, _P, _P, _P, _I64, _P, _P, _P]),
+    "htab_atomic_load": (_I32, [_P, _I64, _P, _P]),
+    "htab_atomic_store": (_I32, [_P, _I64, _P, _P]),
     "htab_vector_create": (_I32, [_I64, ctypes.POINTER(_P)]),
     "htab_vector_destroy": (_I32, [_P]),
     "htab_vector_push_back": (_I32, [_P, _P, _I64, _P, _P]),

@@ -321,7 +321,8 @@ class deque_u32:
 
 class vector_u32:
     """vector<uint32_t> used as a stack (vector.cuh; the excess free list's container) with bulk
-    push_back / pop_back, one atomic per wavefront."""
+    push_back / pop_back; a bulk call reserves its positions once and returns what a loop of single
+    calls would."""
 
     def __init__(self, capacity: int):
         h = ctypes.c_void_p()
@@ -438,3 +439,133 @@ class bitset:
 
     def none(self) -> bool:
         return self.count() == 0
+
+
+_ATOMIC_TYPES = {
+    "int": _lib.HTAB_ATOMIC_I32, "i32": _lib.HTAB_ATOMIC_I32,
+    "unsigned int": _lib.HTAB_ATOMIC_U32, "u32": _lib.HTAB_ATOMIC_U32,
+    "unsigned long long int": _lib.HTAB_ATOMIC_U64, "u64": _lib.HTAB_ATOMIC_U64,
+    "float": _lib.HTAB_ATOMIC_F32, "f32": _lib.HTAB_ATOMIC_F32,
+}
+_ATOMIC_CTYPES = {_lib.HTAB_ATOMIC_I32: ctypes.c_int32, _lib.HTAB_ATOMIC_U32: ctypes.c_uint32,
+                  _lib.HTAB_ATOMIC_U64: ctypes.c_uint64, _lib.HTAB_ATOMIC_F32: ctypes.c_float}
+
+
+def _atomic_torch_dtype(code: int):
+    """device tensors of the values: int32 (also for unsigned int bits), int64 (unsigned long long bits), float32"""
+    torch = _torch()
+    return {_lib.HTAB_ATOMIC_I32: torch.int32, _lib.HTAB_ATOMIC_U32: torch.int32,
+            _lib.HTAB_ATOMIC_U64: torch.int64, _lib.HTAB_ATOMIC_F32: torch.float32}[code]
+
+
+def _atomic_bulk(call, name, code, op, operands, targets, expected, stream):
+    torch = _torch()
+    opc = _lib.HTAB_ATOMIC_OPS[op]
+    ref = operands if operands is not None else targets
+    n = ref.numel() if ref is not None else 0
+    dev = ref.device if ref is not None else "cuda"
+    old = None if op == "store" else torch.empty(n, dtype=_atomic_torch_dtype(code), device=dev)
+    done = torch.empty(n, dtype=torch.uint8, device=dev)
+    check(call(opc, _ptr(targets), _ptr(operands), _ptr(expected), n, _ptr(old), _ptr(done), _stream_ptr(stream)),
+          name)
+    return old, done
+
+
+def atomic_ref_apply(type: str, values, op: str, operands=None, targets=None, expected=None, stream=None):
+    """atomic_ref<T> over the elements of a device tensor: the bulk form of op (see atomic.apply)."""
+    code = _ATOMIC_TYPES[type]
+    lib = _lib.load()
+    return _atomic_bulk(lambda *a: lib.htab_atomic_ref_apply(code, _ptr(values), values.numel(), *a),
+                        "htab_atomic_ref_apply", code, op, operands, targets, expected, stream)
+
+
+class atomic:
+    """atomic<T> (atomic.cuh) for T in int / unsigned int / unsigned long long int / float: count device
+    values (count = 1 is one atomic<T>), bulk device operations, host load() / store().
+
+    apply(op, operands, targets, expected) applies operation i to element targets[i] (element 0 when
+    targets is None) and returns (old, done): the values replaced, and 1 per operation that applied
+    (target in range; for compare_exchange, the value equalled expected[i] and was replaced).  Operands
+    are device tensors of the value type's bits (int32 for int and unsigned int, int64 for unsigned
+    long long, float32 for float)."""
+
+    def __init__(self, type: str = "unsigned int", count: int = 1):
+        self.type_code = _ATOMIC_TYPES[type]
+        h = ctypes.c_void_p()
+        check(_lib.load().htab_atomic_create(self.type_code, int(count), ctypes.byref(h)), "htab_atomic_create")
+        self._h = h
+        self.count = count
+
+    @classmethod
+    def createDeviceObject(cls, type: str = "unsigned int", count: int = 1):
+        return cls(type, count)
+
+    @staticmethod
+    def destroyDeviceObject(obj) -> None:
+        obj.destroy()
+
+    def destroy(self):
+        if self._h:
+            check(_lib.load().htab_atomic_destroy(self._h), "htab_atomic_destroy")
+            self._h = ctypes.c_void_p()
+
+    @property
+    def dtype(self):
+        return _atomic_torch_dtype(self.type_code)
+
+    def data_ptr(self) -> int:
+        p = ctypes.c_void_p()
+        check(_lib.load().htab_atomic_data(self._h, ctypes.byref(p)), "htab_atomic_data")
+        return int(p.value)
+
+    def load(self, index: int = 0, stream=None):
+        v = _ATOMIC_CTYPES[self.type_code]()
+        check(_lib.load().htab_atomic_load(self._h, int(index), ctypes.byref(v), _stream_ptr(stream)),
+              "htab_atomic_load")
+        return v.value
+
+    def store(self, value, index: int = 0, stream=None):
+        v = _ATOMIC_CTYPES[self.type_code](value)
+        check(_lib.load().htab_atomic_store(self._h, int(index), ctypes.byref(v), _stream_ptr(stream)),
+              "htab_atomic_store")
+
+    def apply(self, op: str, operands=None, targets=None, expected=None, stream=None):
+        lib = _lib.load()
+        return _atomic_bulk(lambda *a: lib.htab_atomic_apply(self._h, *a), "htab_atomic_apply", self.type_code, op,
+                            operands, targets, expected, stream)
+
+    def bulk_load(self, targets, stream=None):
+        return self.apply("load", None, targets, stream=stream)[0]
+
+    def exchange(self, operands, targets=None, stream=None):
+        return self.apply("exchange", operands, targets, stream=stream)
+
+    def compare_exchange(self, expected, desired, targets=None, stream=None):
+        return self.apply("compare_exchange", desired, targets, expected, stream=stream)
+
+    def fetch_add(self, operands, targets=None, stream=None):
+        return self.apply("fetch_add", operands, targets, stream=stream)
+
+    def fetch_sub(self, operands, targets=None, stream=None):
+        return self.apply("fetch_sub", operands, targets, stream=stream)
+
+    def fetch_min(self, operands, targets=None, stream=None):
+        return self.apply("fetch_min", operands, targets, stream=stream)
+
+    def fetch_max(self, operands, targets=None, stream=None):
+        return self.apply("fetch_max", operands, targets, stream=stream)
+
+    def fetch_and(self, operands, targets=None, stream=None):
+        return self.apply("fetch_and", operands, targets, stream=stream)
+
+    def fetch_or(self, operands, targets=None, stream=None):
+        return self.apply("fetch_or", operands, targets, stream=stream)
+
+    def fetch_xor(self, operands, targets=None, stream=None):
+        return self.apply("fetch_xor", operands, targets, stream=stream)
+
+    def fetch_inc_mod(self, operands, targets=None, stream=None):
+        return self.apply("fetch_inc_mod", operands, targets, stream=stream)
+
+    def fetch_dec_mod(self, operands, targets=None, stream=None):
+        return self.apply("fetch_dec_mod", operands, targets, stream=stream)

@@ -1,5 +1,5 @@
-"""Runs the C++ source-compatibility tests of the stdgpu facade (tests/cpp/test_facade.hip, built
-by `python -m stdgpu_amd.build`) on the GPU: user kernels call the native device API, host code the
+"""Runs the C++ source-compatibility tests of the stdgpu facade (tests/cpp/test_facade.hip and
+test_atomic.hip, built by `python -m stdgpu_amd.build`) on the GPU: user kernels call the native device API, host code the
 agnostic API, as in the reference's own test suite."""
 import os
 import subprocess
@@ -9,12 +9,14 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-EXE = os.path.join(ROOT, "tests", "cpp", "_build", "test_facade")
+BUILD = os.path.join(ROOT, "tests", "cpp", "_build")
 
 
-def test_cpp_facade_cases():
-    assert os.path.exists(EXE), "build the C++ tests first: python -m stdgpu_amd.build"
-    r = subprocess.run([EXE], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=180)
+@pytest.mark.parametrize("name", ["test_facade", "test_atomic"])
+def test_cpp_facade_cases(name):
+    exe = os.path.join(BUILD, name)
+    assert os.path.exists(exe), "build the C++ tests first: python -m stdgpu_amd.build"
+    r = subprocess.run([exe], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=180)
     print(r.stdout)
     assert r.returncode == 0, r.stdout
     assert "OK: 0 failure(s)" in r.stdout

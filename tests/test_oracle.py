@@ -308,3 +308,97 @@ def test_golden_small_workloads_regenerate_identically():
     assert sorted(stored.files) == sorted(fresh)
     for k in stored.files:
         np.testing.assert_array_equal(stored[k], fresh[k], err_msg=k)
+
+
+# ------------------------------------------------------------------ atomic<T>
+# Known answers of the reference's own atomic tests (tests/stdgpu/atomic.inc): for_each_index over
+# N = 40000 with sequence = 1..N (iota from 1), and N = 50000 with modulus N / 10 = 5000 for the
+# wrapping increment / decrement, each from the stated start value.
+
+ATOMIC_INT_TYPES = ["int", "unsigned int", "unsigned long long int"]
+
+
+@pytest.mark.parametrize("t", ATOMIC_INT_TYPES)
+def test_atomic_sequence_known_answers(t):
+    n = 40000
+    seq = list(range(1, n + 1))
+    total = n * (n + 1) // 2
+    assert po.atomic_sequence(t, "fetch_add", 0, seq)[0] == total  # atomic.inc:850-866
+    assert po.atomic_sequence(t, "fetch_sub", total, seq)[0] == 0  # atomic.inc:1030-1051
+    top = (1 << 31) - 1 if t == "int" else (1 << po.ATOMIC_BITS[t]) - 1
+    assert po.atomic_sequence(t, "fetch_min", top, seq)[0] == 1  # atomic.inc:1197-1213
+    lowest = -(1 << 31) if t == "int" else 0
+    assert po.atomic_sequence(t, "fetch_max", lowest, seq)[0] == n  # atomic.inc:1324-1340
+    bits = po.ATOMIC_BITS[t]
+    pat = [po._wrap(t, 1 << i) for i in range(bits)]
+    ones = po._wrap(t, (1 << bits) - 1)
+    assert po.atomic_sequence(t, "fetch_or", 0, pat)[0] == ones  # one bit per thread (atomic.inc:1479-1495, 1871-1887)
+    assert po.atomic_sequence(t, "fetch_xor", 0, pat)[0] == ones
+    assert po.atomic_sequence(t, "fetch_and", ones, [po._wrap(t, ~(1 << i)) for i in range(bits)])[0] == 0
+    # the reference's compare-exchange loop adds the sequence (atomic.inc:584-703, 670-703)
+    s = 0
+    for a in seq[:2000]:
+        while True:
+            seen = s
+            _, s, hit = po.atomic_step(t, "compare_exchange", s, seen + a, seen)
+            if hit:
+                break
+    assert s == 2000 * 2001 // 2
+
+
+def test_atomic_wrapping_increment_decrement():
+    n, mod = 50000, 5000
+    assert po.atomic_sequence("unsigned int", "fetch_inc_mod", 42, [mod] * n)[0] == 42  # atomic.inc:1990-2007
+    assert po.atomic_sequence("unsigned int", "fetch_dec_mod", 42, [mod] * n)[0] == 42  # atomic.inc:2046-2063
+    # the backend's limit = arg - 1: inc wraps after limit to 0; dec wraps from 0 (or above limit) to limit
+    assert po.atomic_step("unsigned int", "fetch_inc_mod", 4999, 5000)[1] == 0
+    assert po.atomic_step("unsigned int", "fetch_inc_mod", 7000, 5000)[1] == 0
+    assert po.atomic_step("unsigned int", "fetch_inc_mod", 4998, 5000)[1] == 4999
+    assert po.atomic_step("unsigned int", "fetch_dec_mod", 0, 5000)[1] == 4999
+    assert po.atomic_step("unsigned int", "fetch_dec_mod", 7000, 5000)[1] == 4999
+    assert po.atomic_step("unsigned int", "fetch_dec_mod", 1, 5000)[1] == 0
+    # wrapping increment of every start value below the modulus is (s + 1) mod m; decrement (s - 1) mod m
+    for m in (1, 2, 7):
+        for s0 in range(m):
+            assert po.atomic_step("unsigned int", "fetch_inc_mod", s0, m)[1] == (s0 + 1) % m
+            assert po.atomic_step("unsigned int", "fetch_dec_mod", s0, m)[1] == (s0 - 1) % m
+
+
+def test_atomic_integer_wrap_and_float():
+    assert po.atomic_step("int", "fetch_add", (1 << 31) - 1, 1)[1] == -(1 << 31)
+    assert po.atomic_step("unsigned int", "fetch_sub", 0, 1)[1] == (1 << 32) - 1
+    assert po.atomic_step("unsigned long long int", "fetch_add", (1 << 64) - 1, 2)[1] == 1
+    assert po.atomic_step("float", "fetch_add", 1.0, 2.0 ** -30)[1] == 1.0  # float32 rounding
+    assert po.atomic_step("float", "fetch_min", 3.0, -1.5)[1] == -1.5
+    assert po.atomic_step("float", "compare_exchange", -0.0, 5.0, 0.0)[2]  # value comparison: -0 == +0
+    nan = float("nan")
+    assert not po.atomic_step("float", "compare_exchange", nan, 5.0, nan)[2]
+
+
+def test_atomic_linearize_accepts_orders_and_rejects_tampering():
+    rng = np.random.default_rng(3)
+    n = 3000
+    ops = [int(x) for x in rng.integers(1, 1000, size=n)]
+    for op, init in (("fetch_add", 0), ("fetch_max", 0), ("fetch_min", 1 << 20), ("exchange", -1)):
+        args = ops if op != "exchange" else list(range(n))
+        order = rng.permutation(n)
+        s, olds = init, [None] * n
+        for i in order:
+            olds[i], s, _ = po.atomic_step("int", op, s, args[i])
+        assert po.atomic_linearize("int", op, init, args, olds) == s
+    # an add history with two old values swapped (between ops with different addends) has no order
+    s, olds = 0, [None] * n
+    for i in range(n):
+        olds[i], s, _ = po.atomic_step("int", "fetch_add", s, ops[i])
+    j = next(k for k in range(1, n) if ops[k] != ops[0])
+    bad = list(olds)
+    bad[0], bad[j] = bad[j], bad[0]
+    with pytest.raises(AssertionError):
+        po.atomic_linearize("int", "fetch_add", 0, ops, bad)
+    # an exchange history where one operation saw a value nobody wrote
+    s, olds = -1, [None] * 100
+    for i in range(100):
+        olds[i], s, _ = po.atomic_step("int", "exchange", s, i)
+    olds[50] = 12345
+    with pytest.raises(AssertionError):
+        po.atomic_linearize("int", "exchange", -1, list(range(100)), olds)
