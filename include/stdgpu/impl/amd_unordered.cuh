@@ -6,9 +6,11 @@
 // libstdgpu_amd.so and complete before returning; native device calls run the per-thread
 // probing of include/stdgpu_amd/htab_device.hpp on the same layout.
 //
-// Supported: 4- or 8-byte integral keys with stdgpu::hash (identity), and (maps) trivially
-// copyable 4- or 8-byte mapped types.  The layout is not the reference's, so a found entry is
-// not at begin() + bucket(key); bucket() itself keeps the reference's value.
+// Supported: 4- or 8-byte integral keys compared with equal_to<Key> or the transparent equal_to<>,
+// any hasher callable with a Key, and (maps) trivially copyable 4- or 8-byte mapped types.  The layout
+// places keys by their bits (htab_device.hpp), so the hasher only feeds bucket(), which keeps the
+// reference's value for it; a found entry is not at begin() + bucket(key).  With a transparent hasher
+// and key_equal, find / contains / count also take an integral KeyLike (unordered_base.cuh:241-301).
 #ifndef STDGPU_AMD_FACADE_UNORDERED_CUH
 #define STDGPU_AMD_FACADE_UNORDERED_CUH
 
@@ -75,7 +77,23 @@ public:
     }
     static_assert(mapped_ok(), "stdgpu (MI355X library): mapped types are trivially copyable, 4 or 8 bytes "
                                "(uint32->uint64 maps are not provided)");
-    static_assert(std::is_same_v<Hash, hash<Key>>, "stdgpu (MI355X library): keys use stdgpu::hash<Key>");
+    static_assert(std::is_same_v<KeyEqual, equal_to<Key>> || std::is_same_v<KeyEqual, equal_to<void>>,
+                  "stdgpu (MI355X library): keys compare with stdgpu::equal_to<Key> or equal_to<>");
+    static_assert(std::is_invocable_r_v<std::size_t, const Hash&, const Key&>,
+                  "stdgpu (MI355X library): the hasher maps a Key to std::size_t");
+
+    template <typename H, typename = void>
+    struct is_transparent_t : std::false_type
+    {
+    };
+    template <typename H>
+    struct is_transparent_t<H, std::void_t<typename H::is_transparent>> : std::true_type
+    {
+    };
+    // the KeyLike overloads exist for a transparent hasher and key_equal (unordered_base.cuh:241-301)
+    template <typename KeyLike>
+    static constexpr bool keylike_ok = is_transparent_t<Hash>::value && is_transparent_t<KeyEqual>::value &&
+                                       std::is_integral_v<KeyLike> && !std::is_same_v<KeyLike, Key>;
 
     using skey = storage_uint_t<sizeof(Key)>;
     using smapped = typename storage_mapped<Mapped>::type;
@@ -118,6 +136,31 @@ public:
     }
     __device__ bool contains(const key_type& key) const { return find(key) != end(); }
     __device__ index_type count(const key_type& key) const { return contains(key) ? 1 : 0; }
+
+    // KeyLike lookups.  Integral conversion to the common type of Key and KeyLike is one-to-one on Key,
+    // so at most one key compares equal to k, and only Key(k) can: it is looked up when it does.
+    template <typename KeyLike, std::enable_if_t<keylike_ok<KeyLike>, int> = 0>
+    __device__ iterator find(const KeyLike& k)
+    {
+        const key_type c = static_cast<key_type>(k);
+        return KeyEqual{}(c, k) ? find(c) : end();
+    }
+    template <typename KeyLike, std::enable_if_t<keylike_ok<KeyLike>, int> = 0>
+    __device__ const_iterator find(const KeyLike& k) const
+    {
+        const key_type c = static_cast<key_type>(k);
+        return KeyEqual{}(c, k) ? find(c) : end();
+    }
+    template <typename KeyLike, std::enable_if_t<keylike_ok<KeyLike>, int> = 0>
+    __device__ bool contains(const KeyLike& k) const
+    {
+        return find(k) != end();
+    }
+    template <typename KeyLike, std::enable_if_t<keylike_ok<KeyLike>, int> = 0>
+    __device__ index_type count(const KeyLike& k) const
+    {
+        return contains(k) ? 1 : 0;
+    }
 
     __device__ pair<iterator, bool> insert(const value_type& value)
     {

@@ -160,6 +160,53 @@ k_device_size(Table t, int* out)
     if (threadIdx.x == 0 && blockIdx.x == 0) out[0] = t.size();
 }
 
+// the reference's transparent hasher (tests/stdgpu/unordered_map.inc:21-30): stdgpu::hash of the
+// type it is given
+struct int_hash
+{
+    using is_transparent = void;
+    template <typename T>
+    __host__ __device__ std::size_t operator()(const T& key) const
+    {
+        return stdgpu::hash<T>{}(key);
+    }
+};
+
+// transparent lookups against the regular ones (unordered_datastructure.inc:392-456, 509-640,
+// 2204-2300): out[i] = 1 found, 0 not found, 9 when the transparent and regular answers differ
+template <typename Table, typename KeyLike>
+__global__ void
+k_transparent(Table t, const KeyLike* q, std::size_t n, int* out)
+{
+    const std::size_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const Table& ct = t;
+    const KeyLike k = q[i];
+    using key_type = typename Table::key_type;
+    const key_type kk = static_cast<key_type>(k);
+    const bool equal = stdgpu::equal_to<>{}(kk, k);
+    const auto regular = equal ? t.find(kk) : t.end();
+    const auto it = t.find(k);
+    const auto cit = ct.find(k);
+    const bool c = ct.contains(k);
+    const auto cnt = ct.count(k);
+    const bool agree = it == regular && cit == regular && c == (regular != t.end()) && cnt == (c ? 1 : 0);
+    out[i] = agree ? (c ? 1 : 0) : 9;
+}
+
+template <typename Table, typename KeyLike>
+std::vector<int>
+run_transparent(Table& t, const std::vector<KeyLike>& q)
+{
+    KeyLike* dq = to_dev(q);
+    int* out = dev_alloc<int>(q.size());
+    hipLaunchKernelGGL((k_transparent<Table, KeyLike>), dim3(blocks(q.size())), dim3(256), 0, 0, t, dq, q.size(), out);
+    std::vector<int> h = to_host(out, q.size());
+    (void)hipFree(dq);
+    (void)hipFree(out);
+    return h;
+}
+
 // ------------------------------------------------------------------ helpers
 template <typename Table, typename Key>
 std::vector<int>
@@ -584,6 +631,58 @@ examples_closed_form()
     imap_t::destroyDeviceObject(m);
 }
 
+// a key matches a KeyLike exactly when key == keylike under C++'s usual arithmetic conversions
+template <typename Key, typename KeyLike>
+static std::vector<int>
+expected_matches(const std::vector<Key>& keys, const std::vector<KeyLike>& q)
+{
+    std::vector<int> out;
+    for (const KeyLike& x : q)
+    {
+        bool hit = false;
+        for (const Key& k : keys) hit = hit || stdgpu::equal_to<>{}(k, x);
+        out.push_back(hit ? 1 : 0);
+    }
+    return out;
+}
+
+static void
+transparent_keylike_lookups()
+{
+    g_case = "transparent KeyLike find / contains / count";
+    using tmap = stdgpu::unordered_map<int, float, int_hash, stdgpu::equal_to<>>;
+    tmap m = tmap::createDeviceObject(2000);
+    std::vector<int> keys;
+    for (int k = -500; k < 500; k += 3) keys.push_back(k);
+    CHECK(sum(run_insert(m, keys)) == static_cast<int>(keys.size()));
+    std::vector<short> qs;
+    for (int k = -600; k < 600; ++k) qs.push_back(static_cast<short>(k));
+    const std::vector<int> got_s = run_transparent(m, qs);
+    CHECK(got_s == expected_matches(keys, qs));
+    CHECK(sum(got_s) == static_cast<int>(keys.size())); // the count_sum case: every key is found once
+    // a wider KeyLike whose value does not fit Key matches nothing, even where its low bits would
+    std::vector<long long> ql = { 1, -2, (1LL << 32) + 1, -(1LL << 32) - 2, (1LL << 40) - 500, 4, -497 };
+    const std::vector<int> got_l = run_transparent(m, ql);
+    CHECK(got_l == expected_matches(keys, ql));
+    CHECK(got_l[2] == 0 && got_l[3] == 0);
+    tmap::destroyDeviceObject(m);
+
+    // unsigned keys: int -1 equals 4294967295u (converted to unsigned), long long -1 does not
+    using tset = stdgpu::unordered_set<std::uint32_t, int_hash, stdgpu::equal_to<>>;
+    tset s = tset::createDeviceObject(100);
+    const std::vector<std::uint32_t> sk = { 0U, 7U, 4294967295U, 4294967294U };
+    CHECK(sum(run_insert(s, sk)) == 4);
+    const std::vector<int> qi = { -1, -2, 0, 7, 8, -7 };
+    const std::vector<int> got_i = run_transparent(s, qi);
+    CHECK(got_i == expected_matches(sk, qi));
+    CHECK(got_i[0] == 1 && got_i[1] == 1);
+    const std::vector<long long> qll = { -1, 4294967295LL, 7, (1LL << 32) + 7 };
+    const std::vector<int> got_ll = run_transparent(s, qll);
+    CHECK(got_ll == expected_matches(sk, qll));
+    CHECK(got_ll[0] == 0 && got_ll[1] == 1 && got_ll[3] == 0);
+    tset::destroyDeviceObject(s);
+}
+
 int
 main()
 {
@@ -604,7 +703,8 @@ main()
                           set_u32_native_and_bulk,
                           collision_chain_native,
                           accessors_and_policies,
-                          examples_closed_form };
+                          examples_closed_form,
+                          transparent_keylike_lookups };
     for (auto c : cases)
     {
         const int before = g_failures;
