@@ -23,6 +23,8 @@
 
 #include <stdgpu_amd/htab_device.hpp>
 
+#include <type_traits>
+
 #include "wave.hpp"
 
 namespace htab_dev
@@ -488,6 +490,14 @@ insert_body(const table_ref<K, M>& t, const void* __restrict__ values, const voi
                 q[r] = C::load_plain(t.slots + hb[r] * W, li);
             }
         }
+        // Decide every element's claim first, then issue the R compare-and-swaps together so that
+        // they are in flight at once, then act on their results.  Byte r of flp (uniform in the group)
+        // is the lane that claims for element r (0xFF: none); that lane keeps the slot's index in the
+        // bucket in byte r of sjp and, in bit r of tombp, whether the free slot it saw held TOMB rather
+        // than EMPTY.  Packed so that R claims in flight cost a few registers, not 3R.
+        using pack_t = std::conditional_t<(R > 4), uint64_t, uint32_t>;
+        pack_t flp = ~pack_t(0), sjp = 0;
+        uint32_t tombp = 0;
 #pragma unroll
         for (int r = 0; r < R; ++r)
         {
@@ -521,35 +531,59 @@ insert_body(const table_ref<K, M>& t, const void* __restrict__ values, const voi
                                                       &fsel);
                     else
                         fl = ctz64(bf);
-                    // the value travels from the owner lane; the claiming lane stores it
-                    const uint64_t v = shfl64(myv, gbase + rb + r);
-                    uint64_t seen = 0;
-                    uint64_t slot = 0;
+                    flp = (flp & ~(pack_t(0xFF) << (8 * r))) | (static_cast<pack_t>(fl) << (8 * r));
                     if (lane == fl)
                     {
                         const int j = __builtin_ctz(fsel);
-                        const K obs = q[r].key(j);
-                        slot = hb[r] * W + static_cast<uint64_t>(li * SPL + j);
-                        const K got = cas_agent(t.key_ptr(slot), obs, k);
-                        seen = (got == obs) ? 1 : ((got == k) ? 2 : 0);
-                        if (seen == 1) store_mapped<K, M>(t.slots + slot, v);
-                    }
-                    seen = shfl64(seen, fl);
-                    slot = shfl64(slot, fl);
-                    if (seen == 1)
-                    {
-                        if (li == rb + r)
-                        {
-                            myins = true;
-                            myslot = slot;
-                        }
-                    }
-                    else if (seen == 0)
-                    {
-                        pend |= 1U << (rb + r);
+                        sjp |= static_cast<pack_t>(li * SPL + j) << (8 * r);
+                        if (q[r].key(j) == tomb_key<K>()) tombp |= 1U << r;
                     }
                 }
                 else
+                {
+                    pend |= 1U << (rb + r);
+                }
+            }
+        }
+        K got[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+        {
+            got[r] = 0;
+            const int fl = static_cast<int>((flp >> (8 * r)) & 0xFFU);
+            if (lane == fl)
+            {
+                const K o = ((tombp >> r) & 1U) ? tomb_key<K>() : empty_key<K>();
+                got[r] = cas_agent(t.key_ptr(hb[r] * W + static_cast<uint32_t>((sjp >> (8 * r)) & 0xFFU)), o, kr[r]);
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+        {
+            const int fl = static_cast<int>((flp >> (8 * r)) & 0xFFU);
+            if (fl != 0xFF)
+            {
+                // the value travels from the owner lane; the claiming lane stores it
+                const uint64_t v = shfl64(myv, gbase + rb + r);
+                const uint32_t j = static_cast<uint32_t>((sjp >> (8 * r)) & 0xFFU);
+                uint64_t seen = 0;
+                if (lane == fl)
+                {
+                    const K o = ((tombp >> r) & 1U) ? tomb_key<K>() : empty_key<K>();
+                    seen = (got[r] == o) ? 1 : ((got[r] == kr[r]) ? 2 : 0);
+                    if (seen == 1) store_mapped<K, M>(t.slots + hb[r] * W + j, v);
+                }
+                seen = shfl64(seen, fl);
+                if (seen == 1)
+                {
+                    const uint64_t slot = hb[r] * W + shfl32(j, fl);
+                    if (li == rb + r)
+                    {
+                        myins = true;
+                        myslot = slot;
+                    }
+                }
+                else if (seen == 0)
                 {
                     pend |= 1U << (rb + r);
                 }
