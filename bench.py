@@ -27,8 +27,8 @@ BYTES_PER_OP = {"insert": 16 + 64 + 16 + 1, "find": 8 + 1 + 4 + 64, "erase": 8 +
 # HBM bytes per launch measured with rocprofv3 on MI355X for the default workload (2^26 keys,
 # 8-slot buckets, 256-thread workgroups): FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000 KiB, the scales
 # that make both counters read a 2 GiB copy kernel's bytes exactly (profiles/map_u64_u32_2e26_bulk_kernels.md)
-MEASURED_TRAFFIC = {(26, 8, 256): {"insert": (2 * 5502112 + 6715875) * 1024, "find": (2 * 5391966 + 336554) * 1024,
-                                   "erase": (2 * 1272267 + 1146756) * 1024, "reinsert": (2 * 2724414 + 1659933) * 1024}}
+MEASURED_TRAFFIC = {(26, 8, 256): {"insert": (2 * 5466773 + 6707952) * 1024, "find": (2 * 5391950 + 336568) * 1024,
+                                   "erase": (2 * 1272259 + 1146748) * 1024, "reinsert": (2 * 2692081 + 1659669) * 1024}}
 
 
 def parse():
