@@ -27,6 +27,8 @@ BYTES_PER_OP = {"insert": 16 + 64 + 16 + 1, "find": 8 + 1 + 4 + 64, "erase": 8 +
 # HBM bytes per launch measured with rocprofv3 on MI355X for the default workload (2^26 keys,
 # 8-slot buckets, 256-thread workgroups): FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000 KiB, the scales
 # that make both counters read a 2 GiB copy kernel's bytes exactly (profiles/map_u64_u32_2e26_bulk_kernels.md)
+# 128 B lines a find reads per key (TCC misses per dispatch / keys), same profile and workload
+MEASURED_FIND_LINES = {(26, 8, 256): 89040914 / (1 << 26)}
 MEASURED_TRAFFIC = {(26, 8, 256): {"insert": (2 * 5466773 + 6707952) * 1024, "find": (2 * 5391950 + 336568) * 1024,
                                    "erase": (2 * 1272259 + 1146748) * 1024, "reinsert": (2 * 2692081 + 1659669) * 1024}}
 
@@ -285,6 +287,28 @@ def bench_voxel(args) -> dict:
     fl.destroy()
     sa.unordered_map_u64_u32.destroyDeviceObject(tab)
     return res
+
+
+def random_access_bound(args, tab, rates: dict) -> dict:
+    """The bound the table kernels actually meet: random 128 B line reads and random 64-bit CAS on a
+    buffer at least as large as the table, measured now on this GPU (stdgpu_amd/roofline.py).  insert
+    does one CAS per key; find reads MEASURED_FIND_LINES lines per key (rocprof TCC misses)."""
+    from stdgpu_amd import roofline
+
+    table_bytes = tab.info.physical_slots * 16
+    mb = 1024
+    while (mb << 20) < table_bytes:
+        mb *= 2
+    r = roofline.measure(sizes_mb=(mb,), lines=(128,), n=1 << 26)
+    gather = r["gather"][f"128B@{mb}MB"]["G_lines_per_s"]
+    cas = r["cas"][f"{mb}MB"]["G_ops_per_s"]
+    out = {"buffer_mb": mb, "random_line_reads_g_per_s": gather, "random_cas_g_per_s": cas,
+           "insert_keys_per_s_over_cas_rate": round(rates["insert"] / cas, 3)}
+    lines = MEASURED_FIND_LINES.get((args.log2n, args.lanes, args.block))
+    if lines is not None and args.probe_batch == 0:
+        out["find_lines_per_key"] = round(lines, 3)
+        out["find_line_rate_over_gather_rate"] = round(rates["find"] * lines / gather, 3)
+    return out
 
 
 def _timed_sequence(step, names, steps: int, warmup: int):
@@ -598,6 +622,7 @@ def main():
             if args.probe_batch == 0 else None,
             "bytes_per_op": BYTES_PER_OP[dom],
             "avg_launch_ms": round(dom_ms, 4),
+            "random_access": random_access_bound(args, tab, rates),
         },
     }
     # snapshot / restore of the table the step leaves (host memory over PCIe; not part of `value`)
