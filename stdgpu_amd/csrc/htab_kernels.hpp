@@ -371,6 +371,14 @@ fetch_element(const void* values, const void* keys, const void* vals, int64_t i,
 
 constexpr uint64_t kNoSlot = ~0ULL;
 
+// What the home bucket showed, kept by an element's own lane when its probe sequence runs on past the
+// home bucket, so that the walk starts at the next bucket instead of loading the home bucket again:
+// bits 0-7 the index in the bucket of its first free slot in the key's claim order, then flags.
+constexpr uint32_t kWalkSet = 1U << 8;     // the walk starts at the home bucket's successor
+constexpr uint32_t kWalkFree = 1U << 9;    // the home bucket has a free slot (index in bits 0-7)
+constexpr uint32_t kWalkTomb = 1U << 10;   // that slot held TOMB (else EMPTY)
+constexpr uint32_t kWalkAbsent = 1U << 11; // no key runs past the home bucket: the key is absent
+
 // Claim order of a key inside a bucket: slots o, o+1, ..., W-1, 0, ..., o-1, with o taken from the
 // key's hash.  Concurrent inserts of different keys into one bucket then aim at different slots,
 // while every inserter of one key follows one order over its whole probe sequence, which is what
@@ -468,6 +476,7 @@ insert_body(const table_ref<K, M>& t, const void* __restrict__ values, const voi
     }
     const bool probe = valid && !reserved && !at_limit;
     uint32_t pend = 0;
+    uint32_t mywalk = 0; // kWalk* for this lane's element
 
 #pragma unroll
     for (int rb = 0; rb < G; rb += R)
@@ -511,10 +520,11 @@ insert_body(const table_ref<K, M>& t, const void* __restrict__ values, const voi
                 const uint64_t be = ballot(e != 0) & gmask;
                 const uint64_t bf = ballot(f != 0) & gmask;
                 bool ends_home = be != 0;
+                uint32_t ovh = 1;
                 if constexpr (kInline)
                 {
                     // no key runs past a home bucket with ovf == 0: a TOMB there is free to take
-                    const uint32_t ovh = shfl32(li == 0 ? q[r].ovf_word() : 0U, gbase);
+                    ovh = shfl32(li == 0 ? q[r].ovf_word() : 0U, gbase);
                     ends_home = ends_home || (bf != 0 && ovh == 0);
                 }
                 if (bm)
@@ -541,6 +551,28 @@ insert_body(const table_ref<K, M>& t, const void* __restrict__ values, const voi
                 }
                 else
                 {
+                    // the probe sequence runs on past the home bucket: the element's lane keeps what
+                    // this bucket showed, and the walk starts at the next bucket
+                    uint32_t info = kWalkSet | ((kInline && ovh == 0) ? kWalkAbsent : 0U);
+                    if (bf)
+                    {
+                        uint32_t fsel = f;
+                        int fl = 0;
+                        if (ordered)
+                            fl = first_free_in_order<SPL>(f, claim_offset(static_cast<uint64_t>(k), W), 0, W, li,
+                                                          gmask, &fsel);
+                        else
+                            fl = ctz64(bf);
+                        uint32_t part = 0;
+                        if (lane == fl)
+                        {
+                            const int j = __builtin_ctz(fsel);
+                            part = static_cast<uint32_t>(li * SPL + j) | kWalkFree |
+                                   ((q[r].key(j) == tomb_key<K>()) ? kWalkTomb : 0U);
+                        }
+                        info |= shfl32(part, fl);
+                    }
+                    if (li == rb + r) mywalk = info;
                     pend |= 1U << (rb + r);
                 }
             }
@@ -618,6 +650,19 @@ insert_body(const table_ref<K, M>& t, const void* __restrict__ values, const voi
             jmin = 0;
             absent = false;
             fresh = false;
+            // an element sent on by its home bucket (not one that lost a claim there) resumes after it
+            const uint32_t info = shfl32(mywalk, gbase + cur);
+            if ((info & kWalkSet) && t.nbuckets > 1)
+            {
+                b = next_bucket(hb, t.nbuckets);
+                walked = 1;
+                absent = (info & kWalkAbsent) != 0;
+                if (info & kWalkFree)
+                {
+                    ff = hb * W + (info & 0xFFU);
+                    ffobs = (info & kWalkTomb) ? tomb_key<K>() : empty_key<K>();
+                }
+            }
         }
         const bool active = cur >= 0;
         if (ballot(active) == 0) break;
