@@ -6,11 +6,14 @@
 // libstdgpu_amd.so and complete before returning; native device calls run the per-thread
 // probing of include/stdgpu_amd/htab_device.hpp on the same layout.
 //
-// Supported: 4- or 8-byte integral keys compared with equal_to<Key> or the transparent equal_to<>,
-// any hasher callable with a Key, and (maps) trivially copyable 4- or 8-byte mapped types.  The layout
-// places keys by their bits (htab_device.hpp), so the hasher only feeds bucket(), which keeps the
-// reference's value for it; a found entry is not at begin() + bucket(key).  With a transparent hasher
-// and key_equal, find / contains / count also take an integral KeyLike (unordered_base.cuh:241-301).
+// Supported keys: 4- or 8-byte integers, and trivially copyable 4- or 8-byte types whose value is their
+// bits (unique object representations: structs of integers without padding, whose operator== compares
+// every member), compared with equal_to<Key> or the transparent equal_to<>; any hasher callable with a
+// Key; (maps) trivially copyable 4- or 8-byte mapped types, with pair<const Key, T> laid out as the
+// slot (an 8-byte struct key then needs 8-byte alignment).  The layout places keys by their bits
+// (htab_device.hpp), so the hasher only feeds bucket(), which keeps the reference's value for it; a
+// found entry is not at begin() + bucket(key).  With a transparent hasher and key_equal, find /
+// contains / count of an integer key also take an integer KeyLike (unordered_base.cuh:241-301).
 #ifndef STDGPU_AMD_FACADE_UNORDERED_CUH
 #define STDGPU_AMD_FACADE_UNORDERED_CUH
 
@@ -65,8 +68,11 @@ public:
     using iterator = pointer;
     using const_iterator = const_pointer;
 
-    static_assert(std::is_integral_v<Key> && (sizeof(Key) == 4 || sizeof(Key) == 8),
-                  "stdgpu (MI355X library): keys are 4- or 8-byte integers");
+    static_assert((std::is_integral_v<Key> ||
+                   (std::is_trivially_copyable_v<Key> && std::has_unique_object_representations_v<Key>)) &&
+                          (sizeof(Key) == 4 || sizeof(Key) == 8),
+                  "stdgpu (MI355X library): keys are 4- or 8-byte integers, or 4- or 8-byte trivially copyable "
+                  "types without padding bits");
     static constexpr bool mapped_ok()
     {
         if constexpr (std::is_void_v<Mapped>)
@@ -93,7 +99,8 @@ public:
     // the KeyLike overloads exist for a transparent hasher and key_equal (unordered_base.cuh:241-301)
     template <typename KeyLike>
     static constexpr bool keylike_ok = is_transparent_t<Hash>::value && is_transparent_t<KeyEqual>::value &&
-                                       std::is_integral_v<KeyLike> && !std::is_same_v<KeyLike, Key>;
+                                       std::is_integral_v<Key> && std::is_integral_v<KeyLike> &&
+                                       !std::is_same_v<KeyLike, Key>;
 
     using skey = storage_uint_t<sizeof(Key)>;
     using smapped = typename storage_mapped<Mapped>::type;
@@ -342,9 +349,13 @@ protected:
             return v.first;
     }
 
+    // the stored bits of a key: integers by value (sign bits included), other keys as they lie in memory
     __host__ __device__ static skey to_skey(const Key& k)
     {
-        return static_cast<skey>(static_cast<std::make_unsigned_t<Key>>(k));
+        if constexpr (std::is_integral_v<Key>)
+            return static_cast<skey>(static_cast<std::make_unsigned_t<Key>>(k));
+        else
+            return __builtin_bit_cast(skey, k);
     }
 
     htab* _h = nullptr;

@@ -8,9 +8,11 @@
 #include <algorithm>
 #include <cstdint>
 #include <cstdio>
+#include <map>
 #include <numeric>
 #include <random>
 #include <set>
+#include <tuple>
 #include <vector>
 
 #include <stdgpu/unordered_map.cuh>
@@ -631,6 +633,146 @@ examples_closed_form()
     imap_t::destroyDeviceObject(m);
 }
 
+// An 8-byte voxel-block key: the reference tests' vec3int16 (tests/stdgpu/unordered_map.inc:53-71)
+// with an explicit fourth member, so that the key has no padding bits and its value is its bits
+struct alignas(8) block_key
+{
+    std::int16_t x = 0;
+    std::int16_t y = 0;
+    std::int16_t z = 0;
+    std::int16_t w = 0;
+    block_key() = default;
+    __host__ __device__ block_key(std::int16_t a, std::int16_t b, std::int16_t c) : x(a), y(b), z(c) {}
+};
+
+__host__ __device__ inline bool
+operator==(const block_key& a, const block_key& b)
+{
+    return a.x == b.x && a.y == b.y && a.z == b.z && a.w == b.w;
+}
+
+static bool
+operator<(const block_key& a, const block_key& b)
+{
+    return std::make_tuple(a.x, a.y, a.z, a.w) < std::make_tuple(b.x, b.y, b.z, b.w);
+}
+
+// the reference tests' vec_hash (tests/stdgpu/unordered_map.inc:181-206)
+struct block_hash
+{
+    __host__ __device__ std::size_t operator()(const block_key& k) const
+    {
+        return (static_cast<std::size_t>(k.x) * static_cast<std::size_t>(73856093U)) ^
+               (static_cast<std::size_t>(k.y) * static_cast<std::size_t>(19349669U)) ^
+               (static_cast<std::size_t>(k.z) * static_cast<std::size_t>(83492791U));
+    }
+};
+
+using block_set_t = stdgpu::unordered_set<block_key, block_hash, stdgpu::equal_to<>>;
+using block_map_t = stdgpu::unordered_map<block_key, std::uint32_t, block_hash, stdgpu::equal_to<>>;
+
+template <typename Table>
+__global__ void
+k_block_bucket(Table t, const block_key* keys, std::size_t n, int* out)
+{
+    const std::size_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = t.bucket(keys[i]);
+}
+
+__global__ void
+k_block_map_emplace(block_map_t m, const block_key* keys, std::size_t n, int* ins)
+{
+    const std::size_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) ins[i] = m.emplace(keys[i], static_cast<std::uint32_t>(i)).second ? 1 : 0;
+}
+
+// a later launch than the emplaces: a find racing an insert of its key in the same kernel may see the
+// key before its value
+__global__ void
+k_block_map_find(block_map_t m, const block_key* keys, std::size_t n, std::uint32_t* vals)
+{
+    const std::size_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const auto it = m.find(keys[i]);
+    vals[i] = (it != m.end() && it->first == keys[i]) ? it->second : 0xFFFFFFFFU;
+}
+
+static void
+struct_keys()
+{
+    g_case = "8-byte struct keys (voxel blocks) with the reference tests' vec_hash";
+    block_set_t s = block_set_t::createDeviceObject(100000); // the reference fixture's capacity
+    // the pinned pairs of insert_collision / insert_no_collision (unordered_datastructure.inc:899-934)
+    const std::vector<block_key> pinned = { { -7, -3, 15 }, { 7, 3, 15 }, { -5, -15, 13 }, { 5, 15, 13 } };
+    {
+        block_key* dp = to_dev(pinned);
+        int* db = dev_alloc<int>(pinned.size());
+        hipLaunchKernelGGL((k_block_bucket<block_set_t>), dim3(1), dim3(64), 0, 0, s, dp, pinned.size(), db);
+        const std::vector<int> b = to_host(db, pinned.size());
+        CHECK(b[0] == b[1] && b[2] == b[3] && b[0] != b[2]);
+        CHECK(b[0] == s.bucket(pinned[0])); // host and device agree
+        (void)hipFree(dp);
+        (void)hipFree(db);
+    }
+    // random blocks with repeats: native inserts, finds, erases against std::set
+    std::mt19937 rng(5);
+    std::uniform_int_distribution<int> d(-40, 40), dz(-5, 5); // 72,171 blocks: draws repeat often
+    std::vector<block_key> keys;
+    for (int i = 0; i < 60000; ++i)
+        keys.emplace_back(static_cast<std::int16_t>(d(rng)), static_cast<std::int16_t>(d(rng)),
+                          static_cast<std::int16_t>(dz(rng)));
+    std::set<block_key> want(keys.begin(), keys.end());
+    CHECK(sum(run_insert(s, keys)) == static_cast<int>(want.size()));
+    CHECK(s.size() == static_cast<stdgpu::index_t>(want.size()) && s.valid());
+    CHECK(sum(run_find(s, keys)) == static_cast<int>(keys.size()));
+    std::vector<block_key> absent;
+    for (int i = 0; i < 2000; ++i) absent.emplace_back(static_cast<std::int16_t>(1000 + i), 0, 0);
+    CHECK(sum(run_find(s, absent)) == 0);
+    const std::vector<block_key> uniq(want.begin(), want.end());
+    const std::vector<block_key> half(uniq.begin(), uniq.begin() + uniq.size() / 2);
+    CHECK(sum(run_erase(s, half)) == static_cast<int>(half.size()));
+    for (const auto& k : half) want.erase(k);
+    // device_range holds exactly the remaining blocks
+    auto range = s.device_range();
+    CHECK(range.size() == static_cast<stdgpu::index_t>(want.size()));
+    block_key* out = dev_alloc<block_key>(want.size());
+    hipLaunchKernelGGL((k_range_copy<decltype(range), block_key>), dim3(blocks(want.size())), dim3(256), 0, 0, range,
+                       out);
+    std::vector<block_key> got = to_host(out, want.size());
+    std::sort(got.begin(), got.end());
+    CHECK(got == std::vector<block_key>(want.begin(), want.end()));
+    (void)hipFree(out);
+    // bulk erase(begin, end) of the rest through the agnostic API
+    const std::vector<block_key> rest(want.begin(), want.end());
+    block_key* dr = to_dev(rest);
+    s.erase(stdgpu::make_device(dr), stdgpu::make_device(dr + rest.size()));
+    CHECK(s.size() == 0 && s.valid());
+    (void)hipFree(dr);
+    block_set_t::destroyDeviceObject(s);
+
+    // a map from blocks to slot numbers: emplace, find through the returned iterator
+    block_map_t m = block_map_t::createDeviceObject(100000);
+    block_key* dk = to_dev(keys);
+    int* dins = dev_alloc<int>(keys.size());
+    std::uint32_t* dval = dev_alloc<std::uint32_t>(keys.size());
+    hipLaunchKernelGGL(k_block_map_emplace, dim3(blocks(keys.size())), dim3(256), 0, 0, m, dk, keys.size(), dins);
+    hipLaunchKernelGGL(k_block_map_find, dim3(blocks(keys.size())), dim3(256), 0, 0, m, dk, keys.size(), dval);
+    const std::vector<int> ins = to_host(dins, keys.size());
+    const std::vector<std::uint32_t> val = to_host(dval, keys.size());
+    const std::set<block_key> all(keys.begin(), keys.end());
+    CHECK(sum(ins) == static_cast<int>(all.size()) && m.size() == static_cast<stdgpu::index_t>(all.size()));
+    // every occurrence of a block reads the index of the one emplace that inserted it
+    std::map<block_key, std::uint32_t> winner;
+    for (std::size_t i = 0; i < keys.size(); ++i)
+        if (ins[i]) winner[keys[i]] = static_cast<std::uint32_t>(i);
+    for (std::size_t i = 0; i < keys.size(); ++i) CHECK(val[i] == winner[keys[i]]);
+    CHECK(m.valid());
+    (void)hipFree(dk);
+    (void)hipFree(dins);
+    (void)hipFree(dval);
+    block_map_t::destroyDeviceObject(m);
+}
+
 // a key matches a KeyLike exactly when key == keylike under C++'s usual arithmetic conversions
 template <typename Key, typename KeyLike>
 static std::vector<int>
@@ -704,7 +846,8 @@ main()
                           collision_chain_native,
                           accessors_and_policies,
                           examples_closed_form,
-                          transparent_keylike_lookups };
+                          transparent_keylike_lookups,
+                          struct_keys };
     for (auto c : cases)
     {
         const int before = g_failures;
