@@ -99,3 +99,35 @@ def test_restore_rejects_what_is_not_a_snapshot():
     buf[:hs] = np.frombuffer(bytes(hd), dtype=np.uint8)
     assert lib.htab_restore(buf.ctypes.data_as(ctypes.c_void_p), buf.size, ctypes.byref(h)) == _lib.HTAB_E_ARG
     assert b"not a table snapshot" in lib.htab_last_error()
+
+
+def test_container_argument_errors_without_gpu():
+    """vector / deque / bitset / atomic entry points refuse bad arguments before any device work"""
+    lib = _lib.load()
+    h = ctypes.c_void_p()
+    E = _lib.HTAB_E_ARG
+    assert lib.htab_vector_create(0, ctypes.byref(h)) == E
+    assert lib.htab_vector_create((1 << 32) + 1, ctypes.byref(h)) == E
+    assert lib.htab_vector_push_back(None, None, 1, None, None) == E
+    assert lib.htab_deque_create(0, ctypes.byref(h)) == E
+    assert lib.htab_bitset_create(0, ctypes.byref(h)) == E
+    assert lib.htab_bitset_create(10, None) == E
+    assert lib.htab_bitset_update(None, _lib.HTAB_BIT_SET, None, 1, None, None, None) == E
+    assert lib.htab_bitset_count(None, None, None) == E
+    assert lib.htab_atomic_create(7, 1, ctypes.byref(h)) == E  # no such value type
+    assert lib.htab_atomic_create(_lib.HTAB_ATOMIC_U32, 0, ctypes.byref(h)) == E
+    assert lib.htab_atomic_apply(None, 4, None, None, None, 1, None, None, None) == E
+    buf = ctypes.c_void_p(0x1000)  # never dereferenced: every call below is refused first
+    ops = _lib.HTAB_ATOMIC_OPS
+    ref = lib.htab_atomic_ref_apply
+    assert ref(9, buf, 4, ops["fetch_add"], None, buf, None, 1, None, None, None) == E  # type
+    assert ref(_lib.HTAB_ATOMIC_U32, None, 4, ops["fetch_add"], None, buf, None, 1, None, None, None) == E  # data
+    assert ref(_lib.HTAB_ATOMIC_U32, buf, 0, ops["fetch_add"], None, buf, None, 1, None, None, None) == E  # count
+    assert ref(_lib.HTAB_ATOMIC_U32, buf, 4, 13, None, buf, None, 1, None, None, None) == E  # op
+    assert ref(_lib.HTAB_ATOMIC_U32, buf, 4, ops["fetch_add"], None, None, None, 1, None, None, None) == E  # operands
+    assert ref(_lib.HTAB_ATOMIC_U32, buf, 4, ops["compare_exchange"], None, buf, None, 1, None, None, None) == E
+    # operations the reference does not define for the type: bitwise ops on float, the wrapping
+    # increment / decrement on anything but unsigned int
+    assert ref(_lib.HTAB_ATOMIC_F32, buf, 4, ops["fetch_and"], None, buf, None, 1, None, None, None) == E
+    assert ref(_lib.HTAB_ATOMIC_I32, buf, 4, ops["fetch_inc_mod"], None, buf, None, 1, None, None, None) == E
+    assert ref(_lib.HTAB_ATOMIC_U64, buf, 4, ops["fetch_dec_mod"], None, buf, None, 1, None, None, None) == E
