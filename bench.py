@@ -28,9 +28,9 @@ BYTES_PER_OP = {"insert": 16 + 64 + 16 + 1, "find": 8 + 1 + 4 + 64, "erase": 8 +
 # 8-slot buckets, 256-thread workgroups): FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000 KiB, the scales
 # that make both counters read a 2 GiB copy kernel's bytes exactly (profiles/map_u64_u32_2e26_bulk_kernels.md)
 # 128 B lines a find reads per key (TCC misses per dispatch / keys), same profile and workload
-MEASURED_FIND_LINES = {(26, 8, 256): 89040914 / (1 << 26)}
-MEASURED_TRAFFIC = {(26, 8, 256): {"insert": (2 * 5466773 + 6707952) * 1024, "find": (2 * 5391950 + 336568) * 1024,
-                                   "erase": (2 * 1272259 + 1146748) * 1024, "reinsert": (2 * 2692081 + 1659669) * 1024}}
+MEASURED_FIND_LINES = {(26, 8, 256): 89039255 / (1 << 26)}
+MEASURED_TRAFFIC = {(26, 8, 256): {"insert": (2 * 5252402 + 6709960) * 1024, "find": (2 * 5391956 + 336602) * 1024,
+                                   "erase": (2 * 1272247 + 1146738) * 1024, "reinsert": (2 * 2368420 + 1667805) * 1024}}
 
 
 def parse():
