@@ -250,6 +250,36 @@ public:
         check(static_cast<int>(hipStreamSynchronize(policy.stream)) == 0 ? HTAB_OK : HTAB_E_HIP, "erase sync");
     }
 
+    // Bulk lookups over a key range (the reference has none: its callers run find per thread in a
+    // functor, unordered_datastructure.inc:2180-2226).  hits[i] = contains(first[i]); for maps,
+    // values[i] = find(first[i])->second where hits[i] (untouched elsewhere; values may be null).
+    // hits and values are device arrays of last - first entries.
+    template <typename KeyIterator>
+    void contains(KeyIterator first, KeyIterator last, std::uint8_t* hits) const
+    {
+        contains(execution::device, first, last, hits);
+    }
+
+    template <typename ExecutionPolicy, typename KeyIterator,
+              std::enable_if_t<is_execution_policy_v<ExecutionPolicy>, int> = 0>
+    void contains(ExecutionPolicy&& policy, KeyIterator first, KeyIterator last, std::uint8_t* hits) const
+    {
+        bulk_find(policy.stream, first, last, hits, nullptr, "contains");
+    }
+
+    template <typename KeyIterator, typename U = Mapped, std::enable_if_t<!std::is_void_v<U>, int> = 0>
+    void find(KeyIterator first, KeyIterator last, std::uint8_t* hits, U* values) const
+    {
+        find(execution::device, first, last, hits, values);
+    }
+
+    template <typename ExecutionPolicy, typename KeyIterator, typename U = Mapped,
+              std::enable_if_t<is_execution_policy_v<ExecutionPolicy> && !std::is_void_v<U>, int> = 0>
+    void find(ExecutionPolicy&& policy, KeyIterator first, KeyIterator last, std::uint8_t* hits, U* values) const
+    {
+        bulk_find(policy.stream, first, last, hits, values, "find");
+    }
+
     device_indexed_range<value_type> device_range() { return device_range(execution::device); }
     template <typename ExecutionPolicy, std::enable_if_t<is_execution_policy_v<ExecutionPolicy>, int> = 0>
     device_indexed_range<value_type> device_range(ExecutionPolicy&& policy)
@@ -340,6 +370,16 @@ public:
 
 protected:
     __host__ __device__ engine eng() const { return engine::from_view(_view); }
+
+    template <typename KeyIterator>
+    void bulk_find(hipStream_t s, KeyIterator first, KeyIterator last, std::uint8_t* hits, void* values,
+                   const char* what) const
+    {
+        const std::int64_t n = static_cast<std::int64_t>(last - first);
+        if (n > 0 && !hits) throw std::invalid_argument(std::string("stdgpu::") + what + ": hits is null");
+        check(htab_find(_h, raw(first), n, hits, values, s), what);
+        check(static_cast<int>(hipStreamSynchronize(s)) == 0 ? HTAB_OK : HTAB_E_HIP, what);
+    }
 
     __host__ __device__ static const Key& key_of(const value_type& v)
     {

@@ -773,6 +773,61 @@ struct_keys()
     block_map_t::destroyDeviceObject(m);
 }
 
+static void
+bulk_find_and_contains()
+{
+    g_case = "bulk find / contains over key ranges";
+    const std::size_t N = 50000;
+    map_t m = map_t::createDeviceObject(static_cast<stdgpu::index_t>(N));
+    const std::vector<std::uint64_t> k = unique_keys(2 * N, 21);
+    const std::vector<std::uint64_t> present(k.begin(), k.begin() + N);
+    CHECK(sum(run_insert(m, present)) == static_cast<int>(N));
+    // queries: the N present keys, N absent ones, then both reserved key values (absent)
+    std::vector<std::uint64_t> q = k;
+    q.push_back(~0ULL);
+    q.push_back(~0ULL - 1);
+    const std::size_t Q = q.size();
+    std::uint64_t* dq = to_dev(q);
+    std::uint8_t* dh = dev_alloc<std::uint8_t>(Q);
+    std::uint32_t* dv = dev_alloc<std::uint32_t>(Q);
+    HIP_OK(hipMemset(dv, 0xFF, Q * sizeof(std::uint32_t)));
+    m.find(stdgpu::make_device(dq), stdgpu::make_device(dq + Q), dh, dv);
+    const std::vector<std::uint8_t> hits = to_host(dh, Q);
+    const std::vector<std::uint32_t> vals = to_host(dv, Q);
+    for (std::size_t i = 0; i < Q; ++i)
+    {
+        const bool in = i < N;
+        CHECK(hits[i] == (in ? 1 : 0));
+        CHECK(vals[i] == (in ? static_cast<std::uint32_t>(q[i] * 3 + 1) : 0xFFFFFFFFU)); // misses untouched
+    }
+    // the native per-thread find agrees
+    const std::vector<int> native = run_find(m, q);
+    for (std::size_t i = 0; i < Q; ++i) CHECK(native[i] == hits[i]);
+    // contains through a policy with its own stream, after erasing the first half of the present keys
+    hipStream_t st;
+    HIP_OK(hipStreamCreate(&st));
+    stdgpu::execution::device_policy pol{ st };
+    m.erase(pol, stdgpu::make_device(dq), stdgpu::make_device(dq + N / 2));
+    m.contains(pol, stdgpu::make_device(dq), stdgpu::make_device(dq + Q), dh);
+    const std::vector<std::uint8_t> h2 = to_host(dh, Q);
+    for (std::size_t i = 0; i < Q; ++i) CHECK(h2[i] == ((i >= N / 2 && i < N) ? 1 : 0));
+    HIP_OK(hipStreamDestroy(st));
+    // a set answers contains the same way
+    set_t s = set_t::createDeviceObject(1000);
+    const std::vector<std::uint32_t> sk = { 1U, 5U, 9U, 0xFFFFFFFFU };
+    CHECK(sum(run_insert(s, sk)) == 4);
+    const std::vector<std::uint32_t> sq = { 0U, 1U, 5U, 6U, 9U, 0xFFFFFFFFU, 0xFFFFFFFEU };
+    std::uint32_t* dsq = to_dev(sq);
+    s.contains(stdgpu::make_device(dsq), stdgpu::make_device(dsq + sq.size()), dh);
+    CHECK(to_host(dh, sq.size()) == (std::vector<std::uint8_t>{ 0, 1, 1, 0, 1, 1, 0 }));
+    set_t::destroyDeviceObject(s);
+    (void)hipFree(dsq);
+    (void)hipFree(dq);
+    (void)hipFree(dh);
+    (void)hipFree(dv);
+    map_t::destroyDeviceObject(m);
+}
+
 // a key matches a KeyLike exactly when key == keylike under C++'s usual arithmetic conversions
 template <typename Key, typename KeyLike>
 static std::vector<int>
@@ -847,7 +902,8 @@ main()
                           accessors_and_policies,
                           examples_closed_form,
                           transparent_keylike_lookups,
-                          struct_keys };
+                          struct_keys,
+                          bulk_find_and_contains };
     for (auto c : cases)
     {
         const int before = g_failures;
