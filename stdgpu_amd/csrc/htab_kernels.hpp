@@ -7,10 +7,12 @@
 //   1. first probes: the home buckets of R of the group's elements (R = probe batch, a tuning
 //      knob) are loaded back to back, so a wave keeps 64*R/G bucket lines in flight;
 //   2. each element is resolved against its home bucket with __ballot votes: a key match,
-//      an EMPTY slot (end of the probe sequence) or, for insert, a CAS on the first free
-//      slot elected with ctz over the ballot mask;
+//      an EMPTY slot or ovf == 0 (end of the probe sequence) or, for insert, a CAS on the
+//      first free slot elected with ctz over the ballot mask (the R claims are issued back to
+//      back, then their results read);
 //   3. elements the home bucket does not resolve (overflow, lost CAS) continue one group
-//      at a time along the probe sequence;
+//      at a time along the probe sequence, from the next bucket when the home bucket's
+//      findings are kept (find, erase, and inserts that did not lose a claim);
 //   4. results return to the owning lane and leave in one coalesced store; size changes are
 //      summed per workgroup into one atomic on one of 32 counter stripes.
 //
