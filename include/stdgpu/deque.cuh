@@ -1,0 +1,179 @@
+// Source-compatible stdgpu::deque<T> (src/stdgpu/deque.cuh) for 4-byte trivially copyable T, the
+// free-list container of voxel-block pipelines, on gfx950.
+//
+// Host: createDeviceObject / destroyDeviceObject, size / empty / full / capacity / max_size, clear,
+// valid.  Device (user kernels, per thread): push_back / emplace_back, pop_front, size / empty / full,
+// operator[] / at / front / back (include/stdgpu_amd/deque_device.hpp).  The calling lanes of a
+// wavefront share one compare-and-swap on the back or front counter.  A pop whose element is pushed
+// but not yet written, or a push whose slot is still being popped, returns failure at once where the
+// reference spins on a per-slot lock; pushes and pops may be mixed in one kernel.  push_front and
+// pop_back are not provided.  Bulk pushes and pops of many values per call: htab_deque_* in
+// <stdgpu_amd/htab.h>.
+#ifndef STDGPU_DEQUE_CUH
+#define STDGPU_DEQUE_CUH
+
+#include <cstdint>
+#include <stdexcept>
+#include <type_traits>
+#include <utility>
+
+#include <stdgpu/impl/amd_core.h>
+#include <stdgpu_amd/deque_device.hpp>
+
+namespace stdgpu
+{
+
+template <typename T, typename Allocator = safe_device_allocator<T>>
+class deque
+{
+public:
+    static_assert(std::is_trivially_copyable_v<T> && sizeof(T) == 4,
+                  "stdgpu::deque (MI355X library): 4-byte trivially copyable value types");
+
+    using value_type = T;
+    using allocator_type = Allocator;
+    using index_type = index_t;
+    using difference_type = std::ptrdiff_t;
+    using reference = value_type&;
+    using const_reference = const value_type&;
+    using pointer = value_type*;
+    using const_pointer = const value_type*;
+
+    [[nodiscard]] static deque createDeviceObject(const index_t& capacity, const Allocator& allocator = Allocator())
+    {
+        return createDeviceObject(execution::device, capacity, allocator);
+    }
+
+    template <typename ExecutionPolicy, std::enable_if_t<is_execution_policy_v<ExecutionPolicy>, int> = 0>
+    [[nodiscard]] static deque createDeviceObject([[maybe_unused]] ExecutionPolicy&& policy, const index_t& capacity,
+                                                  const Allocator& allocator = Allocator())
+    {
+        if (capacity <= 0) throw std::invalid_argument("stdgpu::deque: capacity must be positive");
+        deque d;
+        d._allocator = allocator;
+        detail::check(htab_deque_create(capacity, &d._h), "deque::createDeviceObject");
+        detail::check(htab_deque_get_view(d._h, &d._view), "deque::createDeviceObject");
+        return d;
+    }
+
+    static void destroyDeviceObject(deque& device_object) { destroyDeviceObject(execution::device, device_object); }
+
+    template <typename ExecutionPolicy, std::enable_if_t<is_execution_policy_v<ExecutionPolicy>, int> = 0>
+    static void destroyDeviceObject([[maybe_unused]] ExecutionPolicy&& policy, deque& device_object)
+    {
+        if (device_object._h) detail::check(htab_deque_destroy(device_object._h), "deque::destroyDeviceObject");
+        device_object._h = nullptr;
+        device_object._view = htab_deque_view{};
+    }
+
+    deque() = default;
+
+    [[nodiscard]] __host__ __device__ allocator_type get_allocator() const noexcept { return _allocator; }
+
+    // ---------------------------------------------------------------- device element access
+    __device__ reference at(const index_type n) { return *reinterpret_cast<pointer>(ref().at(n)); }
+    __device__ const_reference at(const index_type n) const { return *reinterpret_cast<const_pointer>(ref().at(n)); }
+    __device__ reference operator[](const index_type n) { return at(n); }
+    __device__ const_reference operator[](const index_type n) const { return at(n); }
+    __device__ reference front() { return at(0); }
+    __device__ const_reference front() const { return at(0); }
+    __device__ reference back() { return at(static_cast<index_type>(ref().size()) - 1); }
+    __device__ const_reference back() const { return at(static_cast<index_type>(ref().size()) - 1); }
+
+    // ---------------------------------------------------------------- device modifiers
+    __device__ bool push_back(const T& element) { return ref().push_back(__builtin_bit_cast(std::uint32_t, element)); }
+
+    template <class... Args>
+    __device__ bool emplace_back(Args&&... args)
+    {
+        return push_back(T(std::forward<Args>(args)...));
+    }
+
+    // the popped element and true, or T() and false (deque.cuh:234-239)
+    __device__ pair<T, bool> pop_front()
+    {
+        std::uint32_t bits = 0;
+        if (ref().pop_front(&bits)) return pair<T, bool>(__builtin_bit_cast(T, bits), true);
+        return pair<T, bool>(T(), false);
+    }
+
+    // ---------------------------------------------------------------- capacity
+    [[nodiscard]] __host__ __device__ bool empty() const { return size() == 0; }
+    [[nodiscard]] __host__ __device__ bool full() const { return size() == max_size(); }
+
+    __host__ __device__ index_t size() const
+    {
+#if defined(__HIP_DEVICE_COMPILE__)
+        return static_cast<index_t>(ref().size());
+#else
+        return size(execution::device);
+#endif
+    }
+
+    template <typename ExecutionPolicy, std::enable_if_t<is_execution_policy_v<ExecutionPolicy>, int> = 0>
+    index_t size(ExecutionPolicy&& policy) const
+    {
+        std::int64_t n = 0;
+        if (_h) detail::check(htab_deque_size(_h, &n, policy.stream), "deque::size");
+        return static_cast<index_t>(n);
+    }
+
+    template <typename ExecutionPolicy, std::enable_if_t<is_execution_policy_v<ExecutionPolicy>, int> = 0>
+    bool empty(ExecutionPolicy&& policy) const
+    {
+        return size(std::forward<ExecutionPolicy>(policy)) == 0;
+    }
+
+    template <typename ExecutionPolicy, std::enable_if_t<is_execution_policy_v<ExecutionPolicy>, int> = 0>
+    bool full(ExecutionPolicy&& policy) const
+    {
+        return size(std::forward<ExecutionPolicy>(policy)) == max_size();
+    }
+
+    __host__ __device__ index_t max_size() const noexcept { return capacity(); }
+    __host__ __device__ index_t capacity() const noexcept { return static_cast<index_t>(_view.capacity); }
+
+    __host__ __device__ const T* data() const noexcept { return reinterpret_cast<const T*>(_view.data); }
+    __host__ __device__ T* data() noexcept { return reinterpret_cast<T*>(_view.data); }
+
+    // ---------------------------------------------------------------- host modifiers and checks
+    void clear() { clear(execution::device); }
+
+    template <typename ExecutionPolicy, std::enable_if_t<is_execution_policy_v<ExecutionPolicy>, int> = 0>
+    void clear(ExecutionPolicy&& policy)
+    {
+        if (!_h) return;
+        detail::check(htab_deque_clear(_h, policy.stream), "deque::clear");
+        detail::check(hipStreamSynchronize(policy.stream) == hipSuccess ? HTAB_OK : HTAB_E_HIP, "deque::clear");
+    }
+
+    // the counters describe at most capacity elements (deque_detail.cuh:637-670 checks its own layout)
+    [[nodiscard]] bool valid() const { return valid(execution::device); }
+
+    template <typename ExecutionPolicy, std::enable_if_t<is_execution_policy_v<ExecutionPolicy>, int> = 0>
+    [[nodiscard]] bool valid(ExecutionPolicy&& policy) const
+    {
+        if (!_h) return true;
+        std::uint64_t c[2] = { 0, 0 };
+        if (hipMemcpyAsync(&c[0], _view.begin, sizeof(std::uint64_t), hipMemcpyDeviceToHost, policy.stream) !=
+                    hipSuccess ||
+            hipMemcpyAsync(&c[1], _view.end, sizeof(std::uint64_t), hipMemcpyDeviceToHost, policy.stream) !=
+                    hipSuccess ||
+            hipStreamSynchronize(policy.stream) != hipSuccess)
+            return false;
+        return c[0] <= c[1] && c[1] - c[0] <= static_cast<std::uint64_t>(_view.capacity);
+    }
+
+    htab_deque* native_handle() const { return _h; }
+
+private:
+    __host__ __device__ htab_dev::deque_ref ref() const { return htab_dev::deque_ref::from_view(_view); }
+
+    htab_deque* _h = nullptr;
+    htab_deque_view _view{};
+    Allocator _allocator{};
+};
+
+} // namespace stdgpu
+
+#endif // STDGPU_DEQUE_CUH

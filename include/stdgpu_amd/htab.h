@@ -352,10 +352,29 @@ const char* htab_last_error(void);
 /* ------------------------------------------------------------------------------------------
  * Free-list deque<uint32_t> (deque.cuh; deque_detail.cuh:202-260 push_back, 390-448 pop_front),
  * used by voxel-block pipelines to hand out slot numbers.  A bulk call only pushes or only pops and
- * reserves its positions once, so its result is that of a loop of single calls.  Bulk calls only;
- * size follows the reference's clamped size().
+ * reserves its positions once, so its result is that of a loop of single calls; size follows the
+ * reference's clamped size().  User kernels push and pop per thread through the view
+ * (include/stdgpu_amd/deque_device.hpp, the C++ stdgpu::deque): ring position p lives in slot
+ * p mod capacity, with seq[slot] = 2t (free), 2t + 1 (holds position p) for turn t = p / capacity.
  * ------------------------------------------------------------------------------------------ */
 typedef struct htab_deque htab_deque;
+
+typedef struct htab_deque_view
+{
+    uint32_t* data;      /* capacity values */
+    uint32_t* seq;       /* capacity slot states */
+    uint64_t* begin;     /* front counter */
+    uint64_t* end;       /* back counter; size = end - begin */
+    int64_t* ready;      /* elements published and not yet reserved by a pop */
+    int64_t* free_slots; /* slots released and not yet reserved by a push */
+    int64_t capacity;
+} htab_deque_view;
+
+/* device pointers of the deque for user kernels (no reference counterpart: the reference's deque
+ * object is itself the device handle) */
+int htab_deque_get_view(const htab_deque* q, htab_deque_view* out);
+/* clear() (deque_detail.cuh:575-635): no elements, every slot free */
+int htab_deque_clear(htab_deque* q, void* stream);
 
 int htab_deque_create(int64_t capacity, htab_deque** out);
 int htab_deque_destroy(htab_deque* q);

@@ -104,6 +104,18 @@ class htab_snapshot_header(ctypes.Structure):
     ]
 
 
+class htab_deque_view(ctypes.Structure):
+    _fields_ = [
+        ("data", ctypes.c_void_p),
+        ("seq", ctypes.c_void_p),
+        ("begin", ctypes.c_void_p),
+        ("end", ctypes.c_void_p),
+        ("ready", ctypes.c_void_p),
+        ("free_slots", ctypes.c_void_p),
+        ("capacity", ctypes.c_int64),
+    ]
+
+
 class htab_view(ctypes.Structure):
     _fields_ = [
         ("slots", ctypes.c_void_p),
@@ -161,6 +173,8 @@ SIGNATURES = {
     "htab_deque_fill_iota": (_I32, [_P, _P]),
     "htab_deque_size": (_I32, [_P, ctypes.POINTER(_I64), _P]),
     "htab_deque_contents": (_I32, [_P, _P, ctypes.POINTER(_I64), _P]),
+    "htab_deque_get_view": (_I32, [_P, ctypes.POINTER(htab_deque_view)]),
+    "htab_deque_clear": (_I32, [_P, _P]),
     "htab_snapshot_bytes": (_I32, [_P, ctypes.POINTER(_I64)]),
     "htab_snapshot": (_I32, [_P, _P, _I64, _P]),
     "htab_restore": (_I32, [_P, _I64, ctypes.POINTER(_P)]),

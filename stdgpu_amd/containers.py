@@ -286,6 +286,15 @@ class deque_u32:
     def fill_iota(self, stream=None):
         check(_lib.load().htab_deque_fill_iota(self._h, _stream_ptr(stream)), "htab_deque_fill_iota")
 
+    def clear(self, stream=None):
+        check(_lib.load().htab_deque_clear(self._h, _stream_ptr(stream)), "htab_deque_clear")
+
+    def view(self) -> _lib.htab_deque_view:
+        """device pointers for kernels that push and pop per thread (include/stdgpu_amd/deque_device.hpp)"""
+        v = _lib.htab_deque_view()
+        check(_lib.load().htab_deque_get_view(self._h, ctypes.byref(v)), "htab_deque_get_view")
+        return v
+
     def push_back(self, values, pushed=None, stream=None):
         check(
             _lib.load().htab_deque_push_back(self._h, _ptr(values), values.numel(), _ptr(pushed), _stream_ptr(stream)),

@@ -1,0 +1,429 @@
+// Source-compatibility tests of stdgpu::deque used from user kernels, as the reference's voxel
+// pipelines use it (a free list of slot numbers; tests/stdgpu/deque.inc exercises push_back /
+// pop_front from many threads): every case checks that no element is lost or duplicated, against
+// the values the calls reported.  Prints one line per case; exits non-zero if any case fails.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+#include <cstdio>
+#include <numeric>
+#include <vector>
+
+#include <stdgpu/deque.cuh>
+#include <stdgpu/unordered_map.cuh>
+
+using free_list = stdgpu::deque<std::uint32_t>;
+
+static int g_failures = 0;
+static const char* g_case = "";
+
+#define CHECK(cond)                                                                                                    \
+    do                                                                                                                 \
+    {                                                                                                                  \
+        if (!(cond))                                                                                                   \
+        {                                                                                                              \
+            std::printf("FAIL %s: %s (line %d)\n", g_case, #cond, __LINE__);                                          \
+            ++g_failures;                                                                                              \
+            return;                                                                                                    \
+        }                                                                                                              \
+    } while (0)
+
+template <typename T>
+T*
+dev_alloc(std::size_t n)
+{
+    T* p = nullptr;
+    (void)hipMalloc(&p, n * sizeof(T) + 4);
+    (void)hipMemset(p, 0, n * sizeof(T) + 4);
+    return p;
+}
+
+template <typename T>
+std::vector<T>
+to_host(const T* d, std::size_t n)
+{
+    std::vector<T> h(n);
+    if (n) (void)hipMemcpy(h.data(), d, n * sizeof(T), hipMemcpyDeviceToHost);
+    return h;
+}
+
+static unsigned
+blocks(std::size_t n)
+{
+    return static_cast<unsigned>((n + 255) / 256);
+}
+
+// the deque's elements front to back, through the C ABI
+static std::vector<std::uint32_t>
+contents(const free_list& q)
+{
+    std::uint32_t* d = dev_alloc<std::uint32_t>(static_cast<std::size_t>(q.capacity()));
+    std::int64_t n = 0;
+    (void)htab_deque_contents(q.native_handle(), d, &n, nullptr);
+    std::vector<std::uint32_t> h = to_host(d, static_cast<std::size_t>(n));
+    (void)hipFree(d);
+    return h;
+}
+
+__global__ void
+k_push(free_list q, std::uint32_t first, std::size_t n, int* ok)
+{
+    const std::size_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) ok[i] = q.push_back(first + static_cast<std::uint32_t>(i)) ? 1 : 0;
+}
+
+__global__ void
+k_push_values(free_list q, const std::uint32_t* v, std::size_t n, int* ok)
+{
+    const std::size_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) ok[i] = q.push_back(v[i]) ? 1 : 0;
+}
+
+__global__ void
+k_pop(free_list q, std::size_t n, std::uint32_t* out, int* ok)
+{
+    const std::size_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const stdgpu::pair<std::uint32_t, bool> r = q.pop_front();
+    out[i] = r.first;
+    ok[i] = r.second ? 1 : 0;
+}
+
+// even lanes pop, odd lanes push first + i: the two calls diverge inside every wavefront
+__global__ void
+k_mixed(free_list q, std::size_t n, std::uint32_t first, std::uint32_t* out, int* ok)
+{
+    const std::size_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (i % 2 == 0)
+    {
+        const auto r = q.pop_front();
+        out[i] = r.first;
+        ok[i] = r.second ? 1 : 0;
+    }
+    else
+    {
+        out[i] = first + static_cast<std::uint32_t>(i);
+        ok[i] = q.emplace_back(first + static_cast<std::uint32_t>(i)) ? 1 : 0;
+    }
+}
+
+// the voxel-block pattern: pop a free slot, insert (key, slot), push the slot back if the key existed
+__global__ void
+k_voxel(free_list q, stdgpu::unordered_map<std::uint32_t, std::uint32_t> map, std::size_t n, std::uint32_t nkeys,
+        int* got_slot)
+{
+    const std::size_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const auto slot = q.pop_front();
+    got_slot[i] = slot.second ? 1 : 0;
+    if (!slot.second) return;
+    const std::uint32_t key = static_cast<std::uint32_t>((i * 2654435761U) % nkeys);
+    if (!map.emplace(key, slot.first).second) q.push_back(slot.first);
+}
+
+__global__ void
+k_read(free_list q, std::uint32_t* out, int* sz)
+{
+    const int n = q.size();
+    if (blockIdx.x == 0 && threadIdx.x == 0) *sz = n;
+    for (int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x); i < n; i += gridDim.x * blockDim.x)
+        out[i] = q[i];
+}
+
+template <typename Range>
+__global__ void
+k_owned_slots(Range r, std::uint32_t* out)
+{
+    const int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i < r.size()) out[i] = (*(r.begin() + i)).second;
+}
+
+static int
+sum(const std::vector<int>& v)
+{
+    return std::accumulate(v.begin(), v.end(), 0);
+}
+
+static void
+push_to_full_and_pop_to_empty()
+{
+    g_case = "push_back from many threads to full, pop_front to empty";
+    const std::size_t C = 100000, N = C + 5000;
+    free_list q = free_list::createDeviceObject(static_cast<stdgpu::index_t>(C));
+    CHECK(q.empty() && q.size() == 0 && q.capacity() == static_cast<stdgpu::index_t>(C) && q.valid());
+    int* ok = dev_alloc<int>(N);
+    hipLaunchKernelGGL(k_push, dim3(blocks(N)), dim3(256), 0, 0, q, 0U, N, ok);
+    const std::vector<int> pushed = to_host(ok, N);
+    CHECK(sum(pushed) == static_cast<int>(C) && q.full() && q.size() == static_cast<stdgpu::index_t>(C));
+    std::vector<std::uint32_t> want;
+    for (std::size_t i = 0; i < N; ++i)
+        if (pushed[i]) want.push_back(static_cast<std::uint32_t>(i));
+    std::vector<std::uint32_t> have = contents(q);
+    // operator[] from a kernel reads the same elements front to back
+    std::uint32_t* dr = dev_alloc<std::uint32_t>(C);
+    int* dsz = dev_alloc<int>(1);
+    hipLaunchKernelGGL(k_read, dim3(64), dim3(256), 0, 0, q, dr, dsz);
+    CHECK(to_host(dsz, 1)[0] == static_cast<int>(C) && to_host(dr, C) == have);
+    std::sort(have.begin(), have.end());
+    CHECK(have == want);
+    std::uint32_t* out = dev_alloc<std::uint32_t>(N);
+    hipLaunchKernelGGL(k_pop, dim3(blocks(N)), dim3(256), 0, 0, q, N, out, ok);
+    const std::vector<int> popped = to_host(ok, N);
+    std::vector<std::uint32_t> got;
+    const std::vector<std::uint32_t> ho = to_host(out, N);
+    for (std::size_t i = 0; i < N; ++i)
+        if (popped[i]) got.push_back(ho[i]);
+    std::sort(got.begin(), got.end());
+    CHECK(sum(popped) == static_cast<int>(C) && got == want && q.empty() && q.valid());
+    (void)hipFree(ok);
+    (void)hipFree(out);
+    (void)hipFree(dr);
+    (void)hipFree(dsz);
+    free_list::destroyDeviceObject(q);
+}
+
+static void
+mixed_pushes_and_pops_in_diverged_lanes()
+{
+    g_case = "pop_front and push_back in diverged lanes of one kernel";
+    const std::size_t C = 1 << 16, N = 1 << 18;
+    free_list q = free_list::createDeviceObject(static_cast<stdgpu::index_t>(C));
+    (void)htab_deque_fill_iota(q.native_handle(), nullptr);
+    std::uint32_t* out = dev_alloc<std::uint32_t>(N);
+    int* ok = dev_alloc<int>(N);
+    hipLaunchKernelGGL(k_mixed, dim3(blocks(N)), dim3(256), 0, 0, q, N, static_cast<std::uint32_t>(C), out, ok);
+    const std::vector<std::uint32_t> ho = to_host(out, N);
+    const std::vector<int> hk = to_host(ok, N);
+    // (initial - popped) + pushed == final contents, as multisets
+    std::vector<std::uint32_t> expect(C);
+    std::iota(expect.begin(), expect.end(), 0U);
+    std::vector<std::uint32_t> popped;
+    for (std::size_t i = 0; i < N; ++i)
+    {
+        if (!hk[i]) continue;
+        if (i % 2 == 0)
+            popped.push_back(ho[i]);
+        else
+            expect.push_back(ho[i]);
+    }
+    std::sort(expect.begin(), expect.end());
+    std::sort(popped.begin(), popped.end());
+    CHECK(std::adjacent_find(popped.begin(), popped.end()) == popped.end()); // no element popped twice
+    std::vector<std::uint32_t> rest;
+    std::set_difference(expect.begin(), expect.end(), popped.begin(), popped.end(), std::back_inserter(rest));
+    CHECK(rest.size() + popped.size() == expect.size()); // every popped value was in the deque
+    std::vector<std::uint32_t> have = contents(q);
+    std::sort(have.begin(), have.end());
+    CHECK(have == rest && q.valid());
+    CHECK(q.size() == static_cast<stdgpu::index_t>(rest.size()));
+    (void)hipFree(out);
+    (void)hipFree(ok);
+    free_list::destroyDeviceObject(q);
+}
+
+static void
+voxel_free_list_pattern()
+{
+    g_case = "voxel blocks: pop a slot, emplace, push the slot back when the key existed";
+    // more free slots than threads: every pop finds a slot even when all threads pop before any
+    // pushes one back, so every one of the K blocks gets one
+    const std::size_t C = 1 << 19, N = 1 << 18;
+    const std::uint32_t K = 20000;
+    using map_t = stdgpu::unordered_map<std::uint32_t, std::uint32_t>;
+    free_list q = free_list::createDeviceObject(static_cast<stdgpu::index_t>(C));
+    (void)htab_deque_fill_iota(q.native_handle(), nullptr);
+    map_t map = map_t::createDeviceObject(1 << 16);
+    int* got = dev_alloc<int>(N);
+    hipLaunchKernelGGL(k_voxel, dim3(blocks(N)), dim3(256), 0, 0, q, map, N, K, got);
+    CHECK(sum(to_host(got, N)) == static_cast<int>(N));
+    CHECK(map.size() == static_cast<stdgpu::index_t>(K) && map.valid());
+    CHECK(q.size() == static_cast<stdgpu::index_t>(C - K) && q.valid());
+    // every slot number is either in the free list or owned by exactly one block
+    std::vector<std::uint32_t> all = contents(q);
+    auto range = map.device_range();
+    CHECK(range.size() == static_cast<stdgpu::index_t>(K));
+    std::uint32_t* d = dev_alloc<std::uint32_t>(K);
+    hipLaunchKernelGGL((k_owned_slots<decltype(range)>), dim3(blocks(K)), dim3(256), 0, 0, range, d);
+    const std::vector<std::uint32_t> owned = to_host(d, K);
+    all.insert(all.end(), owned.begin(), owned.end());
+    std::sort(all.begin(), all.end());
+    std::vector<std::uint32_t> want(C);
+    std::iota(want.begin(), want.end(), 0U);
+    CHECK(all == want);
+    (void)hipFree(d);
+    (void)hipFree(got);
+    map_t::destroyDeviceObject(map);
+    free_list::destroyDeviceObject(q);
+}
+
+static void
+wrap_around_many_turns()
+{
+    g_case = "pops and pushes around the ring for many turns";
+    const std::size_t C = 1000, N = 700;
+    free_list q = free_list::createDeviceObject(static_cast<stdgpu::index_t>(C));
+    (void)htab_deque_fill_iota(q.native_handle(), nullptr);
+    std::uint32_t* out = dev_alloc<std::uint32_t>(N);
+    int* ok = dev_alloc<int>(N);
+    std::uint8_t* pushed = dev_alloc<std::uint8_t>(N);
+    for (int round = 0; round < 40; ++round)
+    {
+        hipLaunchKernelGGL(k_pop, dim3(blocks(N)), dim3(256), 0, 0, q, N, out, ok);
+        CHECK(sum(to_host(ok, N)) == static_cast<int>(N));
+        CHECK(q.size() == static_cast<stdgpu::index_t>(C - N));
+        // the popped values go back: per thread on even rounds, by the bulk call on odd ones
+        if (round % 2 == 0)
+        {
+            hipLaunchKernelGGL(k_push_values, dim3(blocks(N)), dim3(256), 0, 0, q, out, N, ok);
+            CHECK(sum(to_host(ok, N)) == static_cast<int>(N));
+        }
+        else
+        {
+            CHECK(htab_deque_push_back(q.native_handle(), out, static_cast<std::int64_t>(N), pushed, nullptr) ==
+                  HTAB_OK);
+            const std::vector<std::uint8_t> hp = to_host(pushed, N);
+            CHECK(std::all_of(hp.begin(), hp.end(), [](std::uint8_t x) { return x == 1; }));
+        }
+        CHECK(q.size() == static_cast<stdgpu::index_t>(C) && q.full());
+    }
+    // 40 x 700 pops took the ring around 28 times; it still holds a permutation of 0..C-1
+    std::vector<std::uint32_t> have = contents(q);
+    std::sort(have.begin(), have.end());
+    std::vector<std::uint32_t> want(C);
+    std::iota(want.begin(), want.end(), 0U);
+    CHECK(have == want && q.valid());
+    q.clear();
+    CHECK(q.empty() && q.valid());
+    // a cleared deque takes pushes from turn 0 again
+    hipLaunchKernelGGL(k_push, dim3(blocks(N)), dim3(256), 0, 0, q, 5U, N, ok);
+    CHECK(sum(to_host(ok, N)) == static_cast<int>(N) && q.size() == static_cast<stdgpu::index_t>(N));
+    (void)hipFree(out);
+    (void)hipFree(ok);
+    (void)hipFree(pushed);
+    free_list::destroyDeviceObject(q);
+}
+
+static void
+bulk_and_per_thread_calls_interleave()
+{
+    g_case = "bulk calls and per-thread calls on one deque";
+    const std::size_t C = 4096;
+    free_list q = free_list::createDeviceObject(static_cast<stdgpu::index_t>(C));
+    // per-thread pushes of 0..2999, then a bulk pop of 1000 returns the front 1000 in ring order
+    int* ok = dev_alloc<int>(3000);
+    hipLaunchKernelGGL(k_push, dim3(blocks(3000)), dim3(256), 0, 0, q, 0U, std::size_t(3000), ok);
+    CHECK(sum(to_host(ok, 3000)) == 3000);
+    const std::vector<std::uint32_t> before = contents(q);
+    std::uint32_t* out = dev_alloc<std::uint32_t>(1000);
+    std::uint8_t* popped = dev_alloc<std::uint8_t>(1000);
+    CHECK(htab_deque_pop_front(q.native_handle(), out, 1000, popped, nullptr) == HTAB_OK);
+    const std::vector<std::uint32_t> bulk = to_host(out, 1000);
+    CHECK(std::vector<std::uint32_t>(before.begin(), before.begin() + 1000) == bulk);
+    // bulk push of 2000 more, then per-thread pops take the remaining 2000 + 2000 in some order
+    std::vector<std::uint32_t> extra(2000);
+    std::iota(extra.begin(), extra.end(), 100000U);
+    std::uint32_t* dx = dev_alloc<std::uint32_t>(2000);
+    (void)hipMemcpy(dx, extra.data(), 2000 * sizeof(std::uint32_t), hipMemcpyHostToDevice);
+    std::uint8_t* pushed = dev_alloc<std::uint8_t>(2000);
+    CHECK(htab_deque_push_back(q.native_handle(), dx, 2000, pushed, nullptr) == HTAB_OK);
+    std::uint32_t* out2 = dev_alloc<std::uint32_t>(5000);
+    int* ok2 = dev_alloc<int>(5000);
+    hipLaunchKernelGGL(k_pop, dim3(blocks(5000)), dim3(256), 0, 0, q, std::size_t(5000), out2, ok2);
+    const std::vector<int> hk = to_host(ok2, 5000);
+    const std::vector<std::uint32_t> ho = to_host(out2, 5000);
+    std::vector<std::uint32_t> got;
+    for (std::size_t i = 0; i < 5000; ++i)
+        if (hk[i]) got.push_back(ho[i]);
+    std::vector<std::uint32_t> want(before.begin() + 1000, before.end());
+    want.insert(want.end(), extra.begin(), extra.end());
+    std::sort(got.begin(), got.end());
+    std::sort(want.begin(), want.end());
+    CHECK(sum(hk) == 4000 && got == want && q.empty());
+    (void)hipFree(ok);
+    (void)hipFree(out);
+    (void)hipFree(popped);
+    (void)hipFree(dx);
+    (void)hipFree(pushed);
+    (void)hipFree(out2);
+    (void)hipFree(ok2);
+    free_list::destroyDeviceObject(q);
+}
+
+static void
+per_thread_rates()
+{
+    g_case = "per-thread pop_front / push_back rates, 2^22 elements";
+    const std::size_t C = 1 << 22;
+    free_list q = free_list::createDeviceObject(static_cast<stdgpu::index_t>(C));
+    (void)htab_deque_fill_iota(q.native_handle(), nullptr);
+    std::uint32_t* out = dev_alloc<std::uint32_t>(C);
+    int* ok = dev_alloc<int>(C);
+    hipEvent_t e0, e1, e2;
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    (void)hipEventCreate(&e2);
+    float pop_ms = 0, push_ms = 0;
+    for (int rep = 0; rep < 3; ++rep) // the last repetition is reported
+    {
+        (void)hipEventRecord(e0, nullptr);
+        hipLaunchKernelGGL(k_pop, dim3(blocks(C)), dim3(256), 0, 0, q, C, out, ok);
+        (void)hipEventRecord(e1, nullptr);
+        hipLaunchKernelGGL(k_push_values, dim3(blocks(C)), dim3(256), 0, 0, q, out, C, ok);
+        (void)hipEventRecord(e2, nullptr);
+        (void)hipEventSynchronize(e2);
+        (void)hipEventElapsedTime(&pop_ms, e0, e1);
+        (void)hipEventElapsedTime(&push_ms, e1, e2);
+    }
+    CHECK(sum(to_host(ok, C)) == static_cast<int>(C) && q.full() && q.valid());
+    std::printf("rate: pop_front %.1f G/s, push_back %.1f G/s (%zu elements, one reservation per wavefront)\n",
+                static_cast<double>(C) / pop_ms / 1e6, static_cast<double>(C) / push_ms / 1e6, C);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipEventDestroy(e2);
+    (void)hipFree(out);
+    (void)hipFree(ok);
+    free_list::destroyDeviceObject(q);
+}
+
+static void
+other_value_types()
+{
+    g_case = "deque<int> and deque<float>";
+    auto qi = stdgpu::deque<int>::createDeviceObject(10);
+    auto qf = stdgpu::deque<float>::createDeviceObject(10);
+    CHECK(qi.empty() && qf.empty() && qi.max_size() == 10);
+    stdgpu::deque<int>::destroyDeviceObject(qi);
+    stdgpu::deque<float>::destroyDeviceObject(qf);
+}
+
+int
+main()
+{
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+    {
+        std::printf("FAIL no GPU\n");
+        return 2;
+    }
+    void (*cases[])() = { push_to_full_and_pop_to_empty, mixed_pushes_and_pops_in_diverged_lanes,
+                          voxel_free_list_pattern, wrap_around_many_turns, bulk_and_per_thread_calls_interleave,
+                          per_thread_rates, other_value_types };
+    for (auto c : cases)
+    {
+        const int before = g_failures;
+        try
+        {
+            c();
+        }
+        catch (const std::exception& e)
+        {
+            std::printf("FAIL %s: exception %s\n", g_case, e.what());
+            ++g_failures;
+        }
+        std::printf("%s %s\n", g_failures == before ? "PASS" : "FAIL", g_case);
+    }
+    std::printf("OK: %d failure(s)\n", g_failures);
+    return g_failures == 0 ? 0 : 1;
+}
