@@ -298,6 +298,16 @@ int htab_bitset_test(htab_bitset* bits, const int64_t* positions, int64_t n, uin
 int htab_bitset_all_bits(htab_bitset* bits, int op, void* stream);
 int htab_bitset_count(htab_bitset* bits, int64_t* out, void* stream);
 
+typedef struct htab_bitset_view
+{
+    uint32_t* blocks; /* ceil(nbits / 32) blocks; bit n is bit n % 32 of block n / 32 */
+    int64_t nbits;
+} htab_bitset_view;
+
+/* device pointer and size for user kernels that set / reset / flip / test single bits (the C++
+ * stdgpu::bitset); no reference counterpart: the reference's bitset object is itself the device handle */
+int htab_bitset_get_view(const htab_bitset* bits, htab_bitset_view* out);
+
 /* ------------------------------------------------------------------------------------------
  * atomic<T> / atomic_ref<T> (src/stdgpu/atomic.cuh) for T = int, unsigned int,
  * unsigned long long int, float (atomic.cuh:137-139).  A bulk call applies n operations, each as the

@@ -104,6 +104,10 @@ class htab_snapshot_header(ctypes.Structure):
     ]
 
 
+class htab_bitset_view(ctypes.Structure):
+    _fields_ = [("blocks", ctypes.c_void_p), ("nbits", ctypes.c_int64)]
+
+
 class htab_deque_view(ctypes.Structure):
     _fields_ = [
         ("data", ctypes.c_void_p),
@@ -184,6 +188,7 @@ SIGNATURES = {
     "htab_bitset_test": (_I32, [_P, _P, _I64, _P, _P]),
     "htab_bitset_all_bits": (_I32, [_P, _I32, _P]),
     "htab_bitset_count": (_I32, [_P, ctypes.POINTER(_I64), _P]),
+    "htab_bitset_get_view": (_I32, [_P, ctypes.POINTER(htab_bitset_view)]),
     "htab_atomic_create": (_I32, [_I32, _I64, ctypes.POINTER(_P)]),
     "htab_atomic_destroy": (_I32, [_P]),
     "htab_atomic_data": (_I32, [_P, ctypes.POINTER(_P)]),

@@ -77,7 +77,7 @@ def build_roofline() -> str:
     return build_library(ROOFLINE_LIB, ["roofline.hip"])
 
 
-CPP_TESTS = ["test_facade", "test_atomic", "test_deque"]
+CPP_TESTS = ["test_facade", "test_atomic", "test_deque", "test_bitset"]
 
 
 def build_cpp_tests() -> list[str]:

@@ -187,6 +187,15 @@ htab_bitset_all_bits(htab_bitset* b, int op, void* stream)
 }
 
 int
+htab_bitset_get_view(const htab_bitset* b, htab_bitset_view* out)
+{
+    if (!b || !out) return HTAB_E_ARG;
+    out->blocks = b->blocks;
+    out->nbits = b->nbits;
+    return HTAB_OK;
+}
+
+int
 htab_bitset_count(htab_bitset* b, int64_t* out, void* stream)
 {
     if (!b || !out) return HTAB_E_ARG;

@@ -42,6 +42,7 @@ def test_struct_layouts_match_header():
     assert ctypes.sizeof(_lib.htab_info) == 8 + 8 * 8
     assert ctypes.sizeof(_lib.htab_view) == 3 * 8 + 2 * 8 + 2 * 4 + 2 * 8 + 8
     assert ctypes.sizeof(_lib.htab_deque_view) == 7 * 8
+    assert ctypes.sizeof(_lib.htab_bitset_view) == 2 * 8
 
 
 def test_argument_errors_without_gpu():

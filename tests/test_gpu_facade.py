@@ -1,5 +1,5 @@
 """Runs the C++ source-compatibility tests of the stdgpu facade (tests/cpp/test_facade.hip,
-test_atomic.hip and test_deque.hip, built by `python -m stdgpu_amd.build`) on the GPU: user kernels call the native device API, host code the
+test_atomic.hip, test_deque.hip and test_bitset.hip, built by `python -m stdgpu_amd.build`) on the GPU: user kernels call the native device API, host code the
 agnostic API, as in the reference's own test suite."""
 import os
 import subprocess
@@ -12,7 +12,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 BUILD = os.path.join(ROOT, "tests", "cpp", "_build")
 
 
-@pytest.mark.parametrize("name", ["test_facade", "test_atomic", "test_deque"])
+@pytest.mark.parametrize("name", ["test_facade", "test_atomic", "test_deque", "test_bitset"])
 def test_cpp_facade_cases(name):
     exe = os.path.join(BUILD, name)
     assert os.path.exists(exe), "build the C++ tests first: python -m stdgpu_amd.build"
