@@ -349,7 +349,8 @@ int htab_atomic_destroy(htab_atomic* a);
 int htab_atomic_data(const htab_atomic* a, void** out);
 int htab_atomic_apply(htab_atomic* a, int op, const int64_t* targets, const void* operands, const void* expected,
                       int64_t n, void* old_out, uint8_t* done_out, void* stream);
-/* the same over count values of the given type at data (device memory): atomic_ref<T> */
+/* the same over count values of the given type at data (device memory, aligned to the value size;
+ * HTAB_E_ARG otherwise): atomic_ref<T> */
 int htab_atomic_ref_apply(int type, void* data, int64_t count, int op, const int64_t* targets, const void* operands,
                           const void* expected, int64_t n, void* old_out, uint8_t* done_out, void* stream);
 /* host-side load() / store() of element index (atomic_detail.cuh:461-526); value in host or device memory */

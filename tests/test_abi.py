@@ -133,3 +133,6 @@ def test_container_argument_errors_without_gpu():
     assert ref(_lib.HTAB_ATOMIC_F32, buf, 4, ops["fetch_and"], None, buf, None, 1, None, None, None) == E
     assert ref(_lib.HTAB_ATOMIC_I32, buf, 4, ops["fetch_inc_mod"], None, buf, None, 1, None, None, None) == E
     assert ref(_lib.HTAB_ATOMIC_U64, buf, 4, ops["fetch_dec_mod"], None, buf, None, 1, None, None, None) == E
+    # values must be naturally aligned for the device atomics
+    odd = ctypes.c_void_p(0x1004)
+    assert ref(_lib.HTAB_ATOMIC_U64, odd, 4, ops["fetch_add"], None, buf, None, 1, None, None, None) == E
