@@ -23,7 +23,8 @@ extern "C" {
 
 /* status codes */
 #define HTAB_OK 0
-#define HTAB_E_ARG (-1)     /* invalid argument (null handle, bad kind, size out of range) */
+#define HTAB_E_ARG (-1)     /* invalid argument (null handle, bad kind, size out of range, a key / value
+                               array not aligned to its element size) */
 #define HTAB_E_OOM (-2)     /* device allocation failed */
 #define HTAB_E_HIP (-3)     /* a HIP runtime call failed */
 #define HTAB_E_NODEV (-4)   /* no usable GPU */

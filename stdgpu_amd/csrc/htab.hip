@@ -100,6 +100,23 @@ key_bytes_of(int kind)
     return (kind == HTAB_SET_U32 || kind == HTAB_MAP_U32_U32) ? 4 : 8;
 }
 
+// device loads and stores of keys, values and pairs need them naturally aligned (null passes)
+static bool
+aligned(const void* p, int bytes)
+{
+    return p == nullptr || bytes <= 1 || reinterpret_cast<uintptr_t>(p) % static_cast<uintptr_t>(bytes) == 0;
+}
+
+static int
+check_alignment(int kind, const void* values, const void* keys, const void* vals, const char* what)
+{
+    const int kb = key_bytes_of(kind);
+    const int mb = mapped_bytes_of(kind);
+    if (!aligned(values, kb > mb ? kb : mb) || !aligned(keys, kb) || !aligned(vals, mb))
+        return fail(HTAB_E_ARG, what);
+    return HTAB_OK;
+}
+
 // Reference sizing (src/stdgpu/impl/unordered_base_detail.cuh:34-48, 1263-1273): bucket_count =
 // bit_ceil(ceil(float(capacity) / 1.0f)); excess = ceil(n (1 - (1 - 1/k)^(n-1))) + 1 in long double.
 void
@@ -374,6 +391,8 @@ insert_common(htab* h, const void* values, const void* keys, const void* vals, i
     if (n < 0) return fail(HTAB_E_ARG, "insert: negative count");
     if (n == 0) return HTAB_OK;
     if (!values && !keys) return fail(HTAB_E_ARG, "insert: null input");
+    if (check_alignment(h->kind, values, keys, vals, "insert: keys, values or pairs not aligned to their size"))
+        return HTAB_E_ARG;
     if ((n + h->block - 1) / h->block > 0x7FFFFFFF) return fail(HTAB_E_LIMIT, "insert: too many elements");
     hipStream_t s = static_cast<hipStream_t>(stream);
     htab_call c;
@@ -409,6 +428,8 @@ htab_find(htab* h, const void* keys, int64_t n, uint8_t* hits, void* vals_out, v
     if (n < 0) return fail(HTAB_E_ARG, "htab_find: negative count");
     if (n == 0) return HTAB_OK;
     if (!keys || !hits) return fail(HTAB_E_ARG, "htab_find: null keys or hits");
+    if (check_alignment(h->kind, nullptr, keys, vals_out, "htab_find: keys or values not aligned to their size"))
+        return HTAB_E_ARG;
     if ((n + h->block - 1) / h->block > 0x7FFFFFFF) return fail(HTAB_E_LIMIT, "htab_find: too many elements");
     hipStream_t s = static_cast<hipStream_t>(stream);
     htab_call c;
@@ -429,6 +450,8 @@ htab_erase(htab* h, const void* keys, int64_t n, uint8_t* erased, void* stream)
     if (n < 0) return fail(HTAB_E_ARG, "htab_erase: negative count");
     if (n == 0) return HTAB_OK;
     if (!keys) return fail(HTAB_E_ARG, "htab_erase: null keys");
+    if (check_alignment(h->kind, nullptr, keys, nullptr, "htab_erase: keys not aligned to their size"))
+        return HTAB_E_ARG;
     if ((n + h->block - 1) / h->block > 0x7FFFFFFF) return fail(HTAB_E_LIMIT, "htab_erase: too many elements");
     hipStream_t s = static_cast<hipStream_t>(stream);
     htab_call c;
@@ -448,6 +471,8 @@ htab_assign(htab* h, const void* keys, const void* vals, int64_t n, uint8_t* fou
     if (n < 0) return fail(HTAB_E_ARG, "htab_assign: negative count");
     if (n == 0) return HTAB_OK;
     if (!keys || !vals) return fail(HTAB_E_ARG, "htab_assign: null keys or values");
+    if (check_alignment(h->kind, nullptr, keys, vals, "htab_assign: keys or values not aligned to their size"))
+        return HTAB_E_ARG;
     if ((n + 255) / 256 > 0x7FFFFFFF) return fail(HTAB_E_LIMIT, "htab_assign: too many elements");
     htab_call c;
     c.op = call_op::assign;

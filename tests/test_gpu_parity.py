@@ -438,3 +438,32 @@ def test_config_map_u64_zipf_full_size_properties():
     assert int(ins2.sum()) == w["reinsert"].numel() and tab.size() == w["distinct"]
     assert tab.valid()
     sa.unordered_map_u64_u32.destroyDeviceObject(tab)
+
+
+def test_misaligned_inputs_are_refused():
+    """keys, values and pairs must be aligned to their size; a misaligned pointer is refused before any
+    device work (HTAB_E_ARG) instead of faulting"""
+    import ctypes
+
+    from stdgpu_amd import _lib
+
+    lib = _lib.load()
+    t = sa.unordered_map_u64_u32.createDeviceObject(1000)
+    buf = torch.zeros(64, dtype=torch.int64, device="cuda")
+    base = buf.data_ptr()
+    s = torch.cuda.current_stream().cuda_stream
+    h = t._h
+    E = _lib.HTAB_E_ARG
+    assert lib.htab_insert_kv(h, base + 4, base, 4, None, s) == E  # u64 keys at a 4-byte offset
+    assert lib.htab_insert_kv(h, base, base + 2, 4, None, s) == E  # u32 values at a 2-byte offset
+    assert lib.htab_insert(h, base + 4, 4, None, s) == E  # 16-byte pairs need 8-byte alignment
+    hits = torch.zeros(4, dtype=torch.uint8, device="cuda")
+    assert lib.htab_find(h, base + 4, 4, hits.data_ptr(), None, s) == E
+    assert lib.htab_find(h, base, 4, hits.data_ptr(), base + 1, s) == E
+    assert lib.htab_erase(h, base + 4, 4, None, s) == E
+    assert lib.htab_assign(h, base, base + 2, 4, None, s) == E
+    # aligned calls on the same buffers go through, and the table is untouched by the refused ones
+    assert lib.htab_insert_kv(h, base, base + 256, 4, None, s) == 0
+    torch.cuda.synchronize()
+    assert t.size() == 1 and t.valid()  # four zero keys: one distinct key
+    sa.unordered_map_u64_u32.destroyDeviceObject(t)
