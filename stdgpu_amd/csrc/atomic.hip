@@ -381,8 +381,11 @@ htab_atomic_ref_apply(int type, void* data, int64_t count, int op, const int64_t
                       const void* expected, int64_t n, void* old_out, uint8_t* done_out, void* stream)
 {
     if (type_bytes(type) == 0 || !data || count <= 0 || n < 0) return HTAB_E_ARG;
-    // device atomics need naturally aligned values
-    if (reinterpret_cast<uintptr_t>(data) % type_bytes(type) != 0) return HTAB_E_ARG;
+    // device atomics and loads need naturally aligned values, operands and targets
+    const uintptr_t w = type_bytes(type);
+    for (const void* p : { static_cast<const void*>(data), operands, expected, static_cast<const void*>(old_out) })
+        if (p && reinterpret_cast<uintptr_t>(p) % w != 0) return HTAB_E_ARG;
+    if (targets && reinterpret_cast<uintptr_t>(targets) % 8 != 0) return HTAB_E_ARG;
     if (op < HTAB_ATOMIC_LOAD || op > HTAB_ATOMIC_FETCH_DEC_MOD) return HTAB_E_ARG;
     if (n > 0 && op != HTAB_ATOMIC_LOAD && !operands) return HTAB_E_ARG;
     if (n > 0 && op == HTAB_ATOMIC_COMPARE_EXCHANGE && !expected) return HTAB_E_ARG;

@@ -92,6 +92,12 @@ k_zero_counter(unsigned long long* c)
     *c = 0;
 }
 
+bool
+aligned(const void* p, uintptr_t bytes)
+{
+    return p == nullptr || reinterpret_cast<uintptr_t>(p) % bytes == 0;
+}
+
 int
 bs_status()
 {
@@ -153,6 +159,7 @@ htab_bitset_update(htab_bitset* b, int op, const int64_t* positions, int64_t n, 
                    void* stream)
 {
     if (!b || n < 0 || (n > 0 && !positions) || op < HTAB_BIT_SET || op > HTAB_BIT_FLIP) return HTAB_E_ARG;
+    if (!aligned(positions, 8)) return HTAB_E_ARG;
     if (n == 0) return HTAB_OK;
     const int64_t grid = (n + 255) / 256;
     if (grid > 0x7FFFFFFF) return HTAB_E_LIMIT;
@@ -165,6 +172,7 @@ int
 htab_bitset_test(htab_bitset* b, const int64_t* positions, int64_t n, uint8_t* out, void* stream)
 {
     if (!b || n < 0 || (n > 0 && (!positions || !out))) return HTAB_E_ARG;
+    if (!aligned(positions, 8)) return HTAB_E_ARG;
     if (n == 0) return HTAB_OK;
     const int64_t grid = (n + 255) / 256;
     if (grid > 0x7FFFFFFF) return HTAB_E_LIMIT;

@@ -123,6 +123,12 @@ k_deque_copy(const uint32_t* __restrict__ data, int64_t cap, uint64_t begin, int
     if (i < n) out[i] = data[(begin + static_cast<uint64_t>(i)) % static_cast<uint64_t>(cap)];
 }
 
+bool
+aligned4(const void* p)
+{
+    return p == nullptr || reinterpret_cast<uintptr_t>(p) % 4 == 0;
+}
+
 int
 dq_status()
 {
@@ -197,7 +203,7 @@ htab_deque_destroy(htab_deque* q)
 int
 htab_deque_push_back(htab_deque* q, const uint32_t* values, int64_t n, uint8_t* pushed, void* stream)
 {
-    if (!q || n < 0 || (n > 0 && !values)) return HTAB_E_ARG;
+    if (!q || n < 0 || (n > 0 && !values) || !aligned4(values)) return HTAB_E_ARG;
     if (n == 0) return HTAB_OK;
     const int64_t grid = (n + 255) / 256;
     if (grid > 0x7FFFFFFF) return HTAB_E_LIMIT;
@@ -211,7 +217,7 @@ htab_deque_push_back(htab_deque* q, const uint32_t* values, int64_t n, uint8_t* 
 int
 htab_deque_pop_front(htab_deque* q, uint32_t* out, int64_t n, uint8_t* popped, void* stream)
 {
-    if (!q || n < 0 || (n > 0 && !out)) return HTAB_E_ARG;
+    if (!q || n < 0 || (n > 0 && !out) || !aligned4(out)) return HTAB_E_ARG;
     if (n == 0) return HTAB_OK;
     const int64_t grid = (n + 255) / 256;
     if (grid > 0x7FFFFFFF) return HTAB_E_LIMIT;

@@ -68,6 +68,12 @@ k_vector_iota(uint32_t* __restrict__ data, int64_t* ctl, int64_t cap, uint32_t f
     if (blockIdx.x == 0 && threadIdx.x == 0) ctl[kSize] = cap;
 }
 
+bool
+aligned4(const void* p)
+{
+    return p == nullptr || reinterpret_cast<uintptr_t>(p) % 4 == 0;
+}
+
 int
 vec_status()
 {
@@ -139,7 +145,7 @@ htab_vector_destroy(htab_vector* q)
 int
 htab_vector_push_back(htab_vector* q, const uint32_t* values, int64_t n, uint8_t* pushed, void* stream)
 {
-    if (!q || n < 0 || (n > 0 && !values)) return HTAB_E_ARG;
+    if (!q || n < 0 || (n > 0 && !values) || !aligned4(values)) return HTAB_E_ARG;
     if (n == 0) return HTAB_OK;
     const int64_t grid = (n + 255) / 256;
     if (grid > 0x7FFFFFFF) return HTAB_E_LIMIT;
@@ -153,7 +159,7 @@ htab_vector_push_back(htab_vector* q, const uint32_t* values, int64_t n, uint8_t
 int
 htab_vector_pop_back(htab_vector* q, uint32_t* out, int64_t n, uint8_t* popped, void* stream)
 {
-    if (!q || n < 0 || (n > 0 && !out)) return HTAB_E_ARG;
+    if (!q || n < 0 || (n > 0 && !out) || !aligned4(out)) return HTAB_E_ARG;
     if (n == 0) return HTAB_OK;
     const int64_t grid = (n + 255) / 256;
     if (grid > 0x7FFFFFFF) return HTAB_E_LIMIT;

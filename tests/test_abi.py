@@ -136,3 +136,6 @@ def test_container_argument_errors_without_gpu():
     # values must be naturally aligned for the device atomics
     odd = ctypes.c_void_p(0x1004)
     assert ref(_lib.HTAB_ATOMIC_U64, odd, 4, ops["fetch_add"], None, buf, None, 1, None, None, None) == E
+    assert ref(_lib.HTAB_ATOMIC_U64, buf, 4, ops["fetch_add"], None, odd, None, 1, None, None, None) == E  # operands
+    assert ref(_lib.HTAB_ATOMIC_U32, buf, 4, ops["fetch_add"], ctypes.c_void_p(0x1002), buf, None, 1, None, None,
+               None) == E  # int64 targets
