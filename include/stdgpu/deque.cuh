@@ -4,11 +4,11 @@
 // Host: createDeviceObject / destroyDeviceObject, size / empty / full / capacity / max_size, clear,
 // valid.  Device (user kernels, per thread): push_back / emplace_back, pop_front, size / empty / full,
 // operator[] / at / front / back (include/stdgpu_amd/deque_device.hpp).  The calling lanes of a
-// wavefront share one compare-and-swap on the back or front counter.  A pop whose element is pushed
-// but not yet written, or a push whose slot is still being popped, returns failure at once where the
-// reference spins on a per-slot lock; pushes and pops may be mixed in one kernel.  push_front and
-// pop_back are not provided.  Bulk pushes and pops of many values per call: htab_deque_* in
-// <stdgpu_amd/htab.h>.
+// wavefront reserve from the published-element or released-slot count with one atomic and take
+// consecutive ring positions with one more; a pop whose element is still being written, or a push
+// whose slot is still being read, waits on that slot's sequence word as the reference waits on its
+// per-slot lock.  Pushes and pops may be mixed in one kernel.  push_front and pop_back are not
+// provided.  Bulk pushes and pops of many values per call: htab_deque_* in <stdgpu_amd/htab.h>.
 #ifndef STDGPU_DEQUE_CUH
 #define STDGPU_DEQUE_CUH
 

@@ -117,6 +117,14 @@ check_alignment(int kind, const void* values, const void* keys, const void* vals
     return HTAB_OK;
 }
 
+bool
+segment_aligned(int kind, const htab_segment& g)
+{
+    const int kb = key_bytes_of(kind);
+    const int mb = mapped_bytes_of(kind);
+    return aligned(g.keys, kb) && aligned(g.vals, mb) && aligned(g.vals_out, mb);
+}
+
 // Reference sizing (src/stdgpu/impl/unordered_base_detail.cuh:34-48, 1263-1273): bucket_count =
 // bit_ceil(ceil(float(capacity) / 1.0f)); excess = ceil(n (1 - (1 - 1/k)^(n-1))) + 1 in long double.
 void
@@ -657,7 +665,7 @@ htab_batch_run_segments(htab_batch* b, const htab_segment* segs, int32_t nsegs, 
     for (int32_t g = 0; g < nsegs; ++g)
     {
         if (segs[g].table < 0 || segs[g].table >= b->ntables || segs[g].n < 0 || segs[g].op < HTAB_OP_INSERT ||
-            segs[g].op > HTAB_OP_ERASE || (segs[g].n > 0 && !segs[g].keys))
+            segs[g].op > HTAB_OP_ERASE || (segs[g].n > 0 && !segs[g].keys) || !segment_aligned(b->kind, segs[g]))
             return fail(HTAB_E_ARG, "htab_batch_run_segments: bad segment");
     }
     if (nsegs == 0) return HTAB_OK;

@@ -24,6 +24,8 @@ int launch_status(const char* what);
 int value_bytes_of(int kind);
 int mapped_bytes_of(int kind);
 bool ovf_in_bucket_of(int kind);
+// a segment's keys, values and value outputs are aligned to their element sizes for the table kind
+bool segment_aligned(int kind, const htab_segment& g);
 int key_bytes_of(int kind);
 void reference_sizing(int64_t capacity, int64_t* bucket_count, int64_t* max_size);
 

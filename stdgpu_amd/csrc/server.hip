@@ -137,7 +137,8 @@ htab_server_run(htab_server* s, const htab_segment* reqs, const int64_t* arrival
     for (int64_t i = 0; i < nreq; ++i)
     {
         if (reqs[i].table < 0 || reqs[i].table >= ntables || reqs[i].op < HTAB_OP_INSERT || reqs[i].op > HTAB_OP_ERASE ||
-            reqs[i].n < 0 || reqs[i].n > 0xFFFFFFFFLL || (reqs[i].n > 0 && !reqs[i].keys))
+            reqs[i].n < 0 || reqs[i].n > 0xFFFFFFFFLL || (reqs[i].n > 0 && !reqs[i].keys) ||
+            !segment_aligned(s->batch->kind, reqs[i]))
             return fail(HTAB_E_ARG, "htab_server_run: bad request");
         status[i] = 1; // pending
     }

@@ -234,3 +234,45 @@ def test_per_table_baseline_matches_batched():
     assert torch.equal(f1, f2)
     for t in range(len(bt)):
         assert bt[t].size() == pt[t].size()
+
+
+def test_misaligned_segments_and_requests_are_refused():
+    """the batched and served forms take per-segment pointers: keys, values and value outputs must be
+    aligned to their size, else the whole call is refused (HTAB_E_ARG) before any device work"""
+    import ctypes
+
+    from stdgpu_amd import _lib
+    from stdgpu_amd import serving as sv
+
+    lib = _lib.load()
+    tabs = [sa.unordered_map_u64_u32.createDeviceObject(1000) for _ in range(2)]
+    batch = TableBatch(tabs)
+    srv = sv.Server(tabs, nstreams=1)
+    buf = torch.zeros(256, dtype=torch.int64, device="cuda")
+    base = buf.data_ptr()
+    s = torch.cuda.current_stream().cuda_stream
+    E = _lib.HTAB_E_ARG
+    INS = _lib.HTAB_OP_INSERT
+    bad = [
+        _lib.htab_segment(base + 4, base + 1024, None, None, 4, 1, INS),  # u64 keys at a 4-byte offset
+        _lib.htab_segment(base, base + 1026, None, None, 4, 1, INS),  # u32 values at a 2-byte offset
+        _lib.htab_segment(base, None, base + 1500, base + 1026, 4, 1, _lib.HTAB_OP_FIND),  # value outputs
+    ]
+    good = _lib.htab_segment(base, base + 1024, None, None, 4, 0, INS)
+    arr_ns = np.zeros(2, np.int64)
+    sub, done = np.zeros(2, np.int64), np.zeros(2, np.int64)
+    st = np.zeros(2, np.int32)
+    stats = _lib.htab_server_stats()
+    p = lambda a: a.ctypes.data_as(ctypes.c_void_p)  # noqa: E731
+    for b in bad:
+        segs = (_lib.htab_segment * 2)(good, b)
+        assert lib.htab_batch_run_segments(batch._b, segs, 2, s) == E
+        assert lib.htab_server_run(srv._s, segs, p(arr_ns), 2, p(sub), p(done), p(st), ctypes.byref(stats)) == E
+    torch.cuda.synchronize()
+    assert tabs[0].size() == 0 and tabs[1].size() == 0  # the good segment of a refused call did not run
+    segs = (_lib.htab_segment * 1)(good)
+    assert lib.htab_batch_run_segments(batch._b, segs, 1, s) == 0
+    torch.cuda.synchronize()
+    assert tabs[0].size() == 1 and tabs[1].size() == 0
+    srv.destroy()
+    batch.destroy()
