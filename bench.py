@@ -236,57 +236,13 @@ def bench_served(args) -> dict:
 
 
 def bench_voxel(args) -> dict:
-    """Voxel-hashing pipeline: 30 frames of 2^24 surface points -> 64 mm block keys; per frame bulk
-    insert (new blocks get slots popped from a deque<uint32_t> of capacity 2^23), find of every point's
-    block, erase of blocks more than 20 m from the camera with their slots pushed back."""
-    import torch
-
-    import stdgpu_amd as sa
+    """Voxel-hashing pipeline (stdgpu_amd.voxel.measure): 30 frames of 2^24 surface points -> 64 mm
+    block keys; per frame bulk insert (new blocks get slots popped from a deque<uint32_t> of capacity
+    2^23), find of every point's block, erase of blocks more than 20 m from the camera with their slots
+    pushed back."""
     from stdgpu_amd import voxel as vx
 
-    frames, npts, cap = args.voxel_frames, 1 << args.voxel_log2_points, 1 << 23
-    scene = vx.make_scene(frames=frames)
-    # one untimed frame on throwaway objects: the allocator's first allocations of the frame's
-    # temporaries are not part of a frame
-    wtab, wfl = sa.unordered_map_u64_u32.createDeviceObject(cap), sa.deque_u32(cap)
-    wfl.fill_iota()
-    vx.run_frame(wtab, wfl, vx.frame_keys_device(scene, 0, npts, "cuda"), scene.cams[0], check=False)
-    torch.cuda.synchronize()
-    wfl.destroy()
-    sa.unordered_map_u64_u32.destroyDeviceObject(wtab)
-    tab = sa.unordered_map_u64_u32.createDeviceObject(cap)
-    fl = sa.deque_u32(cap)
-    fl.fill_iota()
-    stream = torch.cuda.current_stream()
-    per_frame_ms, new_blocks, erased = [], [], []
-    for f in range(frames):
-        keys = vx.frame_keys_device(scene, f, npts, "cuda")
-        torch.cuda.synchronize()
-        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        a.record(stream)
-        out = vx.run_frame(tab, fl, keys, scene.cams[f], check=False)
-        b.record(stream)
-        b.synchronize()
-        per_frame_ms.append(a.elapsed_time(b))
-        new_blocks.append(out["new_blocks"])
-        erased.append(out["erased"])
-    ok = vx.free_list_invariant(tab, fl, cap) and tab.valid()
-    ops = frames * 2 * npts + sum(erased)
-    res = {
-        "workload": f"{frames} frames x 2^{args.voxel_log2_points} points on 200 spheres + 50 planes in a 50 m cube, "
-                    "8 mm voxels in 8^3 blocks; insert+slot pop, find, erase > 20 m + slot push",
-        "frames": frames,
-        "gops": round(ops / (sum(per_frame_ms) / 1e3) / 1e9, 3),
-        "frame_ms_mean": round(float(np.mean(per_frame_ms)), 3),
-        "frame_ms_p99": round(float(np.percentile(per_frame_ms, 99)), 3),
-        "new_blocks_per_frame_mean": int(np.mean(new_blocks)),
-        "erased_per_frame_mean": int(np.mean(erased)),
-        "blocks_resident_end": tab.size(),
-        "free_list_and_table_consistent": bool(ok),
-    }
-    fl.destroy()
-    sa.unordered_map_u64_u32.destroyDeviceObject(tab)
-    return res
+    return vx.measure(frames=args.voxel_frames, log2_points=args.voxel_log2_points)
 
 
 def random_access_bound(args, tab, rates: dict) -> dict:

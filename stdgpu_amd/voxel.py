@@ -163,3 +163,72 @@ def free_list_invariant(tab, freelist, capacity: int) -> bool:
     if allv.numel() != capacity:
         return False
     return bool(torch.equal(torch.sort(allv)[0], torch.arange(capacity, device=allv.device)))
+
+
+def measure(frames: int = 30, log2_points: int = 24, capacity_log2: int = 23) -> dict:
+    """Times the frame pipeline on cuda:0: `frames` frames of 2^log2_points points, a map and a
+    free-list deque<uint32_t> of 2^capacity_log2 slots; HIP events around each frame (key generation
+    outside).  Returns the figures the benchmark reports."""
+    import torch
+
+    import stdgpu_amd as sa
+
+    npts, cap = 1 << log2_points, 1 << capacity_log2
+    scene = make_scene(frames=frames)
+    # one untimed frame on throwaway objects: the allocator's first allocations of the frame's
+    # temporaries are not part of a frame
+    wtab, wfl = sa.unordered_map_u64_u32.createDeviceObject(cap), sa.deque_u32(cap)
+    wfl.fill_iota()
+    run_frame(wtab, wfl, frame_keys_device(scene, 0, npts, "cuda"), scene.cams[0], check=False)
+    torch.cuda.synchronize()
+    wfl.destroy()
+    sa.unordered_map_u64_u32.destroyDeviceObject(wtab)
+    tab = sa.unordered_map_u64_u32.createDeviceObject(cap)
+    fl = sa.deque_u32(cap)
+    fl.fill_iota()
+    stream = torch.cuda.current_stream()
+    per_frame_ms, new_blocks, erased = [], [], []
+    for f in range(frames):
+        keys = frame_keys_device(scene, f, npts, "cuda")
+        torch.cuda.synchronize()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        out = run_frame(tab, fl, keys, scene.cams[f], check=False)
+        b.record(stream)
+        b.synchronize()
+        per_frame_ms.append(a.elapsed_time(b))
+        new_blocks.append(out["new_blocks"])
+        erased.append(out["erased"])
+    ok = free_list_invariant(tab, fl, cap) and tab.valid()
+    ops = frames * 2 * npts + sum(erased)
+    res = {
+        "workload": f"{frames} frames x 2^{log2_points} points on 200 spheres + 50 planes in a 50 m cube, "
+                    "8 mm voxels in 8^3 blocks; insert+slot pop, find, erase > 20 m + slot push",
+        "frames": frames,
+        "gops": round(ops / (sum(per_frame_ms) / 1e3) / 1e9, 3),
+        "frame_ms_mean": round(float(np.mean(per_frame_ms)), 3),
+        "frame_ms_p99": round(float(np.percentile(per_frame_ms, 99)), 3),
+        "new_blocks_per_frame_mean": int(np.mean(new_blocks)),
+        "erased_per_frame_mean": int(np.mean(erased)),
+        "blocks_resident_end": tab.size(),
+        "free_list_and_table_consistent": bool(ok),
+    }
+    fl.destroy()
+    sa.unordered_map_u64_u32.destroyDeviceObject(tab)
+    return res
+
+
+def main(argv=None) -> None:
+    """python -m stdgpu_amd.voxel [frames] [log2 points]: the timed frame pipeline alone (for
+    profiling), one JSON line."""
+    import json
+    import sys
+
+    argv = sys.argv[1:] if argv is None else argv
+    frames = int(argv[0]) if argv else 30
+    log2_points = int(argv[1]) if len(argv) > 1 else 24
+    print(json.dumps(measure(frames=frames, log2_points=log2_points)), flush=True)
+
+
+if __name__ == "__main__":
+    main()
