@@ -254,7 +254,10 @@ typedef struct htab_snapshot_header
 int htab_snapshot_bytes(const htab* table, int64_t* bytes);
 /* writes the snapshot into host memory of at least htab_snapshot_bytes bytes; waits for the copies */
 int htab_snapshot(htab* table, void* host_buffer, int64_t bytes, void* stream);
-/* a new table from a snapshot in host memory (HTAB_E_ARG for a buffer that is not a whole snapshot) */
+/* a new table from a snapshot in host memory (HTAB_E_ARG for a buffer that is not a whole snapshot).
+   The header and sizes are checked; the slot bytes are taken as they are (a snapshot altered after
+   htab_snapshot wrote it gives a table that htab_valid may reject, at a cost that grows with how far
+   its keys lie from their home buckets) */
 int htab_restore(const void* host_buffer, int64_t bytes, htab** out);
 
 /* ------------------------------------------------------------------------------------------
