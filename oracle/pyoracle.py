@@ -1,6 +1,18 @@
 """ctypes binding of oracle/liboracle.so, the CPU restatement of stdgpu's hash-table engine
 (oracle/htab_oracle.cpp).  Test infrastructure only: the parity checker and the timed CPU
 baseline; the product never calls it.
+
+Pieces and what pins each (tests/test_oracle.py):
+  * Table (sizing, bucket(), insert / find / erase, valid): reference KATs, exact closed forms,
+    the reference tests' colliding keys, brute force against Python sets and dicts, tiny-table
+    capacity cases, the examples' closed-form sums;
+  * Deque, Vector (the free lists): multiset conservation, FIFO / LIFO order, full and empty;
+  * voxel_insert (the reference's caller pattern, not library code): brute force against a
+    Python deque and dict, exact key -> slot map and free-list order;
+  * atomic_step / atomic_sequence / atomic_linearize (pure Python): the reference atomic tests'
+    closed forms, wrap-around of the modular increment / decrement, and a planted tamper that
+    the linearisation check must reject.
+No function here is left without such a check.
 """
 from __future__ import annotations
 

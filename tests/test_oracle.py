@@ -10,7 +10,10 @@ Each check is chosen so that a plausible slip in the restatement fails it:
   * the tiny-table capacity cases (unordered_datastructure.inc:1375-1739);
   * the examples' closed-form sums (examples/openmp/unordered_map.cpp:99-108,
     unordered_set.cpp:82-87);
-  * multiset conservation of the deque free list.
+  * multiset conservation of the deque free list, stack order of the vector;
+  * the voxel caller pattern (pop, insert, push back) against a pure-Python statement of it;
+  * the reference atomic tests' closed forms (tests/stdgpu/atomic.inc) for the sequential
+    atomic model.
 """
 from decimal import Decimal, getcontext
 import math
@@ -402,3 +405,33 @@ def test_atomic_linearize_accepts_orders_and_rejects_tampering():
     olds[50] = 12345
     with pytest.raises(AssertionError):
         po.atomic_linearize("int", "exchange", -1, list(range(100)), olds)
+
+
+def test_voxel_caller_pattern_bruteforce():
+    """voxel_insert against a pure-Python statement of the caller pattern: per key pop the front slot,
+    insert (key, slot), push the slot back when the key is present; a key that finds the free list
+    empty is skipped.  Exact key -> slot map and free-list order, including the empty-list case."""
+    import collections
+
+    rng = np.random.default_rng(5)
+    for cap, nkeys, universe in ((64, 200, 40), (64, 200, 100), (1000, 5000, 700)):
+        keys = rng.integers(1, universe + 1, size=nkeys, dtype=np.uint64)
+        tab = po.Table("u64", "u32", capacity=4 * universe)
+        q = po.Deque(cap)
+        q.push_back(np.arange(cap, dtype=np.uint32))
+        added = po.voxel_insert(tab, q, keys)
+        free = collections.deque(range(cap))
+        want = {}
+        for k in keys.tolist():
+            if not free:
+                continue
+            s = free.popleft()
+            if k in want:
+                free.append(s)
+            else:
+                want[k] = s
+        assert added == len(want)
+        k, v = tab.contents()
+        assert dict(zip(k.tolist(), v.tolist())) == want
+        assert q.contents().tolist() == list(free)
+        assert tab.valid()
