@@ -15,6 +15,7 @@
 #include <tuple>
 #include <vector>
 
+#include <stdgpu/atomic.cuh>
 #include <stdgpu/unordered_map.cuh>
 #include <stdgpu/unordered_set.cuh>
 
@@ -880,6 +881,157 @@ transparent_keylike_lookups()
     tset::destroyDeviceObject(s);
 }
 
+// The reference fixture's remaining cases with its own key (three int16 coordinates) and hash
+// (vec_hash): hash_inside_range, bucket_number_hits, bucket_number_collisions, empty_size_limits
+// (unordered_datastructure.inc:60-256), insert_while_excess_empty (:1443-1465), the const-range
+// variants insert_const_range / erase_const_range / const_device_range (:1959-2041, 2376-2410) and
+// custom_allocator (:2518-2546).
+__global__ void
+k_count_bucket_hits(block_set_t s, const block_key* keys, std::size_t n, int* hits)
+{
+    const std::size_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) stdgpu::atomic_ref<int>(hits[s.bucket(keys[i])]).fetch_add(1);
+}
+
+// a stateful allocator that counts what the table asks of it
+template <typename T>
+struct counting_allocator
+{
+    using value_type = T;
+    static inline int allocations = 0;
+    static inline int deallocations = 0;
+    int id = 0;
+    counting_allocator() = default;
+    explicit counting_allocator(int i) : id(i) {}
+    template <typename U>
+    counting_allocator(const counting_allocator<U>& o) : id(o.id)
+    {
+    }
+    T* allocate(std::int64_t n)
+    {
+        ++allocations;
+        T* p = nullptr;
+        return hipMalloc(&p, static_cast<std::size_t>(n) * sizeof(T)) == hipSuccess ? p : nullptr;
+    }
+    void deallocate(T* p, std::int64_t /*n*/)
+    {
+        ++deallocations;
+        (void)hipFree(p);
+    }
+};
+
+static std::vector<int>
+bucket_hits(block_set_t& s, std::size_t n, std::mt19937& rng)
+{
+    std::uniform_int_distribution<int> d16(-32768, 32767);
+    std::vector<block_key> keys;
+    for (std::size_t i = 0; i < n; ++i)
+        keys.emplace_back(static_cast<std::int16_t>(d16(rng)), static_cast<std::int16_t>(d16(rng)),
+                          static_cast<std::int16_t>(d16(rng)));
+    block_key* dk = to_dev(keys);
+    std::vector<int> zero(static_cast<std::size_t>(s.bucket_count()), 0);
+    int* dh = to_dev(zero);
+    hipLaunchKernelGGL(k_count_bucket_hits, dim3(blocks(n)), dim3(256), 0, 0, s, dk, n, dh);
+    std::vector<int> h = to_host(dh, zero.size());
+    (void)hipFree(dk);
+    (void)hipFree(dh);
+    return h;
+}
+
+static void
+reference_fixture_cases()
+{
+    g_case = "hash distribution, size limits, const ranges, struct-key tiny table, custom allocator";
+    block_set_t s = block_set_t::createDeviceObject(100000);
+    const stdgpu::index_t bc = s.bucket_count();
+    // empty_size_limits
+    CHECK(s.empty() && !s.full() && s.size() <= s.max_size() && s.load_factor() <= s.max_load_factor());
+    // hash_inside_range: 2^21 random keys on the host
+    std::mt19937 rng(17);
+    std::uniform_int_distribution<int> d16(-32768, 32767);
+    for (int i = 0; i < (1 << 21); ++i)
+    {
+        const stdgpu::index_t b = s.bucket(block_key(static_cast<std::int16_t>(d16(rng)), static_cast<std::int16_t>(d16(rng)),
+                                                     static_cast<std::int16_t>(d16(rng))));
+        CHECK(b >= 0 && b < bc);
+    }
+    // bucket_number_hits: 2 * bucket_count keys hit more than 80% of the buckets
+    {
+        const std::vector<int> h = bucket_hits(s, 2 * static_cast<std::size_t>(bc), rng);
+        CHECK(sum(h) == 2 * bc);
+        const auto hit = std::count_if(h.begin(), h.end(), [](int c) { return c > 0; });
+        CHECK(hit > static_cast<long>(static_cast<float>(bc) * 80.0F / 100.0F));
+    }
+    // bucket_number_collisions: bucket_count keys leave fewer than 40% of the buckets with two or more
+    {
+        const std::vector<int> h = bucket_hits(s, static_cast<std::size_t>(bc), rng);
+        CHECK(sum(h) == bc);
+        const auto coll = std::count_if(h.begin(), h.end(), [](int c) { return c > 1; });
+        CHECK(coll < static_cast<long>(static_cast<float>(bc) * 40.0F / 100.0F));
+    }
+    block_set_t::destroyDeviceObject(s);
+
+    // insert_while_excess_empty: the one-capacity table takes two positions, not a third
+    block_set_t tiny = block_set_t::createDeviceObject(1);
+    const std::vector<block_key> p12 = { { 1, 2, 3 }, { -1, 2, 3 } };
+    CHECK(sum(run_insert(tiny, p12)) == 2);
+    CHECK(tiny.valid() && tiny.size() == 2);
+    CHECK(sum(run_insert(tiny, std::vector<block_key>{ { 1, -2, 3 } })) == 0);
+    CHECK(tiny.valid() && tiny.size() == 2);
+    block_set_t::destroyDeviceObject(tiny);
+
+    // insert_const_range / const_device_range / erase_const_range
+    const std::size_t N = 100000;
+    map_t m = map_t::createDeviceObject(static_cast<stdgpu::index_t>(N));
+    const std::vector<std::uint64_t> k = unique_keys(N, 23);
+    std::vector<map_t::value_type> hv;
+    for (auto x : k) hv.emplace_back(x, static_cast<std::uint32_t>(x * 5));
+    map_t::value_type* dv = dev_alloc<map_t::value_type>(N);
+    HIP_OK(hipMemcpy(dv, hv.data(), N * sizeof(map_t::value_type), hipMemcpyHostToDevice));
+    const map_t::value_type* cdv = dv;
+    m.insert(stdgpu::make_device(cdv), stdgpu::make_device(cdv + N));
+    CHECK(!m.empty() && m.size() == static_cast<stdgpu::index_t>(N) && m.valid());
+    auto crange = static_cast<const map_t&>(m).device_range();
+    CHECK(crange.size() == static_cast<stdgpu::index_t>(N));
+    map_t::value_type* out = dev_alloc<map_t::value_type>(N);
+    hipLaunchKernelGGL((k_range_copy<decltype(crange), map_t::value_type>), dim3(blocks(N)), dim3(256), 0, 0, crange, out);
+    std::vector<map_t::value_type> got(N);
+    HIP_OK(hipMemcpy(got.data(), out, N * sizeof(map_t::value_type), hipMemcpyDeviceToHost));
+    std::vector<std::uint64_t> gk, wk(k);
+    for (auto& p : got)
+    {
+        gk.push_back(p.first);
+        CHECK(p.second == static_cast<std::uint32_t>(p.first * 5));
+    }
+    std::sort(gk.begin(), gk.end());
+    std::sort(wk.begin(), wk.end());
+    CHECK(gk == wk);
+    std::uint64_t* dk = to_dev(k);
+    const std::uint64_t* cdk = dk;
+    m.erase(stdgpu::make_device(cdk), stdgpu::make_device(cdk + N));
+    CHECK(m.empty() && m.size() == 0 && m.valid());
+    (void)hipFree(dk);
+    (void)hipFree(dv);
+    (void)hipFree(out);
+    map_t::destroyDeviceObject(m);
+
+    // custom_allocator: the table keeps the allocator it was given, and get_allocator() returns it
+    using calloc_t = counting_allocator<map_t::value_type>;
+    using cmap_t = stdgpu::unordered_map<std::uint64_t, std::uint32_t, stdgpu::hash<std::uint64_t>,
+                                         stdgpu::equal_to<std::uint64_t>, calloc_t>;
+    calloc_t::allocations = 0;
+    calloc_t::deallocations = 0;
+    cmap_t c = cmap_t::createDeviceObject(10000, calloc_t(42));
+    calloc_t a = c.get_allocator();
+    CHECK(a.id == 42);
+    map_t::value_type* arr = a.allocate(10000);
+    CHECK(arr != nullptr);
+    a.deallocate(arr, 10000);
+    CHECK(calloc_t::allocations == 1 && calloc_t::deallocations == 1);
+    CHECK(sum(run_insert(c, std::vector<std::uint64_t>{ 5, 6, 5 })) == 2 && c.size() == 2 && c.valid());
+    cmap_t::destroyDeviceObject(c);
+}
+
 int
 main()
 {
@@ -903,7 +1055,8 @@ main()
                           examples_closed_form,
                           transparent_keylike_lookups,
                           struct_keys,
-                          bulk_find_and_contains };
+                          bulk_find_and_contains,
+                          reference_fixture_cases };
     for (auto c : cases)
     {
         const int before = g_failures;
