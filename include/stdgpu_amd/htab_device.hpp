@@ -110,6 +110,23 @@ next_bucket(uint64_t b, uint64_t nbuckets)
     return (b + 1 == nbuckets) ? 0 : b + 1;
 }
 
+// Claim order of a key inside a bucket: slots o, o+1, ..., W-1, 0, ..., o-1, with o taken from the
+// key's hash.  Concurrent inserts of different keys into one bucket then aim at different slots,
+// while every inserter of one key follows one order over its whole probe sequence, which is what
+// the no-duplicate argument needs (a key's claims move forward along that order only).  Tables of
+// kOrderedClaimBuckets buckets or more keep o = 0 (plain first-free order): a launch's elements in
+// flight (about 2^19) rarely meet in one of so many buckets, and the offset then only costs ALU work.
+// The choice depends on the table alone, so all inserters of a key in a launch share it.  The native
+// per-thread calls scan a bucket in the same order, so a key that the bulk insert placed is usually
+// found with its first load.
+constexpr uint64_t kOrderedClaimBuckets = 1ULL << 22;
+
+__host__ __device__ inline uint32_t
+claim_offset(uint64_t key, uint32_t W)
+{
+    return static_cast<uint32_t>(mix64(key)) & (W - 1);
+}
+
 // The reference's bucket(): Fibonacci hashing of the identity hash over bucket_count()
 // (src/stdgpu/impl/unordered_base_detail.cuh:56-85, functional_detail.h:60-80).
 __host__ __device__ inline int64_t
@@ -217,6 +234,13 @@ struct table_ref
 
     // ---- native per-thread operations (user kernels; the facade's find/insert/erase) ----
 
+    // where a key's scan of a bucket starts: its claim order (claim_offset); slot j of the scan is
+    // (o + j) mod W, W a power of two
+    __device__ uint32_t scan_start(K key) const
+    {
+        return nbuckets < kOrderedClaimBuckets ? claim_offset(static_cast<uint64_t>(key), W) : 0U;
+    }
+
     // find (unordered_base_detail.cuh:653-682): pointer to the entry or end_ptr().
     __device__ value_type* find(K key) const
     {
@@ -225,14 +249,16 @@ struct table_ref
             return (load_agent(flags) & side_bit(key)) ? slots + side_index(key) : end_ptr();
         }
         uint64_t b = home_bucket(static_cast<uint64_t>(key), nbuckets);
+        const uint32_t o = scan_start(key);
         for (uint64_t step = 0; step < nbuckets; ++step)
         {
             bool saw_empty = false;
             value_type* base = slots + b * W;
             for (uint32_t j = 0; j < W; ++j)
             {
-                const K k = load_agent(key_ptr(b * W + j));
-                if (k == key) return base + j;
+                const uint32_t x = (o + j) & (W - 1);
+                const K k = load_agent(key_ptr(b * W + x));
+                if (k == key) return base + x;
                 saw_empty |= (k == empty_key<K>());
             }
             if (saw_empty || overflow(b) == 0) break;
@@ -241,16 +267,38 @@ struct table_ref
         return end_ptr();
     }
 
-    __device__ bool count_add(int64_t delta) const
+    // Adds delta (+1 or -1, the same at each call site) to the size for every lane of the wavefront
+    // that makes this call at this point, with one atomic for all of them (the size word is the one
+    // every successful insert and erase touches).  Returns false for a lane whose +1 would pass the
+    // exact limit (tracked only when nstripes == 1): lanes in lane order take what is left below the
+    // limit, the rest are taken back with one more atomic.
+    __device__ __forceinline__ bool count_add(int64_t delta) const
     {
-        // returns false if the exact limit would be exceeded (only tracked when nstripes == 1)
+        const uint64_t lanes = __builtin_amdgcn_ballot_w64(true); // the lanes making this call
+        const int lane = static_cast<int>(__builtin_amdgcn_mbcnt_hi(~0U, __builtin_amdgcn_mbcnt_lo(~0U, 0U)));
+        const int leader = __builtin_ctzll(lanes);
+        const int64_t k = __builtin_popcountll(lanes);
+        const int64_t rank = __builtin_popcountll(lanes & ((uint64_t(1) << lane) - 1));
         const uint32_t s = nstripes == 1 ? 0U : (static_cast<uint32_t>(blockIdx.x) & (nstripes - 1));
-        const int64_t old = __hip_atomic_fetch_add(stripes + static_cast<uint64_t>(s) * kStripeStride, delta,
-                                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (nstripes == 1 && delta > 0 && old + delta > static_cast<int64_t>(limit))
+        int64_t old = 0;
+        if (lane == leader)
+            old = __hip_atomic_fetch_add(stripes + static_cast<uint64_t>(s) * kStripeStride, delta * k, __ATOMIC_RELAXED,
+                                         __HIP_MEMORY_SCOPE_AGENT);
+        old = static_cast<int64_t>(
+                static_cast<uint64_t>(static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<uint32_t>(old), leader))) |
+                (static_cast<uint64_t>(static_cast<uint32_t>(
+                         __builtin_amdgcn_readlane(static_cast<uint32_t>(static_cast<uint64_t>(old) >> 32), leader)))
+                 << 32));
+        if (nstripes == 1 && delta > 0)
         {
-            __hip_atomic_fetch_add(stripes, -delta, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            return false;
+            // lanes past the limit: those whose unit lands above it
+            int64_t over = old + k - static_cast<int64_t>(limit);
+            if (over > 0)
+            {
+                if (over > k) over = k;
+                if (lane == leader) __hip_atomic_fetch_add(stripes, -over, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                return rank < k - over;
+            }
         }
         return true;
     }
@@ -282,12 +330,13 @@ struct table_ref
             return p;
         }
         const uint64_t h = home_bucket(static_cast<uint64_t>(key), nbuckets);
+        const uint32_t o = scan_start(key);
         uint64_t b = h;
         for (uint64_t step = 0; step < nbuckets; ++step)
         {
             for (uint32_t j = 0; j < W; ++j)
             {
-                const uint64_t s = b * W + j;
+                const uint64_t s = b * W + ((o + j) & (W - 1));
                 K k = load_agent(key_ptr(s));
                 if (k == key)
                 {
@@ -338,13 +387,14 @@ struct table_ref
             return true;
         }
         const uint64_t h = home_bucket(static_cast<uint64_t>(key), nbuckets);
+        const uint32_t o = scan_start(key);
         uint64_t b = h;
         for (uint64_t step = 0; step < nbuckets; ++step)
         {
             bool saw_empty = false;
             for (uint32_t j = 0; j < W; ++j)
             {
-                const uint64_t s = b * W + j;
+                const uint64_t s = b * W + ((o + j) & (W - 1));
                 const K k = load_agent(key_ptr(s));
                 if (k == key)
                 {

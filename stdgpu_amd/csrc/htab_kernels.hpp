@@ -381,20 +381,7 @@ constexpr uint32_t kWalkFree = 1U << 9;    // the home bucket has a free slot (i
 constexpr uint32_t kWalkTomb = 1U << 10;   // that slot held TOMB (else EMPTY)
 constexpr uint32_t kWalkAbsent = 1U << 11; // no key runs past the home bucket: the key is absent
 
-// Claim order of a key inside a bucket: slots o, o+1, ..., W-1, 0, ..., o-1, with o taken from the
-// key's hash.  Concurrent inserts of different keys into one bucket then aim at different slots,
-// while every inserter of one key follows one order over its whole probe sequence, which is what
-// the no-duplicate argument needs (a key's claims move forward along that order only).  Tables of
-// kOrderedClaimBuckets buckets or more keep o = 0 (plain first-free order): a launch's elements in
-// flight (about 2^19) rarely meet in one of so many buckets, and the offset then only costs ALU work.
-// The choice depends on the table alone, so all inserters of a key in a launch share it.
-constexpr uint64_t kOrderedClaimBuckets = 1ULL << 22;
-
-__device__ __forceinline__ uint32_t
-claim_offset(uint64_t key, uint32_t W)
-{
-    return static_cast<uint32_t>(mix64(key)) & (W - 1);
-}
+// The claim order of a key inside a bucket (claim_offset, kOrderedClaimBuckets): htab_device.hpp.
 
 // bits j of a lane's chunk whose bucket slot li*SPL + j lies in [lo, hi)
 template <int SPL>

@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdint>
 #include <cstdio>
 #include <map>
@@ -1032,6 +1033,85 @@ reference_fixture_cases()
     cmap_t::destroyDeviceObject(c);
 }
 
+// Rates of the native per-thread API from a user kernel (one call per thread, no cooperation):
+// emplace, find of present and of absent keys, erase; 2^22 keys at load 0.8.  Also find of keys that
+// the bulk insert placed (its claim order starts at a key-dependent slot of the bucket).
+template <typename Table, typename Key>
+__global__ void
+k_find_flag(Table t, const Key* keys, std::size_t n, int* found)
+{
+    const std::size_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) found[i] = t.find(keys[i]) != t.end() ? 1 : 0;
+}
+
+static void
+native_rates()
+{
+    g_case = "native per-thread emplace / find / erase rates, 2^22 keys at load 0.8";
+    const std::size_t N = 1 << 22;
+    const stdgpu::index_t cap = static_cast<stdgpu::index_t>(std::ceil(static_cast<double>(N) / 0.8));
+    // distinct keys: a bijective 64-bit mix (splitmix64's finaliser) of 0 .. 2N-1
+    std::vector<std::uint64_t> all(2 * N);
+    for (std::size_t i = 0; i < 2 * N; ++i)
+    {
+        std::uint64_t z = i + 0x9E3779B97F4A7C15ULL;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+        all[i] = z ^ (z >> 31);
+    }
+    const std::vector<std::uint64_t> k(all.begin(), all.begin() + N), absent(all.begin() + N, all.end());
+    std::uint64_t* dk = to_dev(k);
+    std::uint64_t* da = to_dev(absent);
+    int* ok = dev_alloc<int>(N);
+    hipEvent_t e[5];
+    for (auto& x : e) HIP_OK(hipEventCreate(&x));
+    float ms[4] = { 0, 0, 0, 0 };
+    map_t m = map_t::createDeviceObject(cap);
+    for (int rep = 0; rep < 3; ++rep) // the last repetition is reported
+    {
+        m.clear();
+        HIP_OK(hipEventRecord(e[0], nullptr));
+        hipLaunchKernelGGL((k_emplace_keys<map_t, std::uint64_t>), dim3(blocks(N)), dim3(256), 0, 0, m, dk, N, ok);
+        HIP_OK(hipEventRecord(e[1], nullptr));
+        hipLaunchKernelGGL((k_find_flag<map_t, std::uint64_t>), dim3(blocks(N)), dim3(256), 0, 0, m, dk, N, ok);
+        HIP_OK(hipEventRecord(e[2], nullptr));
+        hipLaunchKernelGGL((k_find_flag<map_t, std::uint64_t>), dim3(blocks(N)), dim3(256), 0, 0, m, da, N, ok);
+        HIP_OK(hipEventRecord(e[3], nullptr));
+        hipLaunchKernelGGL((k_erase_keys<map_t, std::uint64_t>), dim3(blocks(N)), dim3(256), 0, 0, m, dk, N, ok);
+        HIP_OK(hipEventRecord(e[4], nullptr));
+        HIP_OK(hipEventSynchronize(e[4]));
+        for (int j = 0; j < 4; ++j) HIP_OK(hipEventElapsedTime(&ms[j], e[j], e[j + 1]));
+    }
+    CHECK(sum(to_host(ok, N)) == static_cast<int>(N) && m.size() == 0 && m.valid());
+    // keys placed by the bulk insert, then found per thread
+    std::vector<map_t::value_type> hv;
+    for (auto x : k) hv.emplace_back(x, static_cast<std::uint32_t>(x));
+    map_t::value_type* dv = dev_alloc<map_t::value_type>(N);
+    HIP_OK(hipMemcpy(dv, hv.data(), N * sizeof(map_t::value_type), hipMemcpyHostToDevice));
+    m.clear();
+    m.insert(stdgpu::make_device(dv), stdgpu::make_device(dv + N));
+    float bulk_ms = 0;
+    for (int rep = 0; rep < 3; ++rep)
+    {
+        HIP_OK(hipEventRecord(e[0], nullptr));
+        hipLaunchKernelGGL((k_find_flag<map_t, std::uint64_t>), dim3(blocks(N)), dim3(256), 0, 0, m, dk, N, ok);
+        HIP_OK(hipEventRecord(e[1], nullptr));
+        HIP_OK(hipEventSynchronize(e[1]));
+        HIP_OK(hipEventElapsedTime(&bulk_ms, e[0], e[1]));
+    }
+    CHECK(sum(to_host(ok, N)) == static_cast<int>(N));
+    const double g = static_cast<double>(N) / 1e6;
+    std::printf("rate: native emplace %.2f, find (present) %.2f, find (absent) %.2f, erase %.2f, find of bulk-inserted "
+                "keys %.2f G/s (%zu keys, capacity %d)\n",
+                g / ms[0], g / ms[1], g / ms[2], g / ms[3], g / bulk_ms, N, cap);
+    for (auto& x : e) (void)hipEventDestroy(x);
+    (void)hipFree(dk);
+    (void)hipFree(da);
+    (void)hipFree(ok);
+    (void)hipFree(dv);
+    map_t::destroyDeviceObject(m);
+}
+
 int
 main()
 {
@@ -1056,7 +1136,8 @@ main()
                           transparent_keylike_lookups,
                           struct_keys,
                           bulk_find_and_contains,
-                          reference_fixture_cases };
+                          reference_fixture_cases,
+                          native_rates };
     for (auto c : cases)
     {
         const int before = g_failures;
