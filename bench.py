@@ -28,8 +28,10 @@ BYTES_PER_OP = {"insert": 16 + 64 + 16 + 1, "find": 8 + 1 + 4 + 64, "erase": 8 +
 # 8-slot buckets, 256-thread workgroups): FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000 KiB, the scales
 # that make both counters read a 2 GiB copy kernel's bytes exactly (profiles/map_u64_u32_2e26_bulk_kernels.md)
 # 128 B lines a find reads per key (TCC misses per dispatch / keys), same profile and workload
-MEASURED_FIND_LINES = {(26, 8, 256): 89039255 / (1 << 26)}
-MEASURED_TRAFFIC = {(26, 8, 256): {"insert": (2 * 5252402 + 6709960) * 1024, "find": (2 * 5391956 + 336602) * 1024,
+MEASURED_FIND_LINES = {(26, 8, 64): 89417762 / (1 << 26), (26, 8, 256): 89039255 / (1 << 26)}
+MEASURED_TRAFFIC = {(26, 8, 64): {"insert": (2 * 5253340 + 6735788) * 1024, "find": (2 * 5392030 + 327680) * 1024,
+                                  "erase": (2 * 1271969 + 1152885) * 1024, "reinsert": (2 * 2372024 + 1676292) * 1024},
+                    (26, 8, 256): {"insert": (2 * 5252402 + 6709960) * 1024, "find": (2 * 5391956 + 336602) * 1024,
                                    "erase": (2 * 1272247 + 1146738) * 1024, "reinsert": (2 * 2368420 + 1667805) * 1024}}
 
 
@@ -40,7 +42,7 @@ def parse():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--log2n", type=int, default=26)
     ap.add_argument("--lanes", type=int, default=8)
-    ap.add_argument("--block", type=int, default=256)
+    ap.add_argument("--block", type=int, default=64)
     ap.add_argument("--cpu-baseline", type=int, default=1, help="time the oracle on a bounded sample")
     ap.add_argument("--cpu-log2n", type=int, default=22)
     ap.add_argument("--probe-batch", type=int, default=0)

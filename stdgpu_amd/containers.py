@@ -52,7 +52,7 @@ def _ptr(t) -> int | None:
 @dataclass
 class Tuning:
     bucket_lanes: int = 8
-    block_size: int = 256
+    block_size: int = 64
     slots_per_capacity: float = 1.0
     probe_batch: int = 0  # 0 = library default
 

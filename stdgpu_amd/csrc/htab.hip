@@ -249,8 +249,8 @@ check_geometry(int kind, int64_t capacity, int lanes, int block, int rb, const c
         return fail(HTAB_E_ARG, "htab_create: bucket_lanes must be 1, 4, 8 or 16");
     if (rb != 0 && ((rb != 2 && rb != 4 && rb != 8) || rb > lanes))
         return fail(HTAB_E_ARG, "htab_create: probe_batch must be 2, 4 or 8 and at most bucket_lanes");
-    if (block != 256 && block != 512 && block != 1024)
-        return fail(HTAB_E_ARG, "htab_create: block_size must be 256, 512 or 1024");
+    if (block != 64 && block != 128 && block != 256 && block != 512 && block != 1024)
+        return fail(HTAB_E_ARG, "htab_create: block_size must be 64, 128, 256, 512 or 1024");
     return HTAB_OK;
 }
 
@@ -259,7 +259,7 @@ htab_create(int kind, int64_t capacity, const htab_tuning* tuning, htab** out)
 {
     if (!out) return fail(HTAB_E_ARG, "htab_create: out is null");
     *out = nullptr;
-    int lanes = 8, block = 256, rb = 0;
+    int lanes = 8, block = 64, rb = 0;
     float spc = 1.0F;
     if (tuning)
     {
@@ -603,7 +603,11 @@ htab_batch_create(htab* const* tables, int32_t ntables, htab_batch** out)
     if (!b) return fail(HTAB_E_OOM, "htab_batch_create: host allocation");
     b->kind = tables[0]->kind;
     b->lanes = tables[0]->lanes;
-    b->block = tables[0]->block;
+    // the persistent kernel's workgroup size is also its chunk size, so it is the size the launch
+    // uses: the tables' own where that kernel exists (512 and 1024 for unordered_map<u64, u32>,
+    // batch_all_blocks in htab_ops.hpp), else 256
+    const int tb = tables[0]->block;
+    b->block = (b->kind == HTAB_MAP_U64_U32 && (tb == 512 || tb == 1024)) ? tb : 256;
     b->rb = tables[0]->rb;
     b->ntables = ntables;
     int dev = 0, cus = 0;

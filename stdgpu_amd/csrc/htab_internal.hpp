@@ -31,6 +31,7 @@ void reference_sizing(int64_t capacity, int64_t* bucket_count, int64_t* max_size
 
 constexpr size_t kCtlBytes = 8192; // stripes (32 x 128 B) + flags + scratch words
 constexpr size_t kFlagsOffset = 4096;
+constexpr size_t kPlanOffset = 4224; // one-stripe tables: whether a bulk insert can reach the limit
 
 #define HTAB_HIP_TRY(expr, what)                                                                                       \
     do                                                                                                                 \
@@ -56,6 +57,13 @@ struct htab
     size_t scratch_bytes = 0;
     int64_t* range_idx = nullptr; // table-owned device_range() indices (limit + 2 entries)
 };
+
+// the device word in the table's control block where k_limit_plan records its decision
+inline uint32_t*
+plan_word(htab* h)
+{
+    return reinterpret_cast<uint32_t*>(static_cast<char*>(h->mem) + htab_host::kPlanOffset);
+}
 
 struct htab_batch
 {

@@ -427,7 +427,8 @@ first_free_in_order(uint32_t f, uint32_t o, uint32_t jmin, uint32_t W, int li, u
 template <typename K, typename M, int G, int R>
 __device__ __forceinline__ void
 insert_body(const table_ref<K, M>& t, const void* __restrict__ values, const void* __restrict__ keys,
-            const void* __restrict__ vals, const int64_t i, const bool valid, uint8_t* __restrict__ inserted)
+            const void* __restrict__ vals, const int64_t i, const bool valid, uint8_t* __restrict__ inserted,
+            const bool striped = false)
 {
     using C = chunk<K, M>;
     using value_type = typename C::value_type;
@@ -446,8 +447,12 @@ insert_body(const table_ref<K, M>& t, const void* __restrict__ values, const voi
     if (valid) fetch_element<K, M>(values, keys, vals, i, &mykey, &myv);
     const bool ordered = t.nbuckets < kOrderedClaimBuckets; // uniform over the launch
 
-    // a table at its exact limit refuses every insert (reference: the !full() loop guard)
-    const bool at_limit = (t.nstripes == 1) && (load_agent(t.stripes) >= static_cast<int64_t>(t.limit));
+    // A table whose limit is below its slots counts its size on one stripe and ranks each insert
+    // against the limit, unless the launch was planned as unable to reach it (striped: counts go to
+    // all stripes, folded back into the first after the launch).  A table at its exact limit refuses
+    // every insert (reference: the !full() loop guard).
+    const bool exact = (t.nstripes == 1) && !striped;
+    const bool at_limit = exact && (load_agent(t.stripes) >= static_cast<int64_t>(t.limit));
 
     bool myins = false;
     uint64_t myslot = kNoSlot;
@@ -774,8 +779,8 @@ insert_body(const table_ref<K, M>& t, const void* __restrict__ values, const voi
     }
 
     int64_t wave_prefix = 0;
-    const int64_t old = block_count_add(t.stripes, t.nstripes, myins ? 1 : 0, s_wave, &s_old, &wave_prefix);
-    if (t.nstripes == 1)
+    const int64_t old = block_count_add(t.stripes, exact ? 1U : kMaxStripes, myins ? 1 : 0, s_wave, &s_old, &wave_prefix);
+    if (exact)
     {
         // exact limit: inserts ranked past it are undone (their slot becomes TOMB)
         const uint64_t bal = ballot(myins);
@@ -808,7 +813,7 @@ insert_body(const table_ref<K, M>& t, const void* __restrict__ values, const voi
 template <typename K, typename M, int G, int R>
 __device__ __forceinline__ void
 erase_body(const table_ref<K, M>& t, const K* __restrict__ keys, const int64_t i, const bool valid,
-           uint8_t* __restrict__ erased)
+           uint8_t* __restrict__ erased, const bool striped = false)
 {
     using C = chunk<K, M>;
     constexpr int SPL = C::SPL;
@@ -967,7 +972,8 @@ erase_body(const table_ref<K, M>& t, const K* __restrict__ keys, const int64_t i
     }
 
     int64_t wave_prefix = 0;
-    block_count_add(t.stripes, t.nstripes, mydel ? -1 : 0, s_wave, &s_old, &wave_prefix);
+    // striped: a one-stripe table's launch counts on all stripes (folded back after the launch)
+    block_count_add(t.stripes, striped ? kMaxStripes : t.nstripes, mydel ? -1 : 0, s_wave, &s_old, &wave_prefix);
     if (valid && erased) erased[i] = mydel ? 1 : 0;
 }
 
@@ -981,21 +987,52 @@ k_find(table_ref<K, M> t, const K* __restrict__ keys, int64_t n, uint8_t* __rest
     find_body<K, M, G, R>(t, keys, i, i < n, hits, vals_out);
 }
 
+// plan: null for tables counted on all stripes; for a one-stripe table, the word k_limit_plan wrote
+// (1: this launch cannot reach the limit, count on all stripes without ranking)
 template <typename K, typename M, int G, int R>
 __global__ __launch_bounds__(1024) void
 k_insert(table_ref<K, M> t, const void* __restrict__ values, const void* __restrict__ keys,
-         const void* __restrict__ vals, int64_t n, uint8_t* __restrict__ inserted)
+         const void* __restrict__ vals, int64_t n, uint8_t* __restrict__ inserted, const uint32_t* __restrict__ plan)
 {
     const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    insert_body<K, M, G, R>(t, values, keys, vals, i, i < n, inserted);
+    const bool striped = plan != nullptr && *plan != 0;
+    insert_body<K, M, G, R>(t, values, keys, vals, i, i < n, inserted, striped);
 }
 
+// striped: a one-stripe table counts this launch on all stripes (folded back by k_limit_fold)
 template <typename K, typename M, int G, int R>
 __global__ __launch_bounds__(1024) void
-k_erase(table_ref<K, M> t, const K* __restrict__ keys, int64_t n, uint8_t* __restrict__ erased)
+k_erase(table_ref<K, M> t, const K* __restrict__ keys, int64_t n, uint8_t* __restrict__ erased, int striped)
 {
     const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    erase_body<K, M, G, R>(t, keys, i, i < n, erased);
+    erase_body<K, M, G, R>(t, keys, i, i < n, erased, striped != 0);
+}
+
+// A one-stripe table (limit below its slots) ranks every insert against the limit on one counter,
+// which serialises one atomic per workgroup on one address.  Before a bulk insert, this decides
+// whether the launch can reach the limit at all (size + n > limit); when it cannot, the launch counts
+// on all stripes like any other table.
+template <typename K, typename M>
+__global__ void
+k_limit_plan(table_ref<K, M> t, int64_t n, uint32_t* plan)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0) *plan = (load_agent(t.stripes) + n <= static_cast<int64_t>(t.limit)) ? 1U : 0U;
+}
+
+// after a striped launch on a one-stripe table: stripes 1.. are added into stripe 0 and cleared
+template <typename K, typename M>
+__global__ void
+k_limit_fold(table_ref<K, M> t)
+{
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    int64_t sum = 0;
+    for (uint32_t s = 1; s < kMaxStripes; ++s)
+    {
+        int64_t* p = t.stripes + static_cast<uint64_t>(s) * kStripeStride;
+        sum += *p;
+        *p = 0;
+    }
+    *t.stripes += sum;
 }
 
 

@@ -39,8 +39,13 @@ struct ops
                       hipStream_t s)
     {
         const int64_t grid = (n + h->block - 1) / h->block;
-        hipLaunchKernelGGL((k_insert<K, M, G, R>), dim3(static_cast<uint32_t>(grid)), dim3(h->block), 0, s,
-                           T::from_view(h->view), values, keys, vals, n, ins);
+        const T t = T::from_view(h->view);
+        const bool one_stripe = h->view.nstripes == 1;
+        uint32_t* plan = one_stripe ? plan_word(h) : nullptr;
+        if (one_stripe) hipLaunchKernelGGL((k_limit_plan<K, M>), dim3(1), dim3(64), 0, s, t, n, plan);
+        hipLaunchKernelGGL((k_insert<K, M, G, R>), dim3(static_cast<uint32_t>(grid)), dim3(h->block), 0, s, t, values,
+                           keys, vals, n, ins, plan);
+        if (one_stripe) hipLaunchKernelGGL((k_limit_fold<K, M>), dim3(1), dim3(64), 0, s, t);
         return launch_status("insert launch");
     }
 
@@ -55,8 +60,11 @@ struct ops
     static int erase(htab* h, const void* keys, int64_t n, uint8_t* del, hipStream_t s)
     {
         const int64_t grid = (n + h->block - 1) / h->block;
-        hipLaunchKernelGGL((k_erase<K, M, G, R>), dim3(static_cast<uint32_t>(grid)), dim3(h->block), 0, s,
-                           T::from_view(h->view), static_cast<const K*>(keys), n, del);
+        const T t = T::from_view(h->view);
+        const bool one_stripe = h->view.nstripes == 1;
+        hipLaunchKernelGGL((k_erase<K, M, G, R>), dim3(static_cast<uint32_t>(grid)), dim3(h->block), 0, s, t,
+                           static_cast<const K*>(keys), n, del, one_stripe ? 1 : 0);
+        if (one_stripe) hipLaunchKernelGGL((k_limit_fold<K, M>), dim3(1), dim3(64), 0, s, t);
         return launch_status("erase launch");
     }
 };

@@ -50,6 +50,11 @@ def test_argument_errors_without_gpu():
     h = ctypes.c_void_p()
     assert lib.htab_create(99, 10, None, ctypes.byref(h)) == _lib.HTAB_E_ARG
     assert lib.htab_create(_lib.HTAB_MAP_U64_U32, 0, None, ctypes.byref(h)) == _lib.HTAB_E_ARG
+    # workgroup sizes: 64, 128, 256, 512 or 1024 (checked before any device work)
+    for bad in (32, 96, 100, 2048):
+        tun = _lib.htab_tuning(8, bad, 1.0, 0)
+        assert lib.htab_create(_lib.HTAB_MAP_U64_U32, 1000, ctypes.byref(tun), ctypes.byref(h)) == _lib.HTAB_E_ARG
+        assert b"block_size" in lib.htab_last_error()
     assert lib.htab_destroy(None) == _lib.HTAB_E_ARG
     assert lib.htab_find(None, None, 0, None, None, None) == _lib.HTAB_E_ARG
     assert b"null" in lib.htab_last_error()

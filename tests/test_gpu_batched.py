@@ -276,3 +276,43 @@ def test_misaligned_segments_and_requests_are_refused():
     assert tabs[0].size() == 1 and tabs[1].size() == 0
     srv.destroy()
     batch.destroy()
+
+
+@pytest.mark.parametrize("block", [64, 512, 1024])
+def test_batched_set_kind_any_workgroup_size(block):
+    """unordered_set<u64> tables (a kind whose persistent kernel exists for 256-thread workgroups
+    only) of any workgroup size: the batch cuts its work into chunks of the workgroup size it
+    launches, so every element of every table is processed exactly as by plain calls."""
+    rng = np.random.default_rng(block)
+    ntab = 24
+    caps = [int(c) for c in rng.integers(500, 20000, size=ntab)]
+    bt = [sa.unordered_set_u64.createDeviceObject(c, sa.Tuning(block_size=block)) for c in caps]
+    pt = [sa.unordered_set_u64.createDeviceObject(c, sa.Tuning(block_size=block)) for c in caps]
+    batch = TableBatch(bt)
+    sizes = [int(0.8 * c) for c in caps]
+    keys = [rng.integers(1, 2**60, size=s, dtype=np.uint64) for s in sizes]
+    off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+    allk = d64(np.concatenate(keys))
+    flags = torch.zeros(int(off[-1]), dtype=torch.uint8, device="cuda")
+    batch.run("insert", d64(off), allk, flags=flags)
+    bflags = flags.cpu().numpy()
+    q = d64(np.concatenate([k[::2] for k in keys]))
+    qoff = np.concatenate([[0], np.cumsum([len(k[::2]) for k in keys])]).astype(np.int64)
+    hits = torch.zeros(int(qoff[-1]), dtype=torch.uint8, device="cuda")
+    batch.run("erase", d64(qoff), q, flags=hits)
+    bh = hits.cpu().numpy()
+    for t, tab in enumerate(pt):
+        a, b = off[t], off[t + 1]
+        f = torch.zeros(int(b - a), dtype=torch.uint8, device="cuda")
+        tab.insert(allk[a:b], inserted=f)
+        np.testing.assert_array_equal(bflags[a:b], f.cpu().numpy())
+        assert int(f.sum()) == len(np.unique(keys[t]))
+        a, b = qoff[t], qoff[t + 1]
+        e = torch.zeros(int(b - a), dtype=torch.uint8, device="cuda")
+        tab.erase(q[a:b], erased=e)
+        np.testing.assert_array_equal(bh[a:b], e.cpu().numpy())
+        ka, _ = bt[t].contents()
+        kb, _ = tab.contents()
+        np.testing.assert_array_equal(np.sort(ka.cpu().numpy().view(np.uint64)), np.sort(kb.cpu().numpy().view(np.uint64)))
+        assert bt[t].size() == tab.size() and bt[t].valid()
+    batch.destroy()

@@ -256,6 +256,40 @@ def test_tiny_table_parallel_one_free(lanes):
     assert u8(h).tolist() == [1, 1]
 
 
+@pytest.mark.parametrize("block", [64, 256])
+def test_one_stripe_table_plans_its_launches(block):
+    """A table whose limit lies below its slots (capacity 2^16 at 2 slots per capacity: the limit is
+    the reference's max_size) counts its size on one stripe.  Bulk launches that cannot reach the
+    limit count on all stripes and fold back into it; a launch that can is ranked exactly, so the
+    table stops at its limit.  A fold that went missing would show as a wrong rank base below."""
+    tab = sa.unordered_map_u64_u32.createDeviceObject(1 << 16, sa.Tuning(block_size=block, slots_per_capacity=2.0))
+    limit = tab.info.limit
+    assert limit == tab.max_size() < tab.info.physical_slots
+    rng = np.random.default_rng(61)
+    keys = np.unique(rng.integers(1, 2**62, size=300000, dtype=np.uint64))[:200000]
+    keys = keys[rng.permutation(len(keys))]
+
+    def ins(k):
+        flags = torch.empty(len(k), dtype=torch.uint8, device="cuda")
+        tab.insert(dev(k, "u64"), dev(k & np.uint64(0xFFFF), "u32"), inserted=flags)
+        return int(u8(flags).sum())
+
+    assert ins(keys[:50000]) == 50000 and tab.size() == 50000       # cannot reach the limit
+    assert ins(keys[50000:90000]) == 40000 and tab.size() == 90000  # cannot reach it either
+    er = torch.empty(20000, dtype=torch.uint8, device="cuda")
+    tab.erase(dev(keys[:20000], "u64"), erased=er)
+    assert int(u8(er).sum()) == 20000 and tab.size() == 70000 and tab.valid()
+    # this one can: exactly limit - 70000 of 60000 new keys go in
+    assert ins(keys[90000:150000]) == limit - 70000
+    assert tab.size() == limit and tab.full() and tab.valid()
+    assert ins(keys[150000:151000]) == 0 and tab.size() == limit
+    tab.erase(dev(keys[20000:20010], "u64"))
+    assert tab.size() == limit - 10
+    assert ins(keys[151000:152000]) == 10 and tab.size() == limit and tab.valid()
+    h, _ = tab.find(dev(keys[20010:90000], "u64"))
+    assert int(u8(h).sum()) == 90000 - 20010
+
+
 @pytest.mark.parametrize("lanes", [1, 4, 8, 16])
 def test_contention_same_key(lanes):
     # insert_multiple_* / erase_multiple_* (unordered_datastructure.inc:1167-1373)
