@@ -137,8 +137,10 @@ def bench_batched(args) -> dict:
         C.unordered_map_u64_u32.destroyDeviceObject(t)
     tabs, batch = fresh()
     ms, out = at.run_phases(batch, w)
-    # plain per-table calls on fresh tables: the same work, one htab_* call per table
-    ptabs, pbatch = fresh()
+    # plain per-table calls on fresh tables of the same geometry: the same work, one htab_* call per
+    # table, with the plain calls' own default workgroup size (the tuned one is the persistent kernel's)
+    ptabs = at.build_tables(w["sizes"], {**best, "block_size": C.Tuning().block_size})
+    pbatch = TableBatch(ptabs)
     s = torch.cuda.Stream()
     pout = {k: torch.zeros_like(v) for k, v in out.items()}
     off = {k: w[f"{k}_offsets"].cpu().numpy() for k in ("ins", "find", "erase")}

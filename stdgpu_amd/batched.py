@@ -114,7 +114,8 @@ def main(argv=None) -> None:
         batch.destroy()
         for t in tabs:
             C.unordered_map_u64_u32.destroyDeviceObject(t)
-    tabs = at.build_tables(w["sizes"], cfg)
+    # the plain calls run with their own default workgroup size (the tuned one is the persistent kernel's)
+    tabs = at.build_tables(w["sizes"], {**cfg, "block_size": C.Tuning().block_size})
     batch = TableBatch(tabs)
     s = torch.cuda.Stream()
     pms = {}
