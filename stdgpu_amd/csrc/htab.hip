@@ -628,8 +628,7 @@ htab_batch_create(htab* const* tables, int32_t ntables, htab_batch** out)
     static_assert(sizeof(htab_dev::batch_segment) == sizeof(htab_segment), "segment layout");
     if (hipMalloc(&b->tables, desc) != hipSuccess ||
         hipMalloc(&b->segs, sizeof(htab_dev::batch_segment) * static_cast<size_t>(b->seg_cap)) != hipSuccess ||
-        hipMalloc(&b->chunk_start, sizeof(int64_t) * (static_cast<size_t>(b->seg_cap) + 1)) != hipSuccess ||
-        hipMalloc(&b->counter, 256) != hipSuccess)
+        hipMalloc(&b->chunk_start, sizeof(int64_t) * (static_cast<size_t>(b->seg_cap) + 1)) != hipSuccess)
     {
         htab_batch_destroy(b);
         return fail(HTAB_E_OOM, "htab_batch_create: device allocation");
@@ -655,7 +654,6 @@ htab_batch_destroy(htab_batch* b)
     if (b->tables) (void)hipFree(b->tables);
     if (b->segs) (void)hipFree(b->segs);
     if (b->chunk_start) (void)hipFree(b->chunk_start);
-    if (b->counter) (void)hipFree(b->counter);
     delete[] b->handles;
     delete b;
     return HTAB_OK;

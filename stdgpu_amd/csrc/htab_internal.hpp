@@ -77,7 +77,6 @@ struct htab_batch
     htab_dev::batch_segment* segs = nullptr; // device segments, seg_cap entries
     int32_t seg_cap = 0;
     int64_t* chunk_start = nullptr;    // seg_cap + 1
-    unsigned long long* counter = nullptr;
     uint32_t grid = 0;                 // persistent workgroups
 };
 

@@ -1041,9 +1041,11 @@ k_limit_fold(table_ref<K, M> t)
 // read from / written to its own device arrays (keys, vals, flags = inserted / hit / erased,
 // vals_out = mapped values of find hits).  Work is cut into chunks of kBatchBlock elements that
 // never straddle two segments; chunk_start[s] counts the chunks of segments before s.  A persistent
-// grid of B-thread workgroups pulls chunks of B elements from one counter until all are done, so
-// every workgroup exits once the counter passes the last chunk.  Segments of one table in one launch
-// must share their operation.
+// grid of B-thread workgroups takes chunks of B elements in a fixed order, workgroup w the chunks
+// w, w + grid, w + 2 grid, ..., and exits past the last one.  Chunks are alike in size, so the fixed
+// order balances as well as a shared work counter did, without one same-address atomic per chunk
+// (those serialise: the counter cost about 9 % of the 4096-table phases).  Segments of one table in
+// one launch must share their operation.
 constexpr int kOpInsert = 0;
 constexpr int kOpFind = 1;
 constexpr int kOpErase = 2;
@@ -1059,18 +1061,13 @@ struct batch_segment
     int32_t op;
 };
 
-// chunk prefix over segments (one workgroup); also resets the work counter
+// chunk prefix over segments (one workgroup)
 __device__ inline void
-plan_chunks(const batch_segment* __restrict__ segs, int32_t nsegs, int chunk, int64_t* __restrict__ chunk_start,
-            unsigned long long* __restrict__ counter)
+plan_chunks(const batch_segment* __restrict__ segs, int32_t nsegs, int chunk, int64_t* __restrict__ chunk_start)
 {
     __shared__ int64_t s_tmp[17];
     __shared__ int64_t s_carry;
-    if (threadIdx.x == 0)
-    {
-        s_carry = 0;
-        *counter = 0;
-    }
+    if (threadIdx.x == 0) s_carry = 0;
     __syncthreads();
     for (int64_t base = 0; base < nsegs; base += blockDim.x)
     {
@@ -1094,10 +1091,9 @@ plan_chunks(const batch_segment* __restrict__ segs, int32_t nsegs, int chunk, in
 // a template so every translation unit may instantiate it (no duplicate symbols)
 template <int Unused = 0>
 __global__ __launch_bounds__(1024) void
-k_batch_plan(const batch_segment* __restrict__ segs, int32_t nsegs, int chunk, int64_t* __restrict__ chunk_start,
-             unsigned long long* __restrict__ counter)
+k_batch_plan(const batch_segment* __restrict__ segs, int32_t nsegs, int chunk, int64_t* __restrict__ chunk_start)
 {
-    plan_chunks(segs, nsegs, chunk, chunk_start, counter);
+    plan_chunks(segs, nsegs, chunk, chunk_start);
 }
 
 // segments from a per-table work-offset array: segment t = table t, elements [offsets[t], offsets[t+1])
@@ -1108,7 +1104,7 @@ __global__ __launch_bounds__(1024) void
 k_batch_plan_offsets(const int64_t* __restrict__ offsets, int32_t ntables, const int8_t* __restrict__ ops, int op,
                      const char* keys, const char* vals, uint8_t* flags, char* vals_out, int key_bytes,
                      int val_bytes, int chunk, batch_segment* __restrict__ segs,
-                     int64_t* __restrict__ chunk_start, unsigned long long* __restrict__ counter)
+                     int64_t* __restrict__ chunk_start)
 {
     for (int32_t t = threadIdx.x; t < ntables; t += blockDim.x)
     {
@@ -1124,22 +1120,21 @@ k_batch_plan_offsets(const int64_t* __restrict__ offsets, int32_t ntables, const
         segs[t] = g;
     }
     __syncthreads();
-    plan_chunks(segs, ntables, chunk, chunk_start, counter);
+    plan_chunks(segs, ntables, chunk, chunk_start);
 }
 
 template <typename K, typename M, int G, int R, int B>
 __global__ __launch_bounds__(B) void
 k_batch(const table_ref<K, M>* __restrict__ tables, const batch_segment* __restrict__ segs, int32_t nsegs,
-        const int64_t* __restrict__ chunk_start, unsigned long long* __restrict__ counter)
+        const int64_t* __restrict__ chunk_start)
 {
     __shared__ int s_seg;
     __shared__ int64_t s_begin;
     const int64_t total_chunks = chunk_start[nsegs];
-    while (true)
+    for (int64_t c = blockIdx.x;; c += gridDim.x)
     {
         if (threadIdx.x == 0)
         {
-            const int64_t c = static_cast<int64_t>(atomicAdd(counter, 1ULL));
             int g = -1;
             int64_t b = 0;
             if (c < total_chunks)

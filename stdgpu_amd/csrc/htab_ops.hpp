@@ -76,8 +76,7 @@ struct batch_ops
     static int launch_rb(htab_batch* b, int32_t nsegs, hipStream_t s)
     {
         hipLaunchKernelGGL((k_batch<K, M, G, R, B>), dim3(b->grid), dim3(B), 0, s,
-                           static_cast<const table_ref<K, M>*>(b->tables), b->segs, nsegs, b->chunk_start,
-                           b->counter);
+                           static_cast<const table_ref<K, M>*>(b->tables), b->segs, nsegs, b->chunk_start);
         return launch_status("batch launch");
     }
 
@@ -132,7 +131,7 @@ struct batch_ops
         hipLaunchKernelGGL(k_batch_plan_offsets<>, dim3(1), dim3(1024), 0, s, offsets, b->ntables, ops, op,
                            static_cast<const char*>(keys), static_cast<const char*>(vals), flags,
                            static_cast<char*>(vals_out), static_cast<int>(sizeof(K)), vb, b->block, b->segs,
-                           b->chunk_start, b->counter);
+                           b->chunk_start);
         int rc = launch_status("batch plan");
         if (rc) return rc;
         return launch(b, b->ntables, s);
@@ -140,8 +139,7 @@ struct batch_ops
 
     static int run_segments(htab_batch* b, int32_t nsegs, hipStream_t s)
     {
-        hipLaunchKernelGGL(k_batch_plan<>, dim3(1), dim3(1024), 0, s, b->segs, nsegs, b->block, b->chunk_start,
-                           b->counter);
+        hipLaunchKernelGGL(k_batch_plan<>, dim3(1), dim3(1024), 0, s, b->segs, nsegs, b->block, b->chunk_start);
         int rc = launch_status("batch plan");
         if (rc) return rc;
         return launch(b, nsegs, s);
