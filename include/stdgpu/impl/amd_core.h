@@ -130,6 +130,11 @@ public:
 
     constexpr __host__ __device__ device_ptr() = default;
     constexpr __host__ __device__ explicit device_ptr(T* p) : _p(p) {}
+    // device_ptr<T> converts to device_ptr<const T> (iterator.h: the const overloads take either)
+    template <typename U, std::enable_if_t<std::is_convertible_v<U*, T*> && !std::is_same_v<U, T>, int> = 0>
+    constexpr __host__ __device__ device_ptr(const device_ptr<U>& other) : _p(other.get())
+    {
+    }
     constexpr __host__ __device__ T* get() const { return _p; }
     __device__ T& operator*() const { return *_p; }
     __device__ T& operator[](difference_type i) const { return _p[i]; }

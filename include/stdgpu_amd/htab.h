@@ -272,9 +272,21 @@ int htab_vector_create(int64_t capacity, htab_vector** out);
 int htab_vector_destroy(htab_vector* vec);
 /* push_back of n values; pushed[i] (optional) = 1 iff value i found room (vector_detail.cuh:181-240) */
 int htab_vector_push_back(htab_vector* vec, const uint32_t* values, int64_t n, uint8_t* pushed, void* stream);
-/* pop_back of up to n values from the top: out[i] = the value, or 0xFFFFFFFF when the stack ran empty;
- * popped[i] (optional) = 1 iff element i got a value (vector_detail.cuh:242-302) */
+/* pop_back of up to n values from the top: out[i] = the value, or 0xFFFFFFFF when the stack ran empty
+ * (out may be null: the values are dropped, as erase(end - n, end) does); popped[i] (optional) = 1 iff
+ * element i got a value (vector_detail.cuh:242-302) */
 int htab_vector_pop_back(htab_vector* vec, uint32_t* out, int64_t n, uint8_t* popped, void* stream);
+/* size = 0 (vector_detail.cuh:579-614 clear); one kernel, no memset (safe inside a captured graph) */
+int htab_vector_clear(htab_vector* vec, void* stream);
+/* device addresses for user kernels (include/stdgpu_amd/vector_device.hpp): the elements, the int64 size
+ * word, and the capacity */
+typedef struct htab_vector_view
+{
+    uint32_t* data;
+    int64_t* size;
+    int64_t capacity;
+} htab_vector_view;
+int htab_vector_get_view(htab_vector* vec, htab_vector_view* out);
 /* vector_clear_iota: element i = first + i, size = capacity (vector_detail.cuh:361-368) */
 int htab_vector_fill_iota(htab_vector* vec, uint32_t first, void* stream);
 /* size() clamped to [0, capacity] (vector_detail.cuh:454-512) */

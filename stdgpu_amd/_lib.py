@@ -120,6 +120,14 @@ class htab_deque_view(ctypes.Structure):
     ]
 
 
+class htab_vector_view(ctypes.Structure):
+    _fields_ = [
+        ("data", ctypes.c_void_p),
+        ("size", ctypes.c_void_p),
+        ("capacity", ctypes.c_int64),
+    ]
+
+
 class htab_view(ctypes.Structure):
     _fields_ = [
         ("slots", ctypes.c_void_p),
@@ -203,6 +211,8 @@ SIGNATURES = {
     "htab_vector_fill_iota": (_I32, [_P, ctypes.c_uint32, _P]),
     "htab_vector_size": (_I32, [_P, ctypes.POINTER(_I64), _P]),
     "htab_vector_contents": (_I32, [_P, _P, ctypes.POINTER(_I64), _P]),
+    "htab_vector_clear": (_I32, [_P, _P]),
+    "htab_vector_get_view": (_I32, [_P, ctypes.POINTER(htab_vector_view)]),
 }
 
 _lib = None

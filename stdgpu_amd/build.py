@@ -21,6 +21,7 @@ HEADERS = [
     os.path.join(ROOT, "include", "stdgpu_amd", "htab_device.hpp"),
     os.path.join(ROOT, "include", "stdgpu_amd", "atomic_device.hpp"),
     os.path.join(ROOT, "include", "stdgpu_amd", "deque_device.hpp"),
+    os.path.join(ROOT, "include", "stdgpu_amd", "vector_device.hpp"),
     os.path.join(CSRC, "htab_kernels.hpp"),
     os.path.join(CSRC, "wave.hpp"),
     os.path.join(CSRC, "htab_internal.hpp"),
@@ -77,7 +78,7 @@ def build_roofline() -> str:
     return build_library(ROOFLINE_LIB, ["roofline.hip"])
 
 
-CPP_TESTS = ["test_facade", "test_atomic", "test_deque", "test_bitset"]
+CPP_TESTS = ["test_facade", "test_atomic", "test_deque", "test_bitset", "test_vector"]
 
 
 def build_cpp_tests() -> list[str]:

@@ -371,6 +371,15 @@ class vector_u32:
         check(_lib.load().htab_vector_size(self._h, ctypes.byref(out), _stream_ptr(stream)), "htab_vector_size")
         return out.value
 
+    def clear(self, stream=None):
+        check(_lib.load().htab_vector_clear(self._h, _stream_ptr(stream)), "htab_vector_clear")
+
+    def view(self) -> _lib.htab_vector_view:
+        """device addresses for user kernels (elements, size word, capacity)"""
+        v = _lib.htab_vector_view()
+        check(_lib.load().htab_vector_get_view(self._h, ctypes.byref(v)), "htab_vector_get_view")
+        return v
+
     def contents(self, stream=None):
         """elements from bottom to top (int32 device tensor of uint32 bits)"""
         torch = _torch()

@@ -53,8 +53,14 @@ k_vector_pop(const uint32_t* __restrict__ data, const int64_t* __restrict__ ctl,
     const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const bool ok = i < ctl[kResCount];
-    out[i] = ok ? data[ctl[kResBase] - 1 - i] : 0xFFFFFFFFU;
+    if (out) out[i] = ok ? data[ctl[kResBase] - 1 - i] : 0xFFFFFFFFU;
     if (popped) popped[i] = ok ? 1 : 0;
+}
+
+__global__ void
+k_vector_clear(int64_t* ctl)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0) ctl[kSize] = 0;
 }
 
 // vector_clear_iota (vector_detail.cuh:361-368): data[i] = first + i, size = capacity
@@ -159,7 +165,7 @@ htab_vector_push_back(htab_vector* q, const uint32_t* values, int64_t n, uint8_t
 int
 htab_vector_pop_back(htab_vector* q, uint32_t* out, int64_t n, uint8_t* popped, void* stream)
 {
-    if (!q || n < 0 || (n > 0 && !out) || !aligned4(out)) return HTAB_E_ARG;
+    if (!q || n < 0 || !aligned4(out)) return HTAB_E_ARG;
     if (n == 0) return HTAB_OK;
     const int64_t grid = (n + 255) / 256;
     if (grid > 0x7FFFFFFF) return HTAB_E_LIMIT;
@@ -168,6 +174,24 @@ htab_vector_pop_back(htab_vector* q, uint32_t* out, int64_t n, uint8_t* popped, 
     hipLaunchKernelGGL(k_vector_pop, dim3(static_cast<uint32_t>(grid)), dim3(256), 0, s, q->data, q->ctl, out, n,
                        popped);
     return vec_status();
+}
+
+int
+htab_vector_clear(htab_vector* q, void* stream)
+{
+    if (!q) return HTAB_E_ARG;
+    hipLaunchKernelGGL(k_vector_clear, dim3(1), dim3(64), 0, static_cast<hipStream_t>(stream), q->ctl);
+    return vec_status();
+}
+
+int
+htab_vector_get_view(htab_vector* q, htab_vector_view* out)
+{
+    if (!q || !out) return HTAB_E_ARG;
+    out->data = q->data;
+    out->size = q->ctl + kSize;
+    out->capacity = q->capacity;
+    return HTAB_OK;
 }
 
 int

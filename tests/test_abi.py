@@ -42,6 +42,7 @@ def test_struct_layouts_match_header():
     assert ctypes.sizeof(_lib.htab_info) == 8 + 8 * 8
     assert ctypes.sizeof(_lib.htab_view) == 3 * 8 + 2 * 8 + 2 * 4 + 2 * 8 + 8
     assert ctypes.sizeof(_lib.htab_deque_view) == 7 * 8
+    assert ctypes.sizeof(_lib.htab_vector_view) == 3 * 8
     assert ctypes.sizeof(_lib.htab_bitset_view) == 2 * 8
 
 
@@ -116,6 +117,8 @@ def test_container_argument_errors_without_gpu():
     assert lib.htab_vector_create(0, ctypes.byref(h)) == E
     assert lib.htab_vector_create((1 << 32) + 1, ctypes.byref(h)) == E
     assert lib.htab_vector_push_back(None, None, 1, None, None) == E
+    assert lib.htab_vector_clear(None, None) == E
+    assert lib.htab_vector_get_view(None, None) == E
     assert lib.htab_deque_create(0, ctypes.byref(h)) == E
     assert lib.htab_bitset_create(0, ctypes.byref(h)) == E
     assert lib.htab_bitset_create(10, None) == E

@@ -12,7 +12,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 BUILD = os.path.join(ROOT, "tests", "cpp", "_build")
 
 
-@pytest.mark.parametrize("name", ["test_facade", "test_atomic", "test_deque", "test_bitset"])
+@pytest.mark.parametrize("name", ["test_facade", "test_atomic", "test_deque", "test_bitset", "test_vector"])
 def test_cpp_facade_cases(name):
     exe = os.path.join(BUILD, name)
     assert os.path.exists(exe), "build the C++ tests first: python -m stdgpu_amd.build"
