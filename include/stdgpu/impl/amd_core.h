@@ -1,11 +1,11 @@
 // Shared vocabulary of the source-compatible facade: index_t, pair, hash, equal_to,
-// execution policies, allocator tag, device_ptr and device_indexed_range.
+// execution policies, allocator tag, device_ptr, device_range and device_indexed_range.
 //
 // Names and meanings follow the reference headers they stand in for (paths relative to the
 // stdgpu source tree): cstddef.h:55-59 (index_t), utility.h:43 (pair), functional.h and
 // impl/functional_detail.h:60-125 (hash, equal_to), execution.h:43-92 (execution::device),
 // memory.h:340 (safe_device_allocator), iterator.h:43-184 (device_ptr, make_device) and
-// ranges.h:285 (device_indexed_range).  Compiled with hipcc (gfx950).
+// ranges.h (device_range, device_indexed_range :285).  Compiled with hipcc (gfx950).
 #ifndef STDGPU_AMD_FACADE_CORE_H
 #define STDGPU_AMD_FACADE_CORE_H
 
@@ -233,6 +233,23 @@ private:
     const std::int64_t* _idx;
     index_t _n;
     T* _values;
+};
+
+// a contiguous device range [begin, begin + size) (ranges.h device_range), e.g. a vector's elements
+template <typename T>
+class device_range
+{
+public:
+    constexpr __host__ __device__ device_range() = default;
+    constexpr __host__ __device__ device_range(T* first, index_t n) : _first(first), _n(n) {}
+    constexpr __host__ __device__ device_ptr<T> begin() const { return device_ptr<T>(_first); }
+    constexpr __host__ __device__ device_ptr<T> end() const { return device_ptr<T>(_first + _n); }
+    constexpr __host__ __device__ index_t size() const { return _n; }
+    constexpr __host__ __device__ bool empty() const { return _n == 0; }
+
+private:
+    T* _first = nullptr;
+    index_t _n = 0;
 };
 
 } // namespace stdgpu

@@ -3,7 +3,7 @@
 //
 // Host: createDeviceObject / destroyDeviceObject, insert at the end and erase up to the end from
 // device ranges (the only positions the reference implements, vector.cuh:73-75), clear, size / empty
-// / full / capacity / max_size, shrink_to_fit, device_begin / device_end, valid.  Device (user
+// / full / capacity / max_size, shrink_to_fit, device_begin / device_end / device_range, valid.  Device (user
 // kernels, per thread): push_back / emplace_back / pop_back, size / empty / full, operator[] / at /
 // front / back (include/stdgpu_amd/vector_device.hpp).  The calling lanes of a wavefront take their
 // positions with one atomic.  A kernel pushes or pops a given vector, not both: a pop could take a
@@ -202,6 +202,22 @@ public:
     device_ptr<const T> device_end() const { return device_ptr<const T>(data() + size()); }
     device_ptr<const T> device_cbegin() const { return device_begin(); }
     device_ptr<const T> device_cend() const { return device_end(); }
+
+    // the elements as a range (vector.cuh device_range; a host call, it reads the size)
+    stdgpu::device_range<T> device_range() { return stdgpu::device_range<T>(data(), size()); }
+    stdgpu::device_range<const T> device_range() const { return stdgpu::device_range<const T>(data(), size()); }
+
+    template <typename ExecutionPolicy, std::enable_if_t<is_execution_policy_v<ExecutionPolicy>, int> = 0>
+    stdgpu::device_range<T> device_range(ExecutionPolicy&& policy)
+    {
+        return stdgpu::device_range<T>(data(), size(std::forward<ExecutionPolicy>(policy)));
+    }
+
+    template <typename ExecutionPolicy, std::enable_if_t<is_execution_policy_v<ExecutionPolicy>, int> = 0>
+    stdgpu::device_range<const T> device_range(ExecutionPolicy&& policy) const
+    {
+        return stdgpu::device_range<const T>(data(), size(std::forward<ExecutionPolicy>(policy)));
+    }
 
     // the size word lies in [0, capacity] (vector_detail.cuh:616-640 checks its own layout)
     [[nodiscard]] bool valid() const { return valid(execution::device); }

@@ -101,6 +101,14 @@ k_push_odd(ivec v, const int* values, std::size_t n, int* ok)
     if (i < n && (i & 1)) ok[i] = v.push_back(values[i]) ? 1 : 0;
 }
 
+template <typename Range>
+__global__ void
+k_copy_range(Range r, int* out)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < r.size()) out[i] = r.begin()[i];
+}
+
 __global__ void
 k_access(ivec v, int* out)
 {
@@ -319,7 +327,7 @@ insert_erase_clear()
 static void
 element_access()
 {
-    g_case = "front / back / at / operator[] / data / device_begin / device_end / shrink_to_fit (vector.inc:1079-1200, 1558-1695)";
+    g_case = "front / back / at / operator[] / data / device_begin / device_end / device_range / shrink_to_fit (vector.inc:1079-1200, 1558-1755)";
     const stdgpu::index_t N = 10000;
     ivec v = ivec::createDeviceObject(N);
     fill(v, N);
@@ -332,6 +340,17 @@ element_access()
     CHECK(v.data() == c.data());
     CHECK(v.device_begin().get() == v.data() && v.device_end().get() == v.data() + N);
     CHECK(c.device_end() == c.device_cend() && c.device_begin() == c.device_cbegin());
+    // non_const_device_range / const_device_range (vector.inc:1696-1755)
+    int* copy = dev_alloc<int>(static_cast<std::size_t>(N));
+    auto r = v.device_range();
+    CHECK(r.size() == N && !r.empty() && r.begin().get() == v.data() && r.end().get() == v.data() + N);
+    hipLaunchKernelGGL(k_copy_range<decltype(r)>, dim3(blocks(static_cast<std::size_t>(N))), dim3(256), 0, 0, r, copy);
+    CHECK(to_host(copy, static_cast<std::size_t>(N)) == iota_from(1, static_cast<std::size_t>(N)));
+    auto cr = c.device_range();
+    (void)hipMemset(copy, 0, static_cast<std::size_t>(N) * sizeof(int));
+    hipLaunchKernelGGL(k_copy_range<decltype(cr)>, dim3(blocks(static_cast<std::size_t>(N))), dim3(256), 0, 0, cr, copy);
+    CHECK(to_host(copy, static_cast<std::size_t>(N)) == iota_from(1, static_cast<std::size_t>(N)));
+    (void)hipFree(copy);
     (void)pop(v, 100);
     v.shrink_to_fit();
     CHECK(v.size() == N - 100 && (v.capacity() == N - 100 || v.capacity() == N) && v.valid());
