@@ -1039,7 +1039,7 @@ k_limit_fold(table_ref<K, M> t)
 // ================================================================== batched: many tables, one launch
 // A launch works through segments; segment s applies operation op to n elements of table `table`,
 // read from / written to its own device arrays (keys, vals, flags = inserted / hit / erased,
-// vals_out = mapped values of find hits).  Work is cut into chunks of kBatchBlock elements that
+// vals_out = mapped values of find hits).  Work is cut into chunks of B elements (the workgroup size) that
 // never straddle two segments; chunk_start[s] counts the chunks of segments before s.  A persistent
 // grid of B-thread workgroups takes chunks of B elements in a fixed order, workgroup w the chunks
 // w, w + grid, w + 2 grid, ..., and exits past the last one.  Chunks are alike in size, so the fixed
