@@ -96,3 +96,31 @@ def test_bulk_calls_equal_a_sequential_loop():
         g.destroy()
     rv.close()
     rq.close()
+
+
+def test_clear_view_and_dropping_pops():
+    """clear() empties the stack in one kernel; the view names the elements, the size word and the
+    capacity; a pop with no output buffer drops the top values (the C++ facade's erase)"""
+    import ctypes
+
+    from stdgpu_amd import _lib
+
+    v = sa.vector_u32(1000)
+    v.fill_iota(5)
+    assert v.size() == 1000
+    view = v.view()
+    assert view.capacity == 1000 and view.data and view.size
+    assert ctypes.sizeof(_lib.htab_vector_view) == 24
+    flags = torch.zeros(300, dtype=torch.uint8, device="cuda")
+    s = torch.cuda.current_stream().cuda_stream
+    assert _lib.load().htab_vector_pop_back(v._h, None, 300, flags.data_ptr(), s) == 0
+    torch.cuda.synchronize()
+    assert int(flags.sum()) == 300 and v.size() == 700
+    assert (host(v.contents()) == np.arange(5, 705, dtype=np.uint32)).all()
+    v.clear()
+    torch.cuda.synchronize()
+    assert v.size() == 0 and v.contents().numel() == 0
+    pushed = torch.zeros(3, dtype=torch.uint8, device="cuda")
+    v.push_back(dev([7, 8, 9]), pushed=pushed)
+    assert int(pushed.sum()) == 3 and v.size() == 3
+    v.destroy()
