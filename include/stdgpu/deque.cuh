@@ -2,8 +2,9 @@
 // free-list container of voxel-block pipelines, on gfx950.
 //
 // Host: createDeviceObject / destroyDeviceObject, size / empty / full / capacity / max_size, clear,
-// valid.  Device (user kernels, per thread): push_back / emplace_back, pop_front, size / empty / full,
-// operator[] / at / front / back (include/stdgpu_amd/deque_device.hpp).  The calling lanes of a
+// valid, device_range (front to back over the ring).  Device (user kernels, per thread): push_back /
+// emplace_back, pop_front, size / empty / full, operator[] / at / front / back
+// (include/stdgpu_amd/deque_device.hpp).  The calling lanes of a
 // wavefront reserve from the published-element or released-slot count with one atomic and take
 // consecutive ring positions with one more; a pop whose element is still being written, or a push
 // whose slot is still being read, waits on that slot's sequence word as the reference waits on its
@@ -22,6 +23,69 @@
 
 namespace stdgpu
 {
+
+// Random-access iterator over a ring of `capacity` slots: element i is data[(start + i) mod capacity]
+template <typename T>
+class ring_iterator
+{
+public:
+    using value_type = std::remove_cv_t<T>;
+    using difference_type = std::ptrdiff_t;
+    using reference = T&;
+    using pointer = T*;
+    using iterator_category = std::random_access_iterator_tag;
+
+    constexpr __host__ __device__ ring_iterator(T* data, std::uint64_t capacity, std::uint64_t pos)
+      : _data(data), _cap(capacity), _pos(pos)
+    {
+    }
+    __device__ T& operator*() const { return _data[_pos % _cap]; }
+    __device__ T& operator[](difference_type i) const { return _data[(_pos + static_cast<std::uint64_t>(i)) % _cap]; }
+    constexpr __host__ __device__ ring_iterator operator+(difference_type d) const
+    {
+        return ring_iterator(_data, _cap, _pos + static_cast<std::uint64_t>(d));
+    }
+    constexpr __host__ __device__ difference_type operator-(const ring_iterator& o) const
+    {
+        return static_cast<difference_type>(_pos - o._pos);
+    }
+    constexpr __host__ __device__ ring_iterator& operator++()
+    {
+        ++_pos;
+        return *this;
+    }
+    constexpr __host__ __device__ bool operator==(const ring_iterator& o) const { return _pos == o._pos; }
+    constexpr __host__ __device__ bool operator!=(const ring_iterator& o) const { return _pos != o._pos; }
+
+private:
+    T* _data;
+    std::uint64_t _cap;
+    std::uint64_t _pos;
+};
+
+// the elements of a deque front to back, as a range over its ring (deque.cuh device_range)
+template <typename T>
+class ring_range
+{
+public:
+    constexpr __host__ __device__ ring_range(T* data, std::uint64_t capacity, std::uint64_t first, index_t n)
+      : _data(data), _cap(capacity), _first(first), _n(n)
+    {
+    }
+    constexpr __host__ __device__ ring_iterator<T> begin() const { return ring_iterator<T>(_data, _cap, _first); }
+    constexpr __host__ __device__ ring_iterator<T> end() const
+    {
+        return ring_iterator<T>(_data, _cap, _first + static_cast<std::uint64_t>(_n));
+    }
+    constexpr __host__ __device__ index_t size() const { return _n; }
+    constexpr __host__ __device__ bool empty() const { return _n == 0; }
+
+private:
+    T* _data;
+    std::uint64_t _cap;
+    std::uint64_t _first;
+    index_t _n;
+};
 
 template <typename T, typename Allocator = safe_device_allocator<T>>
 class deque
@@ -164,10 +228,42 @@ public:
         return c[0] <= c[1] && c[1] - c[0] <= static_cast<std::uint64_t>(_view.capacity);
     }
 
+    // the elements front to back (a host call: it reads the front counter and the size)
+    ring_range<T> device_range() { return device_range(execution::device); }
+    ring_range<const T> device_range() const { return device_range(execution::device); }
+
+    template <typename ExecutionPolicy, std::enable_if_t<is_execution_policy_v<ExecutionPolicy>, int> = 0>
+    ring_range<T> device_range(ExecutionPolicy&& policy)
+    {
+        const std::uint64_t first = front_position(policy.stream);
+        return ring_range<T>(data(), static_cast<std::uint64_t>(_view.capacity), first, size(policy));
+    }
+
+    template <typename ExecutionPolicy, std::enable_if_t<is_execution_policy_v<ExecutionPolicy>, int> = 0>
+    ring_range<const T> device_range(ExecutionPolicy&& policy) const
+    {
+        const std::uint64_t first = front_position(policy.stream);
+        return ring_range<const T>(data(), static_cast<std::uint64_t>(_view.capacity), first, size(policy));
+    }
+
     htab_deque* native_handle() const { return _h; }
 
 private:
     __host__ __device__ htab_dev::deque_ref ref() const { return htab_dev::deque_ref::from_view(_view); }
+
+    std::uint64_t front_position(hipStream_t s) const
+    {
+        std::uint64_t b = 0;
+        if (_h)
+        {
+            detail::check(hipMemcpyAsync(&b, _view.begin, sizeof(b), hipMemcpyDeviceToHost, s) == hipSuccess &&
+                                          hipStreamSynchronize(s) == hipSuccess
+                                  ? HTAB_OK
+                                  : HTAB_E_HIP,
+                          "deque::device_range");
+        }
+        return b;
+    }
 
     htab_deque* _h = nullptr;
     htab_deque_view _view{};

@@ -146,6 +146,55 @@ sum(const std::vector<int>& v)
     return std::accumulate(v.begin(), v.end(), 0);
 }
 
+template <typename Range>
+__global__ void
+k_copy_ring(Range r, std::uint32_t* out)
+{
+    const int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i < r.size()) out[i] = r.begin()[i];
+}
+
+// (non_)const_device_range, _full, _circular (deque.inc): the range holds the elements front to back,
+// also after the ring has wrapped
+static void
+device_ranges()
+{
+    g_case = "device_range, const device_range: partly filled, full, wrapped around the ring (deque.inc)";
+    const std::size_t N = 10000;
+    for (int shape = 0; shape < 3; ++shape)
+    {
+        const std::size_t cap = shape == 0 ? 2 * N : (shape == 1 ? N : N + 1);
+        free_list q = free_list::createDeviceObject(static_cast<stdgpu::index_t>(cap));
+        int* ok = dev_alloc<int>(N);
+        std::uint32_t* tmp = dev_alloc<std::uint32_t>(N);
+        hipLaunchKernelGGL(k_push, dim3(blocks(N)), dim3(256), 0, 0, q, 1U, N, ok);
+        std::vector<std::uint32_t> want = contents(q);
+        if (shape == 2)
+        {
+            // pop a quarter from the front and push it back: the elements now wrap around the ring end
+            const std::size_t off = N / 4;
+            hipLaunchKernelGGL(k_pop, dim3(blocks(off)), dim3(256), 0, 0, q, off, tmp, ok);
+            hipLaunchKernelGGL(k_push, dim3(blocks(off)), dim3(256), 0, 0, q, 1U, off, ok);
+            want = contents(q);
+        }
+        CHECK(want.size() == N && q.size() == static_cast<stdgpu::index_t>(N) && q.valid());
+        std::uint32_t* got = dev_alloc<std::uint32_t>(N);
+        auto r = q.device_range();
+        CHECK(r.size() == static_cast<stdgpu::index_t>(N) && !r.empty() && r.end() - r.begin() == static_cast<std::ptrdiff_t>(N));
+        hipLaunchKernelGGL(k_copy_ring<decltype(r)>, dim3(blocks(N)), dim3(256), 0, 0, r, got);
+        CHECK(to_host(got, N) == want); // front to back, element by element
+        const free_list& cq = q;
+        auto cr = cq.device_range();
+        (void)hipMemset(got, 0, N * sizeof(std::uint32_t));
+        hipLaunchKernelGGL(k_copy_ring<decltype(cr)>, dim3(blocks(N)), dim3(256), 0, 0, cr, got);
+        CHECK(to_host(got, N) == want);
+        (void)hipFree(got);
+        (void)hipFree(tmp);
+        (void)hipFree(ok);
+        free_list::destroyDeviceObject(q);
+    }
+}
+
 static void
 push_to_full_and_pop_to_empty()
 {
@@ -409,7 +458,7 @@ main()
     }
     void (*cases[])() = { push_to_full_and_pop_to_empty, mixed_pushes_and_pops_in_diverged_lanes,
                           voxel_free_list_pattern, wrap_around_many_turns, bulk_and_per_thread_calls_interleave,
-                          per_thread_rates, other_value_types };
+                          per_thread_rates, other_value_types, device_ranges };
     for (auto c : cases)
     {
         const int before = g_failures;
