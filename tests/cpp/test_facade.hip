@@ -9,6 +9,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
+#include <limits>
 #include <map>
 #include <numeric>
 #include <random>
@@ -1112,6 +1113,61 @@ native_rates()
     map_t::destroyDeviceObject(m);
 }
 
+// hash of the integer types (functional.cpp:120-211): the identity widened to size_t, signed types
+// sign-extended (functional_detail.h:60-80), so every value of a small type hashes apart and a million
+// random values of a wide type keep more than 90% distinct hashes, as the reference tests require;
+// equal_to and its transparent form (functional.cpp:457-506)
+template <typename T>
+static bool
+hash_small_type_ok()
+{
+    stdgpu::hash<T> h;
+    std::set<std::size_t> seen;
+    for (long long v = static_cast<long long>(std::numeric_limits<T>::lowest());
+         v <= static_cast<long long>(std::numeric_limits<T>::max()); ++v)
+    {
+        const T x = static_cast<T>(v);
+        if (h(x) != static_cast<std::size_t>(x)) return false;
+        seen.insert(h(x));
+    }
+    return seen.size() == static_cast<std::size_t>(static_cast<long long>(std::numeric_limits<T>::max()) -
+                                                   static_cast<long long>(std::numeric_limits<T>::lowest()) + 1);
+}
+
+template <typename T>
+static bool
+hash_wide_type_ok()
+{
+    stdgpu::hash<T> h;
+    std::mt19937_64 rng(sizeof(T) * 7 + std::is_signed_v<T>);
+    std::uniform_int_distribution<T> dist(std::numeric_limits<T>::lowest(), std::numeric_limits<T>::max());
+    std::set<std::size_t> seen;
+    const int N = 1000000;
+    for (int i = 0; i < N; ++i)
+    {
+        const T x = dist(rng);
+        if (h(x) != static_cast<std::size_t>(x)) return false;
+        seen.insert(h(x));
+    }
+    if (h(static_cast<T>(-1)) != static_cast<std::size_t>(static_cast<T>(-1))) return false; // sign extension
+    return static_cast<int>(seen.size()) > N * 90 / 100;
+}
+
+static void
+hash_and_equal_to_of_integers()
+{
+    g_case = "hash of every integer type and equal_to (functional.cpp:120-211, 457-506)";
+    CHECK(hash_small_type_ok<bool>() && hash_small_type_ok<char>() && hash_small_type_ok<signed char>());
+    CHECK(hash_small_type_ok<unsigned char>() && hash_small_type_ok<short>() && hash_small_type_ok<unsigned short>());
+    CHECK(hash_wide_type_ok<int>() && hash_wide_type_ok<unsigned int>() && hash_wide_type_ok<long>());
+    CHECK(hash_wide_type_ok<unsigned long>() && hash_wide_type_ok<long long>() &&
+          hash_wide_type_ok<unsigned long long>());
+    CHECK(stdgpu::hash<int>{}(-5) == ~std::size_t{ 4 }); // -5 sign-extended
+    stdgpu::equal_to<int> eq;
+    stdgpu::equal_to<> teq;
+    CHECK(eq(3, 3) && !eq(3, 4) && teq(3, 3L) && !teq(3U, 4ULL));
+}
+
 int
 main()
 {
@@ -1137,7 +1193,8 @@ main()
                           struct_keys,
                           bulk_find_and_contains,
                           reference_fixture_cases,
-                          native_rates };
+                          native_rates,
+                          hash_and_equal_to_of_integers };
     for (auto c : cases)
     {
         const int before = g_failures;
