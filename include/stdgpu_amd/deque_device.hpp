@@ -130,7 +130,7 @@ private:
     __device__ static wave_call begin_call()
     {
         wave_call w;
-        w.lanes = __builtin_amdgcn_read_exec();
+        w.lanes = __builtin_amdgcn_ballot_w64(true); // the lanes making this call (a convergent vote)
         w.lane = static_cast<int>(__builtin_amdgcn_mbcnt_hi(~0U, __builtin_amdgcn_mbcnt_lo(~0U, 0U)));
         w.leader = __builtin_ctzll(w.lanes);
         w.k = static_cast<uint32_t>(__builtin_popcountll(w.lanes));
