@@ -45,7 +45,8 @@ typedef struct htab_tuning
 {
     int32_t bucket_lanes;      /* lanes that probe one bucket together: 1, 4, 8 or 16 (bucket = 16*lanes bytes); default 8 */
     int32_t block_size;        /* threads per workgroup of the bulk kernels: 64, 128, 256, 512 or 1024;
-                                  default 64 (the batched form's persistent kernel uses 256 for 64 and 128) */
+                                  default 64 (the batched form's persistent kernel takes 64, 512 and 1024
+                                  for unordered_map<u64, u32>, and 256 otherwise) */
     float slots_per_capacity;  /* physical slots = ceil(capacity * this); default 1.0 (load 0.8 at 0.8*capacity keys) */
     int32_t probe_batch;       /* elements a lane group probes together: 2, 4 or 8 (at most bucket_lanes);
                                 * default: 8 for find, 4 for insert and erase (1 with one-lane buckets) */

@@ -5,7 +5,7 @@ Sweeps, on the workload itself:
   * probe group: lanes whose bucket loads are issued together = bucket lanes x probe batch,
     16, 32 or 64 (probe batch 2, 4 or 8, at most the bucket lanes); with one-slot buckets every
     lane probes its own element (probe group 1);
-  * workgroup size: 256, 512 or 1024 (also the chunk of elements a workgroup takes at a time);
+  * workgroup size: 64, 256, 512 or 1024 (also the chunk of elements a workgroup takes at a time);
   * load factor: 0.5, 0.7, 0.8 or 0.9 (physical slots = keys / load).
 Each candidate builds its tables, runs the insert, find and erase phases once as captured HIP
 graphs and is scored by the sum of the three phase times (HIP events on the launch stream).
@@ -20,7 +20,7 @@ from .batched import TableBatch
 
 BUCKET_LANES = (1, 4, 8, 16)
 PROBE_GROUPS = (16, 32, 64)
-BLOCKS = (256, 512, 1024)
+BLOCKS = (64, 256, 512, 1024)
 LOADS = (0.5, 0.7, 0.8, 0.9)
 
 

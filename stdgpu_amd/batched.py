@@ -102,8 +102,8 @@ def main(argv=None) -> None:
     from . import workloads_device as wd
 
     args = sys.argv[1:] if argv is None else argv
-    # default: the autotuner's pick in the benchmark runs (8-slot buckets, probe batch 8, 256 threads, load 0.5)
-    g, rb, blk, load = (int(args[0]), int(args[1]), int(args[2]), float(args[3])) if len(args) >= 4 else (8, 8, 256, 0.5)
+    # default: the autotuner's pick in the benchmark runs (8-slot buckets, probe batch 8, 64 threads, load 0.5)
+    g, rb, blk, load = (int(args[0]), int(args[1]), int(args[2]), float(args[3])) if len(args) >= 4 else (8, 8, 64, 0.5)
     cfg = {"bucket_lanes": g, "probe_batch": rb, "probe_group": g * rb, "block_size": blk, "load": load}
     w = wd.batched_maps("cuda", ntables=4096)
     out = {"config": cfg}

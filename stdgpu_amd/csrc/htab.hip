@@ -604,10 +604,10 @@ htab_batch_create(htab* const* tables, int32_t ntables, htab_batch** out)
     b->kind = tables[0]->kind;
     b->lanes = tables[0]->lanes;
     // the persistent kernel's workgroup size is also its chunk size, so it is the size the launch
-    // uses: the tables' own where that kernel exists (512 and 1024 for unordered_map<u64, u32>,
+    // uses: the tables' own where that kernel exists (64, 512 and 1024 for unordered_map<u64, u32>,
     // batch_all_blocks in htab_ops.hpp), else 256
     const int tb = tables[0]->block;
-    b->block = (b->kind == HTAB_MAP_U64_U32 && (tb == 512 || tb == 1024)) ? tb : 256;
+    b->block = (b->kind == HTAB_MAP_U64_U32 && (tb == 64 || tb == 512 || tb == 1024)) ? tb : 256;
     b->rb = tables[0]->rb;
     b->ntables = ntables;
     int dev = 0, cus = 0;

@@ -118,6 +118,7 @@ struct batch_ops
     {
         if constexpr (batch_all_blocks<K, M>)
         {
+            if (b->block == 64) return launch_b<64>(b, nsegs, s);
             if (b->block == 512) return launch_b<512>(b, nsegs, s);
             if (b->block == 1024) return launch_b<1024>(b, nsegs, s);
         }
