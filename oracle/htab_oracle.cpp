@@ -40,13 +40,26 @@ namespace oracle
 using index_t = std::int32_t;
 
 // ---------------------------------------------------------------- atomics (OpenMP backend cost model)
+// Every atomic_ref operation of the reference defaults to memory_order_seq_cst, which puts a
+// backend fence (`omp flush`) before and after the backend call (atomic_detail.cuh:34-97 and
+// :436-456, 490-505, 540-600; openmp/impl/atomic_detail.h:33-36).  The fences order nothing that
+// the critical sections do not already order here, but they are part of what the reference's
+// OpenMP path costs, and this restatement is also the timed CPU baseline.
+static inline void
+seq_cst_fence()
+{
+#pragma omp flush
+}
+
 template <typename T>
 static inline T
 a_load(const T* p)
 {
     T v;
+    seq_cst_fence();
 #pragma omp critical
     v = *p;
+    seq_cst_fence();
     return v;
 }
 
@@ -54,8 +67,10 @@ template <typename T>
 static inline void
 a_store(T* p, T v)
 {
+    seq_cst_fence();
 #pragma omp critical
     *p = v;
+    seq_cst_fence();
 }
 
 template <typename T>
@@ -63,11 +78,13 @@ static inline T
 a_fetch_add(T* p, T arg)
 {
     T old;
+    seq_cst_fence();
 #pragma omp atomic capture
     {
         old = *p;
         *p += arg;
     }
+    seq_cst_fence();
     return old;
 }
 
@@ -76,11 +93,13 @@ static inline T
 a_fetch_sub(T* p, T arg)
 {
     T old;
+    seq_cst_fence();
 #pragma omp atomic capture
     {
         old = *p;
         *p -= arg;
     }
+    seq_cst_fence();
     return old;
 }
 
@@ -89,11 +108,13 @@ static inline T
 a_fetch_or(T* p, T arg)
 {
     T old;
+    seq_cst_fence();
 #pragma omp atomic capture
     {
         old = *p;
         *p |= arg;
     }
+    seq_cst_fence();
     return old;
 }
 
@@ -102,11 +123,13 @@ static inline T
 a_fetch_and(T* p, T arg)
 {
     T old;
+    seq_cst_fence();
 #pragma omp atomic capture
     {
         old = *p;
         *p &= arg;
     }
+    seq_cst_fence();
     return old;
 }
 
@@ -116,11 +139,13 @@ a_fetch_inc_mod(unsigned int* p, unsigned int capacity)
 {
     unsigned int old;
     const unsigned int arg = capacity - 1;
+    seq_cst_fence();
 #pragma omp critical
     {
         old = *p;
         *p = (old >= arg) ? 0U : old + 1U;
     }
+    seq_cst_fence();
     return old;
 }
 
@@ -311,7 +336,7 @@ struct traits
     static void make(value_type* dst, K k, const void* vals, std::int64_t i)
     {
         dst->first = k;
-        dst->second = static_cast<const M*>(vals)[i];
+        dst->second = vals ? static_cast<const M*>(vals)[i] : M{};
     }
     static void out(void* vals, std::int64_t i, const value_type& v) { static_cast<M*>(vals)[i] = v.second; }
     static constexpr bool is_map = true;
@@ -345,6 +370,7 @@ struct table_base
     virtual int valid() const = 0;
     virtual void clear() = 0;
     virtual std::int64_t excess_free() const = 0;
+    virtual int corrupt(int what) = 0;
     index_t bucket_count_ = 0;
     index_t total_ = 0;
     virtual index_t bucket(std::uint64_t key) const = 0;
@@ -582,35 +608,120 @@ struct table final : table_base
         return c;
     }
 
-    // valid() invariants (unordered_base_detail.cuh:210-468, 1192-1218)
+    // valid() (unordered_base_detail.cuh:1192-1218): the reference's seven checks in its order,
+    // each stopping the rest (its && chain), so a looping chain never reaches the finds below.
     int valid() const override
     {
-        // offsets stay inside the table
+        if (total_ <= 0) return 1;
+        // offset_range_valid (:211-256): every linked entry i + offsets[i] lies inside the table
         for (index_t i = 0; i < total_; ++i)
-            if (offsets[i] != 0 && (i + offsets[i] < 0 || i + offsets[i] >= total_)) return 0;
-        // every chain from a bucket head is loop-free: each excess slot is visited at most once
-        unsigned char* visits = static_cast<unsigned char*>(std::calloc(static_cast<std::size_t>(total_), 1));
+        {
+            const std::int64_t linked = static_cast<std::int64_t>(i) + offsets[i];
+            if (linked < 0 || linked >= total_) return 0;
+        }
+        // loop_free (:259-351): walk the chain of every bucket, counting visits of each position,
+        // the bucket head included; a walk stops at a position seen twice; all counts must be <= 1
+        int* visits = static_cast<int*>(std::calloc(static_cast<std::size_t>(total_), sizeof(int)));
         int ok = 1;
-        for (index_t b = 0; b < bucket_count_ && ok; ++b)
+        for (index_t b = 0; b < bucket_count_; ++b)
         {
             index_t i = b;
-            index_t steps = 0;
+            ++visits[i];
             while (offsets[i] != 0)
             {
                 i += offsets[i];
-                if (++visits[i] > 1 || ++steps > total_) { ok = 0; break; }
+                if (++visits[i] > 1) break;
             }
         }
+        for (index_t i = 0; i < total_ && ok; ++i)
+            if (visits[i] > 1) ok = 0;
         std::free(visits);
         if (!ok) return 0;
-        // every occupied value is reachable through find() and unique
+        // values_reachable (:353-399): every occupied value is found by contains()
+        for (index_t i = 0; i < total_; ++i)
+            if (occupied.test(i) && !contains(T::key(values[i]))) return 0;
+        // unique (:401-452): find() of every occupied value returns its own position
         for (index_t i = 0; i < total_; ++i)
             if (occupied.test(i) && find_pos(T::key(values[i])) != i) return 0;
-        // occupied count matches the bits; no lock held; excess stack consistent
-        if (occupied.count() != occupied_count) return 0;
+        // occupied_count_valid (:454-468): size() equals the number of set occupancy bits
+        if (occupied.count() != a_load(&occupied_count)) return 0;
+        // _locks.valid() (mutex_detail.cuh:136-152): no lock bit set
         if (locks.count() != 0) return 0;
-        if (excess.occ.count() != excess.size()) return 0;
+        // _excess_list_positions.valid() (vector_detail.cuh:614-636, 776-796): size inside
+        // [0, capacity], size() equal to the occupancy count, no lock bit set
+        if (excess.cap != 0)
+        {
+            const int raw = a_load(&excess.size_);
+            if (raw < 0 || raw > excess.cap) return 0;
+            if (excess.occ.count() != excess.size()) return 0;
+            if (excess.locks.count() != 0) return 0;
+        }
         return 1;
+    }
+
+    // Test-only: breaks one invariant of a filled table so that a test can require valid() == 0
+    // for it.  Returns 1 when the table had the structure the case needs (a chain with an excess
+    // entry, a free excess position), else 0 and leaves the table unchanged.
+    int corrupt(int what) override
+    {
+        // a bucket whose chain holds at least one excess entry: head b, first linked entry e
+        index_t b = -1;
+        for (index_t i = 0; i < bucket_count_; ++i)
+            if (offsets[i] != 0 && occupied.test(i)) { b = i; break; }
+        switch (what)
+        {
+            case 0: // loop: the chain's last entry links back to its bucket head
+            {
+                if (b < 0) return 0;
+                const index_t end = list_end(b);
+                offsets[end] = b - end;
+                return 1;
+            }
+            case 1: // an offset pointing past the last position
+            {
+                if (b < 0) return 0;
+                const index_t end = list_end(b);
+                offsets[end] = total_ - end;
+                return 1;
+            }
+            case 2: // an occupied entry that no chain reaches (size kept equal to the bit count)
+            {
+                if (excess.size() == 0) return 0;
+                const index_t pos = excess.data[excess.size() - 1]; // a free excess position
+                K fresh = static_cast<K>(1);
+                while (contains(fresh) || bucket_of(fresh) == pos) fresh = static_cast<K>(fresh + 1);
+                T::make(&values[pos], fresh, nullptr, -1);
+                occupied.set(pos);
+                a_fetch_add(&occupied_count, 1);
+                return 1;
+            }
+            case 3: // a duplicate: the first excess entry of a chain gets its head's key
+            {
+                if (b < 0) return 0;
+                const index_t e = b + offsets[b];
+                set_key(&values[e], T::key(values[b]));
+                return 1;
+            }
+            case 4: // size() one above the number of occupied entries
+                a_fetch_add(&occupied_count, 1);
+                return 1;
+            case 5: // a lock left held
+                locks.set(total_ / 2);
+                return 1;
+            case 6: // the excess list's size one above its occupancy count
+                if (excess.cap == 0 || excess.full()) return 0;
+                a_fetch_add(&excess.size_, 1);
+                return 1;
+            default: return 0;
+        }
+    }
+
+    static void set_key(value_type* v, K k)
+    {
+        if constexpr (T::is_map)
+            v->first = k;
+        else
+            *v = k;
     }
 
     // unordered_base_detail.cuh:1220-1252
@@ -750,6 +861,7 @@ std::int64_t orc_size(void* t) { return static_cast<table_base*>(t)->size(); }
 std::int64_t orc_contents(void* t, void* keys, void* vals) { return static_cast<table_base*>(t)->contents(keys, vals); }
 int orc_valid(void* t) { return static_cast<table_base*>(t)->valid(); }
 void orc_clear(void* t) { static_cast<table_base*>(t)->clear(); }
+int orc_corrupt(void* t, int what) { return static_cast<table_base*>(t)->corrupt(what); }
 std::int64_t orc_bucket_count(void* t) { return static_cast<table_base*>(t)->bucket_count_; }
 std::int64_t orc_max_size(void* t) { return static_cast<table_base*>(t)->total_; }
 std::int64_t orc_excess_free(void* t) { return static_cast<table_base*>(t)->excess_free(); }

@@ -49,6 +49,7 @@ def load():
         "orc_contents": (I64, [P, P, P]),
         "orc_valid": (I32, [P]),
         "orc_clear": (None, [P]),
+        "orc_corrupt": (I32, [P, I32]),
         "orc_bucket_count": (I64, [P]),
         "orc_max_size": (I64, [P]),
         "orc_excess_free": (I64, [P]),
@@ -154,6 +155,14 @@ class Table:
 
     def clear(self):
         load().orc_clear(self._t)
+
+    # test-only: break one valid() invariant (0 looping chain, 1 offset out of range,
+    # 2 unreachable entry, 3 duplicate key, 4 size above the occupied count, 5 lock held,
+    # 6 excess list size above its occupancy); False when the table lacks the structure needed
+    CORRUPTIONS = ("loop", "offset_out_of_range", "unreachable", "duplicate", "count", "lock", "excess_list")
+
+    def corrupt(self, what: str) -> bool:
+        return bool(load().orc_corrupt(self._t, self.CORRUPTIONS.index(what)))
 
     def bucket_count(self) -> int:
         return load().orc_bucket_count(self._t)

@@ -6,7 +6,8 @@ Each check is chosen so that a plausible slip in the restatement fails it:
   * the reference tests' pinned colliding / non-colliding key pairs
     (tests/stdgpu/unordered_datastructure.inc:899-934, vec_hash unordered_map.inc:181-206);
   * brute-force set semantics against Python sets, single- and multi-threaded;
-  * the reference's valid() invariants after every step;
+  * the reference's valid() invariants after every step, and valid() false for a table with each
+    invariant broken on its own (a test-only corruption hook); clear() against a fresh table;
   * the tiny-table capacity cases (unordered_datastructure.inc:1375-1739);
   * the examples' closed-form sums (examples/openmp/unordered_map.cpp:99-108,
     unordered_set.cpp:82-87);
@@ -175,6 +176,72 @@ def test_set_semantics_bruteforce(kind, threads):
         assert sk.tolist() == sorted(ref)
         if mapped is not None:
             assert sv.tolist() == [ref[k] for k in sorted(ref)]
+
+
+def _filled_with_chains(kind, cap=64, n=60, seed=7):
+    """a small table whose buckets carry chains into the excess area"""
+    key, mapped = kind
+    t = po.Table(key, mapped, cap)
+    rng = np.random.default_rng(seed)
+    keys = np.unique(_rand_keys(rng, key, 4 * n))[:n]
+    assert t.insert(keys, _val(key, keys, mapped)).sum() == n
+    assert t.valid()
+    return t, keys
+
+
+@pytest.mark.parametrize("kind", KINDS)
+@pytest.mark.parametrize("what", po.Table.CORRUPTIONS)
+def test_valid_rejects_each_broken_invariant(kind, what):
+    """unordered_base_detail.cuh:1192-1218: each of the reference's checks, broken on its own
+    (a looping chain, an offset out of range, an unreachable entry, a duplicate key, a size above
+    the occupied count, a held lock, an inconsistent excess list), makes valid() false."""
+    t, _ = _filled_with_chains(kind)
+    assert t.corrupt(what), "table lacks the structure this corruption needs"
+    assert not t.valid()
+
+
+@pytest.mark.parametrize("what", ["unreachable", "duplicate"])
+def test_valid_rejects_corruption_the_earlier_checks_pass(what):
+    """An unreachable entry keeps size == the bit count, and a duplicate keeps every entry
+    reachable: only the reachability and uniqueness checks can reject them.  (Reachability is
+    implied by uniqueness in the reference too: find() of an unreachable key returns end(), whose
+    position differs from the entry's, so no corruption fails one check and passes the other.)"""
+    t, keys = _filled_with_chains(("u64", "u32"))
+    assert t.corrupt(what)
+    k, _ = t.contents()
+    assert len(k) == t.size()  # the occupied count still equals the number of occupied entries
+    if what == "duplicate":
+        h, _ = t.find(k)
+        assert h.all()  # every stored key is still reachable
+    assert not t.valid()
+
+
+@pytest.mark.parametrize("kind", KINDS)
+def test_clear_restores_a_fresh_table(kind):
+    """unordered_base_detail.cuh:1220-1252: clear() empties the table, resets the excess list to
+    all of [bucket_count, max_size), and the table behaves as a fresh one afterwards."""
+    key, mapped = kind
+    t, keys = _filled_with_chains(kind, cap=256, n=200, seed=9)
+    t.erase(keys[::3])
+    assert t.excess_free() < t.max_size() - t.bucket_count()
+    t.clear()
+    assert t.size() == 0 and t.valid()
+    assert len(t.contents()[0]) == 0
+    assert t.excess_free() == t.max_size() - t.bucket_count()
+    h, _ = t.find(keys)
+    assert h.sum() == 0
+    # a later insert equals a fresh table's (identical layout for a sequential insert)
+    fresh = po.Table(key, mapped, 256)
+    rng = np.random.default_rng(10)
+    more = np.unique(_rand_keys(rng, key, 400))[:220]
+    vals = _val(key, more, mapped)
+    assert t.insert(more, vals).tolist() == fresh.insert(more, vals).tolist()
+    a, av = t.sorted_contents()
+    b, bv = fresh.sorted_contents()
+    np.testing.assert_array_equal(a, b)
+    if mapped is not None:
+        np.testing.assert_array_equal(av, bv)
+    assert t.size() == fresh.size() and t.excess_free() == fresh.excess_free() and t.valid()
 
 
 def test_tiny_table_capacity_cases():
