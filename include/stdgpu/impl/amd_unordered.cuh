@@ -121,6 +121,18 @@ public:
         out._allocator = allocator;
     }
 
+    // takes over a table made through the C ABI (for the tunings createDeviceObject has no parameter
+    // for); the kind must match Key and Mapped, and destroyDeviceObject frees it
+    static void adopt(amd_unordered_base& out, htab* h, const Allocator& allocator)
+    {
+        htab_info info{};
+        check(htab_get_info(h, &info), "adopt");
+        if (info.kind != htab_kind<Key, Mapped>()) throw std::invalid_argument("stdgpu: adopt: table of another kind");
+        out._h = h;
+        check(htab_get_view(h, &out._view), "adopt");
+        out._allocator = allocator;
+    }
+
     static void destroy(amd_unordered_base& obj)
     {
         if (obj._h) check(htab_destroy(obj._h), "destroyDeviceObject");
@@ -200,22 +212,28 @@ public:
         return static_cast<index_type>(htab_dev::reference_bucket(hasher{}(key), _view.ref_bucket_count));
     }
 
-    // bucket_size(n): entries in the physical buckets that map onto reference bucket n, so that
-    // the sizes over [0, bucket_count()) add up to size() as in the reference
+    // bucket_size(n) (unordered_base_detail.cuh:566-593): the number of stored keys k with
+    // bucket(k) == n, which is what the reference counts along bucket n's chain.  This layout places
+    // keys by their bits, not by bucket(), so the call reads every slot: O(max_size()) work per call.
+    // All lanes of a wavefront read the same slots in the same order, so a kernel that asks for many
+    // buckets at once (as the reference's bucket_size_sum case does) shares each load.
     __device__ index_type bucket_size(index_type n) const
     {
         const engine e = eng();
-        const std::uint64_t bc = static_cast<std::uint64_t>(_view.ref_bucket_count);
-        const std::uint64_t nb = e.nbuckets;
-        std::uint64_t lo = (static_cast<std::uint64_t>(n) * nb + bc - 1) / bc;
-        std::uint64_t hi = ((static_cast<std::uint64_t>(n) + 1) * nb + bc - 1) / bc;
+        const std::uint64_t ns = e.nslots();
+        const skey* keys = reinterpret_cast<const skey*>(e.slots);
+        constexpr std::uint64_t step = sizeof(slot_type) / sizeof(skey);
         index_type c = 0;
-        for (std::uint64_t b = lo; b < hi && b < nb; ++b) c += e.physical_bucket_size(b);
-        if (n == 0)
+        for (std::uint64_t s = 0; s < ns; ++s)
         {
-            const std::uint32_t f = htab_dev::load_agent(e.flags);
-            c += static_cast<index_type>((f & 1U) + ((f >> 1) & 1U));
+            const skey k = keys[s * step];
+            if (k == htab_dev::empty_key<skey>() || k == htab_dev::tomb_key<skey>()) continue;
+            c += bucket(from_skey(k)) == n ? 1 : 0;
         }
+        // the two reserved key values live in side slots with a presence bit each
+        const std::uint32_t f = htab_dev::load_agent(e.flags);
+        if ((f & 1U) && bucket(from_skey(htab_dev::empty_key<skey>())) == n) ++c;
+        if ((f & 2U) && bucket(from_skey(htab_dev::tomb_key<skey>())) == n) ++c;
         return c;
     }
 
@@ -396,6 +414,15 @@ protected:
             return static_cast<skey>(static_cast<std::make_unsigned_t<Key>>(k));
         else
             return __builtin_bit_cast(skey, k);
+    }
+
+    // a key from its stored bits (the inverse of to_skey)
+    __host__ __device__ static Key from_skey(const skey k)
+    {
+        if constexpr (std::is_integral_v<Key>)
+            return static_cast<Key>(static_cast<std::make_unsigned_t<Key>>(k));
+        else
+            return __builtin_bit_cast(Key, k);
     }
 
     htab* _h = nullptr;

@@ -40,6 +40,15 @@ public:
         return m;
     }
 
+    // not in the reference: a facade object over a table made with htab_create (own tuning);
+    // destroyDeviceObject frees it
+    [[nodiscard]] static unordered_map adopt(htab* table, const Allocator& allocator = Allocator())
+    {
+        unordered_map m;
+        base::adopt(m, table, allocator);
+        return m;
+    }
+
     static void destroyDeviceObject(unordered_map& device_object) { base::destroy(device_object); }
 
     template <typename ExecutionPolicy, std::enable_if_t<is_execution_policy_v<ExecutionPolicy>, int> = 0>

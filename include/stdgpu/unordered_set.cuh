@@ -36,6 +36,15 @@ public:
         return s;
     }
 
+    // not in the reference: a facade object over a table made with htab_create (own tuning);
+    // destroyDeviceObject frees it
+    [[nodiscard]] static unordered_set adopt(htab* table, const Allocator& allocator = Allocator())
+    {
+        unordered_set m;
+        base::adopt(m, table, allocator);
+        return m;
+    }
+
     static void destroyDeviceObject(unordered_set& device_object) { base::destroy(device_object); }
 
     template <typename ExecutionPolicy, std::enable_if_t<is_execution_policy_v<ExecutionPolicy>, int> = 0>

@@ -198,9 +198,9 @@ int htab_graph_destroy(htab_graph* graph);
  * While requests arrive, a scheduler keeps up to two launches per stream in flight; each launch
  * takes the longest same-operation run at the head of every table's queue.  A launch waits on the
  * device for the previous launch of each of its tables, so per table the requests take effect in
- * arrival order while launches run back to back.  An insert that might not fit the table's free
- * slots waits for the table's launches in flight and is refused (status HTAB_E_LIMIT, no effect)
- * when it still cannot be guaranteed to fit.
+ * arrival order while launches run back to back.  An insert that may reach the table's limit waits
+ * until the table has no launch in flight and then runs with no other request of that table: it
+ * inserts what a plain call would (keys past the limit are not inserted); no request is refused.
  * ------------------------------------------------------------------------------------------ */
 typedef struct htab_server htab_server;
 
@@ -219,8 +219,8 @@ int htab_server_create(htab* const* tables, int32_t ntables, int32_t nstreams, i
                        htab_server** out);
 int htab_server_destroy(htab_server* server);
 /* Open-loop replay of a request log: request i is submitted at arrival_ns[i] after the start.
- * Returns per request its submit and completion times (ns since the start) and status (0 or
- * HTAB_E_LIMIT when refused).  Blocks until every request has completed. */
+ * Returns per request its submit and completion times (ns since the start) and status (0 once
+ * the request has taken effect).  Blocks until every request has completed. */
 int htab_server_run(htab_server* server, const htab_segment* requests, const int64_t* arrival_ns, int64_t nreq,
                     int64_t* submit_ns, int64_t* done_ns, int32_t* status, htab_server_stats* stats);
 /* The same requests as plain calls, one per request in log order on `stream` (skipping requests whose

@@ -216,7 +216,13 @@ struct table_ref
             return ovf + b;
     }
 
-    __device__ uint32_t overflow(uint64_t b) const { return load_agent(ovf_ptr(b)); }
+    // The top bit of a bucket's ovf word is the lock that native inserts of keys homed there hold
+    // while they claim a slot (see insert()); it is clear whenever no native call runs, so the bulk
+    // kernels, valid() and snapshots only ever see counts.
+    static constexpr uint32_t kOvfLock = 0x80000000U;
+    static constexpr uint32_t kOvfCount = 0x7FFFFFFFU;
+
+    __device__ uint32_t overflow(uint64_t b) const { return load_agent(ovf_ptr(b)) & kOvfCount; }
 
     // adds d to ovf of the `count` buckets starting at h (cyclic)
     __device__ void overflow_add(uint64_t h, uint64_t count, int32_t d) const
@@ -303,12 +309,38 @@ struct table_ref
         return true;
     }
 
-    // insert (unordered_base_detail.cuh:708-809, 967-994): never overwrites; claims only EMPTY
-    // slots, so it stays duplicate-free when it runs concurrently with erase() in one kernel.
-    // Every bucket it walks past without an EMPTY slot gets its ovf raised before the key can
-    // appear further on (and lowered again if the key is not inserted), so a concurrent find()
-    // that sees the key's slot also sees the way there.
-    // Returns {slot pointer, true} on insertion, {end_ptr(), false} otherwise.
+    // Waits until every memory operation this wavefront issued has completed.  The table's words
+    // change only through agent-scope atomics, which are performed coherently for the whole device,
+    // so once they have completed every later coherent load on any CU sees them; this orders a lock
+    // release after the claim it protects without the L2 write-back of an agent-scope release fence.
+    __device__ __forceinline__ static void settle()
+    {
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        __builtin_amdgcn_s_waitcnt(0);
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    }
+
+    // insert (unordered_base_detail.cuh:708-809, 967-994): never overwrites; reuses erased (TOMB)
+    // slots as well as EMPTY ones, so emplace / erase churn from user kernels never wears a table
+    // out.  Returns the slot and *inserted = true on insertion, end_ptr() and false when the key is
+    // present or the table is at its limit (try_insert reports a present key with end(),
+    // unordered_base_detail.cuh:714, 805).
+    //
+    // Duplicate-freedom.  A present key is returned by a lock-free find first.  Otherwise the call
+    // takes the lock bit of the key's home bucket (the top bit of that bucket's ovf word), so at
+    // most one insert of a key runs its claim at a time; erase and find never wait for it.  Under
+    // the lock it scans the probe sequence to its end (a bucket that holds an EMPTY slot or has
+    // ovf == 0), so a copy of the key anywhere on it is seen, whichever call put it there (native,
+    // or a bulk insert before a bulk erase emptied slots in front of it), and claims the first free
+    // slot of that scan by CAS, or the next free slot on after a CAS lost to another key.  Before
+    // the claim, ovf is raised on every bucket the key runs through, so a find() that sees the key
+    // also sees the way there.  EMPTY never appears during a kernel that runs native calls (native
+    // erase writes TOMB), so a bucket walked past holds no EMPTY slot and keeps invariant (E).
+    //
+    // Forward progress.  The lock is tried in a loop whose exit is uniform over the wavefront (a
+    // ballot of the lanes still waiting), and a lane that gets the lock finishes its whole claim
+    // and releases it in the same iteration, so lanes of one wavefront waiting on each other's
+    // lock always make progress.
     template <typename WriteValue>
     __device__ value_type* insert(K key, const WriteValue& write_value, bool* inserted) const
     {
@@ -329,48 +361,155 @@ struct table_ref
             *inserted = true;
             return p;
         }
+        if (find(key) != end_ptr()) return end_ptr();
         const uint64_t h = home_bucket(static_cast<uint64_t>(key), nbuckets);
-        const uint32_t o = scan_start(key);
-        uint64_t b = h;
-        for (uint64_t step = 0; step < nbuckets; ++step)
+        uint32_t* lock = ovf_ptr(h);
+        const int lane = static_cast<int>(__builtin_amdgcn_mbcnt_hi(~0U, __builtin_amdgcn_mbcnt_lo(~0U, 0U)));
+        value_type* result = end_ptr();
+        bool waiting = true, first = true;
+        while (__builtin_amdgcn_ballot_w64(waiting) != 0)
         {
+            // A waiting call that can be decided without the lock leaves: the table reached its
+            // limit, or another call inserted the key meanwhile (many calls of one key, or many keys
+            // into a full tiny table, would otherwise take the lock one after another only to find
+            // that out).  Otherwise it tries the lock when it looks free.
+            bool ready = false;
+            if (waiting)
+            {
+                if (!first && ((nstripes == 1 && load_agent(stripes) >= static_cast<int64_t>(limit)) ||
+                               find(key) != end_ptr()))
+                    waiting = false;
+                else
+                    ready = !(load_agent(lock) & kOvfLock);
+            }
+            first = false;
+            // of the lanes of this wavefront that try one lock word, only the lowest does
+            const uint64_t cand = __builtin_amdgcn_ballot_w64(ready);
+            const int leader = cand ? __builtin_ctzll(cand) : 0;
+            const uint64_t la = static_cast<uint64_t>(reinterpret_cast<uintptr_t>(lock));
+            const uint64_t leader_lock =
+                    static_cast<uint64_t>(static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<uint32_t>(la), leader))) |
+                    (static_cast<uint64_t>(static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<uint32_t>(la >> 32), leader)))
+                     << 32);
+            bool got = false;
+            if (ready && (lane == leader || la != leader_lock))
+            {
+                const uint32_t old = __hip_atomic_fetch_or(lock, kOvfLock, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                got = !(old & kOvfLock);
+                if (got)
+                {
+                    result = claim_locked(key, h, write_value, inserted);
+                    settle();
+                    __hip_atomic_fetch_and(lock, ~kOvfLock, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    waiting = false;
+                }
+            }
+            // no lane of the wavefront got its lock: back off before looking again
+            if (__builtin_amdgcn_ballot_w64(got) == 0) __builtin_amdgcn_s_sleep(4);
+        }
+        return result;
+    }
+
+    // the body of insert() under the home-bucket lock
+    template <typename WriteValue>
+    __device__ value_type* claim_locked(K key, uint64_t h, const WriteValue& write_value, bool* inserted) const
+    {
+        constexpr uint64_t kNone = ~uint64_t(0);
+        if (nstripes == 1 && load_agent(stripes) >= static_cast<int64_t>(limit)) return end_ptr();
+        const uint32_t o = scan_start(key);
+        // 1. scan to the end of the probe sequence: the key, and the first free slot in scan order
+        uint64_t b = h, step = 0;
+        uint64_t fslot = kNone, fstep = 0;
+        uint32_t fj = 0;
+        K fseen = empty_key<K>();
+        for (; step < nbuckets; ++step)
+        {
+            bool saw_empty = false;
             for (uint32_t j = 0; j < W; ++j)
             {
                 const uint64_t s = b * W + ((o + j) & (W - 1));
-                K k = load_agent(key_ptr(s));
-                if (k == key)
+                const K k = load_agent(key_ptr(s));
+                if (k == key) return end_ptr();
+                if (fslot == kNone && (k == empty_key<K>() || k == tomb_key<K>()))
                 {
-                    overflow_add(h, step, -1);
-                    return end_ptr();
+                    fslot = s;
+                    fstep = step;
+                    fj = j;
+                    fseen = k;
                 }
-                if (k == empty_key<K>())
-                {
-                    const K seen = cas_agent(key_ptr(s), empty_key<K>(), key);
-                    if (seen == empty_key<K>())
-                    {
-                        if (!count_add(1))
-                        {
-                            __hip_atomic_store(key_ptr(s), tomb_key<K>(), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                            overflow_add(h, step, -1);
-                            return end_ptr();
-                        }
-                        write_value(slots + s);
-                        *inserted = true;
-                        return slots + s;
-                    }
-                    if (seen == key)
-                    {
-                        overflow_add(h, step, -1);
-                        return end_ptr();
-                    }
-                    // lost the slot to another key: keep scanning from the next slot
-                }
+                saw_empty |= (k == empty_key<K>());
             }
-            // no EMPTY slot left here: the key's probe sequence runs through this bucket
-            __hip_atomic_fetch_add(ovf_ptr(b), 1U, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (saw_empty || overflow(b) == 0) break;
             b = next_bucket(b, nbuckets);
         }
-        overflow_add(h, nbuckets, -1);
+        // 2. absent: claim the first free slot from the scan's first one on (the next bucket on
+        //    when the scan saw none), raising ovf over the buckets before the one claimed in
+        uint64_t raised = 0; // ovf raised on buckets h .. h + raised - 1
+        if (fslot == kNone)
+        {
+            if (step >= nbuckets) return end_ptr(); // every bucket scanned, none free
+            fstep = step + 1;
+            fj = 0;
+            fseen = empty_key<K>();
+        }
+        uint64_t cstep = fstep; // bucket (steps from h) and order position of the next try
+        uint32_t cj = fj;
+        bool have = fslot != kNone;
+        while (cstep < nbuckets)
+        {
+            const uint64_t cb = h + cstep >= nbuckets ? h + cstep - nbuckets : h + cstep;
+            if (!have)
+            {
+                // look for the first free slot of bucket cb from order position cj on
+                for (; cj < W; ++cj)
+                {
+                    const uint64_t s = cb * W + ((o + cj) & (W - 1));
+                    const K k = load_agent(key_ptr(s));
+                    if (k == empty_key<K>() || k == tomb_key<K>())
+                    {
+                        fslot = s;
+                        fseen = k;
+                        have = true;
+                        break;
+                    }
+                }
+                if (!have)
+                {
+                    ++cstep;
+                    cj = 0;
+                    continue;
+                }
+            }
+            if (raised < cstep)
+            {
+                for (; raised < cstep; ++raised)
+                {
+                    const uint64_t rb = h + raised >= nbuckets ? h + raised - nbuckets : h + raised;
+                    __hip_atomic_fetch_add(ovf_ptr(rb), 1U, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                }
+                // the raises complete before the key can be seen: an erase of the key lowers these
+                // counts only after it has seen the key, so no count ever dips below zero (into the
+                // lock bit)
+                settle();
+            }
+            const K seen = cas_agent(key_ptr(fslot), fseen, key);
+            if (seen == fseen)
+            {
+                if (!count_add(1))
+                {
+                    __hip_atomic_store(key_ptr(fslot), tomb_key<K>(), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    overflow_add(h, raised, -1);
+                    return end_ptr();
+                }
+                write_value(slots + fslot);
+                *inserted = true;
+                return slots + fslot;
+            }
+            // lost to another key (only keys of other home buckets claim here meanwhile): go on
+            have = false;
+            ++cj;
+        }
+        overflow_add(h, raised, -1);
         return end_ptr();
     }
 
@@ -409,18 +548,6 @@ struct table_ref
             b = next_bucket(b, nbuckets);
         }
         return false;
-    }
-
-    // bucket_size(n) analogue for this layout: stored entries in physical bucket n.
-    __device__ int32_t physical_bucket_size(uint64_t n) const
-    {
-        int32_t c = 0;
-        for (uint32_t j = 0; j < W; ++j)
-        {
-            const K k = load_agent(key_ptr(n * W + j));
-            c += (k != empty_key<K>() && k != tomb_key<K>()) ? 1 : 0;
-        }
-        return c;
     }
 };
 

@@ -11,6 +11,7 @@
 #include <new>
 #include <string>
 #include <type_traits>
+#include <vector>
 
 #include "htab_internal.hpp"
 
@@ -664,11 +665,19 @@ htab_batch_run_segments(htab_batch* b, const htab_segment* segs, int32_t nsegs, 
 {
     if (!b || (nsegs > 0 && !segs) || nsegs < 0) return fail(HTAB_E_ARG, "htab_batch_run_segments: bad argument");
     if (nsegs > b->seg_cap) return fail(HTAB_E_LIMIT, "htab_batch_run_segments: more segments than capacity");
+    // the kernels take each table's work in a launch to be one operation (an insert relies on no
+    // slot being freed meanwhile, an erase may free slots outright), so segments of one table that
+    // name different operations are refused before any device work
+    std::vector<int8_t> table_op(static_cast<size_t>(b->ntables), static_cast<int8_t>(-1));
     for (int32_t g = 0; g < nsegs; ++g)
     {
         if (segs[g].table < 0 || segs[g].table >= b->ntables || segs[g].n < 0 || segs[g].op < HTAB_OP_INSERT ||
             segs[g].op > HTAB_OP_ERASE || (segs[g].n > 0 && !segs[g].keys) || !segment_aligned(b->kind, segs[g]))
             return fail(HTAB_E_ARG, "htab_batch_run_segments: bad segment");
+        int8_t& o = table_op[static_cast<size_t>(segs[g].table)];
+        if (o >= 0 && o != static_cast<int8_t>(segs[g].op))
+            return fail(HTAB_E_ARG, "htab_batch_run_segments: segments of one table name different operations");
+        o = static_cast<int8_t>(segs[g].op);
     }
     if (nsegs == 0) return HTAB_OK;
     hipStream_t s = static_cast<hipStream_t>(stream);
@@ -687,6 +696,7 @@ htab_batch_run_per_table(htab_batch* b, int op, const int64_t* off, const void* 
                          void* vals_out, void* stream)
 {
     if (!b || !off || !keys) return fail(HTAB_E_ARG, "htab_batch_run_per_table: null argument");
+    if (op == HTAB_OP_FIND && !flags) return fail(HTAB_E_ARG, "htab_batch_run_per_table: find needs flags (hits)");
     const int kb = key_bytes_of(b->kind);
     const int vb = mapped_bytes_of(b->kind);
     for (int32_t t = 0; t < b->ntables; ++t)

@@ -14,10 +14,11 @@
 // then one single-workgroup kernel that stores the sizes of the launch's tables into host-mapped
 // memory, and an event record: no copies and no planning kernel.
 //
-// Admission: an insert is taken only if the table's known size plus the inserts in flight plus its
-// n fits the table's limit.  If it does not fit while launches on the table are in flight, the
-// table waits for them (its size is then exact); if it does not fit with nothing in flight it is
-// refused with HTAB_E_LIMIT and has no effect.
+// Admission: inserts are coalesced while the table's known size plus the inserts in flight plus
+// their n stay within the table's limit.  An insert that may reach the limit (n counts every key,
+// also those already present) waits until the table has no launch in flight and then runs with no
+// other request of that table; its kernel ranks every new key against the exact limit, so it gets
+// what a plain call would (keys past the limit are not inserted) and no request is ever refused.
 //
 // No reference counterpart: the reference serves one blocking bulk call at a time.
 #include <hip/hip_runtime.h>
@@ -246,18 +247,18 @@ htab_server_run(htab_server* s, const htab_segment* reqs, const int64_t* arrival
                         const int64_t r = q.front();
                         const int64_t n = reqs[r].n;
                         if (took && elems + n > s->max_launch_elems) break;
+                        bool alone = false;
                         if (op == HTAB_OP_INSERT &&
                             known[static_cast<size_t>(t)] + inflight_ins[static_cast<size_t>(t)] + n >
                                     s->limit[static_cast<size_t>(t)])
                         {
-                            // wait for the table's launches in flight (its size is exact afterwards) ...
+                            // the request may reach the table's limit (n counts keys that may be
+                            // present already): wait for the table's launches in flight, then run it
+                            // with no other request of the table, so earlier requests take their
+                            // slots first, as in the in-order plain calls; the insert kernel ranks
+                            // each key against the exact limit
                             if (took || table_launches[static_cast<size_t>(t)] > 0) break;
-                            // ... or refuse: cannot fit, no effect on the table
-                            q.pop_front();
-                            status[r] = HTAB_E_LIMIT;
-                            done_ns[r] = now_ns(t0);
-                            ++finished;
-                            continue;
+                            alone = true;
                         }
                         q.pop_front();
                         if (!took) run_first[static_cast<size_t>(t)] = r;
@@ -266,6 +267,7 @@ htab_server_run(htab_server* s, const htab_segment* reqs, const int64_t* arrival
                         if (op == HTAB_OP_INSERT) inflight_ins[static_cast<size_t>(t)] += n;
                         elems += n;
                         took = true;
+                        if (alone) break;
                     }
                     if (took) l.tables.push_back(t);
                     if (elems >= s->max_launch_elems || static_cast<int32_t>(segs.size()) >= kMaxLaunchSegs) break;

@@ -154,9 +154,11 @@ def same_contents(a, b) -> bool:
     return bool(torch.equal(ka, kb) and torch.equal(va, vb))
 
 
-def verify_against_replay(caps, pools: Pools, log: RequestLog, served_tables, status, flags, vals) -> dict:
+def verify_against_replay(caps, pools: Pools, log: RequestLog, served_tables, status, flags, vals,
+                          only_tables=None) -> dict:
     """Replays the log in order through plain calls on fresh tables and compares every find's
-    hit vector and values and every table's final contents."""
+    hit vector and values and every table's final contents (of the tables in only_tables, when
+    given)."""
     import torch
 
     fresh = make_tables(caps)
@@ -171,12 +173,13 @@ def verify_against_replay(caps, pools: Pools, log: RequestLog, served_tables, st
     ok = status == 0
     for i in range(len(log.n)):
         n = int(log.n[i])
-        if ok[i] and OPS[int(log.op[i])] == _lib.HTAB_OP_FIND:
+        if ok[i] and OPS[int(log.op[i])] == _lib.HTAB_OP_FIND and (only_tables is None or int(log.table[i]) in only_tables):
             a, b = flags[off: off + n], rflags[off: off + n]
             if not torch.equal(a, b) or not torch.equal(vals[off: off + n][a == 1], rvals[off: off + n][b == 1]):
                 finds_ok = False
         off += n
-    contents_ok = all(same_contents(served_tables[t], fresh[t]) for t in range(len(caps)))
+    check_tables = range(len(caps)) if only_tables is None else only_tables
+    contents_ok = all(same_contents(served_tables[t], fresh[t]) for t in check_tables)
     for t in fresh:
         C.unordered_map_u64_u32.destroyDeviceObject(t)
     return {"finds_identical": finds_ok, "contents_identical": contents_ok}

@@ -1168,6 +1168,212 @@ hash_and_equal_to_of_integers()
     CHECK(eq(3, 3) && !eq(3, 4) && teq(3, 3L) && !teq(3U, 4ULL));
 }
 
+// ------------------------------------------------------------------ native churn, bulk / native mixing
+template <typename Table, typename Key>
+__global__ void
+k_mixed_emplace_erase(Table t, const Key* ins, const Key* del, std::size_t n, int* ok)
+{
+    // lane i of an even-numbered pair emplaces ins[i / 2], of an odd one erases del[i / 2], in one kernel
+    const std::size_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 2 * n) return;
+    if (i % 2 == 0)
+        ok[i] = t.emplace(ins[i / 2], static_cast<typename Table::mapped_type>(ins[i / 2] * 3 + 1)).second ? 1 : 0;
+    else
+        ok[i] = static_cast<int>(t.erase(del[i / 2]));
+}
+
+static std::vector<std::uint64_t>
+mixed_keys(std::size_t n, std::uint64_t salt)
+{
+    // distinct keys: a bijective 64-bit mix of salt * 2^32 + i
+    std::vector<std::uint64_t> out(n);
+    for (std::size_t i = 0; i < n; ++i)
+    {
+        std::uint64_t z = (salt << 32) + i + 0x9E3779B97F4A7C15ULL;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+        out[i] = (z ^ (z >> 31)) >> 2; // clear of the reserved key values
+    }
+    return out;
+}
+
+// the table equals the model: size, valid(), every model key found with its value, erased keys absent
+static bool
+same_as_model(map_t& m, const std::map<std::uint64_t, std::uint32_t>& model, const std::vector<std::uint64_t>& gone)
+{
+    if (m.size() != static_cast<stdgpu::index_t>(model.size()) || !m.valid()) return false;
+    std::vector<std::uint64_t> keys;
+    for (const auto& kv : model) keys.push_back(kv.first);
+    std::vector<std::uint64_t> vals;
+    const std::vector<int> f = run_find(m, keys, &vals);
+    std::size_t i = 0;
+    for (const auto& kv : model)
+    {
+        if (f[i] != 1 || vals[i] != kv.second) return false;
+        ++i;
+    }
+    if (!gone.empty() && sum(run_find(m, gone)) != 0) return false;
+    return true;
+}
+
+static void
+native_churn_reuses_freed_slots()
+{
+    g_case = "native emplace / erase churn of 4x the physical slots without clear()";
+    const stdgpu::index_t cap = 1 << 16;
+    map_t m = map_t::createDeviceObject(cap);
+    htab_info info{};
+    CHECK(htab_get_info(m.native_handle(), &info) == HTAB_OK);
+    const std::size_t slots = static_cast<std::size_t>(info.physical_slots);
+    const std::size_t live = slots / 2, batch = slots / 8;
+    const std::vector<std::uint64_t> pool = mixed_keys(live + 4 * slots + batch, 11);
+    std::map<std::uint64_t, std::uint32_t> model;
+    std::size_t next = 0, oldest = 0;
+    auto value_of = [](std::uint64_t k) { return static_cast<std::uint32_t>(k * 3 + 1); };
+    // fill to half the slots
+    {
+        std::vector<std::uint64_t> first(pool.begin(), pool.begin() + live);
+        CHECK(sum(run_insert(m, first, true)) == static_cast<int>(live));
+        for (auto k : first) model[k] = value_of(k);
+        next = live;
+    }
+    hipEvent_t e0, e1;
+    HIP_OK(hipEventCreate(&e0));
+    HIP_OK(hipEventCreate(&e1));
+    std::vector<float> round_ms;
+    std::size_t inserted_total = 0;
+    int round = 0;
+    while (inserted_total < 4 * slots)
+    {
+        // one kernel: emplace `batch` fresh keys in even lanes, erase the `batch` oldest in odd lanes
+        std::vector<std::uint64_t> ins(pool.begin() + next, pool.begin() + next + batch);
+        std::vector<std::uint64_t> del(pool.begin() + oldest, pool.begin() + oldest + batch);
+        std::uint64_t* di = to_dev(ins);
+        std::uint64_t* dd = to_dev(del);
+        int* ok = dev_alloc<int>(2 * batch);
+        HIP_OK(hipEventRecord(e0, nullptr));
+        hipLaunchKernelGGL((k_mixed_emplace_erase<map_t, std::uint64_t>), dim3(blocks(2 * batch)), dim3(256), 0, 0, m, di,
+                           dd, batch, ok);
+        HIP_OK(hipEventRecord(e1, nullptr));
+        HIP_OK(hipEventSynchronize(e1));
+        float ms = 0;
+        HIP_OK(hipEventElapsedTime(&ms, e0, e1));
+        round_ms.push_back(ms);
+        const std::vector<int> f = to_host(ok, 2 * batch);
+        (void)hipFree(di);
+        (void)hipFree(dd);
+        (void)hipFree(ok);
+        CHECK(sum(f) == static_cast<int>(2 * batch)); // every fresh key inserted, every old key erased
+        for (auto k : ins) model[k] = value_of(k);
+        for (auto k : del) model.erase(k);
+        next += batch;
+        oldest += batch;
+        inserted_total += batch;
+        // present keys are refused, absent ones are not erased
+        std::vector<std::uint64_t> again(pool.begin() + oldest, pool.begin() + oldest + batch / 8);
+        CHECK(sum(run_insert(m, again, true)) == 0);
+        std::vector<std::uint64_t> gone(pool.begin() + oldest - batch, pool.begin() + oldest - batch + batch / 8);
+        CHECK(sum(run_erase(m, gone)) == 0);
+        if (round % 4 == 0 || inserted_total >= 4 * slots) CHECK(same_as_model(m, model, gone));
+        ++round;
+    }
+    CHECK(same_as_model(m, model, std::vector<std::uint64_t>(pool.begin(), pool.begin() + oldest)));
+    // flat: the last quarter of the rounds is not slower than twice the first quarter
+    const std::size_t q = round_ms.size() / 4;
+    double a = 0, b = 0;
+    for (std::size_t i = 1; i <= q; ++i) a += round_ms[i];
+    for (std::size_t i = round_ms.size() - q; i < round_ms.size(); ++i) b += round_ms[i];
+    std::printf("rate: churn rounds %zu (%zu inserts into %zu slots), first quarter %.3f ms, last quarter %.3f ms per round\n",
+                round_ms.size(), inserted_total, slots, a / q, b / q);
+    CHECK(b <= 2.0 * a + 0.05 * q);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    map_t::destroyDeviceObject(m);
+}
+
+static void
+native_insert_after_bulk_erase()
+{
+    g_case = "native insert after a bulk erase (duplicate-free, every kind of freed slot)";
+    const std::size_t N = 1 << 16;
+    const stdgpu::index_t cap = static_cast<stdgpu::index_t>(std::ceil(static_cast<double>(N) / 0.8));
+    for (int lanes : { 1, 4, 8 })
+    {
+        htab_tuning tun{};
+        tun.bucket_lanes = lanes;
+        tun.block_size = 64;
+        tun.slots_per_capacity = 1.0F;
+        htab* h = nullptr;
+        CHECK(htab_create(HTAB_MAP_U64_U32, cap, &tun, &h) == HTAB_OK);
+        const std::vector<std::uint64_t> k = mixed_keys(N, 23);
+        std::vector<map_t::value_type> hv;
+        for (auto x : k) hv.emplace_back(x, static_cast<std::uint32_t>(x * 3 + 1));
+        map_t::value_type* dv = dev_alloc<map_t::value_type>(N);
+        HIP_OK(hipMemcpy(dv, hv.data(), N * sizeof(map_t::value_type), hipMemcpyHostToDevice));
+        CHECK(htab_insert(h, dv, static_cast<std::int64_t>(N), nullptr, nullptr) == HTAB_OK);
+        std::vector<std::uint64_t> half;
+        for (std::size_t i = 0; i < N; i += 2) half.push_back(k[i]);
+        std::uint64_t* dh = to_dev(half);
+        CHECK(htab_erase(h, dh, static_cast<std::int64_t>(half.size()), nullptr, nullptr) == HTAB_OK);
+        HIP_OK(hipDeviceSynchronize());
+        (void)hipFree(dh);
+        (void)hipFree(dv);
+        map_t m = map_t::adopt(h);
+        // every key once per thread, plus each key by three more threads of other wavefronts
+        std::vector<std::uint64_t> all = k;
+        for (int r = 0; r < 3; ++r) all.insert(all.end(), k.begin(), k.end());
+        const std::vector<int> ok = run_insert(m, all, true);
+        std::map<std::uint64_t, int> wins;
+        for (std::size_t i = 0; i < all.size(); ++i) wins[all[i]] += ok[i];
+        for (std::size_t i = 0; i < N; ++i) CHECK(wins[k[i]] == (i % 2 == 0 ? 1 : 0));
+        std::map<std::uint64_t, std::uint32_t> model;
+        for (auto x : k) model[x] = static_cast<std::uint32_t>(x * 3 + 1);
+        CHECK(same_as_model(m, model, {}));
+        map_t::destroyDeviceObject(m);
+    }
+}
+
+static void
+bucket_size_is_the_reference_count()
+{
+    g_case = "bucket_size(n) equals the number of stored keys with bucket(k) == n";
+    // the reference fixture's table (capacity 100000) holding 10^4 keys, the two reserved values included
+    map_t m = map_t::createDeviceObject(100000);
+    std::vector<std::uint64_t> k = unique_keys(10000 - 2, 41);
+    k.push_back(~std::uint64_t(0));
+    k.push_back(~std::uint64_t(0) - 1);
+    CHECK(sum(run_insert(m, k, true)) == 10000);
+    std::vector<int> want(m.bucket_count(), 0);
+    for (auto x : k) ++want[m.bucket(x)];
+    int* sizes = dev_alloc<int>(m.bucket_count());
+    hipLaunchKernelGGL((k_bucket_sizes<map_t>), dim3(blocks(m.bucket_count())), dim3(256), 0, 0, m, sizes);
+    const std::vector<int> got = to_host(sizes, m.bucket_count());
+    (void)hipFree(sizes);
+    CHECK(got == want);
+    // after erasing a third of the keys
+    std::vector<std::uint64_t> gone;
+    for (std::size_t i = 0; i < k.size(); i += 3) gone.push_back(k[i]);
+    CHECK(sum(run_erase(m, gone)) == static_cast<int>(gone.size()));
+    for (auto x : gone) --want[m.bucket(x)];
+    sizes = dev_alloc<int>(m.bucket_count());
+    hipLaunchKernelGGL((k_bucket_sizes<map_t>), dim3(blocks(m.bucket_count())), dim3(256), 0, 0, m, sizes);
+    CHECK(to_host(sizes, m.bucket_count()) == want);
+    (void)hipFree(sizes);
+    map_t::destroyDeviceObject(m);
+    // signed keys (sign-extended identity hash) in a set
+    iset_t s = iset_t::createDeviceObject(20000);
+    std::vector<int> ik;
+    for (int i = -5000; i < 5000; ++i) ik.push_back(i * 7919);
+    CHECK(sum(run_insert(s, ik)) == 10000);
+    std::vector<int> iw(s.bucket_count(), 0);
+    for (auto x : ik) ++iw[s.bucket(x)];
+    sizes = dev_alloc<int>(s.bucket_count());
+    hipLaunchKernelGGL((k_bucket_sizes<iset_t>), dim3(blocks(s.bucket_count())), dim3(256), 0, 0, s, sizes);
+    CHECK(to_host(sizes, s.bucket_count()) == iw);
+    (void)hipFree(sizes);
+    iset_t::destroyDeviceObject(s);
+}
+
 int
 main()
 {
@@ -1194,7 +1400,10 @@ main()
                           bulk_find_and_contains,
                           reference_fixture_cases,
                           native_rates,
-                          hash_and_equal_to_of_integers };
+                          hash_and_equal_to_of_integers,
+                          native_churn_reuses_freed_slots,
+                          native_insert_after_bulk_erase,
+                          bucket_size_is_the_reference_count };
     for (auto c : cases)
     {
         const int before = g_failures;

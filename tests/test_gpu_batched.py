@@ -316,3 +316,36 @@ def test_batched_set_kind_any_workgroup_size(block):
         np.testing.assert_array_equal(np.sort(ka.cpu().numpy().view(np.uint64)), np.sort(kb.cpu().numpy().view(np.uint64)))
         assert bt[t].size() == tab.size() and bt[t].valid()
     batch.destroy()
+
+
+def test_mixed_operations_on_one_table_and_find_without_hits_are_refused():
+    """segments of one table naming different operations, and a per-table find without a hit array
+    (table 0 empty, so the first table with work starts past element 0), are refused with
+    HTAB_E_ARG before any device work"""
+    import ctypes
+
+    from stdgpu_amd import _lib
+
+    lib = _lib.load()
+    tabs = [sa.unordered_map_u64_u32.createDeviceObject(1000) for _ in range(2)]
+    batch = TableBatch(tabs)
+    buf = torch.zeros(256, dtype=torch.int64, device="cuda")
+    base = buf.data_ptr()
+    s = torch.cuda.current_stream().cuda_stream
+    E = _lib.HTAB_E_ARG
+    ins = _lib.htab_segment(base, base + 1024, None, None, 4, 1, _lib.HTAB_OP_INSERT)
+    era = _lib.htab_segment(base + 64, None, None, None, 4, 1, _lib.HTAB_OP_ERASE)
+    other = _lib.htab_segment(base + 64, None, None, None, 4, 0, _lib.HTAB_OP_ERASE)
+    segs = (_lib.htab_segment * 3)(ins, other, era)
+    assert lib.htab_batch_run_segments(batch._b, segs, 3, s) == E
+    torch.cuda.synchronize()
+    assert tabs[0].size() == 0 and tabs[1].size() == 0
+    # one op per table (different across tables) is accepted
+    segs = (_lib.htab_segment * 2)(ins, other)
+    assert lib.htab_batch_run_segments(batch._b, segs, 2, s) == 0
+    torch.cuda.synchronize()
+    assert tabs[1].size() == 1  # the buffer holds four copies of key 0
+    off = (ctypes.c_int64 * 3)(0, 0, 4)
+    assert lib.htab_batch_run_per_table(batch._b, _lib.HTAB_OP_FIND, off, ctypes.c_void_p(base), None, None, None,
+                                        ctypes.c_void_p(s)) == E
+    batch.destroy()

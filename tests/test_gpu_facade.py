@@ -16,7 +16,11 @@ BUILD = os.path.join(ROOT, "tests", "cpp", "_build")
 def test_cpp_facade_cases(name):
     exe = os.path.join(BUILD, name)
     assert os.path.exists(exe), "build the C++ tests first: python -m stdgpu_amd.build"
-    r = subprocess.run([exe], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=180)
+    try:
+        r = subprocess.run([exe], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=180)
+    except subprocess.TimeoutExpired as e:
+        out = e.stdout.decode() if isinstance(e.stdout, bytes) else (e.stdout or "")
+        pytest.fail(f"{name} did not finish in 180 s; cases reported so far:\n{out}")
     print(r.stdout)
     assert r.returncode == 0, r.stdout
     assert "OK: 0 failure(s)" in r.stdout

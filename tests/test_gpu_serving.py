@@ -26,16 +26,32 @@ def test_served_results_equal_in_order_replay(rate, nstreams):
     assert chk["contents_identical"], chk
 
 
-def test_admission_refuses_inserts_that_cannot_fit():
-    """a table whose pool is larger than its capacity: inserts beyond the free slots are refused,
-    and the refused requests never take effect (the replay skips them and still matches)."""
+def test_inserts_reaching_the_limit_are_served_not_refused():
+    """table 0's pool is four times its capacity, table 1's fits: no request is refused; table 0 ends
+    exactly at its limit with valid() holding (which keys of the request that crosses the limit get
+    in is decided by the race, as inside one plain bulk call), and every request of table 1 gives the
+    in-order replay's result."""
     caps = np.array([300, 5000], np.int64)
     pools = sv.Pools(caps, seed=3, fill=4.0)  # 1200 keys for a 300-slot table
     log = sv.make_log(pools.sizes, nreq=200, rate_per_s=None, seed=9, log2_lo=5, log2_hi=9, mix=(0.7, 0.3, 0.0))
     tables, (flags, vals, status), res = sv.served_run(caps, pools, log, nstreams=2)
-    assert res["refused"] > 0
-    for t in tables:
-        assert t.size() <= t.info.limit
+    assert res["refused"] == 0 and (status == 0).all()
+    assert tables[0].size() == tables[0].info.limit and tables[0].valid()
+    assert tables[1].size() <= tables[1].info.limit and tables[1].valid()
+    chk = sv.verify_against_replay(caps, pools, log, tables, status, flags, vals, only_tables=[1])
+    sv.destroy_tables(tables)
+    assert chk["finds_identical"] and chk["contents_identical"], chk
+
+
+def test_mostly_present_inserts_near_the_limit_are_applied():
+    """a table filled close to its limit receives inserts whose keys are mostly present already: n
+    exceeds the free slots, the new keys do not; every request is applied and the result equals the
+    in-order replay"""
+    caps = np.array([4096, 4096], np.int64)
+    pools = sv.Pools(caps, seed=5, fill=0.95)
+    log = sv.make_log(pools.sizes, nreq=300, rate_per_s=None, seed=12, log2_lo=8, log2_hi=11, mix=(0.8, 0.2, 0.0))
+    tables, (flags, vals, status), res = sv.served_run(caps, pools, log, nstreams=2)
+    assert res["refused"] == 0 and (status == 0).all()
     chk = sv.verify_against_replay(caps, pools, log, tables, status, flags, vals)
     sv.destroy_tables(tables)
     assert chk["finds_identical"] and chk["contents_identical"], chk
