@@ -33,6 +33,8 @@ def test_inserts_reaching_the_limit_are_served_not_refused():
     in-order replay's result."""
     caps = np.array([300, 5000], np.int64)
     pools = sv.Pools(caps, seed=3, fill=4.0)  # 1200 keys for a 300-slot table
+    pools.sizes[1] = 4000  # ... and 4000 for the 5000-slot one
+    pools.keys[1], pools.vals[1] = pools.keys[1][:4000], pools.vals[1][:4000]
     log = sv.make_log(pools.sizes, nreq=200, rate_per_s=None, seed=9, log2_lo=5, log2_hi=9, mix=(0.7, 0.3, 0.0))
     tables, (flags, vals, status), res = sv.served_run(caps, pools, log, nstreams=2)
     assert res["refused"] == 0 and (status == 0).all()
