@@ -65,7 +65,7 @@ def parse():
     ap.add_argument("--cpu-baseline", type=int, default=1, help="time the oracle on a bounded sample")
     ap.add_argument("--cpu-log2n", type=int, default=20, help="log2 keys of the CPU thread sweep")
     ap.add_argument("--cpu-full-log2n", type=int, default=26, help="log2 keys of the one full-size CPU run")
-    ap.add_argument("--cpu-full-budget-s", type=float, default=150.0,
+    ap.add_argument("--cpu-full-budget-s", type=float, default=120.0,
                     help="skip the full-size CPU run when the sweep predicts it takes longer")
     ap.add_argument("--probe-batch", type=int, default=0)
     ap.add_argument("--batched", type=int, default=1, help="also measure the batched form")
@@ -95,6 +95,14 @@ ORACLE_CONFIG1_S_XEON = {1: {"insert": 0.607, "find": 0.153, "erase": 0.205},
                          8: {"insert": 2.071, "find": 0.389, "erase": 0.500}}
 
 
+_T0 = time.perf_counter()
+
+
+def progress(what: str) -> None:
+    """one line per finished section on stderr (the JSON result stays alone on stdout)"""
+    print(f"bench: {what} ({time.perf_counter() - _T0:.1f} s)", file=sys.stderr, flush=True)
+
+
 def _cpu_model() -> str:
     try:
         with open("/proc/cpuinfo") as f:
@@ -106,16 +114,20 @@ def _cpu_model() -> str:
     return "unknown CPU"
 
 
-def _oracle_sequence(w, th: int) -> float:
+def _oracle_sequence(w, th: int, report=None) -> float:
     """seconds for the map sequence (insert, find, erase 25%, re-insert) on the oracle"""
     from oracle import pyoracle as po
     from stdgpu_amd import workloads as wl
 
     t = po.Table("u64", "u32", w.capacity)
+    say = report or (lambda what: None)
     t0 = time.perf_counter()
     t.insert(w.keys, w.values, threads=th)
+    say("insert")
     t.find(w.queries, threads=th)
+    say("find")
     t.erase(w.erase, threads=th)
+    say("erase")
     t.insert(w.reinsert, wl.value_of(w.reinsert), threads=th)
     dt = time.perf_counter() - t0
     t.close()
@@ -158,6 +170,7 @@ def cpu_baseline(sweep_log2n: int, full_log2n: int, full_budget_s: float) -> dic
         dt = _oracle_sequence(w, th)
         rate = ops / dt / 1e9
         per_threads[th] = round(rate, 6)
+        progress(f"cpu baseline sweep, {th} threads: {rate:.6f} Gops/s")
         if best is None or rate > best[0]:
             best = (rate, th, dt)
     del w
@@ -167,10 +180,12 @@ def cpu_baseline(sweep_log2n: int, full_log2n: int, full_budget_s: float) -> dic
     full_ops = 2 * (1 << full_log2n) + 2 * (1 << (full_log2n - 2))
     if full_log2n > sweep_log2n and full_ops / (best[0] * 1e9) <= full_budget_s:
         wf = wl.map_u64_uniform(n=1 << full_log2n, seed=1)
-        dt = _oracle_sequence(wf, best[1])
+        progress(f"cpu baseline: 2^{full_log2n} workload made, running it at {best[1]} threads")
+        dt = _oracle_sequence(wf, best[1], report=lambda what: progress(f"cpu baseline: 2^{full_log2n} {what} done"))
         fops = len(wf.keys) + len(wf.queries) + len(wf.erase) + len(wf.reinsert)
         del wf
         full = {"log2n": full_log2n, "threads": best[1], "seconds": round(dt, 2), "gops": round(fops / dt / 1e9, 6)}
+        progress(f"cpu baseline: 2^{full_log2n} sequence in {dt:.1f} s")
         value = fops / dt / 1e9
         sample = (f"the full map sequence at 2^{full_log2n} keys (insert, find at 90% hit, erase 25%, re-insert) once, "
                   f"at the best thread count ({best[1]}) of a sweep at 2^{sweep_log2n} keys over {cand}")
@@ -583,19 +598,24 @@ def main():
     stream = torch.cuda.current_stream()
     names = ["clear", "insert", "find", "erase", "reinsert"]
 
+    # The reference's bulk insert(begin, end) and erase(begin, end) return nothing; the timed step
+    # calls them that way (no per-element flags).  `with_flags` below times the same step with the
+    # per-element inserted / erased flags the C ABI can also return.
+    flag_args = {"ins": None, "er": None}
+
     def step(ev=None):
         if ev is not None:
             ev[0].record(stream)
         tab.clear()
         if ev is not None:
             ev[1].record(stream)
-        tab.insert(keys, vals, inserted=ins_flags)
+        tab.insert(keys, vals, inserted=flag_args["ins"])
         if ev is not None:
             ev[2].record(stream)
         tab.find(queries, hits=hits, values_out=got)
         if ev is not None:
             ev[3].record(stream)
-        tab.erase(erase, erased=er_flags)
+        tab.erase(erase, erased=flag_args["er"])
         if ev is not None:
             ev[4].record(stream)
         tab.insert(reins, reins_v)
@@ -616,6 +636,7 @@ def main():
         step(events[s])
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
+    progress(f"timed steps done: {wall * 1e3 / args.steps:.3f} ms per step")
 
     per = {k: [] for k in names}
     for ev in events:
@@ -675,6 +696,24 @@ def main():
             "random_access": random_access_bound(args, tab, rates),
         },
     }
+    # the same step with per-element inserted / erased flags (per-key insert path), untimed by `value`
+    flag_args["ins"], flag_args["er"] = ins_flags, er_flags
+    for _ in range(1):
+        step()
+    fev = [[torch.cuda.Event(enable_timing=True) for _ in range(6)] for _ in range(3)]
+    for s_ in range(3):
+        step(fev[s_])
+    torch.cuda.synchronize()
+    fper = {k: [fev[s_][j].elapsed_time(fev[s_][j + 1]) for s_ in range(3)] for j, k in enumerate(names)}
+    fstep = float(np.mean([sum(fper[k][s_] for k in names) for s_ in range(3)]))
+    out["with_flags"] = {
+        "what": "the same step with per-element inserted / erased flags returned by insert and erase",
+        "gops": round(total_ops / (fstep / 1e3) / 1e9, 4),
+        "ms_per_step": round(fstep, 4),
+        "per_op_gops": {k: round(ops[k] / (np.mean(fper[k]) / 1e3) / 1e9, 3) for k in ops},
+        "inserted_flags_ok": int(ins_flags.sum()) == n,
+    }
+    flag_args["ins"], flag_args["er"] = None, None
     # snapshot / restore of the table the step leaves (host memory over PCIe; not part of `value`)
     t0 = time.perf_counter()
     snap = tab.snapshot()
@@ -691,18 +730,25 @@ def main():
     del keys, vals, queries, erase, reins, reins_v, hits, got, ins_flags, er_flags
     if args.zipf:
         out["zipf"] = bench_zipf(args)
+        progress("zipf done")
     if args.set_u32:
         out["set_u32"] = bench_set_u32(args)
+        progress("set_u32 done")
     if args.free_lists:
         out["free_lists"] = bench_free_lists(args)
+        progress("free_lists done")
     if args.batched:
         out["batched"] = bench_batched(args)
+        progress("batched done")
     if args.served:
         out["served"] = bench_served(args)
+        progress("served done")
     if args.voxel:
         out["voxel"] = bench_voxel(args)
+        progress("voxel done")
     if args.cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.cpu_log2n, args.cpu_full_log2n, args.cpu_full_budget_s)
+    progress("all sections done")
     print(json.dumps(out), flush=True)
 
 

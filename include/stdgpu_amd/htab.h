@@ -50,6 +50,10 @@ typedef struct htab_tuning
     float slots_per_capacity;  /* physical slots = ceil(capacity * this); default 1.0 (load 0.8 at 0.8*capacity keys) */
     int32_t probe_batch;       /* elements a lane group probes together: 2, 4 or 8 (at most bucket_lanes);
                                 * default: 8 for find, 4 for insert and erase (1 with one-lane buckets) */
+    int32_t insert_path;       /* bulk inserts without per-element flags on tables whose limit is their
+                                * slot count: 0 = partitioned by table region when the launch is large
+                                * for the table (default), 1 = always one compare-and-swap per key,
+                                * 2 = partitioned whenever it applies (for tests) */
 } htab_tuning;
 
 /* Sizes a caller can query.  bucket_count / max_size follow the reference formulas so

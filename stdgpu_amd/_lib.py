@@ -48,6 +48,7 @@ class htab_tuning(ctypes.Structure):
         ("block_size", ctypes.c_int32),
         ("slots_per_capacity", ctypes.c_float),
         ("probe_batch", ctypes.c_int32),
+        ("insert_path", ctypes.c_int32),
     ]
 
 

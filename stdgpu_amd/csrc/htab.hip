@@ -260,7 +260,7 @@ htab_create(int kind, int64_t capacity, const htab_tuning* tuning, htab** out)
 {
     if (!out) return fail(HTAB_E_ARG, "htab_create: out is null");
     *out = nullptr;
-    int lanes = 8, block = 64, rb = 0;
+    int lanes = 8, block = 64, rb = 0, ipath = 0;
     float spc = 1.0F;
     if (tuning)
     {
@@ -268,9 +268,11 @@ htab_create(int kind, int64_t capacity, const htab_tuning* tuning, htab** out)
         if (tuning->block_size) block = tuning->block_size;
         if (tuning->slots_per_capacity > 0.0F) spc = tuning->slots_per_capacity;
         rb = tuning->probe_batch;
+        ipath = tuning->insert_path;
     }
     int rc = check_geometry(kind, capacity, lanes, block, rb, "htab_create");
     if (rc != HTAB_OK) return rc;
+    if (ipath < 0 || ipath > 2) return fail(HTAB_E_ARG, "htab_create: insert_path must be 0, 1 or 2");
     if (!(spc >= 0.25F && spc <= 16.0F)) return fail(HTAB_E_ARG, "htab_create: slots_per_capacity out of [0.25, 16]");
 
     const int64_t W = 16LL * lanes / value_bytes_of(kind);
@@ -280,7 +282,9 @@ htab_create(int kind, int64_t capacity, const htab_tuning* tuning, htab** out)
     // room for two entries at least, as the reference's capacity-1 table (one bucket head plus one
     // excess entry); only one-slot buckets need the second bucket for it
     if (nbuckets * W < 2) nbuckets = (2 + W - 1) / W;
-    return build_table(kind, capacity, lanes, block, rb, nbuckets, out);
+    rc = build_table(kind, capacity, lanes, block, rb, nbuckets, out);
+    if (rc == HTAB_OK) (*out)->insert_path = ipath;
+    return rc;
 }
 
 // ---------------------------------------------------------------- snapshot / restore
@@ -387,6 +391,7 @@ htab_destroy(htab* h)
     if (h->mem) e1 = hipFree(h->mem);
     if (h->scratch) e2 = hipFree(h->scratch);
     if (h->range_idx && e2 == hipSuccess) e2 = hipFree(h->range_idx);
+    if (h->part && e2 == hipSuccess) e2 = hipFree(h->part);
     delete h;
     if (e1 != hipSuccess) return fail_hip(e1, "htab_destroy");
     if (e2 != hipSuccess) return fail_hip(e2, "htab_destroy scratch");

@@ -13,6 +13,7 @@
 #include <stdgpu_amd/htab.h>
 
 #include "htab_kernels.hpp"
+#include "htab_region.hpp"
 
 namespace htab_host
 {
@@ -56,6 +57,9 @@ struct htab
     int64_t* scratch = nullptr; // range/valid scratch, grown on demand
     size_t scratch_bytes = 0;
     int64_t* range_idx = nullptr; // table-owned device_range() indices (limit + 2 entries)
+    int insert_path = 0;          // htab_tuning::insert_path
+    void* part = nullptr;         // partitioned-insert scratch, grown on demand
+    size_t part_bytes = 0;
 };
 
 // the device word in the table's control block where k_limit_plan records its decision

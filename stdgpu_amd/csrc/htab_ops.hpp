@@ -35,9 +35,82 @@ struct ops
 {
     using T = table_ref<K, M>;
 
+    // The partitioned path (htab_region.hpp) for a launch without per-element flags on a table whose
+    // limit is its slot count: when forced, or by default when the launch brings enough keys per
+    // region (each region costs a 32 KB read and write; a per-key insert moves about 250 B per key).
+    static bool use_regions(const htab* h, int64_t n, const uint8_t* ins)
+    {
+        if (ins || h->view.nstripes == 1 || n <= 0 || n > 0x7FFFFFFFLL || h->insert_path == 1) return false;
+        const uint64_t nreg = (h->view.nbuckets + region_buckets<G>() - 1) / region_buckets<G>();
+        if (nreg > kMaxRegions) return false;
+        if (h->insert_path == 2) return true;
+        return nreg >= 1024 && static_cast<uint64_t>(n) >= 192 * nreg;
+    }
+
+    static int insert_regions(htab* h, const void* values, const void* keys, const void* vals, int64_t n, hipStream_t s)
+    {
+        using VT = std::conditional_t<std::is_void_v<M>, char, M>;
+        constexpr size_t kb = sizeof(K), vb = slot_traits<K, M>::is_map ? sizeof(VT) : 0;
+        const T t = T::from_view(h->view);
+        constexpr uint32_t rbk = region_buckets<G>();
+        const uint32_t nreg = static_cast<uint32_t>((t.nbuckets + rbk - 1) / rbk);
+        const uint32_t nsuper = (nreg + kSubBins - 1) / kSubBins;
+        const size_t nrs = static_cast<size_t>(nsuper) * kSubBins;
+        auto up = [](size_t x) { return (x + 255) / 256 * 256; };
+        const size_t o_count = up(sizeof(region_ctl));
+        const size_t o_start = o_count + up(nrs * 4);
+        const size_t o_cursor = o_start + up((nrs + 1) * 4);
+        const size_t o_ak = o_cursor + up(nrs * 4);
+        const size_t o_av = o_ak + up(static_cast<size_t>(n) * kb);
+        const size_t o_bk = o_av + up(static_cast<size_t>(n) * vb);
+        const size_t o_bv = o_bk + up(static_cast<size_t>(n) * kb);
+        const size_t need = o_bv + up(static_cast<size_t>(n) * vb);
+        if (h->part_bytes < need)
+        {
+            if (h->part) (void)hipFree(h->part);
+            h->part = nullptr;
+            h->part_bytes = 0;
+            HTAB_HIP_TRY(hipMalloc(&h->part, need), "partitioned insert scratch");
+            h->part_bytes = need;
+        }
+        char* base = static_cast<char*>(h->part);
+        region_ctl* ctl = reinterpret_cast<region_ctl*>(base);
+        region_scratch sc{ctl,
+                          reinterpret_cast<uint32_t*>(base + o_count),
+                          reinterpret_cast<uint32_t*>(base + o_start),
+                          reinterpret_cast<uint32_t*>(base + o_cursor),
+                          base + o_ak,
+                          vb ? base + o_av : nullptr,
+                          base + o_bk,
+                          vb ? base + o_bv : nullptr};
+        const uint32_t nchunks = static_cast<uint32_t>((n + kPartChunk - 1) / kPartChunk);
+        uint32_t rgrid = static_cast<uint32_t>((nrs + 255) / 256);
+        if (rgrid < 2) rgrid = 2;
+        if (rgrid > 1024) rgrid = 1024;
+        hipLaunchKernelGGL(k_region_reset<>, dim3(rgrid), dim3(256), 0, s, ctl, sc.region_count, static_cast<uint32_t>(nrs));
+        hipLaunchKernelGGL((k_part1_count<K, M>), dim3(nchunks), dim3(kPartThreads), 0, s, t, values, keys, vals, n, rbk,
+                           nsuper, ctl);
+        hipLaunchKernelGGL(k_part1_scan<>, dim3(1), dim3(kPartThreads), 0, s, ctl, nsuper);
+        hipLaunchKernelGGL((k_part1_scatter<K, M>), dim3(nchunks), dim3(kPartThreads), 0, s, t, values, keys, vals, n,
+                           rbk, nsuper, ctl, sc);
+        hipLaunchKernelGGL((k_part2_count<K, M>), dim3(nchunks + nsuper), dim3(kPartThreads), 0, s, t, rbk, nsuper, ctl,
+                           sc);
+        hipLaunchKernelGGL(k_part2_scan<>, dim3(nsuper), dim3(kSubBins), 0, s, ctl, nsuper, sc);
+        hipLaunchKernelGGL((k_part2_scatter<K, M>), dim3(nchunks + nsuper), dim3(kPartThreads), 0, s, t, rbk, nsuper, ctl,
+                           sc);
+        hipLaunchKernelGGL((k_region_insert<K, M, G>), dim3(nreg), dim3(kRegionThreads), 0, s, t, ctl, sc);
+        // what the regions could not place, and the two reserved key values, by the per-key kernel
+        hipLaunchKernelGGL((k_insert_list<K, M, G, R>), dim3(256), dim3(256), 0, s, t, sc.a_keys, sc.a_vals,
+                           &ctl->deferred, n, 0);
+        hipLaunchKernelGGL((k_insert_list<K, M, G, R>), dim3(16), dim3(256), 0, s, t, sc.b_keys, sc.b_vals,
+                           &ctl->reserved, n, 1);
+        return launch_status("partitioned insert launch");
+    }
+
     static int insert(htab* h, const void* values, const void* keys, const void* vals, int64_t n, uint8_t* ins,
                       hipStream_t s)
     {
+        if (use_regions(h, n, ins)) return insert_regions(h, values, keys, vals, n, s);
         const int64_t grid = (n + h->block - 1) / h->block;
         const T t = T::from_view(h->view);
         const bool one_stripe = h->view.nstripes == 1;
@@ -235,10 +308,10 @@ struct table_ops
         HTAB_HIP_TRY(hipStreamSynchronize(s), "valid sync");
         int64_t size = 0;
         for (uint32_t i = 0; i < kMaxStripes; ++i) size += stripes[i * kStripeStride];
-        *out = (res[0] == 0 && static_cast<int64_t>(res[1]) == size && size >= 0 &&
-                size <= static_cast<int64_t>(h->view.limit))
-                       ? 1
-                       : 0;
+        // a table whose limit is its slot count also holds the two reserved key values in their side
+        // slots; a table with an exact limit counts them against it
+        const int64_t bound = static_cast<int64_t>(h->view.limit) + (h->view.nstripes == 1 ? 0 : 2);
+        *out = (res[0] == 0 && static_cast<int64_t>(res[1]) == size && size >= 0 && size <= bound) ? 1 : 0;
         return HTAB_OK;
     }
 };

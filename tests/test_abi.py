@@ -38,7 +38,7 @@ def test_python_binding_covers_header():
 
 def test_struct_layouts_match_header():
     # field order/size of the C structs as seen by ctypes (plain 64-bit ABI)
-    assert ctypes.sizeof(_lib.htab_tuning) == 16
+    assert ctypes.sizeof(_lib.htab_tuning) == 20
     assert ctypes.sizeof(_lib.htab_info) == 8 + 8 * 8
     assert ctypes.sizeof(_lib.htab_view) == 3 * 8 + 2 * 8 + 2 * 4 + 2 * 8 + 8
     assert ctypes.sizeof(_lib.htab_deque_view) == 7 * 8
