@@ -51,9 +51,9 @@ typedef struct htab_tuning
     int32_t probe_batch;       /* elements a lane group probes together: 2, 4 or 8 (at most bucket_lanes);
                                 * default: 8 for find, 4 for insert and erase (1 with one-lane buckets) */
     int32_t insert_path;       /* bulk inserts without per-element flags on tables whose limit is their
-                                * slot count: 0 = partitioned by table region when the launch is large
-                                * for the table (default), 1 = always one compare-and-swap per key,
-                                * 2 = partitioned whenever it applies (for tests) */
+                                * slot count: 0 (default) and 1 = one compare-and-swap per key,
+                                * 2 = keys partitioned by table region and each region inserted in
+                                * LDS (tables of at most 65536 regions of 32 KB) */
 } htab_tuning;
 
 /* Sizes a caller can query.  bucket_count / max_size follow the reference formulas so

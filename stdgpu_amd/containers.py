@@ -55,7 +55,7 @@ class Tuning:
     block_size: int = 64
     slots_per_capacity: float = 1.0
     probe_batch: int = 0  # 0 = library default
-    insert_path: int = 0  # 0 = by launch size, 1 = per-key CAS, 2 = partitioned whenever it applies
+    insert_path: int = 0  # 0 and 1 = per-key CAS, 2 = partitioned by table region
 
     def to_c(self) -> _lib.htab_tuning:
         return _lib.htab_tuning(self.bucket_lanes, self.block_size, self.slots_per_capacity, self.probe_batch,

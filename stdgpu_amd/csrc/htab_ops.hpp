@@ -3,6 +3,7 @@
 #ifndef STDGPU_AMD_HTAB_OPS_HPP
 #define STDGPU_AMD_HTAB_OPS_HPP
 
+#include <cmath>
 #include <new>
 #include <type_traits>
 
@@ -36,15 +37,13 @@ struct ops
     using T = table_ref<K, M>;
 
     // The partitioned path (htab_region.hpp) for a launch without per-element flags on a table whose
-    // limit is its slot count: when forced, or by default when the launch brings enough keys per
-    // region (each region costs a 32 KB read and write; a per-key insert moves about 250 B per key).
+    // limit is its slot count, when the table asks for it.  It is not the default: at 2^26 keys into
+    // 2^23 buckets its five launches take 5.3 ms where one CAS per key takes 4.1 ms.
     static bool use_regions(const htab* h, int64_t n, const uint8_t* ins)
     {
-        if (ins || h->view.nstripes == 1 || n <= 0 || n > 0x7FFFFFFFLL || h->insert_path == 1) return false;
+        if (ins || h->view.nstripes == 1 || n <= 0 || n > 0x7FFFFFFFLL || h->insert_path != 2) return false;
         const uint64_t nreg = (h->view.nbuckets + region_buckets<G>() - 1) / region_buckets<G>();
-        if (nreg > kMaxRegions) return false;
-        if (h->insert_path == 2) return true;
-        return nreg >= 1024 && static_cast<uint64_t>(n) >= 192 * nreg;
+        return nreg <= kMaxRegions;
     }
 
     static int insert_regions(htab* h, const void* values, const void* keys, const void* vals, int64_t n, hipStream_t s)
@@ -56,54 +55,68 @@ struct ops
         const uint32_t nreg = static_cast<uint32_t>((t.nbuckets + rbk - 1) / rbk);
         const uint32_t nsuper = (nreg + kSubBins - 1) / kSubBins;
         const size_t nrs = static_cast<size_t>(nsuper) * kSubBins;
+        // bin capacities: a bin's expected share of uniformly hashed keys, plus 6 % (super-bins) or
+        // 12.5 % (regions) and four standard deviations; what a bin cannot take spills
+        const double per_region = static_cast<double>(n) / nreg;
+        const double e1 = per_region * kSubBins;
+        const uint32_t cap1 = static_cast<uint32_t>(std::ceil(e1 * 1.0625 + 4.0 * std::sqrt(e1) + 64.0));
+        const uint32_t cap2 = static_cast<uint32_t>(std::ceil(per_region * 1.125 + 4.0 * std::sqrt(per_region) + 32.0));
         auto up = [](size_t x) { return (x + 255) / 256 * 256; };
-        const size_t o_count = up(sizeof(region_ctl));
-        const size_t o_start = o_count + up(nrs * 4);
-        const size_t o_cursor = o_start + up((nrs + 1) * 4);
-        const size_t o_ak = o_cursor + up(nrs * 4);
-        const size_t o_av = o_ak + up(static_cast<size_t>(n) * kb);
-        const size_t o_bk = o_av + up(static_cast<size_t>(n) * vb);
-        const size_t o_bv = o_bk + up(static_cast<size_t>(n) * kb);
-        const size_t need = o_bv + up(static_cast<size_t>(n) * vb);
-        if (h->part_bytes < need)
+        const size_t na = static_cast<size_t>(nsuper) * cap1, nbb = nrs * cap2, ns = static_cast<size_t>(n);
+        size_t off = up(sizeof(region_ctl));
+        const size_t o_fill = off;
+        off += up(nrs * 4);
+        const size_t o_ak = off;
+        off += up(na * kb);
+        const size_t o_av = off;
+        off += up(na * vb);
+        const size_t o_bk = off;
+        off += up(nbb * kb);
+        const size_t o_bv = off;
+        off += up(nbb * vb);
+        const size_t o_sk = off;
+        off += up(ns * kb);
+        const size_t o_sv = off;
+        off += up(ns * vb);
+        // the deferred keys reuse buffer A once pass 2 has read it: at most every element defers
+        if (na < ns) return fail(HTAB_E_LIMIT, "partitioned insert: super-bin capacity below the launch");
+        if (h->part_bytes < off)
         {
             if (h->part) (void)hipFree(h->part);
             h->part = nullptr;
             h->part_bytes = 0;
-            HTAB_HIP_TRY(hipMalloc(&h->part, need), "partitioned insert scratch");
-            h->part_bytes = need;
+            HTAB_HIP_TRY(hipMalloc(&h->part, off), "partitioned insert scratch");
+            h->part_bytes = off;
         }
         char* base = static_cast<char*>(h->part);
         region_ctl* ctl = reinterpret_cast<region_ctl*>(base);
         region_scratch sc{ctl,
-                          reinterpret_cast<uint32_t*>(base + o_count),
-                          reinterpret_cast<uint32_t*>(base + o_start),
-                          reinterpret_cast<uint32_t*>(base + o_cursor),
+                          reinterpret_cast<uint32_t*>(base + o_fill),
                           base + o_ak,
                           vb ? base + o_av : nullptr,
                           base + o_bk,
-                          vb ? base + o_bv : nullptr};
+                          vb ? base + o_bv : nullptr,
+                          base + o_sk,
+                          vb ? base + o_sv : nullptr,
+                          cap1,
+                          cap2};
         const uint32_t nchunks = static_cast<uint32_t>((n + kPartChunk - 1) / kPartChunk);
+        const uint32_t chunks_per_super = (cap1 + kPartChunk - 1) / kPartChunk;
         uint32_t rgrid = static_cast<uint32_t>((nrs + 255) / 256);
         if (rgrid < 2) rgrid = 2;
         if (rgrid > 1024) rgrid = 1024;
-        hipLaunchKernelGGL(k_region_reset<>, dim3(rgrid), dim3(256), 0, s, ctl, sc.region_count, static_cast<uint32_t>(nrs));
-        hipLaunchKernelGGL((k_part1_count<K, M>), dim3(nchunks), dim3(kPartThreads), 0, s, t, values, keys, vals, n, rbk,
-                           nsuper, ctl);
-        hipLaunchKernelGGL(k_part1_scan<>, dim3(1), dim3(kPartThreads), 0, s, ctl, nsuper);
+        hipLaunchKernelGGL(k_region_reset<>, dim3(rgrid), dim3(256), 0, s, ctl, sc.region_fill, static_cast<uint32_t>(nrs));
         hipLaunchKernelGGL((k_part1_scatter<K, M>), dim3(nchunks), dim3(kPartThreads), 0, s, t, values, keys, vals, n,
-                           rbk, nsuper, ctl, sc);
-        hipLaunchKernelGGL((k_part2_count<K, M>), dim3(nchunks + nsuper), dim3(kPartThreads), 0, s, t, rbk, nsuper, ctl,
-                           sc);
-        hipLaunchKernelGGL(k_part2_scan<>, dim3(nsuper), dim3(kSubBins), 0, s, ctl, nsuper, sc);
-        hipLaunchKernelGGL((k_part2_scatter<K, M>), dim3(nchunks + nsuper), dim3(kPartThreads), 0, s, t, rbk, nsuper, ctl,
-                           sc);
-        hipLaunchKernelGGL((k_region_insert<K, M, G>), dim3(nreg), dim3(kRegionThreads), 0, s, t, ctl, sc);
-        // what the regions could not place, and the two reserved key values, by the per-key kernel
-        hipLaunchKernelGGL((k_insert_list<K, M, G, R>), dim3(256), dim3(256), 0, s, t, sc.a_keys, sc.a_vals,
-                           &ctl->deferred, n, 0);
-        hipLaunchKernelGGL((k_insert_list<K, M, G, R>), dim3(16), dim3(256), 0, s, t, sc.b_keys, sc.b_vals,
-                           &ctl->reserved, n, 1);
+                           rbk, nsuper, sc);
+        hipLaunchKernelGGL((k_part2_scatter<K, M>), dim3(nsuper * chunks_per_super), dim3(kPartThreads), 0, s, t, rbk,
+                           chunks_per_super, sc);
+        hipLaunchKernelGGL((k_region_insert<K, M, G>), dim3(nreg), dim3(kRegionThreads), 0, s, t, sc);
+        // what the regions could not place, what the bins could not take and the two reserved key
+        // values, by the per-key kernel
+        hipLaunchKernelGGL((k_insert_list<K, M, G, R>), dim3(1024), dim3(256), 0, s, t, sc.a_keys, sc.a_vals,
+                           &ctl->deferred);
+        hipLaunchKernelGGL((k_insert_list<K, M, G, R>), dim3(1024), dim3(256), 0, s, t, sc.s_keys, sc.s_vals,
+                           &ctl->spilled);
         return launch_status("partitioned insert launch");
     }
 

@@ -2,24 +2,26 @@
 // its size on all stripes, runs as streaming passes instead of one random compare-and-swap per key.
 //
 // The bucket array is cut into regions of kRegionBytes (32 KB: 256 buckets of 128 B).  The keys are
-// partitioned by the region of their home bucket in two LDS-staged counting passes (by region / 128,
-// then by region % 128), each reading and writing the keys once with coalesced runs.  Then one
-// workgroup per region copies the region into LDS, inserts the region's keys there with LDS atomics
-// (the same claim protocol as the per-key kernel, so copies of one key in the launch still produce
-// exactly one stored key), and writes the region back.  A key whose probe sequence runs past the
-// region's last bucket (to the next region, which another workgroup owns meanwhile) is deferred, and
-// so are the two reserved key values; the per-key kernel inserts the deferred keys afterwards, on
-// the whole table.
+// partitioned by the region of their home bucket in two LDS-staged scatter passes (by region / 128
+// into super-bins, then by region % 128), each reading and writing the keys once in coalesced runs.
+// Every bin has a fixed capacity (its expected share plus slack), so no counting pass precedes a
+// scatter; the keys a full bin cannot take (a hot key of a skewed input, or bad luck) spill to a list.
+// Then one workgroup per region copies the region into LDS, inserts the region's keys there with LDS
+// atomics (the claim protocol of the per-key kernel, so copies of one key in a launch still leave one
+// stored key) and writes the region back.  A key whose probe sequence runs past the region's last
+// bucket (into a region another workgroup owns meanwhile) is deferred.  The per-key kernel then
+// inserts the deferred keys, the spilled keys and the two reserved key values on the whole table.
 //
 // Why: random 64-bit CAS is performed memory-side on gfx950 (about 18 G/s over a table in HBM,
 // 27 G/s inside the 256 MB Infinity Cache, no faster when each XCD keeps to its own slice), so the
-// per-key kernel is bound near 18 G keys/s; the partitioned form moves about 8 GB for 2^26 keys into
-// a 1.34 GB table at streaming rates.
+// per-key kernel is bound near 18 G keys/s; the partitioned form streams the keys twice and the table
+// once.
 //
 // Correctness of the split: keys of one home bucket land in one region, and a key is inserted in the
 // LDS copy only when its probe sequence ends inside the region (a bucket with an EMPTY slot or
-// ovf == 0) or it is found there; slots only fill during an insert launch, so copies of one key
-// either all resolve inside the region (against each other, by LDS CAS) or all defer.
+// ovf == 0) or it is found there; slots only fill during an insert launch, so copies of one key that
+// reach the region either all resolve there (against each other, by LDS CAS) or all defer, and copies
+// that spill or defer meet the region's result in the table when the per-key kernel runs.
 #ifndef STDGPU_AMD_HTAB_REGION_HPP
 #define STDGPU_AMD_HTAB_REGION_HPP
 
@@ -33,8 +35,8 @@ constexpr uint32_t kSubBins = 128;   // regions per super-bin (digit of the seco
 constexpr uint32_t kMaxSuper = 512;  // super-bins (digit of the first pass)
 constexpr uint32_t kMaxRegions = kSubBins * kMaxSuper;
 constexpr int kPartThreads = 256;
-constexpr int kPartItems = 16;                            // elements per thread of a partition chunk
-constexpr int kPartChunk = kPartThreads * kPartItems;     // 4096
+constexpr int kPartItems = 8;                             // elements per thread of a scatter chunk
+constexpr int kPartChunk = kPartThreads * kPartItems;     // 2048
 constexpr int kRegionThreads = 256;
 
 template <int G>
@@ -44,28 +46,25 @@ region_buckets()
     return static_cast<uint32_t>(kRegionBytes / (16 * G));
 }
 
-// Device bookkeeping of one partitioned insert, at the start of the partition scratch.
+// Device bookkeeping of one partitioned insert.
 struct region_ctl
 {
-    uint32_t super_count[kMaxSuper];
-    uint32_t super_start[kMaxSuper + 1];
-    uint32_t super_cursor[kMaxSuper];
-    uint32_t chunk_start[kMaxSuper + 1]; // second pass: first chunk of each super-bin
-    uint32_t reserved;                   // elements holding a reserved key value (top of buffer B)
-    uint32_t deferred;                   // elements the region kernel left (bottom of buffer A)
+    uint32_t super_fill[kMaxSuper]; // elements offered to each super-bin (may pass its capacity)
+    uint32_t spilled;               // elements in the spill list
+    uint32_t deferred;              // elements the region kernel left (in buffer A once pass 2 is done)
 };
 
-// scratch layout: [region_ctl | region_count | region_start | region_cursor | A keys | A vals | B keys | B vals]
 struct region_scratch
 {
     region_ctl* ctl;
-    uint32_t* region_count; // S * kSubBins
-    uint32_t* region_start; // S * kSubBins + 1
-    uint32_t* region_cursor;
-    void* a_keys;
+    uint32_t* region_fill; // elements offered to each region (may pass its capacity)
+    void* a_keys;          // super-bin d: entries [d * cap1, d * cap1 + min(fill, cap1))
     void* a_vals;
-    void* b_keys;
+    void* b_keys;          // region r: entries [r * cap2, r * cap2 + min(fill, cap2))
     void* b_vals;
+    void* s_keys;          // spill list
+    void* s_vals;
+    uint32_t cap1, cap2;
 };
 
 template <typename K, typename M>
@@ -99,7 +98,7 @@ lds_exclusive_scan(uint32_t* a, uint32_t n, uint32_t* s_tmp)
     const uint32_t t = threadIdx.x;
     const uint32_t i0 = 2 * t, i1 = 2 * t + 1;
     const uint32_t x0 = i0 < n ? a[i0] : 0U, x1 = i1 < n ? a[i1] : 0U;
-    uint32_t v = x0 + x1;
+    const uint32_t v = x0 + x1;
     const int lane = lane_id();
     const int wid = static_cast<int>(t) >> 6;
     uint32_t inc = v;
@@ -134,78 +133,21 @@ lds_exclusive_scan(uint32_t* a, uint32_t n, uint32_t* s_tmp)
 // ------------------------------------------------------------------ bookkeeping reset (no memset)
 template <int = 0>
 __global__ __launch_bounds__(256) void
-k_region_reset(region_ctl* ctl, uint32_t* region_count, uint32_t nregion_slots)
+k_region_reset(region_ctl* ctl, uint32_t* region_fill, uint32_t nregion_slots)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < kMaxSuper) ctl->super_count[i] = 0;
+    if (i < kMaxSuper) ctl->super_fill[i] = 0;
     if (i == 0)
     {
-        ctl->reserved = 0;
+        ctl->spilled = 0;
         ctl->deferred = 0;
     }
-    for (uint32_t r = i; r < nregion_slots; r += gridDim.x * blockDim.x) region_count[r] = 0;
+    for (uint32_t r = i; r < nregion_slots; r += gridDim.x * blockDim.x) region_fill[r] = 0;
 }
 
-// ------------------------------------------------------------------ pass 1: by super-bin
-template <typename K, typename M>
-__global__ __launch_bounds__(kPartThreads) void
-k_part1_count(table_ref<K, M> t, const void* __restrict__ values, const void* __restrict__ keys,
-              const void* __restrict__ vals, int64_t n, uint32_t rb, uint32_t nsuper, region_ctl* ctl)
-{
-    __shared__ uint32_t hist[kMaxSuper];
-    for (uint32_t d = threadIdx.x; d < nsuper; d += kPartThreads) hist[d] = 0;
-    __syncthreads();
-    const int64_t base = static_cast<int64_t>(blockIdx.x) * kPartChunk;
-#pragma unroll 4
-    for (int j = 0; j < kPartItems; ++j)
-    {
-        const int64_t e = base + j * kPartThreads + threadIdx.x;
-        if (e < n)
-        {
-            K k;
-            uint64_t v;
-            fetch_element<K, M>(values, keys, vals, e, &k, &v);
-            if (!table_ref<K, M>::is_reserved(k))
-                atomicAdd(&hist[region_of(static_cast<uint64_t>(k), t.nbuckets, rb) / kSubBins], 1U);
-        }
-    }
-    __syncthreads();
-    for (uint32_t d = threadIdx.x; d < nsuper; d += kPartThreads)
-        if (hist[d]) atomicAdd(&ctl->super_count[d], hist[d]);
-}
-
-// one workgroup: super-bin offsets, and the second pass's chunk plan
-template <int = 0>
-__global__ __launch_bounds__(kPartThreads) void
-k_part1_scan(region_ctl* ctl, uint32_t nsuper)
-{
-    __shared__ uint32_t a[kMaxSuper];
-    __shared__ uint32_t c[kMaxSuper];
-    __shared__ uint32_t tmp[kPartThreads / 64 + 1];
-    for (uint32_t d = threadIdx.x; d < nsuper; d += kPartThreads)
-    {
-        a[d] = ctl->super_count[d];
-        c[d] = (ctl->super_count[d] + kPartChunk - 1) / kPartChunk;
-    }
-    __syncthreads();
-    const uint32_t total = lds_exclusive_scan(a, nsuper, tmp);
-    const uint32_t chunks = lds_exclusive_scan(c, nsuper, tmp);
-    for (uint32_t d = threadIdx.x; d < nsuper; d += kPartThreads)
-    {
-        ctl->super_start[d] = a[d];
-        ctl->super_cursor[d] = a[d];
-        ctl->chunk_start[d] = c[d];
-    }
-    if (threadIdx.x == 0)
-    {
-        ctl->super_start[nsuper] = total;
-        ctl->chunk_start[nsuper] = chunks;
-    }
-}
-
-// Shared body of both scatter passes: the chunk's elements (their digits already known) are ordered
-// by digit in LDS, each digit's run gets a range of the output by one atomic per digit, and the runs
-// leave with consecutive threads writing consecutive elements.
+// Shared body of both scatter passes: a chunk's elements are ordered by digit in LDS, each digit's run
+// reserves a range of its bin with one atomic, and the runs leave with consecutive threads writing
+// consecutive entries; the part of a run past its bin's capacity goes to the spill list.
 template <typename K, typename M, int NB>
 struct scatter_lds
 {
@@ -213,175 +155,62 @@ struct scatter_lds
     uint32_t pre[NB];
     uint32_t gbase[NB];
     uint32_t tmp[kPartThreads / 64 + 1];
+    uint32_t spill_base;
     K key[kPartChunk];
     std::conditional_t<slot_traits<K, M>::is_map, std::conditional_t<std::is_void_v<M>, char, M>, char>
             val[slot_traits<K, M>::is_map ? kPartChunk : 1];
 };
 
-// ------------------------------------------------------------------ pass 1 scatter
-template <typename K, typename M>
-__global__ __launch_bounds__(kPartThreads) void
-k_part1_scatter(table_ref<K, M> t, const void* __restrict__ values, const void* __restrict__ keys,
-                const void* __restrict__ vals, int64_t n, uint32_t rb, uint32_t nsuper, region_ctl* ctl,
-                region_scratch sc)
+// digit of a key in a pass: its super-bin (pass 1) or its region within the super-bin (pass 2)
+template <bool kFirst>
+__device__ __forceinline__ uint32_t
+pass_digit(uint64_t key, uint64_t nbuckets, uint32_t rb)
 {
-    __shared__ scatter_lds<K, M, kMaxSuper> s;
-    for (uint32_t d = threadIdx.x; d < nsuper; d += kPartThreads) s.hist[d] = 0;
+    const uint32_t r = region_of(key, nbuckets, rb);
+    return kFirst ? r / kSubBins : r % kSubBins;
+}
+
+// The thread's elements k / v where ok; bin (bin_base + digit) holds entries [bin * cap, bin * cap +
+// cap) of (out_keys, out_vals), and fill[digit] counts what was offered to it.
+template <typename K, typename M, bool kFirst>
+__device__ __forceinline__ void
+scatter_chunk(scatter_lds<K, M, kFirst ? kMaxSuper : kSubBins>& s, const K (&k)[kPartItems],
+              const uint64_t (&v)[kPartItems], const bool (&ok)[kPartItems], uint32_t ndigits, uint64_t nbuckets,
+              uint32_t rb, uint32_t* fill, uint64_t bin_base, uint32_t cap, void* out_keys, void* out_vals,
+              region_ctl* ctl, const region_scratch& sc)
+{
+    constexpr bool kMap = slot_traits<K, M>::is_map;
+    for (uint32_t d = threadIdx.x; d < ndigits; d += kPartThreads) s.hist[d] = 0;
+    if (threadIdx.x == 0) s.spill_base = 0;
     __syncthreads();
-    const int64_t base = static_cast<int64_t>(blockIdx.x) * kPartChunk;
-    K k[kPartItems];
-    uint64_t v[kPartItems];
     uint32_t dr[kPartItems]; // digit << 16 | rank within the chunk's run of that digit
 #pragma unroll
     for (int j = 0; j < kPartItems; ++j)
     {
-        const int64_t e = base + j * kPartThreads + threadIdx.x;
         dr[j] = ~0U;
-        if (e < n)
+        if (ok[j])
         {
-            fetch_element<K, M>(values, keys, vals, e, &k[j], &v[j]);
-            if (table_ref<K, M>::is_reserved(k[j]))
-            {
-                // the two reserved key values go to the per-key kernel, from the top of buffer B
-                const uint32_t p = atomicAdd(&ctl->reserved, 1U);
-                region_elem<K, M>::put(sc.b_keys, sc.b_vals, static_cast<uint64_t>(n) - 1 - p, k[j], v[j]);
-            }
-            else
-            {
-                const uint32_t d = region_of(static_cast<uint64_t>(k[j]), t.nbuckets, rb) / kSubBins;
-                dr[j] = (d << 16) | atomicAdd(&s.hist[d], 1U);
-            }
-        }
-    }
-    __syncthreads();
-    for (uint32_t d = threadIdx.x; d < nsuper; d += kPartThreads)
-    {
-        s.pre[d] = s.hist[d];
-        if (s.hist[d]) s.gbase[d] = atomicAdd(&ctl->super_cursor[d], s.hist[d]);
-    }
-    __syncthreads();
-    const uint32_t total = lds_exclusive_scan(s.pre, nsuper, s.tmp);
-#pragma unroll
-    for (int j = 0; j < kPartItems; ++j)
-    {
-        if (dr[j] != ~0U)
-        {
-            const uint32_t p = s.pre[dr[j] >> 16] + (dr[j] & 0xFFFFU);
-            s.key[p] = k[j];
-            if constexpr (slot_traits<K, M>::is_map) s.val[p] = static_cast<M>(v[j]);
-        }
-    }
-    __syncthreads();
-    for (uint32_t p = threadIdx.x; p < total; p += kPartThreads)
-    {
-        const K kk = s.key[p];
-        const uint32_t d = region_of(static_cast<uint64_t>(kk), t.nbuckets, rb) / kSubBins;
-        const uint64_t g = static_cast<uint64_t>(s.gbase[d]) + (p - s.pre[d]);
-        region_elem<K, M>::put(sc.a_keys, sc.a_vals, g, kk, slot_traits<K, M>::is_map ? static_cast<uint64_t>(s.val[p]) : 0);
-    }
-}
-
-// ------------------------------------------------------------------ pass 2: by region inside a super-bin
-// workgroup -> (super-bin, element range) from the chunk plan; false for workgroups past the plan
-__device__ __forceinline__ bool
-pass2_range(const region_ctl* ctl, uint32_t nsuper, uint32_t* seg, uint32_t* lo, uint32_t* hi)
-{
-    const uint32_t c = blockIdx.x;
-    if (c >= ctl->chunk_start[nsuper]) return false;
-    uint32_t a = 0, b = nsuper - 1; // last super-bin whose first chunk is <= c
-    while (a < b)
-    {
-        const uint32_t mid = (a + b + 1) >> 1;
-        if (ctl->chunk_start[mid] <= c)
-            a = mid;
-        else
-            b = mid - 1;
-    }
-    *seg = a;
-    *lo = ctl->super_start[a] + (c - ctl->chunk_start[a]) * static_cast<uint32_t>(kPartChunk);
-    const uint32_t end = ctl->super_start[a + 1];
-    *hi = (*lo + kPartChunk < end) ? *lo + kPartChunk : end;
-    return true;
-}
-
-template <typename K, typename M>
-__global__ __launch_bounds__(kPartThreads) void
-k_part2_count(table_ref<K, M> t, uint32_t rb, uint32_t nsuper, region_ctl* ctl, region_scratch sc)
-{
-    __shared__ uint32_t hist[kSubBins];
-    uint32_t seg = 0, lo = 0, hi = 0;
-    if (!pass2_range(ctl, nsuper, &seg, &lo, &hi)) return;
-    for (uint32_t d = threadIdx.x; d < kSubBins; d += kPartThreads) hist[d] = 0;
-    __syncthreads();
-    for (uint32_t e = lo + threadIdx.x; e < hi; e += kPartThreads)
-    {
-        const K k = static_cast<const K*>(sc.a_keys)[e];
-        atomicAdd(&hist[region_of(static_cast<uint64_t>(k), t.nbuckets, rb) % kSubBins], 1U);
-    }
-    __syncthreads();
-    for (uint32_t d = threadIdx.x; d < kSubBins; d += kPartThreads)
-        if (hist[d]) atomicAdd(&sc.region_count[seg * kSubBins + d], hist[d]);
-}
-
-// one workgroup of kSubBins threads per super-bin: region offsets
-template <int = 0>
-__global__ __launch_bounds__(kSubBins) void
-k_part2_scan(const region_ctl* ctl, uint32_t nsuper, region_scratch sc)
-{
-    __shared__ uint32_t tmp[kSubBins / 64 + 1];
-    const uint32_t seg = blockIdx.x;
-    const uint32_t r = seg * kSubBins + threadIdx.x;
-    const uint32_t c = sc.region_count[r];
-    const int lane = lane_id();
-    const int wid = static_cast<int>(threadIdx.x) >> 6;
-    uint32_t inc = c;
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1)
-    {
-        const uint32_t y = shfl32(inc, lane >= off ? lane - off : lane);
-        if (lane >= off) inc += y;
-    }
-    if (lane == 63) tmp[wid] = inc;
-    __syncthreads();
-    const uint32_t before = wid == 0 ? 0U : tmp[0];
-    const uint32_t start = ctl->super_start[seg] + before + inc - c;
-    sc.region_start[r] = start;
-    sc.region_cursor[r] = start;
-    if (seg == 0 && threadIdx.x == 0) sc.region_start[nsuper * kSubBins] = ctl->super_start[nsuper];
-}
-
-template <typename K, typename M>
-__global__ __launch_bounds__(kPartThreads) void
-k_part2_scatter(table_ref<K, M> t, uint32_t rb, uint32_t nsuper, region_ctl* ctl, region_scratch sc)
-{
-    __shared__ scatter_lds<K, M, kSubBins> s;
-    uint32_t seg = 0, lo = 0, hi = 0;
-    if (!pass2_range(ctl, nsuper, &seg, &lo, &hi)) return;
-    for (uint32_t d = threadIdx.x; d < kSubBins; d += kPartThreads) s.hist[d] = 0;
-    __syncthreads();
-    K k[kPartItems];
-    uint64_t v[kPartItems];
-    uint32_t dr[kPartItems];
-#pragma unroll
-    for (int j = 0; j < kPartItems; ++j)
-    {
-        const uint32_t e = lo + j * kPartThreads + threadIdx.x;
-        dr[j] = ~0U;
-        if (e < hi)
-        {
-            region_elem<K, M>::get(sc.a_keys, sc.a_vals, e, &k[j], &v[j]);
-            const uint32_t d = region_of(static_cast<uint64_t>(k[j]), t.nbuckets, rb) % kSubBins;
+            const uint32_t d = pass_digit<kFirst>(static_cast<uint64_t>(k[j]), nbuckets, rb);
             dr[j] = (d << 16) | atomicAdd(&s.hist[d], 1U);
         }
     }
     __syncthreads();
-    for (uint32_t d = threadIdx.x; d < kSubBins; d += kPartThreads)
+    // reserve each run in its bin; what passes the bin's capacity takes a range of the spill list
+    uint32_t spill_mine = 0;
+    for (uint32_t d = threadIdx.x; d < ndigits; d += kPartThreads)
     {
         s.pre[d] = s.hist[d];
-        if (s.hist[d]) s.gbase[d] = atomicAdd(&sc.region_cursor[seg * kSubBins + d], s.hist[d]);
+        if (s.hist[d])
+        {
+            const uint32_t g = atomicAdd(&fill[d], s.hist[d]);
+            s.gbase[d] = g;
+            if (g + s.hist[d] > cap) spill_mine += g >= cap ? s.hist[d] : g + s.hist[d] - cap;
+        }
     }
+    if (spill_mine) atomicAdd(&s.spill_base, spill_mine);
     __syncthreads();
-    const uint32_t total = lds_exclusive_scan(s.pre, kSubBins, s.tmp);
+    if (threadIdx.x == 0 && s.spill_base) s.spill_base = atomicAdd(&ctl->spilled, s.spill_base);
+    const uint32_t total = lds_exclusive_scan(s.pre, ndigits, s.tmp); // its barriers publish spill_base
 #pragma unroll
     for (int j = 0; j < kPartItems; ++j)
     {
@@ -389,17 +218,84 @@ k_part2_scatter(table_ref<K, M> t, uint32_t rb, uint32_t nsuper, region_ctl* ctl
         {
             const uint32_t p = s.pre[dr[j] >> 16] + (dr[j] & 0xFFFFU);
             s.key[p] = k[j];
-            if constexpr (slot_traits<K, M>::is_map) s.val[p] = static_cast<M>(v[j]);
+            if constexpr (kMap) s.val[p] = static_cast<M>(v[j]);
         }
     }
     __syncthreads();
     for (uint32_t p = threadIdx.x; p < total; p += kPartThreads)
     {
         const K kk = s.key[p];
-        const uint32_t d = region_of(static_cast<uint64_t>(kk), t.nbuckets, rb) % kSubBins;
-        const uint64_t g = static_cast<uint64_t>(s.gbase[d]) + (p - s.pre[d]);
-        region_elem<K, M>::put(sc.b_keys, sc.b_vals, g, kk, slot_traits<K, M>::is_map ? static_cast<uint64_t>(s.val[p]) : 0);
+        const uint32_t d = pass_digit<kFirst>(static_cast<uint64_t>(kk), nbuckets, rb);
+        const uint32_t pos = s.gbase[d] + (p - s.pre[d]);
+        const uint64_t vv = kMap ? static_cast<uint64_t>(s.val[p]) : 0;
+        if (pos < cap)
+            region_elem<K, M>::put(out_keys, out_vals, (bin_base + d) * cap + pos, kk, vv);
+        else
+            region_elem<K, M>::put(sc.s_keys, sc.s_vals, atomicAdd(&s.spill_base, 1U), kk, vv);
     }
+}
+
+// ------------------------------------------------------------------ pass 1: input -> super-bins
+template <typename K, typename M>
+__global__ __launch_bounds__(kPartThreads) void
+k_part1_scatter(table_ref<K, M> t, const void* __restrict__ values, const void* __restrict__ keys,
+                const void* __restrict__ vals, int64_t n, uint32_t rb, uint32_t nsuper, region_scratch sc)
+{
+    __shared__ scatter_lds<K, M, kMaxSuper> s;
+    const int64_t base = static_cast<int64_t>(blockIdx.x) * kPartChunk;
+    K k[kPartItems];
+    uint64_t v[kPartItems];
+    bool ok[kPartItems];
+#pragma unroll
+    for (int j = 0; j < kPartItems; ++j)
+    {
+        const int64_t e = base + j * kPartThreads + threadIdx.x;
+        ok[j] = e < n;
+        k[j] = empty_key<K>();
+        v[j] = 0;
+        if (ok[j]) fetch_element<K, M>(values, keys, vals, e, &k[j], &v[j]);
+    }
+#pragma unroll
+    for (int j = 0; j < kPartItems; ++j)
+    {
+        if (ok[j] && table_ref<K, M>::is_reserved(k[j]))
+        {
+            // the two reserved key values go to the per-key kernel through the spill list
+            region_elem<K, M>::put(sc.s_keys, sc.s_vals, atomicAdd(&sc.ctl->spilled, 1U), k[j], v[j]);
+            ok[j] = false;
+        }
+    }
+    scatter_chunk<K, M, true>(s, k, v, ok, nsuper, t.nbuckets, rb, sc.ctl->super_fill, 0, sc.cap1, sc.a_keys,
+                              sc.a_vals, sc.ctl, sc);
+}
+
+// ------------------------------------------------------------------ pass 2: super-bin -> regions
+// workgroup c takes chunk c % chunks_per_super of super-bin c / chunks_per_super
+template <typename K, typename M>
+__global__ __launch_bounds__(kPartThreads) void
+k_part2_scatter(table_ref<K, M> t, uint32_t rb, uint32_t chunks_per_super, region_scratch sc)
+{
+    __shared__ scatter_lds<K, M, kSubBins> s;
+    const uint32_t d = blockIdx.x / chunks_per_super;
+    const uint32_t c = blockIdx.x % chunks_per_super;
+    const uint32_t f = sc.ctl->super_fill[d];
+    const uint32_t count = f < sc.cap1 ? f : sc.cap1;
+    const uint32_t lo = c * static_cast<uint32_t>(kPartChunk);
+    if (lo >= count) return;
+    K k[kPartItems];
+    uint64_t v[kPartItems];
+    bool ok[kPartItems];
+#pragma unroll
+    for (int j = 0; j < kPartItems; ++j)
+    {
+        const uint32_t e = lo + j * kPartThreads + threadIdx.x;
+        ok[j] = e < count;
+        k[j] = empty_key<K>();
+        v[j] = 0;
+        if (ok[j]) region_elem<K, M>::get(sc.a_keys, sc.a_vals, static_cast<uint64_t>(d) * sc.cap1 + e, &k[j], &v[j]);
+    }
+    scatter_chunk<K, M, false>(s, k, v, ok, kSubBins, t.nbuckets, rb, sc.region_fill + d * kSubBins,
+                               static_cast<uint64_t>(d) * kSubBins, sc.cap2, sc.b_keys, sc.b_vals, sc.ctl, sc);
 }
 
 // ------------------------------------------------------------------ the region insert
@@ -431,19 +327,33 @@ enum class region_result
 // or ovf == 0) comparing keys and noting the first free slot in the key's claim order, then claim the
 // first free slot from there on by CAS, comparing keys on the way; a CAS lost to another key resumes
 // after that slot.  A sequence that leaves the region is deferred before anything is written.
-template <typename K, typename M, int W>
+// In LDS, the 16-byte pieces of bucket b are rotated by b (piece q at position (q + b) mod G): lanes
+// that read the same slot of different buckets then spread over the banks instead of all meeting in
+// the two banks a 128-byte stride maps one slot to.  The copy in and out applies and undoes it.
+template <int G>
+__device__ __forceinline__ uint32_t
+lds_piece(uint32_t b, uint32_t q)
+{
+    return b * G + ((q + b) & (G - 1));
+}
+
+template <typename K, typename M, int G, int W>
 __device__ __forceinline__ region_result
 region_insert_one(typename slot_traits<K, M>::value_type* lslots, uint32_t* lovf, uint32_t nb, uint32_t lb, uint32_t o,
                   K key, uint64_t val)
 {
     using value_type = typename slot_traits<K, M>::value_type;
     constexpr bool kInline = slot_traits<K, M>::ovf_in_bucket;
+    constexpr uint32_t kSeg = W < 16 ? W : 16; // keys of a bucket read together
+    constexpr uint32_t SPL = W / G;            // slots per 16-byte piece
+    char* lbytes = reinterpret_cast<char*>(lslots);
     auto kp = [&](uint32_t b, uint32_t j) -> K* {
-        return reinterpret_cast<K*>(lslots + b * W + ((o + j) & (W - 1)));
+        const uint32_t x = (o + j) & (W - 1);
+        return reinterpret_cast<K*>(lbytes + lds_piece<G>(b, x / SPL) * 16 + (x % SPL) * sizeof(value_type));
     };
     auto ovp = [&](uint32_t b) -> uint32_t* {
         if constexpr (kInline)
-            return reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(lslots + b * W) + slot_traits<K, M>::ovf_offset);
+            return reinterpret_cast<uint32_t*>(lbytes + lds_piece<G>(b, 0) * 16 + slot_traits<K, M>::ovf_offset);
         else
             return lovf + b;
     };
@@ -453,18 +363,27 @@ region_insert_one(typename slot_traits<K, M>::value_type* lslots, uint32_t* lovf
     while (true)
     {
         bool saw_empty = false;
-        for (uint32_t j = 0; j < W; ++j)
+        const uint32_t ov = lds_load(ovp(b));
+        for (uint32_t j0 = 0; j0 < W; j0 += kSeg)
         {
-            const K kk = lds_load(kp(b, j));
-            if (kk == key) return region_result::present;
-            if (fb < 0 && (kk == empty_key<K>() || kk == tomb_key<K>()))
+            K kk[kSeg];
+#pragma unroll
+            for (uint32_t j = 0; j < kSeg; ++j) kk[j] = lds_load(kp(b, j0 + j));
+            bool hit = false;
+#pragma unroll
+            for (uint32_t j = 0; j < kSeg; ++j)
             {
-                fb = static_cast<int32_t>(b);
-                fj = j;
+                hit |= kk[j] == key;
+                if (fb < 0 && (kk[j] == empty_key<K>() || kk[j] == tomb_key<K>()))
+                {
+                    fb = static_cast<int32_t>(b);
+                    fj = j0 + j;
+                }
+                saw_empty |= kk[j] == empty_key<K>();
             }
-            saw_empty |= kk == empty_key<K>();
+            if (hit) return region_result::present;
         }
-        if (saw_empty || lds_load(ovp(b)) == 0) break; // absent
+        if (saw_empty || ov == 0) break; // absent
         if (b + 1 >= nb) return region_result::deferred; // the sequence runs on into the next region
         ++b;
     }
@@ -529,50 +448,90 @@ struct region_lds<K, M, G, true>
 
 template <typename K, typename M, int G>
 __global__ __launch_bounds__(kRegionThreads) void
-k_region_insert(table_ref<K, M> t, region_ctl* ctl, region_scratch sc)
+k_region_insert(table_ref<K, M> t, region_scratch sc)
 {
     using L = region_lds<K, M, G>;
     using value_type = typename slot_traits<K, M>::value_type;
     constexpr uint32_t RB = L::RB;
     constexpr uint32_t W = L::W;
     constexpr bool kInline = slot_traits<K, M>::ovf_in_bucket;
+    constexpr uint32_t kPieces = static_cast<uint32_t>(kRegionBytes / 16 / kRegionThreads); // 16 B per thread each
     __shared__ L s;
     const uint32_t r = blockIdx.x;
-    const uint32_t lo = sc.region_start[r], hi = sc.region_start[r + 1];
-    if (lo == hi) return; // no key of this launch lands here: the region is not touched
+    const uint32_t f = sc.region_fill[r];
+    const uint32_t count = f < sc.cap2 ? f : sc.cap2;
+    if (count == 0) return; // no key of this launch lands here: the region is not touched
+    const uint64_t kbase = static_cast<uint64_t>(r) * sc.cap2;
     const uint64_t b0 = static_cast<uint64_t>(r) * RB;
     const uint32_t nb = static_cast<uint32_t>((t.nbuckets - b0) < RB ? (t.nbuckets - b0) : RB);
-    // region -> LDS (16-byte pieces, coalesced)
+    const uint32_t n16 = nb * W * static_cast<uint32_t>(sizeof(value_type)) / 16;
+    // the region and this thread's first keys are loaded together (all in flight at once); the keys
+    // then come through a ring of four rounds, each round's load issued four rounds ahead of its use
+    // (the region goes straight to LDS: each lane's 16 bytes land at its wavefront's base + lane * 16)
     const uint4* g = reinterpret_cast<const uint4*>(t.slots + b0 * W);
     uint4* l = reinterpret_cast<uint4*>(s.slots);
-    const uint32_t n16 = nb * W * static_cast<uint32_t>(sizeof(value_type)) / 16;
-    for (uint32_t i = threadIdx.x; i < n16; i += kRegionThreads) l[i] = g[i];
+    const uint32_t wbase = threadIdx.x & ~63U;
+#pragma unroll
+    for (uint32_t j = 0; j < kPieces; ++j)
+    {
+        const uint32_t p = j * kRegionThreads + threadIdx.x; // LDS position of this lane's piece
+        const uint32_t b = p / G, q = (p - (p / G) * G - b) & (G - 1);
+        if (p < n16) __builtin_amdgcn_global_load_lds(g + b * G + q, l + j * kRegionThreads + wbase, 16, 0, 0);
+    }
+    K k0 = 0, k1 = 0, k2 = 0, k3 = 0;
+    uint64_t v0 = 0, v1 = 0, v2 = 0, v3 = 0;
+    auto fetch = [&](uint32_t q, K* k, uint64_t* v) {
+        const uint32_t e = q * kRegionThreads + threadIdx.x;
+        if (e < count) region_elem<K, M>::get(sc.b_keys, sc.b_vals, kbase + e, k, v);
+    };
+    fetch(0, &k0, &v0);
+    fetch(1, &k1, &v1);
+    fetch(2, &k2, &v2);
+    fetch(3, &k3, &v3);
     uint32_t* lovf = s.ovf_array();
     if constexpr (!kInline)
         for (uint32_t i = threadIdx.x; i < nb; i += kRegionThreads) lovf[i] = t.ovf[b0 + i];
+    __builtin_amdgcn_s_waitcnt(0); // the direct loads count as vector memory operations
     __syncthreads();
     const bool ordered = t.nbuckets < kOrderedClaimBuckets;
     int64_t added = 0;
-    for (uint32_t e = lo + threadIdx.x; e < hi; e += kRegionThreads)
-    {
-        K k;
-        uint64_t v;
-        region_elem<K, M>::get(sc.b_keys, sc.b_vals, e, &k, &v);
-        const uint32_t lb = static_cast<uint32_t>(home_bucket(static_cast<uint64_t>(k), t.nbuckets) - b0);
+    auto one = [&](uint32_t q, K k, uint64_t v) {
+        const uint32_t e = q * kRegionThreads + threadIdx.x;
+        if (e >= count) return;
+        const uint64_t hb = home_bucket(static_cast<uint64_t>(k), t.nbuckets);
         const uint32_t o = ordered ? claim_offset(static_cast<uint64_t>(k), W) : 0U;
-        const region_result res = region_insert_one<K, M, W>(s.slots, lovf, nb, lb, o, k, v);
+        // every key of the region's bin is homed in the region; one that is not (never, by the
+        // partition) would go to the per-key kernel rather than address LDS outside the copy
+        const region_result res = (hb >= b0 && hb - b0 < nb)
+                                          ? region_insert_one<K, M, G, W>(s.slots, lovf, nb, static_cast<uint32_t>(hb - b0), o, k, v)
+                                          : region_result::deferred;
         if (res == region_result::inserted)
             ++added;
         else if (res == region_result::deferred)
-        {
-            const uint32_t p = atomicAdd(&ctl->deferred, 1U);
-            region_elem<K, M>::put(sc.a_keys, sc.a_vals, p, k, v);
-        }
+            region_elem<K, M>::put(sc.a_keys, sc.a_vals, atomicAdd(&sc.ctl->deferred, 1U), k, v);
+    };
+    const uint32_t rounds = (count + kRegionThreads - 1) / kRegionThreads;
+    for (uint32_t q = 0; q < rounds; q += 4)
+    {
+        one(q, k0, v0);
+        fetch(q + 4, &k0, &v0);
+        one(q + 1, k1, v1);
+        fetch(q + 5, &k1, &v1);
+        one(q + 2, k2, v2);
+        fetch(q + 6, &k2, &v2);
+        one(q + 3, k3, v3);
+        fetch(q + 7, &k3, &v3);
     }
     __syncthreads();
     // LDS -> region
     uint4* gw = reinterpret_cast<uint4*>(t.slots + b0 * W);
-    for (uint32_t i = threadIdx.x; i < n16; i += kRegionThreads) gw[i] = l[i];
+#pragma unroll
+    for (uint32_t j = 0; j < kPieces; ++j)
+    {
+        const uint32_t p = j * kRegionThreads + threadIdx.x;
+        const uint32_t b = p / G, q = (p - (p / G) * G - b) & (G - 1);
+        if (p < n16) gw[b * G + q] = l[p];
+    }
     if constexpr (!kInline)
         for (uint32_t i = threadIdx.x; i < nb; i += kRegionThreads) t.ovf[b0 + i] = lovf[i];
     // one size update per workgroup (the wave sums reuse the LDS copy once it has been written back)
@@ -593,20 +552,26 @@ k_region_insert(table_ref<K, M> t, region_ctl* ctl, region_scratch sc)
     }
 }
 
-// The per-key kernel over a list whose length lives in device memory: the deferred elements
-// (bottom of buffer A) or the reserved-key elements (top of buffer B, from_top).
+// The per-key kernel over a list whose length lives in device memory (the deferred keys, or the
+// spilled ones).  A short list (the deferred keys: those whose probe sequences run long) gives each
+// lane group one key, so no group walks several long sequences one after another; a long list (keys
+// a skewed input spilled, mostly copies of hot keys) takes one key per lane.
 template <typename K, typename M, int G, int R>
 __global__ __launch_bounds__(256) void
 k_insert_list(table_ref<K, M> t, const void* __restrict__ keys, const void* __restrict__ vals,
-              const uint32_t* __restrict__ count, int64_t n_total, int from_top)
+              const uint32_t* __restrict__ count)
 {
     const int64_t cnt = static_cast<int64_t>(*count);
-    const int64_t start = from_top ? n_total - cnt : 0;
-    const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
-    for (int64_t base = static_cast<int64_t>(blockIdx.x) * blockDim.x; base < cnt; base += stride)
+    const int64_t nthreads = static_cast<int64_t>(gridDim.x) * blockDim.x;
+    const bool sparse = cnt * G <= 4 * nthreads;
+    const int64_t per = sparse ? G : 1; // threads per key
+    const int64_t stride = nthreads / per;
+    const int lane = lane_id();
+    for (int64_t base = static_cast<int64_t>(blockIdx.x) * (blockDim.x / per); base < cnt; base += stride)
     {
-        const int64_t j = base + threadIdx.x;
-        insert_body<K, M, G, R>(t, nullptr, keys, vals, start + j, j < cnt, nullptr, false);
+        const int64_t j = base + static_cast<int64_t>(threadIdx.x) / per;
+        const bool mine = (sparse ? (lane & (G - 1)) == 0 : true) && j < cnt;
+        insert_body<K, M, G, R>(t, nullptr, keys, vals, j, mine, nullptr, false);
     }
 }
 

@@ -144,6 +144,21 @@ def test_partitioned_insert_near_full_and_reserved_keys(name):
     tw.close()
 
 
+@pytest.mark.parametrize("lanes,cap", [(1, 1 << 20), (8, 1 << 22)])
+def test_partitioned_insert_many_super_bins(lanes, cap):
+    """tables of 512 and 2048 regions (4 and 16 super-bins of the first pass), filled in two launches"""
+    rng = np.random.default_rng(lanes)
+    tw = Twin("map_u64_u32", cap, lanes)
+    first = np.unique(keys_of(rng, np.uint64, int(cap * 0.5)))
+    tw.insert(first)
+    tw.check(np.concatenate([first[:1000], keys_of(rng, np.uint64, 1000)]))
+    tw.erase(first[::4])
+    more = np.concatenate([first[::4], keys_of(rng, np.uint64, int(cap * 0.25))])
+    tw.insert(more)
+    tw.check(np.concatenate([more[:1000], first[:1000]]))
+    tw.close()
+
+
 def test_partitioned_insert_one_region_wraps():
     """a table of one region: keys homed at its last buckets run on to bucket 0"""
     rng = np.random.default_rng(9)
@@ -157,9 +172,9 @@ def test_partitioned_insert_one_region_wraps():
     tw.close()
 
 
-def test_default_path_full_size_sequence():
-    """2^26 keys at load 0.8 with the default choice (partitioned: insert and re-insert are large
-    for the table): size, valid(), every key found, erased keys absent, re-inserted found"""
+def test_partitioned_full_size_sequence():
+    """2^26 keys at load 0.8 through the partitioned path: size, valid(), every key found, erased
+    keys absent, re-inserted found"""
     from stdgpu_amd import workloads_device as wd
 
     n = 1 << 26
@@ -169,7 +184,7 @@ def test_default_path_full_size_sequence():
     erase = keys[perm[: n // 4]].contiguous()
     w = {"capacity": int(np.ceil(n / 0.8)), "keys": keys, "values": wd.value_of(keys), "erase": erase,
          "reinsert": erase, "reinsert_values": wd.value_of(erase)}
-    tab = sa.unordered_map_u64_u32.createDeviceObject(w["capacity"])
+    tab = sa.unordered_map_u64_u32.createDeviceObject(w["capacity"], sa.Tuning(insert_path=2))
     tab.insert(w["keys"], w["values"])
     assert tab.size() == n and tab.valid()
     hits = torch.empty(n, dtype=torch.uint8, device="cuda")
