@@ -134,6 +134,19 @@ int htab_range(htab* table, int64_t* idx_out, int64_t cap, int64_t* n_out, void*
  * destruction.  Synchronises the stream. */
 int htab_range_owned(htab* table, const int64_t** idx_out, int64_t* n_out, void* stream);
 
+/* Largest table htab_range_noalloc serves: HTAB_NOALLOC_RANGE_TILES * 4096 slots (134,213,632). */
+#define HTAB_NOALLOC_RANGE_TILES 32767
+
+/* device_range() that allocates nothing and does not synchronise.  No reference counterpart: for
+ * callers that cannot allocate (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  Writes the ascending slot indices of all stored entries to idx_out (cap
+ * entries, device memory) and their number to *n_out_dev (device memory) once the stream reaches
+ * the call; the caller synchronises when it needs them.  Scratch: a static array of
+ * HTAB_NOALLOC_RANGE_TILES + 1 int64 (256 KiB) per GPU, held until the process ends; a table with
+ * more than HTAB_NOALLOC_RANGE_TILES * 4096 slots is refused with HTAB_E_LIMIT before any device
+ * work.  Calls that overlap on one GPU share that array and race. */
+int htab_range_noalloc(htab* table, int64_t* idx_out, int64_t cap, int64_t* n_out_dev, void* stream);
+
 /* Contents as two dense arrays (keys, mapped values; vals_out NULL for sets) in slot order,
  * the data the reference's device_range() exposes.  Synchronises the stream. */
 int htab_contents(htab* table, void* keys_out, void* vals_out, int64_t cap, int64_t* n_out, void* stream);

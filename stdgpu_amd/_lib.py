@@ -18,6 +18,8 @@ HTAB_E_HIP = -3
 HTAB_E_NODEV = -4
 HTAB_E_LIMIT = -5
 
+HTAB_NOALLOC_RANGE_TILES = 32767  # htab.h: largest table htab_range_noalloc serves, in 4096-slot tiles
+
 HTAB_SET_U32 = 0
 HTAB_SET_U64 = 1
 HTAB_MAP_U64_U32 = 2
@@ -161,6 +163,7 @@ SIGNATURES = {
     "htab_clear": (_I32, [_P, _P]),
     "htab_range": (_I32, [_P, _P, _I64, ctypes.POINTER(_I64), _P]),
     "htab_range_owned": (_I32, [_P, ctypes.POINTER(_P), ctypes.POINTER(_I64), _P]),
+    "htab_range_noalloc": (_I32, [_P, _P, _I64, _P, _P]),
     "htab_contents": (_I32, [_P, _P, _P, _I64, ctypes.POINTER(_I64), _P]),
     "htab_valid": (_I32, [_P, ctypes.POINTER(ctypes.c_int32), _P]),
     "htab_bucket": (_I32, [_P, ctypes.c_uint64, ctypes.POINTER(_I64)]),

@@ -14,7 +14,7 @@ PKG = os.path.join(ROOT, "stdgpu_amd")
 CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(PKG, "libstdgpu_amd.so")
 ROOFLINE_LIB = os.path.join(PKG, "libstdgpu_amd_roofline.so")
-SOURCES = ["htab.hip", "deque.hip", "vector.hip", "bitset.hip", "atomic.hip", "server.hip", "kind_set_u32.hip", "kind_set_u64.hip", "kind_map_u64_u32.hip",
+SOURCES = ["htab.hip", "noalloc.hip", "deque.hip", "vector.hip", "bitset.hip", "atomic.hip", "server.hip", "kind_set_u32.hip", "kind_set_u64.hip", "kind_map_u64_u32.hip",
            "kind_map_u32_u32.hip", "kind_map_u64_u64.hip"]
 HEADERS = [
     os.path.join(ROOT, "include", "stdgpu_amd", "htab.h"),

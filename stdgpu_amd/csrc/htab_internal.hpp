@@ -134,6 +134,8 @@ struct htab_call
     const htab_dev::serve_args* serve = nullptr;       // served launch: segments and chunk plan
     const htab_dev::serve_tables* serve_tabs = nullptr; // served launch: its tables (size readback)
     int64_t* sizes_out = nullptr;                      // served launch: host-mapped table sizes
+    int64_t* scratch_ext = nullptr;                    // range: tile scratch given by the caller (no allocation)
+    int64_t* count_dev = nullptr;                      // range: entry count written to device memory (no sync)
     hipStream_t s = nullptr;
 };
 

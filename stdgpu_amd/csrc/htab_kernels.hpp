@@ -1366,6 +1366,14 @@ k_scan_tiles(int64_t* counts, int64_t ntiles)
     if (threadIdx.x == 0) counts[ntiles] = s_carry;
 }
 
+// the grand total of a range into caller device memory (the allocation-free device_range)
+template <int Unused = 0>
+__global__ __launch_bounds__(64) void
+k_copy_count(const int64_t* __restrict__ src, int64_t* __restrict__ dst)
+{
+    if (threadIdx.x == 0) *dst = *src;
+}
+
 template <typename K, typename M>
 __global__ __launch_bounds__(kRangeThreads) void
 k_range_scatter(table_ref<K, M> t, uint64_t total_slots, const int64_t* __restrict__ tile_offsets,

@@ -254,34 +254,47 @@ struct table_ops
         return launch_status("clear launch");
     }
 
+    // device_range (and contents).  scratch_ext, when given, holds the tile counts instead of the
+    // table's own scratch (nothing is allocated); count_dev, when given, receives the entry count on
+    // the device and the call returns without waiting for the stream.
     static int range(htab* h, int64_t* idx_out, void* keys_out, void* vals_out, int64_t cap, int64_t* n_out,
-                     hipStream_t s)
+                     hipStream_t s, int64_t* scratch_ext = nullptr, int64_t* count_dev = nullptr)
     {
         const T t = T::from_view(h->view);
         const uint64_t total = t.nslots() + 2;
         const uint64_t ntiles = (total + kRangeTile - 1) / kRangeTile;
-        const size_t need = (ntiles + 1) * sizeof(int64_t);
-        if (h->scratch_bytes < need)
+        int64_t* tiles = scratch_ext;
+        if (!tiles)
         {
-            if (h->scratch) (void)hipFree(h->scratch);
-            h->scratch = nullptr;
-            h->scratch_bytes = 0;
-            HTAB_HIP_TRY(hipMalloc(&h->scratch, need), "range scratch");
-            h->scratch_bytes = need;
+            const size_t need = (ntiles + 1) * sizeof(int64_t);
+            if (h->scratch_bytes < need)
+            {
+                if (h->scratch) (void)hipFree(h->scratch);
+                h->scratch = nullptr;
+                h->scratch_bytes = 0;
+                HTAB_HIP_TRY(hipMalloc(&h->scratch, need), "range scratch");
+                h->scratch_bytes = need;
+            }
+            tiles = h->scratch;
         }
         hipLaunchKernelGGL((k_range_count<K, M>), dim3(static_cast<uint32_t>(ntiles)), dim3(kRangeThreads), 0, s, t,
-                           total, h->scratch);
+                           total, tiles);
         int rc = launch_status("range count");
         if (rc) return rc;
-        hipLaunchKernelGGL(k_scan_tiles<>, dim3(1), dim3(1024), 0, s, h->scratch, static_cast<int64_t>(ntiles));
+        hipLaunchKernelGGL(k_scan_tiles<>, dim3(1), dim3(1024), 0, s, tiles, static_cast<int64_t>(ntiles));
         rc = launch_status("range scan");
         if (rc) return rc;
         hipLaunchKernelGGL((k_range_scatter<K, M>), dim3(static_cast<uint32_t>(ntiles)), dim3(kRangeThreads), 0, s, t,
-                           total, h->scratch, idx_out, static_cast<K*>(keys_out), vals_out, cap);
+                           total, tiles, idx_out, static_cast<K*>(keys_out), vals_out, cap);
         rc = launch_status("range scatter");
         if (rc) return rc;
+        if (count_dev)
+        {
+            hipLaunchKernelGGL(k_copy_count<>, dim3(1), dim3(64), 0, s, tiles + ntiles, count_dev);
+            return launch_status("range count store");
+        }
         int64_t n = 0;
-        HTAB_HIP_TRY(hipMemcpyAsync(&n, h->scratch + ntiles, sizeof(int64_t), hipMemcpyDeviceToHost, s), "range count copy");
+        HTAB_HIP_TRY(hipMemcpyAsync(&n, tiles + ntiles, sizeof(int64_t), hipMemcpyDeviceToHost, s), "range count copy");
         HTAB_HIP_TRY(hipStreamSynchronize(s), "range sync");
         *n_out = n;
         return HTAB_OK;
@@ -405,7 +418,7 @@ dispatch_kind(htab_call& c)
             return launch_status("served sizes");
         case call_op::clear: return table_ops<K, M>::clear(c.h, c.s);
         case call_op::range: return table_ops<K, M>::range(c.h, c.idx_out, const_cast<void*>(c.keys), c.vals_out, c.cap,
-                                                           c.n_out, c.s);
+                                                           c.n_out, c.s, c.scratch_ext, c.count_dev);
         case call_op::valid: return table_ops<K, M>::valid(c.h, c.valid_out, c.s);
         default: break;
     }
