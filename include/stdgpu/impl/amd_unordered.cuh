@@ -6,14 +6,27 @@
 // libstdgpu_amd.so and complete before returning; native device calls run the per-thread
 // probing of include/stdgpu_amd/htab_device.hpp on the same layout.
 //
-// Supported keys: 4- or 8-byte integers, and trivially copyable 4- or 8-byte types whose value is their
-// bits (unique object representations: structs of integers without padding, whose operator== compares
-// every member), compared with equal_to<Key> or the transparent equal_to<>; any hasher callable with a
-// Key; (maps) trivially copyable 4- or 8-byte mapped types, with pair<const Key, T> laid out as the
-// slot (an 8-byte struct key then needs 8-byte alignment).  The layout places keys by their bits
-// (htab_device.hpp), so the hasher only feeds bucket(), which keeps the reference's value for it; a
-// found entry is not at begin() + bucket(key).  With a transparent hasher and key_equal, find /
-// contains / count of an integer key also take an integer KeyLike (unordered_base.cuh:241-301).
+// Supported keys: integers of up to 8 bytes, and trivially copyable types of up to 8 bytes whose value
+// is their bits (unique object representations: structs of integers without padding, whose operator==
+// compares every member, such as the reference tests' 6-byte vec3int16), compared with equal_to<Key>
+// or the transparent equal_to<>; any hasher callable with a Key.  A key is stored in a 4- or 8-byte
+// slot key, zero-extended.  Mapped types: any trivially copyable type, empty ones included.
+//
+// Two storage forms:
+//   * aliased: value_type is laid out exactly as the table's slot (4- or 8-byte keys with 4- or 8-byte
+//     mapped types of the library's kinds, sets of 4- or 8-byte keys, and maps with an empty mapped
+//     type whose pair is 4 or 8 bytes, such as pair<const vec3int16, dummy>): iterators point into
+//     the slots;
+//   * payload: otherwise (sets of 6-byte keys, uint32 -> uint64 maps, 12-byte pairs, ...), the table
+//     keeps the keys in a set kind and a value_type record per slot in a parallel payload array
+//     (htab_tuning::payload_bytes); iterators point into the payload.
+// Placement: by the key's bits (htab_device.hpp), so the hasher only feeds bucket(), which keeps the
+// reference's value for it; a found entry is not at begin() + bucket(key).  With a transparent hasher
+// and key_equal, find / contains / count take a KeyLike (unordered_base.cuh:241-301): an integer
+// KeyLike of an integer key by conversion; for any other key the table is placed by the hasher's value
+// instead (htab_tuning::caller_hashes), since hash(k) == hash(KeyLike) is what ties a KeyLike to its
+// key.  Such tables, and payload tables, run their bulk insert / erase / find as per-element native
+// calls in kernels of this header, as the reference's bulk calls do (unordered_base_detail.cuh:996-1066).
 #ifndef STDGPU_AMD_FACADE_UNORDERED_CUH
 #define STDGPU_AMD_FACADE_UNORDERED_CUH
 
@@ -25,12 +38,42 @@ namespace stdgpu::detail
 template <std::size_t N>
 using storage_uint_t = std::conditional_t<N == 4, std::uint32_t, std::uint64_t>;
 
+// the slot key of a Key: 4 bytes for keys of up to 4 bytes, else 8
+template <typename Key>
+using slot_key_t = std::conditional_t<(sizeof(Key) <= 4), std::uint32_t, std::uint64_t>;
+
+// mapped types the engine stores in the slot itself (the library's map kinds): 4 or 8 trivially
+// copyable bytes, not uint32 -> uint64
+template <typename Key, typename Mapped>
+constexpr bool
+slot_mapped()
+{
+    if constexpr (std::is_void_v<Mapped>)
+        return false;
+    else if constexpr (std::is_empty_v<Mapped>)
+        return false;
+    else
+        return std::is_trivially_copyable_v<Mapped> && (sizeof(Mapped) == 4 || sizeof(Mapped) == 8) &&
+               sizeof(Key) == sizeof(slot_key_t<Key>) && !(sizeof(Key) == 4 && sizeof(Mapped) == 8);
+}
+
+template <typename Key, typename Mapped, bool kSlotMapped = slot_mapped<Key, Mapped>()>
+struct storage_mapped
+{
+    using type = void; // keys only in the slot (sets, empty or payload-held mapped values)
+};
+template <typename Key, typename Mapped>
+struct storage_mapped<Key, Mapped, true>
+{
+    using type = storage_uint_t<sizeof(Mapped)>;
+};
+
 template <typename Key, typename Mapped>
 constexpr int
 htab_kind()
 {
-    if constexpr (std::is_void_v<Mapped>)
-        return sizeof(Key) == 4 ? HTAB_SET_U32 : HTAB_SET_U64;
+    if constexpr (!slot_mapped<Key, Mapped>())
+        return sizeof(slot_key_t<Key>) == 4 ? HTAB_SET_U32 : HTAB_SET_U64;
     else if constexpr (sizeof(Key) == 8 && sizeof(Mapped) == 4)
         return HTAB_MAP_U64_U32;
     else if constexpr (sizeof(Key) == 4 && sizeof(Mapped) == 4)
@@ -39,16 +82,47 @@ htab_kind()
         return HTAB_MAP_U64_U64;
 }
 
-template <typename Mapped>
-struct storage_mapped
+template <typename Table, typename Value>
+__global__ void
+k_facade_insert(Table t, const Value* values, std::int64_t n)
 {
-    using type = storage_uint_t<sizeof(Mapped)>;
-};
-template <>
-struct storage_mapped<void>
+    const std::int64_t i = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) (void)t.insert(values[i]);
+}
+
+template <typename Table, typename Key>
+__global__ void
+k_facade_erase(Table t, const Key* keys, std::int64_t n)
 {
-    using type = void;
-};
+    const std::int64_t i = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) (void)t.erase(keys[i]);
+}
+
+template <typename Table, typename Key, typename Mapped>
+__global__ void
+k_facade_find(Table t, const Key* keys, std::int64_t n, std::uint8_t* hits, Mapped* values)
+{
+    const std::int64_t i = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const auto it = t.find(keys[i]);
+    const bool hit = it != t.end();
+    hits[i] = hit ? 1 : 0;
+    if constexpr (!std::is_void_v<Mapped>)
+    {
+        if (hit && values) values[i] = it->second;
+    }
+}
+
+// the caller's hash of every slot's key (0 for free slots), for valid() of a table placed by hashes
+template <typename Table>
+__global__ void
+k_facade_slot_hashes(Table t, std::uint64_t* out)
+{
+    const std::uint64_t n = t.eng().nslots();
+    const std::uint64_t stride = static_cast<std::uint64_t>(gridDim.x) * blockDim.x;
+    for (std::uint64_t s = static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; s < n; s += stride)
+        out[s] = t.slot_hash(s);
+}
 
 template <typename Key, typename Value, typename Mapped, typename Hash, typename KeyEqual, typename Allocator>
 class amd_unordered_base
@@ -70,19 +144,19 @@ public:
 
     static_assert((std::is_integral_v<Key> ||
                    (std::is_trivially_copyable_v<Key> && std::has_unique_object_representations_v<Key>)) &&
-                          (sizeof(Key) == 4 || sizeof(Key) == 8),
-                  "stdgpu (MI355X library): keys are 4- or 8-byte integers, or 4- or 8-byte trivially copyable "
-                  "types without padding bits");
+                          sizeof(Key) <= 8,
+                  "stdgpu (MI355X library): keys are integers, or trivially copyable types without padding "
+                  "bits, of at most 8 bytes");
     static constexpr bool mapped_ok()
     {
         if constexpr (std::is_void_v<Mapped>)
             return true;
         else
-            return std::is_trivially_copyable_v<Mapped> && (sizeof(Mapped) == 4 || sizeof(Mapped) == 8) &&
-                   !(sizeof(Key) == 4 && sizeof(Mapped) == 8);
+            return std::is_trivially_copyable_v<Mapped>;
     }
-    static_assert(mapped_ok(), "stdgpu (MI355X library): mapped types are trivially copyable, 4 or 8 bytes "
-                               "(uint32->uint64 maps are not provided)");
+    static_assert(mapped_ok(), "stdgpu (MI355X library): mapped types are trivially copyable");
+    static_assert(std::is_trivially_copyable_v<Value> || !std::is_void_v<Mapped>,
+                  "stdgpu (MI355X library): value types are trivially copyable");
     static_assert(std::is_same_v<KeyEqual, equal_to<Key>> || std::is_same_v<KeyEqual, equal_to<void>>,
                   "stdgpu (MI355X library): keys compare with stdgpu::equal_to<Key> or equal_to<>");
     static_assert(std::is_invocable_r_v<std::size_t, const Hash&, const Key&>,
@@ -96,17 +170,40 @@ public:
     struct is_transparent_t<H, std::void_t<typename H::is_transparent>> : std::true_type
     {
     };
-    // the KeyLike overloads exist for a transparent hasher and key_equal (unordered_base.cuh:241-301)
+    static constexpr bool kTransparent = is_transparent_t<Hash>::value && is_transparent_t<KeyEqual>::value;
+    // the KeyLike overloads exist for a transparent hasher and key_equal (unordered_base.cuh:241-301):
+    // an integer KeyLike of an integer key, or any KeyLike the hasher and key_equal take for a table
+    // placed by the hasher's value
     template <typename KeyLike>
-    static constexpr bool keylike_ok = is_transparent_t<Hash>::value && is_transparent_t<KeyEqual>::value &&
-                                       std::is_integral_v<Key> && std::is_integral_v<KeyLike> &&
+    static constexpr bool keylike_ok = kTransparent && std::is_integral_v<Key> && std::is_integral_v<KeyLike> &&
                                        !std::is_same_v<KeyLike, Key>;
+    // a table placed by the hasher's value: transparent lookups of a non-integer key
+    static constexpr bool kHashed = kTransparent && !std::is_integral_v<Key>;
+    template <typename KeyLike>
+    static constexpr bool keylike_hashed_ok =
+            kHashed && !std::is_same_v<std::remove_cv_t<KeyLike>, Key> &&
+            std::is_invocable_r_v<std::size_t, const Hash&, const KeyLike&> &&
+            std::is_invocable_r_v<bool, const KeyEqual&, const Key&, const KeyLike&>;
 
-    using skey = storage_uint_t<sizeof(Key)>;
-    using smapped = typename storage_mapped<Mapped>::type;
+    using skey = slot_key_t<Key>;
+    using smapped = typename storage_mapped<Key, Mapped>::type;
     using engine = htab_dev::table_ref<skey, smapped>;
     using slot_type = typename engine::value_type;
-    static_assert(sizeof(slot_type) == sizeof(value_type), "value_type layout differs from the slot layout");
+    // aliased: value_type is the slot (the key at offset 0, a slot-held mapped value at the slot's);
+    // otherwise each slot has a value_type record in the payload array
+    static constexpr bool aliased()
+    {
+        if constexpr (std::is_void_v<Mapped>)
+            return sizeof(Key) == sizeof(skey);
+        else if constexpr (std::is_void_v<smapped>)
+            return std::is_empty_v<Mapped> && sizeof(value_type) == sizeof(skey);
+        else
+            return sizeof(slot_type) == sizeof(value_type);
+    }
+    static constexpr bool kAliased = aliased();
+    static_assert(kAliased || sizeof(value_type) <= 256, "stdgpu (MI355X library): value_type of at most 256 bytes");
+    // bulk calls run as native calls in this header's kernels (tables placed by hashes, payload tables)
+    static constexpr bool kNativeBulk = kHashed || !kAliased;
 
     amd_unordered_base() = default;
 
@@ -115,7 +212,10 @@ public:
     {
         if (capacity <= 0) throw std::invalid_argument("stdgpu: capacity must be positive");
         htab* h = nullptr;
-        check(htab_create(htab_kind<Key, Mapped>(), capacity, nullptr, &h), "createDeviceObject");
+        htab_tuning tun{};
+        tun.caller_hashes = kHashed ? 1 : 0;
+        tun.payload_bytes = kAliased ? 0 : static_cast<std::int32_t>(sizeof(value_type));
+        check(htab_create(htab_kind<Key, Mapped>(), capacity, &tun, &h), "createDeviceObject");
         out._h = h;
         check(htab_get_view(h, &out._view), "createDeviceObject");
         out._allocator = allocator;
@@ -128,6 +228,7 @@ public:
         htab_info info{};
         check(htab_get_info(h, &info), "adopt");
         if (info.kind != htab_kind<Key, Mapped>()) throw std::invalid_argument("stdgpu: adopt: table of another kind");
+        if (kNativeBulk) throw std::invalid_argument("stdgpu: adopt: this key / value type needs its own table layout");
         out._h = h;
         check(htab_get_view(h, &out._view), "adopt");
         out._allocator = allocator;
@@ -141,17 +242,37 @@ public:
     }
 
     // ---------------------------------------------------------------- native (device) API
-    __device__ iterator begin() noexcept { return reinterpret_cast<iterator>(_view.slots); }
-    __device__ const_iterator begin() const noexcept { return reinterpret_cast<const_iterator>(_view.slots); }
+    __device__ iterator begin() noexcept { return entries(); }
+    __device__ const_iterator begin() const noexcept { return entries(); }
     __device__ const_iterator cbegin() const noexcept { return begin(); }
-    __device__ iterator end() noexcept { return reinterpret_cast<iterator>(eng().end_ptr()); }
-    __device__ const_iterator end() const noexcept { return reinterpret_cast<const_iterator>(eng().end_ptr()); }
+    __device__ iterator end() noexcept { return entry(eng().end_ptr()); }
+    __device__ const_iterator end() const noexcept { return entry(eng().end_ptr()); }
     __device__ const_iterator cend() const noexcept { return end(); }
 
-    __device__ iterator find(const key_type& key) { return reinterpret_cast<iterator>(eng().find(to_skey(key))); }
-    __device__ const_iterator find(const key_type& key) const
+    __device__ iterator find(const key_type& key) { return entry(find_slot(key)); }
+    __device__ const_iterator find(const key_type& key) const { return entry(find_slot(key)); }
+
+    // transparent lookups of a table placed by the hasher's value: the KeyLike's hash gives the home
+    // bucket, and key_equal compares it with each stored key on the probe sequence
+    template <typename KeyLike, std::enable_if_t<keylike_hashed_ok<KeyLike>, int> = 0>
+    __device__ iterator find(const KeyLike& k)
     {
-        return reinterpret_cast<const_iterator>(eng().find(to_skey(key)));
+        return entry(find_like(k));
+    }
+    template <typename KeyLike, std::enable_if_t<keylike_hashed_ok<KeyLike>, int> = 0>
+    __device__ const_iterator find(const KeyLike& k) const
+    {
+        return entry(find_like(k));
+    }
+    template <typename KeyLike, std::enable_if_t<keylike_hashed_ok<KeyLike>, int> = 0>
+    __device__ bool contains(const KeyLike& k) const
+    {
+        return find(k) != end();
+    }
+    template <typename KeyLike, std::enable_if_t<keylike_hashed_ok<KeyLike>, int> = 0>
+    __device__ index_type count(const KeyLike& k) const
+    {
+        return contains(k) ? 1 : 0;
     }
     __device__ bool contains(const key_type& key) const { return find(key) != end(); }
     __device__ index_type count(const key_type& key) const { return contains(key) ? 1 : 0; }
@@ -184,18 +305,20 @@ public:
     __device__ pair<iterator, bool> insert(const value_type& value)
     {
         bool ins = false;
-        slot_type* p = eng().insert(
-                to_skey(key_of(value)),
-                [&value](slot_type* slot) {
-                    if constexpr (!std::is_void_v<Mapped>)
-                    {
-                        smapped bits;
-                        __builtin_memcpy(&bits, &value.second, sizeof(bits));
-                        slot->second = bits;
-                    }
-                },
-                &ins);
-        return pair<iterator, bool>(reinterpret_cast<iterator>(p), ins);
+        const engine e = eng();
+        const auto write = [&](slot_type* slot) {
+            if constexpr (!kAliased)
+                __builtin_memcpy(static_cast<void*>(entry(slot)), &value, sizeof(value_type));
+            else if constexpr (!std::is_void_v<smapped>)
+            {
+                smapped bits;
+                __builtin_memcpy(&bits, &value.second, sizeof(bits));
+                slot->second = bits;
+            }
+        };
+        const skey k = to_skey(key_of(value));
+        slot_type* p = kHashed ? e.insert_at(k, home_of(key_of(value)), write, &ins) : e.insert(k, write, &ins);
+        return pair<iterator, bool>(entry(p), ins);
     }
 
     template <class... Args>
@@ -204,7 +327,12 @@ public:
         return insert(value_type(std::forward<Args>(args)...));
     }
 
-    __device__ index_type erase(const key_type& key) { return eng().erase(to_skey(key)) ? 1 : 0; }
+    __device__ index_type erase(const key_type& key)
+    {
+        const engine e = eng();
+        const bool done = kHashed ? e.erase_at(to_skey(key), home_of(key)) : e.erase(to_skey(key));
+        return done ? 1 : 0;
+    }
 
     // bucket(): the reference's Fibonacci bucket over bucket_count() (unordered_base_detail.cuh:547-564)
     __host__ __device__ index_type bucket(const key_type& key) const
@@ -249,7 +377,15 @@ public:
     void insert(ExecutionPolicy&& policy, ValueIterator first, ValueIterator last)
     {
         const std::int64_t n = static_cast<std::int64_t>(last - first);
-        check(htab_insert(_h, raw(first), n, nullptr, policy.stream), "insert");
+        if constexpr (kNativeBulk)
+        {
+            if (n > 0)
+                hipLaunchKernelGGL((k_facade_insert<amd_unordered_base, value_type>), dim3(grid_of(n)), dim3(256), 0,
+                                   policy.stream, *this, static_cast<const value_type*>(raw(first)), n);
+            check(static_cast<int>(hipGetLastError()) == 0 ? HTAB_OK : HTAB_E_HIP, "insert launch");
+        }
+        else
+            check(htab_insert(_h, raw(first), n, nullptr, policy.stream), "insert");
         check(static_cast<int>(hipStreamSynchronize(policy.stream)) == 0 ? HTAB_OK : HTAB_E_HIP, "insert sync");
     }
 
@@ -264,7 +400,15 @@ public:
     void erase(ExecutionPolicy&& policy, KeyIterator first, KeyIterator last)
     {
         const std::int64_t n = static_cast<std::int64_t>(last - first);
-        check(htab_erase(_h, raw(first), n, nullptr, policy.stream), "erase");
+        if constexpr (kNativeBulk)
+        {
+            if (n > 0)
+                hipLaunchKernelGGL((k_facade_erase<amd_unordered_base, key_type>), dim3(grid_of(n)), dim3(256), 0,
+                                   policy.stream, *this, static_cast<const key_type*>(raw(first)), n);
+            check(static_cast<int>(hipGetLastError()) == 0 ? HTAB_OK : HTAB_E_HIP, "erase launch");
+        }
+        else
+            check(htab_erase(_h, raw(first), n, nullptr, policy.stream), "erase");
         check(static_cast<int>(hipStreamSynchronize(policy.stream)) == 0 ? HTAB_OK : HTAB_E_HIP, "erase sync");
     }
 
@@ -305,7 +449,7 @@ public:
         const std::int64_t* idx = nullptr;
         std::int64_t n = 0;
         if (_h) check(htab_range_owned(_h, &idx, &n, policy.stream), "device_range");
-        return device_indexed_range<value_type>(idx, static_cast<index_t>(n), reinterpret_cast<value_type*>(_view.slots));
+        return device_indexed_range<value_type>(idx, static_cast<index_t>(n), entries_host());
     }
     device_indexed_range<const value_type> device_range() const { return device_range(execution::device); }
     template <typename ExecutionPolicy, std::enable_if_t<is_execution_policy_v<ExecutionPolicy>, int> = 0>
@@ -314,8 +458,7 @@ public:
         const std::int64_t* idx = nullptr;
         std::int64_t n = 0;
         if (_h) check(htab_range_owned(_h, &idx, &n, policy.stream), "device_range");
-        return device_indexed_range<const value_type>(idx, static_cast<index_t>(n),
-                                                      reinterpret_cast<const value_type*>(_view.slots));
+        return device_indexed_range<const value_type>(idx, static_cast<index_t>(n), entries_host());
     }
 
     void clear() { clear(execution::device); }
@@ -333,7 +476,18 @@ public:
     {
         if (!_h) return true; // an empty (never created) container is valid
         std::int32_t ok = 0;
-        check(htab_valid(_h, &ok, policy.stream), "valid");
+        if constexpr (kHashed)
+        {
+            // the hasher's value of every stored key, then the engine's checks against those
+            std::uint64_t* hashes = nullptr;
+            check(htab_hash_buffer(_h, &hashes), "valid");
+            hipLaunchKernelGGL((k_facade_slot_hashes<amd_unordered_base>), dim3(1024), dim3(256), 0, policy.stream, *this,
+                               hashes);
+            check(static_cast<int>(hipGetLastError()) == 0 ? HTAB_OK : HTAB_E_HIP, "valid launch");
+            check(htab_valid_hashed(_h, hashes, &ok, policy.stream), "valid");
+        }
+        else
+            check(htab_valid(_h, &ok, policy.stream), "valid");
         return ok != 0;
     }
 
@@ -386,8 +540,53 @@ public:
     htab* native_handle() const { return _h; }
     __host__ __device__ const htab_view& native_view() const { return _view; }
 
-protected:
     __host__ __device__ engine eng() const { return engine::from_view(_view); }
+
+    // the caller's hash of the key in slot s (0 for a free slot): what a table placed by hashes was
+    // placed by (valid())
+    __device__ std::uint64_t slot_hash(std::uint64_t s) const
+    {
+        const skey k = *eng().key_ptr(s);
+        if (k == htab_dev::empty_key<skey>() || k == htab_dev::tomb_key<skey>()) return 0;
+        return static_cast<std::uint64_t>(hasher{}(from_skey(k)));
+    }
+
+protected:
+    static unsigned grid_of(std::int64_t n) { return static_cast<unsigned>((n + 255) / 256); }
+
+    // home bucket of a key in a table placed by the hasher's value
+    __device__ std::uint64_t home_of(const key_type& key) const
+    {
+        return eng().home_of_hash(static_cast<std::uint64_t>(hasher{}(key)));
+    }
+
+    __device__ slot_type* find_slot(const key_type& key) const
+    {
+        const engine e = eng();
+        return kHashed ? e.find_at(to_skey(key), home_of(key)) : e.find(to_skey(key));
+    }
+
+    template <typename KeyLike>
+    __device__ slot_type* find_like(const KeyLike& k) const
+    {
+        const engine e = eng();
+        return e.find_match(e.home_of_hash(static_cast<std::uint64_t>(hasher{}(k))),
+                            [&k](skey bits) { return key_equal{}(from_skey(bits), k); });
+    }
+
+    // the entry of a slot: the slot itself (aliased) or its payload record
+    __host__ __device__ value_type* entries() const
+    {
+        if constexpr (kAliased)
+            return reinterpret_cast<value_type*>(_view.slots);
+        else
+            return static_cast<value_type*>(_view.payload);
+    }
+    __host__ value_type* entries_host() const { return entries(); }
+    __host__ __device__ value_type* entry(const slot_type* p) const
+    {
+        return entries() + (p - static_cast<const slot_type*>(_view.slots));
+    }
 
     template <typename KeyIterator>
     void bulk_find(hipStream_t s, KeyIterator first, KeyIterator last, std::uint8_t* hits, void* values,
@@ -395,7 +594,16 @@ protected:
     {
         const std::int64_t n = static_cast<std::int64_t>(last - first);
         if (n > 0 && !hits) throw std::invalid_argument(std::string("stdgpu::") + what + ": hits is null");
-        check(htab_find(_h, raw(first), n, hits, values, s), what);
+        if constexpr (kNativeBulk)
+        {
+            using M = std::conditional_t<std::is_void_v<Mapped>, void, std::remove_cv_t<Mapped>>;
+            if (n > 0)
+                hipLaunchKernelGGL((k_facade_find<amd_unordered_base, key_type, M>), dim3(grid_of(n)), dim3(256), 0, s,
+                                   *this, static_cast<const key_type*>(raw(first)), n, hits, static_cast<M*>(values));
+            check(static_cast<int>(hipGetLastError()) == 0 ? HTAB_OK : HTAB_E_HIP, what);
+        }
+        else
+            check(htab_find(_h, raw(first), n, hits, values, s), what);
         check(static_cast<int>(hipStreamSynchronize(s)) == 0 ? HTAB_OK : HTAB_E_HIP, what);
     }
 
@@ -407,13 +615,20 @@ protected:
             return v.first;
     }
 
-    // the stored bits of a key: integers by value (sign bits included), other keys as they lie in memory
+    // the stored bits of a key: integers by value (sign bits included), other keys as they lie in
+    // memory, zero-extended to the slot key
     __host__ __device__ static skey to_skey(const Key& k)
     {
         if constexpr (std::is_integral_v<Key>)
             return static_cast<skey>(static_cast<std::make_unsigned_t<Key>>(k));
-        else
+        else if constexpr (sizeof(Key) == sizeof(skey))
             return __builtin_bit_cast(skey, k);
+        else
+        {
+            skey bits = 0;
+            __builtin_memcpy(&bits, &k, sizeof(Key));
+            return bits;
+        }
     }
 
     // a key from its stored bits (the inverse of to_skey)
@@ -421,8 +636,14 @@ protected:
     {
         if constexpr (std::is_integral_v<Key>)
             return static_cast<Key>(static_cast<std::make_unsigned_t<Key>>(k));
-        else
+        else if constexpr (sizeof(Key) == sizeof(skey))
             return __builtin_bit_cast(Key, k);
+        else
+        {
+            alignas(Key) unsigned char raw_key[sizeof(Key)];
+            __builtin_memcpy(raw_key, &k, sizeof(Key));
+            return *reinterpret_cast<const Key*>(raw_key);
+        }
     }
 
     htab* _h = nullptr;

@@ -54,6 +54,14 @@ typedef struct htab_tuning
                                 * slot count: 0 (default) and 1 = one compare-and-swap per key,
                                 * 2 = keys partitioned by table region and each region inserted in
                                 * LDS (tables of at most 65536 regions of 32 KB) */
+    int32_t caller_hashes;     /* nonzero: keys are placed by a 64-bit hash their caller computes (the C++
+                                * facade's hasher, for transparent lookups of struct keys), so only the
+                                * native per-thread calls (which take that hash) and htab_valid_hashed
+                                * work on the table; the bulk insert / find / erase / assign calls, htab_valid
+                                * and the batched and served forms refuse it */
+    int32_t payload_bytes;     /* 0, or 1..256: bytes of a caller-owned record per slot, kept in a second
+                                * array of (physical_slots + 2) records parallel to the slots (the facade
+                                * stores a value_type there when it cannot alias the slot layout) */
 } htab_tuning;
 
 /* Sizes a caller can query.  bucket_count / max_size follow the reference formulas so
@@ -86,6 +94,8 @@ typedef struct htab_view
     int64_t ref_max_size;      /* reference max_size() */
     uint32_t* ovf;             /* per bucket: stored keys whose probe sequence runs through it; NULL
                                   when the count lives in the padding of each bucket's first slot */
+    void* payload;             /* (physical_slots + 2) records of payload_bytes parallel to the slots, or NULL */
+    int64_t payload_bytes;     /* htab_tuning::payload_bytes */
 } htab_view;
 
 /* unordered_map::createDeviceObject / unordered_set::createDeviceObject
@@ -116,6 +126,16 @@ int htab_erase(htab* table, const void* keys, int64_t n, uint8_t* erased, void* 
  * find(key)->second = value (unordered_map.cuh:288-317) over a range.  Maps only; each key at most
  * once per call.  found[i] (optional) = 1 iff keys[i] was present. */
 int htab_assign(htab* table, const void* keys, const void* vals, int64_t n, uint8_t* found, void* stream);
+
+/* valid() of a table made with caller_hashes (unordered_base_detail.cuh:1192-1218): hashes[s] is the
+ * caller's hash of the key stored in slot s, for every slot s < physical_slots that holds a key (other
+ * entries are not read), typically written by the caller into htab_hash_buffer's array.  Also accepted
+ * for a table without caller hashes, whose hash of a key is then its bits. */
+int htab_valid_hashed(htab* table, const uint64_t* hashes, int32_t* out, void* stream);
+
+/* A device array of physical_slots + 2 uint64 owned by the table (allocated on the first call, freed
+ * with the table), for the per-slot hashes htab_valid_hashed reads.  No reference counterpart. */
+int htab_hash_buffer(htab* table, uint64_t** out);
 
 /* size() (unordered_base_detail.cuh:1110-1132): synchronises the stream, then reads the count. */
 int htab_size(htab* table, int64_t* out, void* stream);
@@ -251,12 +271,13 @@ int htab_server_replay_plain(htab_server* server, const htab_segment* requests, 
  * Byte layout, little-endian: htab_snapshot_header, then the slot area
  * ((physical_buckets * slots per bucket + 2) * value_bytes bytes: buckets in order, the two side slots
  * of the reserved key values last), then the overflow-count array (physical_buckets uint32, present
- * when ovf_bytes > 0).  A restored table has the snapshot's geometry, contents and size.
+ * when ovf_bytes > 0), then the payload records (present when payload_bytes > 0).  A restored table
+ * has the snapshot's geometry, options, contents and size.
  * ------------------------------------------------------------------------------------------ */
 typedef struct htab_snapshot_header
 {
     char magic[8];            /* "HTABSNP1" */
-    int32_t version;          /* 1 */
+    int32_t version;          /* 2 (1: before the away counters; restored only for layouts without them) */
     int32_t kind;
     int32_t bucket_lanes;
     int32_t block_size;
@@ -267,6 +288,9 @@ typedef struct htab_snapshot_header
     int64_t size;
     int64_t slot_bytes;
     int64_t ovf_bytes;
+    int64_t payload_bytes;    /* bytes of the payload area after the counts (version 2; 0 without one) */
+    int32_t record_bytes;     /* htab_tuning::payload_bytes (version 2) */
+    int32_t caller_hashes;    /* htab_tuning::caller_hashes (version 2) */
 } htab_snapshot_header;
 
 /* bytes a snapshot of the table takes (header included) */

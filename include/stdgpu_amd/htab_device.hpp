@@ -19,6 +19,12 @@
 //     so the EMPTY test needs no extra load, and TOMBs never lengthen a probe sequence through
 //     buckets that no key overflows.
 //   * Home bucket = (hi32(mix64(key)) * nbuckets) >> 32, mix64 = MurmurHash3's fmix64.
+//   * Away counts (in-bucket layouts with W > 1): the padding words of slots 1..W-1 of bucket h hold
+//     2(W-1) 16-bit counters; counter f counts the stored keys whose home bucket is h, whose
+//     fingerprint (away_index) is f, and that are stored in a later bucket.  A key that is not in its
+//     home bucket and whose counter there is 0 is absent, so most probes of absent keys (finds that
+//     miss, inserts of new keys into a home bucket with ovf > 0) end at the home bucket although
+//     ovf > 0.  A counter at or above kAwaySaturated is never lowered again (it only says "maybe").
 #ifndef STDGPU_AMD_HTAB_DEVICE_HPP
 #define STDGPU_AMD_HTAB_DEVICE_HPP
 
@@ -103,6 +109,24 @@ home_bucket(uint64_t key, uint64_t nbuckets)
     const uint64_t h = mix64(key) >> 32;
     return (h * nbuckets) >> 32;
 }
+
+// counter index of a key among the `nctr` (at most 30) away counters of its home bucket: a 32-bit
+// multiplicative hash of the key's two halves folded together, independent of mix64, which places the
+// key (home bucket, claim offset), and cheap enough to recompute in registers wherever it is needed
+__host__ __device__ inline uint32_t
+away_index(uint64_t key, uint32_t nctr)
+{
+    const uint32_t x = (static_cast<uint32_t>(key) ^ static_cast<uint32_t>(key >> 32)) * 0x9E3779B1U;
+    return ((x >> 8) * nctr) >> 24;
+}
+
+// A counter at or above kAwaySaturated is never lowered: the keys it stands for are no longer all
+// counted.  The bulk insert raises a counter with a plain atomic add (no wait for the result) when
+// it saw it below kAwayFastLimit at the start of the launch, else with a compare-and-swap that stops
+// at kAwaySaturated; a field can only carry into its neighbour after more than 0xC000 plain adds to
+// one counter in one launch, i.e. that many keys of one home bucket and one fingerprint stored away.
+constexpr uint32_t kAwaySaturated = 0x8000U;
+constexpr uint32_t kAwayFastLimit = 0x4000U;
 
 __host__ __device__ inline uint64_t
 next_bucket(uint64_t b, uint64_t nbuckets)
@@ -238,6 +262,37 @@ struct table_ref
     // buckets from h up to (not including) f, cyclically
     __device__ uint64_t distance(uint64_t h, uint64_t f) const { return f >= h ? f - h : f + nbuckets - h; }
 
+    // ---- away counts (see the layout notes at the top of this file); 0 counters without them
+    __host__ __device__ uint32_t away_counters() const { return traits::ovf_in_bucket ? 2U * (W - 1) : 0U; }
+    __device__ uint32_t away_slot(K key) const { return away_index(static_cast<uint64_t>(key), away_counters()); }
+    __device__ uint32_t* away_word(uint64_t h, uint32_t f) const
+    {
+        return reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(slots + h * W + 1 + f / 2) + traits::ovf_offset);
+    }
+    // counter f of bucket h (coherent read)
+    __device__ uint32_t away(uint64_t h, uint32_t f) const { return (load_agent(away_word(h, f)) >> (16 * (f & 1))) & 0xFFFFU; }
+    // counter f of bucket h += d (d = +1 or -1) by compare-and-swap: a raise stops at kAwaySaturated,
+    // a saturated counter is not lowered, and no neighbouring counter ever sees a carry or a borrow
+    __device__ void away_add(uint64_t h, uint32_t f, int d) const
+    {
+        uint32_t* w = away_word(h, f);
+        const uint32_t sh = 16 * (f & 1);
+        uint32_t old = load_agent(w);
+        while (((old >> sh) & 0xFFFFU) < kAwaySaturated)
+        {
+            const uint32_t want = d > 0 ? old + (1U << sh) : old - (1U << sh);
+            const uint32_t seen = cas_agent(w, old, want);
+            if (seen == old) break;
+            old = seen;
+        }
+    }
+    // plain atomic raise / lower of counter f (the bulk kernels, under the rules at kAwaySaturated)
+    __device__ void away_add_fast(uint64_t h, uint32_t f, int d) const
+    {
+        const uint32_t inc = 1U << (16 * (f & 1));
+        __hip_atomic_fetch_add(away_word(h, f), d > 0 ? inc : 0U - inc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+
     // ---- native per-thread operations (user kernels; the facade's find/insert/erase) ----
 
     // where a key's scan of a bucket starts: its claim order (claim_offset); slot j of the scan is
@@ -248,13 +303,20 @@ struct table_ref
     }
 
     // find (unordered_base_detail.cuh:653-682): pointer to the entry or end_ptr().
-    __device__ value_type* find(K key) const
+    __device__ value_type* find(K key) const { return find_at(key, home_bucket(static_cast<uint64_t>(key), nbuckets)); }
+
+    // home bucket of a key addressed by a caller's hash instead of its bits (home_of_hash(key bits) is
+    // the home bucket of the bits, so integer keys hashed by identity land where the bulk kernels put them)
+    __host__ __device__ uint64_t home_of_hash(uint64_t hash) const { return home_bucket(hash, nbuckets); }
+
+    // find() with the home bucket given (a table addressed by caller hashes)
+    __device__ value_type* find_at(K key, uint64_t home) const
     {
         if (is_reserved(key))
         {
             return (load_agent(flags) & side_bit(key)) ? slots + side_index(key) : end_ptr();
         }
-        uint64_t b = home_bucket(static_cast<uint64_t>(key), nbuckets);
+        uint64_t b = home;
         const uint32_t o = scan_start(key);
         for (uint64_t step = 0; step < nbuckets; ++step)
         {
@@ -266,6 +328,38 @@ struct table_ref
                 const K k = load_agent(key_ptr(b * W + x));
                 if (k == key) return base + x;
                 saw_empty |= (k == empty_key<K>());
+            }
+            if (saw_empty || overflow(b) == 0) break;
+            // not in its home bucket and no key of that bucket with its fingerprint stored further on
+            if (step == 0 && away_counters() != 0 && away(b, away_slot(key)) == 0) break;
+            b = next_bucket(b, nbuckets);
+        }
+        return end_ptr();
+    }
+
+    // The stored entry whose key bits satisfy `match`, looked up from home bucket `home` (a transparent
+    // lookup: a KeyLike compares with stored keys through the caller's key_equal, so neither the key's
+    // bits nor its fingerprint are known, and the probe sequence is read to its end).  At most one
+    // stored key may match (the caller's key_equal is consistent with its hasher).
+    template <typename Match>
+    __device__ value_type* find_match(uint64_t home, const Match& match) const
+    {
+        const uint32_t f = load_agent(flags);
+        if ((f & 1U) && match(empty_key<K>())) return slots + side_index(empty_key<K>());
+        if ((f & 2U) && match(tomb_key<K>())) return slots + side_index(tomb_key<K>());
+        uint64_t b = home;
+        for (uint64_t step = 0; step < nbuckets; ++step)
+        {
+            bool saw_empty = false;
+            for (uint32_t j = 0; j < W; ++j)
+            {
+                const K k = load_agent(key_ptr(b * W + j));
+                if (k == empty_key<K>())
+                {
+                    saw_empty = true;
+                    continue;
+                }
+                if (k != tomb_key<K>() && match(k)) return slots + b * W + j;
             }
             if (saw_empty || overflow(b) == 0) break;
             b = next_bucket(b, nbuckets);
@@ -344,6 +438,13 @@ struct table_ref
     template <typename WriteValue>
     __device__ value_type* insert(K key, const WriteValue& write_value, bool* inserted) const
     {
+        return insert_at(key, home_bucket(static_cast<uint64_t>(key), nbuckets), write_value, inserted);
+    }
+
+    // insert() with the home bucket given (a table addressed by caller hashes)
+    template <typename WriteValue>
+    __device__ value_type* insert_at(K key, uint64_t h, const WriteValue& write_value, bool* inserted) const
+    {
         *inserted = false;
         if (nstripes == 1 && load_agent(stripes) >= static_cast<int64_t>(limit)) return end_ptr();
         if (is_reserved(key))
@@ -361,8 +462,7 @@ struct table_ref
             *inserted = true;
             return p;
         }
-        if (find(key) != end_ptr()) return end_ptr();
-        const uint64_t h = home_bucket(static_cast<uint64_t>(key), nbuckets);
+        if (find_at(key, h) != end_ptr()) return end_ptr();
         uint32_t* lock = ovf_ptr(h);
         const int lane = static_cast<int>(__builtin_amdgcn_mbcnt_hi(~0U, __builtin_amdgcn_mbcnt_lo(~0U, 0U)));
         value_type* result = end_ptr();
@@ -377,7 +477,7 @@ struct table_ref
             if (waiting)
             {
                 if (!first && ((nstripes == 1 && load_agent(stripes) >= static_cast<int64_t>(limit)) ||
-                               find(key) != end_ptr()))
+                               find_at(key, h) != end_ptr()))
                     waiting = false;
                 else
                     ready = !(load_agent(lock) & kOvfLock);
@@ -440,11 +540,16 @@ struct table_ref
                 saw_empty |= (k == empty_key<K>());
             }
             if (saw_empty || overflow(b) == 0) break;
+            // the away count ends the scan at the home bucket as in find()
+            if (step == 0 && away_counters() != 0 && away(b, away_slot(key)) == 0) break;
             b = next_bucket(b, nbuckets);
         }
         // 2. absent: claim the first free slot from the scan's first one on (the next bucket on
-        //    when the scan saw none), raising ovf over the buckets before the one claimed in
+        //    when the scan saw none), raising ovf over the buckets before the one claimed in (and
+        //    the key's away count in its home bucket when that is not the one claimed in)
         uint64_t raised = 0; // ovf raised on buckets h .. h + raised - 1
+        const bool counts_away = away_counters() != 0;
+        const uint32_t af = counts_away ? away_slot(key) : 0U;
         if (fslot == kNone)
         {
             if (step >= nbuckets) return end_ptr(); // every bucket scanned, none free
@@ -482,6 +587,7 @@ struct table_ref
             }
             if (raised < cstep)
             {
+                if (raised == 0 && counts_away) away_add(h, af, 1);
                 for (; raised < cstep; ++raised)
                 {
                     const uint64_t rb = h + raised >= nbuckets ? h + raised - nbuckets : h + raised;
@@ -499,6 +605,7 @@ struct table_ref
                 {
                     __hip_atomic_store(key_ptr(fslot), tomb_key<K>(), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                     overflow_add(h, raised, -1);
+                    if (raised && counts_away) away_add(h, af, -1);
                     return end_ptr();
                 }
                 write_value(slots + fslot);
@@ -510,12 +617,16 @@ struct table_ref
             ++cj;
         }
         overflow_add(h, raised, -1);
+        if (raised && counts_away) away_add(h, af, -1);
         return end_ptr();
     }
 
     // erase (unordered_base_detail.cuh:811-913, 1019-1043): marks the slot TOMB; exactly one of
     // several concurrent erasers of one key succeeds.
-    __device__ bool erase(K key) const
+    __device__ bool erase(K key) const { return erase_at(key, home_bucket(static_cast<uint64_t>(key), nbuckets)); }
+
+    // erase() with the home bucket given (a table addressed by caller hashes)
+    __device__ bool erase_at(K key, uint64_t h) const
     {
         if (is_reserved(key))
         {
@@ -525,7 +636,6 @@ struct table_ref
             count_add(-1);
             return true;
         }
-        const uint64_t h = home_bucket(static_cast<uint64_t>(key), nbuckets);
         const uint32_t o = scan_start(key);
         uint64_t b = h;
         for (uint64_t step = 0; step < nbuckets; ++step)
@@ -540,11 +650,13 @@ struct table_ref
                     if (cas_agent(key_ptr(s), key, tomb_key<K>()) != key) return false;
                     count_add(-1);
                     overflow_add(h, step, -1);
+                    if (step != 0 && away_counters() != 0) away_add(h, away_slot(key), -1);
                     return true;
                 }
                 saw_empty |= (k == empty_key<K>());
             }
             if (saw_empty || overflow(b) == 0) break;
+            if (step == 0 && away_counters() != 0 && away(b, away_slot(key)) == 0) break;
             b = next_bucket(b, nbuckets);
         }
         return false;

@@ -51,6 +51,8 @@ class htab_tuning(ctypes.Structure):
         ("slots_per_capacity", ctypes.c_float),
         ("probe_batch", ctypes.c_int32),
         ("insert_path", ctypes.c_int32),
+        ("caller_hashes", ctypes.c_int32),
+        ("payload_bytes", ctypes.c_int32),
     ]
 
 
@@ -104,6 +106,9 @@ class htab_snapshot_header(ctypes.Structure):
         ("size", ctypes.c_int64),
         ("slot_bytes", ctypes.c_int64),
         ("ovf_bytes", ctypes.c_int64),
+        ("payload_bytes", ctypes.c_int64),
+        ("record_bytes", ctypes.c_int32),
+        ("caller_hashes", ctypes.c_int32),
     ]
 
 
@@ -143,6 +148,8 @@ class htab_view(ctypes.Structure):
         ("ref_bucket_count", ctypes.c_int64),
         ("ref_max_size", ctypes.c_int64),
         ("ovf", ctypes.c_void_p),
+        ("payload", ctypes.c_void_p),
+        ("payload_bytes", ctypes.c_int64),
     ]
 
 
@@ -166,6 +173,8 @@ SIGNATURES = {
     "htab_range_noalloc": (_I32, [_P, _P, _I64, _P, _P]),
     "htab_contents": (_I32, [_P, _P, _P, _I64, ctypes.POINTER(_I64), _P]),
     "htab_valid": (_I32, [_P, ctypes.POINTER(ctypes.c_int32), _P]),
+    "htab_valid_hashed": (_I32, [_P, _P, ctypes.POINTER(ctypes.c_int32), _P]),
+    "htab_hash_buffer": (_I32, [_P, ctypes.POINTER(_P)]),
     "htab_bucket": (_I32, [_P, ctypes.c_uint64, ctypes.POINTER(_I64)]),
     "htab_get_info": (_I32, [_P, ctypes.POINTER(htab_info)]),
     "htab_get_view": (_I32, [_P, ctypes.POINTER(htab_view)]),

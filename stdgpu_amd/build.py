@@ -78,7 +78,7 @@ def build_roofline() -> str:
     return build_library(ROOFLINE_LIB, ["roofline.hip"])
 
 
-CPP_TESTS = ["test_facade", "test_atomic", "test_deque", "test_bitset", "test_vector"]
+CPP_TESTS = ["test_facade", "test_atomic", "test_deque", "test_bitset", "test_vector", "test_reference_fixture"]
 
 
 def build_cpp_tests() -> list[str]:

@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstddef>
 #include <cstdio>
 #include <cstring>
 #include <new>
@@ -157,7 +158,7 @@ htab_last_error(void)
 // A table of a given geometry: allocation, view, sizes, and every slot EMPTY.  htab_create derives
 // nbuckets from the capacity; htab_restore takes it from a snapshot.
 static int
-build_table(int kind, int64_t capacity, int lanes, int block, int rb, int64_t nbuckets, htab** out)
+build_table(int kind, int64_t capacity, int lanes, int block, int rb, int64_t nbuckets, int payload, htab** out)
 {
     const int vb = value_bytes_of(kind);
     int ndev = 0;
@@ -183,7 +184,9 @@ build_table(int kind, int64_t capacity, int lanes, int block, int rb, int64_t nb
     // the ovf count array (none when the count lives in the first slot's padding)
     const size_t ovf_bytes =
             ovf_in_bucket_of(kind) ? 0 : ((static_cast<size_t>(nbuckets) * sizeof(uint32_t) + 255) / 256) * 256;
-    h->bytes = kCtlBytes + area_padded + ovf_bytes;
+    // records of the caller's payload, parallel to the slots (side slots included)
+    const size_t payload_area = payload ? ((static_cast<size_t>(nslots + 2) * payload + 255) / 256) * 256 : 0;
+    h->bytes = kCtlBytes + area_padded + ovf_bytes + payload_area;
     hipError_t e = hipMalloc(&h->mem, h->bytes);
     if (e != hipSuccess)
     {
@@ -195,6 +198,8 @@ build_table(int kind, int64_t capacity, int lanes, int block, int rb, int64_t nb
     h->view.stripes = reinterpret_cast<int64_t*>(base);
     h->view.flags = reinterpret_cast<uint32_t*>(base + kFlagsOffset);
     h->view.ovf = ovf_bytes ? reinterpret_cast<uint32_t*>(base + kCtlBytes + area_padded) : nullptr;
+    h->view.payload = payload ? base + kCtlBytes + area_padded + ovf_bytes : nullptr;
+    h->view.payload_bytes = payload;
     h->view.nbuckets = static_cast<uint64_t>(nbuckets);
     h->view.limit = static_cast<uint64_t>(limit);
     h->view.nstripes = (limit < nslots) ? 1U : kMaxStripes;
@@ -260,7 +265,8 @@ htab_create(int kind, int64_t capacity, const htab_tuning* tuning, htab** out)
 {
     if (!out) return fail(HTAB_E_ARG, "htab_create: out is null");
     *out = nullptr;
-    int lanes = 8, block = 64, rb = 0, ipath = 0;
+    int lanes = 8, block = 64, rb = 0, ipath = 0, payload = 0;
+    bool hashed = false;
     float spc = 1.0F;
     if (tuning)
     {
@@ -269,10 +275,13 @@ htab_create(int kind, int64_t capacity, const htab_tuning* tuning, htab** out)
         if (tuning->slots_per_capacity > 0.0F) spc = tuning->slots_per_capacity;
         rb = tuning->probe_batch;
         ipath = tuning->insert_path;
+        hashed = tuning->caller_hashes != 0;
+        payload = tuning->payload_bytes;
     }
     int rc = check_geometry(kind, capacity, lanes, block, rb, "htab_create");
     if (rc != HTAB_OK) return rc;
     if (ipath < 0 || ipath > 2) return fail(HTAB_E_ARG, "htab_create: insert_path must be 0, 1 or 2");
+    if (payload < 0 || payload > 256) return fail(HTAB_E_ARG, "htab_create: payload_bytes must be in [0, 256]");
     if (!(spc >= 0.25F && spc <= 16.0F)) return fail(HTAB_E_ARG, "htab_create: slots_per_capacity out of [0.25, 16]");
 
     const int64_t W = 16LL * lanes / value_bytes_of(kind);
@@ -282,8 +291,12 @@ htab_create(int kind, int64_t capacity, const htab_tuning* tuning, htab** out)
     // room for two entries at least, as the reference's capacity-1 table (one bucket head plus one
     // excess entry); only one-slot buckets need the second bucket for it
     if (nbuckets * W < 2) nbuckets = (2 + W - 1) / W;
-    rc = build_table(kind, capacity, lanes, block, rb, nbuckets, out);
-    if (rc == HTAB_OK) (*out)->insert_path = ipath;
+    rc = build_table(kind, capacity, lanes, block, rb, nbuckets, payload, out);
+    if (rc == HTAB_OK)
+    {
+        (*out)->insert_path = ipath;
+        (*out)->caller_hashes = hashed;
+    }
     return rc;
 }
 
@@ -300,11 +313,18 @@ ovf_array_bytes(const htab* h)
     return h->view.ovf ? h->info.physical_buckets * static_cast<int64_t>(sizeof(uint32_t)) : 0;
 }
 
+static int64_t
+payload_area_bytes(const htab* h)
+{
+    return (h->info.physical_slots + 2) * h->view.payload_bytes;
+}
+
 int
 htab_snapshot_bytes(const htab* h, int64_t* bytes)
 {
     if (!h || !bytes) return fail(HTAB_E_ARG, "htab_snapshot_bytes: null argument");
-    *bytes = static_cast<int64_t>(sizeof(htab_snapshot_header)) + slot_area_bytes(h) + ovf_array_bytes(h);
+    *bytes = static_cast<int64_t>(sizeof(htab_snapshot_header)) + slot_area_bytes(h) + ovf_array_bytes(h) +
+             payload_area_bytes(h);
     return HTAB_OK;
 }
 
@@ -318,7 +338,7 @@ htab_snapshot(htab* h, void* host_buffer, int64_t bytes, void* stream)
     hipStream_t s = static_cast<hipStream_t>(stream);
     htab_snapshot_header hd{};
     std::memcpy(hd.magic, "HTABSNP1", 8);
-    hd.version = 1;
+    hd.version = 2;
     hd.kind = h->kind;
     hd.bucket_lanes = h->lanes;
     hd.block_size = h->block;
@@ -327,6 +347,9 @@ htab_snapshot(htab* h, void* host_buffer, int64_t bytes, void* stream)
     hd.physical_buckets = h->info.physical_buckets;
     hd.slot_bytes = slot_area_bytes(h);
     hd.ovf_bytes = ovf_array_bytes(h);
+    hd.payload_bytes = payload_area_bytes(h);
+    hd.record_bytes = static_cast<int32_t>(h->view.payload_bytes);
+    hd.caller_hashes = h->caller_hashes ? 1 : 0;
     int rc = htab_size(h, &hd.size, stream);
     if (rc != HTAB_OK) return rc;
     char* out = static_cast<char*>(host_buffer);
@@ -338,6 +361,10 @@ htab_snapshot(htab* h, void* host_buffer, int64_t bytes, void* stream)
         HTAB_HIP_TRY(hipMemcpyAsync(out + sizeof(hd) + hd.slot_bytes, h->view.ovf, static_cast<size_t>(hd.ovf_bytes),
                                     hipMemcpyDeviceToHost, s),
                      "htab_snapshot counts");
+    if (hd.payload_bytes)
+        HTAB_HIP_TRY(hipMemcpyAsync(out + sizeof(hd) + hd.slot_bytes + hd.ovf_bytes, h->view.payload,
+                                    static_cast<size_t>(hd.payload_bytes), hipMemcpyDeviceToHost, s),
+                     "htab_snapshot payload");
     HTAB_HIP_TRY(hipStreamSynchronize(s), "htab_snapshot sync");
     std::memcpy(out, &hd, sizeof(hd));
     return HTAB_OK;
@@ -348,11 +375,23 @@ htab_restore(const void* host_buffer, int64_t bytes, htab** out)
 {
     if (!host_buffer || !out) return fail(HTAB_E_ARG, "htab_restore: null argument");
     *out = nullptr;
-    if (bytes < static_cast<int64_t>(sizeof(htab_snapshot_header))) return fail(HTAB_E_ARG, "htab_restore: truncated");
+    // version 1 headers end before payload_bytes (72 bytes)
+    constexpr int64_t kV1Header = static_cast<int64_t>(offsetof(htab_snapshot_header, payload_bytes));
+    if (bytes < kV1Header) return fail(HTAB_E_ARG, "htab_restore: truncated");
     htab_snapshot_header hd{};
-    std::memcpy(&hd, host_buffer, sizeof(hd));
-    if (std::memcmp(hd.magic, "HTABSNP1", 8) != 0 || hd.version != 1)
+    std::memcpy(&hd, host_buffer, static_cast<size_t>(kV1Header));
+    if (std::memcmp(hd.magic, "HTABSNP1", 8) != 0 || (hd.version != 1 && hd.version != 2))
         return fail(HTAB_E_ARG, "htab_restore: not a table snapshot");
+    // version 1 predates the away counters in the slot padding of in-bucket layouts: its tables of
+    // such a layout would restore with all counters at zero, which makes stored keys unreachable
+    if (hd.version == 1 && ovf_in_bucket_of(hd.kind) && hd.bucket_lanes > 1)
+        return fail(HTAB_E_ARG, "htab_restore: version 1 snapshot of a layout with away counters");
+    const int64_t header_bytes = hd.version == 1 ? kV1Header : static_cast<int64_t>(sizeof(hd));
+    if (hd.version == 2)
+    {
+        if (bytes < header_bytes) return fail(HTAB_E_ARG, "htab_restore: truncated");
+        std::memcpy(&hd, host_buffer, sizeof(hd));
+    }
     int rc = check_geometry(hd.kind, hd.capacity, hd.bucket_lanes, hd.block_size, hd.probe_batch, "htab_restore");
     if (rc != HTAB_OK) return rc;
     const int vb = value_bytes_of(hd.kind);
@@ -360,16 +399,21 @@ htab_restore(const void* host_buffer, int64_t bytes, htab** out)
     if (hd.physical_buckets < 1 || hd.physical_buckets >= (int64_t(1) << 32) ||
         hd.slot_bytes != (hd.physical_buckets * W + 2) * vb ||
         hd.ovf_bytes != (ovf_in_bucket_of(hd.kind) ? 0 : hd.physical_buckets * static_cast<int64_t>(sizeof(uint32_t))) ||
-        hd.size < 0 || hd.size > hd.physical_buckets * W + 2 ||
-        bytes < static_cast<int64_t>(sizeof(hd)) + hd.slot_bytes + hd.ovf_bytes)
+        hd.size < 0 || hd.size > hd.physical_buckets * W + 2 || hd.record_bytes < 0 || hd.record_bytes > 256 ||
+        hd.payload_bytes != (hd.physical_buckets * W + 2) * hd.record_bytes ||
+        bytes < header_bytes + hd.slot_bytes + hd.ovf_bytes + hd.payload_bytes)
         return fail(HTAB_E_ARG, "htab_restore: inconsistent snapshot");
     htab* h = nullptr;
-    rc = build_table(hd.kind, hd.capacity, hd.bucket_lanes, hd.block_size, hd.probe_batch, hd.physical_buckets, &h);
+    rc = build_table(hd.kind, hd.capacity, hd.bucket_lanes, hd.block_size, hd.probe_batch, hd.physical_buckets,
+                     hd.record_bytes, &h);
     if (rc != HTAB_OK) return rc;
-    const char* in = static_cast<const char*>(host_buffer);
-    hipError_t e = hipMemcpy(h->view.slots, in + sizeof(hd), static_cast<size_t>(hd.slot_bytes), hipMemcpyHostToDevice);
+    h->caller_hashes = hd.caller_hashes != 0;
+    const char* in = static_cast<const char*>(host_buffer) + header_bytes;
+    hipError_t e = hipMemcpy(h->view.slots, in, static_cast<size_t>(hd.slot_bytes), hipMemcpyHostToDevice);
     if (e == hipSuccess && hd.ovf_bytes)
-        e = hipMemcpy(h->view.ovf, in + sizeof(hd) + hd.slot_bytes, static_cast<size_t>(hd.ovf_bytes),
+        e = hipMemcpy(h->view.ovf, in + hd.slot_bytes, static_cast<size_t>(hd.ovf_bytes), hipMemcpyHostToDevice);
+    if (e == hipSuccess && hd.payload_bytes)
+        e = hipMemcpy(h->view.payload, in + hd.slot_bytes + hd.ovf_bytes, static_cast<size_t>(hd.payload_bytes),
                       hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(h->view.flags, &hd.flags, sizeof(uint32_t), hipMemcpyHostToDevice);
     // the size goes into the first stripe (the others are zero after creation)
@@ -392,6 +436,7 @@ htab_destroy(htab* h)
     if (h->scratch) e2 = hipFree(h->scratch);
     if (h->range_idx && e2 == hipSuccess) e2 = hipFree(h->range_idx);
     if (h->part && e2 == hipSuccess) e2 = hipFree(h->part);
+    if (h->hash_buf && e2 == hipSuccess) e2 = hipFree(h->hash_buf);
     delete h;
     if (e1 != hipSuccess) return fail_hip(e1, "htab_destroy");
     if (e2 != hipSuccess) return fail_hip(e2, "htab_destroy scratch");
@@ -402,6 +447,7 @@ static int
 insert_common(htab* h, const void* values, const void* keys, const void* vals, int64_t n, uint8_t* ins, void* stream)
 {
     if (!h) return fail(HTAB_E_ARG, "insert: null handle");
+    if (refuse_caller_hashed(h, "insert: the table places keys by caller hashes (native calls only)")) return HTAB_E_ARG;
     if (n < 0) return fail(HTAB_E_ARG, "insert: negative count");
     if (n == 0) return HTAB_OK;
     if (!values && !keys) return fail(HTAB_E_ARG, "insert: null input");
@@ -439,6 +485,7 @@ int
 htab_find(htab* h, const void* keys, int64_t n, uint8_t* hits, void* vals_out, void* stream)
 {
     if (!h) return fail(HTAB_E_ARG, "htab_find: null handle");
+    if (refuse_caller_hashed(h, "htab_find: the table places keys by caller hashes (native calls only)")) return HTAB_E_ARG;
     if (n < 0) return fail(HTAB_E_ARG, "htab_find: negative count");
     if (n == 0) return HTAB_OK;
     if (!keys || !hits) return fail(HTAB_E_ARG, "htab_find: null keys or hits");
@@ -461,6 +508,7 @@ int
 htab_erase(htab* h, const void* keys, int64_t n, uint8_t* erased, void* stream)
 {
     if (!h) return fail(HTAB_E_ARG, "htab_erase: null handle");
+    if (refuse_caller_hashed(h, "htab_erase: the table places keys by caller hashes (native calls only)")) return HTAB_E_ARG;
     if (n < 0) return fail(HTAB_E_ARG, "htab_erase: negative count");
     if (n == 0) return HTAB_OK;
     if (!keys) return fail(HTAB_E_ARG, "htab_erase: null keys");
@@ -482,6 +530,7 @@ int
 htab_assign(htab* h, const void* keys, const void* vals, int64_t n, uint8_t* found, void* stream)
 {
     if (!h) return fail(HTAB_E_ARG, "htab_assign: null handle");
+    if (refuse_caller_hashed(h, "htab_assign: the table places keys by caller hashes (native calls only)")) return HTAB_E_ARG;
     if (n < 0) return fail(HTAB_E_ARG, "htab_assign: negative count");
     if (n == 0) return HTAB_OK;
     if (!keys || !vals) return fail(HTAB_E_ARG, "htab_assign: null keys or values");
@@ -575,6 +624,8 @@ int
 htab_valid(htab* h, int32_t* out, void* stream)
 {
     if (!h || !out) return fail(HTAB_E_ARG, "htab_valid: null argument");
+    if (refuse_caller_hashed(h, "htab_valid: the table places keys by caller hashes (htab_valid_hashed)"))
+        return HTAB_E_ARG;
     hipStream_t s = static_cast<hipStream_t>(stream);
     htab_call c;
     c.op = call_op::valid;
@@ -582,6 +633,34 @@ htab_valid(htab* h, int32_t* out, void* stream)
     c.valid_out = out;
     c.s = s;
     return dispatch(h->kind, c);
+}
+
+int
+htab_valid_hashed(htab* h, const uint64_t* hashes, int32_t* out, void* stream)
+{
+    if (!h || !hashes || !out) return fail(HTAB_E_ARG, "htab_valid_hashed: null argument");
+    if (!aligned(hashes, 8)) return fail(HTAB_E_ARG, "htab_valid_hashed: hashes not aligned to 8 bytes");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    htab_call c;
+    c.op = call_op::valid;
+    c.h = h;
+    c.valid_out = out;
+    c.hashes = hashes;
+    c.s = s;
+    return dispatch(h->kind, c);
+}
+
+int
+htab_hash_buffer(htab* h, uint64_t** out)
+{
+    if (!h || !out) return fail(HTAB_E_ARG, "htab_hash_buffer: null argument");
+    if (!h->hash_buf)
+    {
+        const size_t bytes = static_cast<size_t>(h->info.physical_slots + 2) * sizeof(uint64_t);
+        HTAB_HIP_TRY(hipMalloc(&h->hash_buf, bytes), "htab_hash_buffer");
+    }
+    *out = h->hash_buf;
+    return HTAB_OK;
 }
 
 int
@@ -601,6 +680,8 @@ htab_batch_create(htab* const* tables, int32_t ntables, htab_batch** out)
     for (int32_t t = 0; t < ntables; ++t)
     {
         if (!tables[t]) return fail(HTAB_E_ARG, "htab_batch_create: null table");
+        if (refuse_caller_hashed(tables[t], "htab_batch_create: a table places keys by caller hashes"))
+            return HTAB_E_ARG;
         if (tables[t]->kind != tables[0]->kind || tables[t]->lanes != tables[0]->lanes ||
             tables[t]->block != tables[0]->block || tables[t]->rb != tables[0]->rb)
             return fail(HTAB_E_ARG, "htab_batch_create: tables differ in kind, bucket_lanes, block_size or probe_batch");

@@ -60,7 +60,18 @@ struct htab
     int insert_path = 0;          // htab_tuning::insert_path
     void* part = nullptr;         // partitioned-insert scratch, grown on demand
     size_t part_bytes = 0;
+    bool caller_hashes = false;   // htab_tuning::caller_hashes: only native calls and htab_valid_hashed
+    uint64_t* hash_buf = nullptr; // htab_hash_buffer's array, allocated on demand
 };
+
+// the bulk entry points that place keys by their bits refuse a table placed by caller hashes
+inline int
+refuse_caller_hashed(const htab* h, const char* who)
+{
+    if (h && h->caller_hashes)
+        return htab_host::fail(HTAB_E_ARG, who);
+    return HTAB_OK;
+}
 
 // the device word in the table's control block where k_limit_plan records its decision
 inline uint32_t*
@@ -136,6 +147,7 @@ struct htab_call
     int64_t* sizes_out = nullptr;                      // served launch: host-mapped table sizes
     int64_t* scratch_ext = nullptr;                    // range: tile scratch given by the caller (no allocation)
     int64_t* count_dev = nullptr;                      // range: entry count written to device memory (no sync)
+    const uint64_t* hashes = nullptr;                  // valid: the caller's hash of each slot's key (or null)
     hipStream_t s = nullptr;
 };
 

@@ -111,6 +111,19 @@ struct chunk
     }
 };
 
+// The away counter of key k in the bucket whose chunks the group holds (in-bucket layouts with G > 1
+// slots per bucket: lane j of the group holds slot j, whose padding word carries counters 2(j-1) and
+// 2j-1).  Called with k uniform over the group.
+template <typename K, typename M, int G>
+__device__ __forceinline__ uint32_t
+group_away(const chunk<K, M>& q, K k, int gbase)
+{
+    static_assert(G > 1, "a one-slot bucket has no away counters");
+    const uint32_t f = away_index(static_cast<uint64_t>(k), 2U * (G - 1));
+    const uint32_t w = shfl32(q.ovf_word(), gbase + 1 + static_cast<int>(f / 2));
+    return (w >> (16 * (f & 1))) & 0xFFFFU;
+}
+
 template <typename K, typename M>
 __device__ inline void
 store_mapped(typename slot_traits<K, M>::value_type* p, uint64_t v)
@@ -205,6 +218,7 @@ find_body(const table_ref<K, M>& t, const K* __restrict__ keys, const int64_t i,
     const uint64_t gmask = ((G == 64) ? ~0ULL : ((1ULL << G) - 1)) << gbase;
     const uint32_t W = static_cast<uint32_t>(G * SPL);
     constexpr bool kInline = slot_traits<K, M>::ovf_in_bucket;
+    constexpr bool kAway = kInline && G > 1;
 
     const K mykey = valid ? keys[i] : empty_key<K>();
     bool myhit = false;
@@ -261,10 +275,16 @@ find_body(const table_ref<K, M>& t, const K* __restrict__ keys, const int64_t i,
                 }
                 else if (!be)
                 {
-                    // no key runs past a home bucket with ovf == 0 (read with the bucket when in-bucket)
+                    // no key runs past a home bucket with ovf == 0 (read with the bucket when in-bucket),
+                    // and none of this bucket's keys with k's fingerprint when its away counter is 0
                     if constexpr (kInline)
                     {
-                        if (shfl32(li == 0 ? q[r].ovf_word() : 0U, gbase) != 0) pend |= 1U << (rb + r);
+                        bool more = shfl32(li == 0 ? q[r].ovf_word() : 0U, gbase) != 0;
+                        if constexpr (kAway)
+                        {
+                            if (more) more = group_away<K, M, G>(q[r], k, gbase) != 0;
+                        }
+                        if (more) pend |= 1U << (rb + r);
                     }
                     else
                     {
@@ -380,6 +400,7 @@ constexpr uint32_t kWalkSet = 1U << 8;     // the walk starts at the home bucket
 constexpr uint32_t kWalkFree = 1U << 9;    // the home bucket has a free slot (index in bits 0-7)
 constexpr uint32_t kWalkTomb = 1U << 10;   // that slot held TOMB (else EMPTY)
 constexpr uint32_t kWalkAbsent = 1U << 11; // no key runs past the home bucket: the key is absent
+constexpr uint32_t kWalkAwayHigh = 1U << 12; // the key's away counter there was seen at kAwayFastLimit or above
 
 // The claim order of a key inside a bucket (claim_offset, kOrderedClaimBuckets): htab_device.hpp.
 
@@ -441,6 +462,7 @@ insert_body(const table_ref<K, M>& t, const void* __restrict__ values, const voi
     const uint64_t gmask = ((G == 64) ? ~0ULL : ((1ULL << G) - 1)) << gbase;
     const uint32_t W = static_cast<uint32_t>(G * SPL);
     constexpr bool kInline = slot_traits<K, M>::ovf_in_bucket;
+    constexpr bool kAway = kInline && G > 1;
 
     K mykey = empty_key<K>();
     uint64_t myv = 0;
@@ -478,7 +500,7 @@ insert_body(const table_ref<K, M>& t, const void* __restrict__ values, const voi
         C q[R];
         K kr[R];
         bool act[R];
-        uint64_t hb[R];
+        uint32_t hb[R]; // home buckets (a table has fewer than 2^32 buckets)
 #pragma unroll
         for (int r = 0; r < R; ++r)
         {
@@ -487,10 +509,10 @@ insert_body(const table_ref<K, M>& t, const void* __restrict__ values, const voi
             hb[r] = 0;
             if (act[r])
             {
-                hb[r] = home_bucket(static_cast<uint64_t>(kr[r]), t.nbuckets);
+                hb[r] = static_cast<uint32_t>(home_bucket(static_cast<uint64_t>(kr[r]), t.nbuckets));
                 // a stale view is harmless here: a slot seen free but taken fails its CAS, and an
                 // insert-only launch never frees a slot (the in-bucket count comes with the load)
-                q[r] = C::load_plain(t.slots + hb[r] * W, li);
+                q[r] = C::load_plain(t.slots + static_cast<uint64_t>(hb[r]) * W, li);
             }
         }
         // Decide every element's claim first, then issue the R compare-and-swaps together so that
@@ -517,7 +539,9 @@ insert_body(const table_ref<K, M>& t, const void* __restrict__ values, const voi
                 uint32_t ovh = 1;
                 if constexpr (kInline)
                 {
-                    // no key runs past a home bucket with ovf == 0: a TOMB there is free to take
+                    // no key runs past a home bucket with ovf == 0: a TOMB there is free to take.  The
+                    // same holds for k alone when the away counter of its fingerprint is 0 (no key of
+                    // this bucket with that fingerprint is stored further on), so ovh then reads 0 too.
                     ovh = shfl32(li == 0 ? q[r].ovf_word() : 0U, gbase);
                     ends_home = ends_home || (bf != 0 && ovh == 0);
                 }
@@ -548,6 +572,15 @@ insert_body(const table_ref<K, M>& t, const void* __restrict__ values, const voi
                     // the probe sequence runs on past the home bucket: the element's lane keeps what
                     // this bucket showed, and the walk starts at the next bucket
                     uint32_t info = kWalkSet | ((kInline && ovh == 0) ? kWalkAbsent : 0U);
+                    if constexpr (kAway)
+                    {
+                        if (ovh != 0)
+                        {
+                            const uint32_t aw = group_away<K, M, G>(q[r], k, gbase);
+                            if (aw == 0) info |= kWalkAbsent;
+                            if (aw >= kAwayFastLimit) info |= kWalkAwayHigh;
+                        }
+                    }
                     if (bf)
                     {
                         uint32_t fsel = f;
@@ -580,7 +613,7 @@ insert_body(const table_ref<K, M>& t, const void* __restrict__ values, const voi
             if (lane == fl)
             {
                 const K o = ((tombp >> r) & 1U) ? tomb_key<K>() : empty_key<K>();
-                got[r] = cas_agent(t.key_ptr(hb[r] * W + static_cast<uint32_t>((sjp >> (8 * r)) & 0xFFU)), o, kr[r]);
+                got[r] = cas_agent(t.key_ptr(static_cast<uint64_t>(hb[r]) * W + static_cast<uint32_t>((sjp >> (8 * r)) & 0xFFU)), o, kr[r]);
             }
         }
 #pragma unroll
@@ -597,12 +630,12 @@ insert_body(const table_ref<K, M>& t, const void* __restrict__ values, const voi
                 {
                     const K o = ((tombp >> r) & 1U) ? tomb_key<K>() : empty_key<K>();
                     seen = (got[r] == o) ? 1 : ((got[r] == kr[r]) ? 2 : 0);
-                    if (seen == 1) store_mapped<K, M>(t.slots + hb[r] * W + j, v);
+                    if (seen == 1) store_mapped<K, M>(t.slots + static_cast<uint64_t>(hb[r]) * W + j, v);
                 }
                 seen = shfl64(seen, fl);
                 if (seen == 1)
                 {
-                    const uint64_t slot = hb[r] * W + shfl32(j, fl);
+                    const uint64_t slot = static_cast<uint64_t>(hb[r]) * W + shfl32(j, fl);
                     if (li == rb + r)
                     {
                         myins = true;
@@ -628,6 +661,7 @@ insert_body(const table_ref<K, M>& t, const void* __restrict__ values, const voi
     K ffobs = 0;
     uint32_t jmin = 0, co = 0; // order position to resume from; the key's claim offset
     bool absent = false, fresh = false;
+    bool away_fast = false; // the key's away counter in its home bucket was seen below kAwayFastLimit
     while (true)
     {
         if (cur < 0 && pend)
@@ -646,6 +680,7 @@ insert_body(const table_ref<K, M>& t, const void* __restrict__ values, const voi
             fresh = false;
             // an element sent on by its home bucket (not one that lost a claim there) resumes after it
             const uint32_t info = shfl32(mywalk, gbase + cur);
+            away_fast = (info & kWalkSet) && !(info & kWalkAwayHigh);
             if ((info & kWalkSet) && t.nbuckets > 1)
             {
                 b = next_bucket(hb, t.nbuckets);
@@ -662,116 +697,139 @@ insert_body(const table_ref<K, M>& t, const void* __restrict__ values, const voi
         if (ballot(active) == 0) break;
         if (active)
         {
-            // plain (L2-cacheable) loads until the element loses a CAS, coherent loads after that: a
-            // stale view only shows taken slots as free or counts as too low, and every claim is
-            // checked by its CAS against the value it saw
-            C q;
-            uint32_t ovl = 0;
-            if (fresh)
-            {
-                q = C::load_keys(t.slots + b * W, li);
-                ovl = li == 0 ? t.overflow(b) : 0U;
-            }
-            else
-            {
-                q = C::load_plain(t.slots + b * W, li);
-                if constexpr (kInline)
-                    ovl = li == 0 ? q.ovf_word() : 0U;
-                else
-                    ovl = li == 0 ? t.ovf[b] : 0U;
-            }
-            const uint32_t ov = shfl32(ovl, gbase);
             const K k = ck;
-            // the whole bucket is checked for the key and for EMPTY slots (slots passed after a lost
-            // CAS hold other keys for good); free slots count from order position jmin on
-            const uint32_t m = q.mask([k](K x) { return x == k; });
-            const uint32_t e = q.mask([](K x) { return x == empty_key<K>(); });
-            const uint32_t f = q.mask([](K x) { return x == empty_key<K>() || x == tomb_key<K>(); });
-            const uint64_t bm = ballot(m != 0) & gmask;
-            const uint64_t be = ballot(e != 0) & gmask;
-            uint32_t fsel = 0;
-            const int fl = first_free_in_order<SPL>(f, co, jmin, W, li, gmask, &fsel);
-            if (bm)
+            // an element whose home bucket proved it absent and offered a free slot claims at once
+            bool claim = absent && ff != kNoSlot;
+            if (!claim)
             {
-                cur = -1; // present
-            }
-            else
-            {
-                if (ff == kNoSlot && fl >= 0)
+                // plain (L2-cacheable) loads until the element loses a CAS, coherent loads after that:
+                // a stale view only shows taken slots as free or counts as too low, and every claim is
+                // checked by its CAS against the value it saw
+                C q;
+                uint32_t ovl = 0;
+                if (fresh)
                 {
-                    uint64_t cand = 0, obs = 0;
-                    if (lane == fl)
-                    {
-                        const int j = __builtin_ctz(fsel);
-                        cand = b * W + static_cast<uint64_t>(li * SPL + j);
-                        obs = static_cast<uint64_t>(q.key(j));
-                    }
-                    ff = shfl64(cand, fl);
-                    ffobs = static_cast<K>(shfl64(obs, fl));
-                }
-                // no key runs past a bucket with ovf == 0, and a bucket holding an EMPTY slot has ovf == 0
-                if (be || ov == 0) absent = true;
-                if (absent && ff != kNoSlot)
-                {
-                    // claim ff (the group's first lane issues the CAS)
-                    uint64_t res = 0;
-                    if (li == 0)
-                    {
-                        const K got = cas_agent(t.key_ptr(ff), ffobs, k);
-                        res = (got == ffobs) ? 1 : ((got == k) ? 2 : 0);
-                        if (res == 1) store_mapped<K, M>(t.slots + ff, cv);
-                    }
-                    res = shfl64(res, gbase);
-                    if (res == 1)
-                    {
-                        // the key runs through every bucket from its home bucket to ff's
-                        const uint64_t fb = ff / W;
-                        const uint64_t dist = t.distance(hb, fb);
-                        for (uint64_t d = static_cast<uint64_t>(li); d < dist; d += G)
-                        {
-                            uint64_t x = hb + d;
-                            if (x >= t.nbuckets) x -= t.nbuckets;
-                            __hip_atomic_fetch_add(t.ovf_ptr(x), 1U, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                        }
-                        if (li == cur)
-                        {
-                            myins = true;
-                            myslot = ff;
-                        }
-                        cur = -1;
-                    }
-                    else if (res == 2)
-                    {
-                        cur = -1;
-                    }
-                    else
-                    {
-                        // lost ff to another key: every slot before ff in the key's order holds another
-                        // key (an insert-only launch never frees a slot), so the scan resumes right after
-                        // ff; ff only moves forward, so this terminates
-                        b = ff / W;
-                        jmin = ((static_cast<uint32_t>(ff % W) + W - co) & (W - 1)) + 1;
-                        ff = kNoSlot;
-                        fresh = true;
-                        if (jmin == W)
-                        {
-                            b = next_bucket(b, t.nbuckets);
-                            jmin = 0;
-                            if (b == hb) cur = -1; // wrapped around: no free slot left
-                        }
-                    }
+                    q = C::load_keys(t.slots + b * W, li);
+                    ovl = li == 0 ? t.overflow(b) : 0U;
                 }
                 else
                 {
-                    b = next_bucket(b, t.nbuckets);
-                    jmin = 0;
-                    ++walked;
-                    // every probe sequence is finite: give up after a full wrap, or early when the
-                    // table has reached its limit
-                    if (b == hb ||
-                        ((walked & 63) == 0 && stripe_sum(t.stripes, t.nstripes) >= static_cast<int64_t>(t.limit)))
+                    q = C::load_plain(t.slots + b * W, li);
+                    if constexpr (kInline)
+                        ovl = li == 0 ? q.ovf_word() : 0U;
+                    else
+                        ovl = li == 0 ? t.ovf[b] : 0U;
+                }
+                const uint32_t ov = shfl32(ovl, gbase);
+                // the whole bucket is checked for the key and for EMPTY slots (slots passed after a
+                // lost CAS hold other keys for good); free slots count from order position jmin on
+                const uint32_t m = q.mask([k](K x) { return x == k; });
+                const uint32_t e = q.mask([](K x) { return x == empty_key<K>(); });
+                const uint32_t f = q.mask([](K x) { return x == empty_key<K>() || x == tomb_key<K>(); });
+                const uint64_t bm = ballot(m != 0) & gmask;
+                const uint64_t be = ballot(e != 0) & gmask;
+                uint32_t fsel = 0;
+                const int fl = first_free_in_order<SPL>(f, co, jmin, W, li, gmask, &fsel);
+                if (bm)
+                {
+                    cur = -1; // present
+                }
+                else
+                {
+                    if (ff == kNoSlot && fl >= 0)
                     {
-                        cur = -1;
+                        uint64_t cand = 0, obs = 0;
+                        if (lane == fl)
+                        {
+                            const int j = __builtin_ctz(fsel);
+                            cand = b * W + static_cast<uint64_t>(li * SPL + j);
+                            obs = static_cast<uint64_t>(q.key(j));
+                        }
+                        ff = shfl64(cand, fl);
+                        ffobs = static_cast<K>(shfl64(obs, fl));
+                    }
+                    // no key runs past a bucket with ovf == 0, and a bucket holding an EMPTY slot has
+                    // ovf == 0; nor is k stored past its home bucket when its away counter there is 0
+                    if (be || ov == 0) absent = true;
+                    if constexpr (kAway)
+                    {
+                        if (!absent && !fresh && b == hb && group_away<K, M, G>(q, k, gbase) == 0) absent = true;
+                    }
+                    claim = absent && ff != kNoSlot;
+                    if (!claim)
+                    {
+                        b = next_bucket(b, t.nbuckets);
+                        jmin = 0;
+                        ++walked;
+                        // every probe sequence is finite: give up after a full wrap, or early when the
+                        // table has reached its limit
+                        if (b == hb ||
+                            ((walked & 63) == 0 && stripe_sum(t.stripes, t.nstripes) >= static_cast<int64_t>(t.limit)))
+                        {
+                            cur = -1;
+                        }
+                    }
+                }
+            }
+            if (claim)
+            {
+                // claim ff (the group's first lane issues the CAS)
+                uint64_t res = 0;
+                if (li == 0)
+                {
+                    const K got = cas_agent(t.key_ptr(ff), ffobs, k);
+                    res = (got == ffobs) ? 1 : ((got == k) ? 2 : 0);
+                    if (res == 1) store_mapped<K, M>(t.slots + ff, cv);
+                }
+                res = shfl64(res, gbase);
+                if (res == 1)
+                {
+                    // the key runs through every bucket from its home bucket to ff's
+                    const uint64_t fb = ff / W;
+                    const uint64_t dist = t.distance(hb, fb);
+                    for (uint64_t d = static_cast<uint64_t>(li); d < dist; d += G)
+                    {
+                        uint64_t x = hb + d;
+                        if (x >= t.nbuckets) x -= t.nbuckets;
+                        __hip_atomic_fetch_add(t.ovf_ptr(x), 1U, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    }
+                    // stored past its home bucket: that bucket's away counter of k's fingerprint (a plain
+                    // add, not waited for, when the counter was seen well below saturation)
+                    if constexpr (kAway)
+                    {
+                        if (li == 0 && dist != 0)
+                        {
+                            if (away_fast)
+                                t.away_add_fast(hb, t.away_slot(k), 1);
+                            else
+                                t.away_add(hb, t.away_slot(k), 1);
+                        }
+                    }
+                    if (li == cur)
+                    {
+                        myins = true;
+                        myslot = ff;
+                    }
+                    cur = -1;
+                }
+                else if (res == 2)
+                {
+                    cur = -1;
+                }
+                else
+                {
+                    // lost ff to another key: every slot before ff in the key's order holds another key
+                    // (an insert-only launch never frees a slot), so the scan resumes right after ff; ff
+                    // only moves forward, so this terminates
+                    b = ff / W;
+                    jmin = ((static_cast<uint32_t>(ff % W) + W - co) & (W - 1)) + 1;
+                    ff = kNoSlot;
+                    fresh = true;
+                    if (jmin == W)
+                    {
+                        b = next_bucket(b, t.nbuckets);
+                        jmin = 0;
+                        if (b == hb) cur = -1; // wrapped around: no free slot left
                     }
                 }
             }
@@ -797,6 +855,10 @@ insert_body(const table_ref<K, M>& t, const void* __restrict__ values, const voi
                 __hip_atomic_store(t.key_ptr(myslot), tomb_key<K>(), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 const uint64_t h = home_bucket(static_cast<uint64_t>(mykey), t.nbuckets);
                 t.overflow_add(h, t.distance(h, myslot / W), -1);
+                if constexpr (kAway)
+                {
+                    if (myslot / W != h) t.away_add(h, t.away_slot(mykey), -1);
+                }
             }
             __hip_atomic_fetch_add(t.stripes, static_cast<int64_t>(-1), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             myins = false;
@@ -825,9 +887,11 @@ erase_body(const table_ref<K, M>& t, const K* __restrict__ keys, const int64_t i
     const uint64_t gmask = ((G == 64) ? ~0ULL : ((1ULL << G) - 1)) << gbase;
     const uint32_t W = static_cast<uint32_t>(G * SPL);
     constexpr bool kInline = slot_traits<K, M>::ovf_in_bucket;
+    constexpr bool kAway = kInline && G > 1;
 
     const K mykey = valid ? keys[i] : empty_key<K>();
     bool mydel = false;
+    bool mysat = false; // the key's away counter in its home bucket was seen saturated
     const bool reserved = valid && table_ref<K, M>::is_reserved(mykey);
     if (reserved)
     {
@@ -888,7 +952,15 @@ erase_body(const table_ref<K, M>& t, const K* __restrict__ keys, const int64_t i
                 }
                 else if (!ends_home)
                 {
-                    pend |= 1U << (rb + r);
+                    // k is stored further on only if its away counter in the home bucket says so
+                    bool more = true;
+                    if constexpr (kAway)
+                    {
+                        const uint32_t aw = group_away<K, M, G>(q[r], k, gbase);
+                        more = aw != 0;
+                        if (li == rb + r) mysat = aw >= kAwaySaturated;
+                    }
+                    if (more) pend |= 1U << (rb + r);
                 }
             }
         }
@@ -899,6 +971,7 @@ erase_body(const table_ref<K, M>& t, const K* __restrict__ keys, const int64_t i
     int cur = -1;
     K ck = 0;
     uint64_t b = 0, p = 0, hb = 0, steps = 0;
+    bool sat = false;
     while (true)
     {
         if (cur < 0 && pend)
@@ -906,6 +979,7 @@ erase_body(const table_ref<K, M>& t, const K* __restrict__ keys, const int64_t i
             cur = __builtin_ctz(pend);
             pend &= pend - 1;
             ck = shfl(mykey, gbase + cur);
+            sat = shfl32(mysat ? 1U : 0U, gbase + cur) != 0;
             hb = home_bucket(static_cast<uint64_t>(ck), t.nbuckets);
             p = hb;
             b = next_bucket(hb, t.nbuckets);
@@ -954,6 +1028,12 @@ erase_body(const table_ref<K, M>& t, const K* __restrict__ keys, const int64_t i
                         if (x >= t.nbuckets) x -= t.nbuckets;
                         __hip_atomic_fetch_add(t.ovf_ptr(x), static_cast<uint32_t>(-1), __ATOMIC_RELAXED,
                                                __HIP_MEMORY_SCOPE_AGENT);
+                    }
+                    // b != hb here: lower the home bucket's away counter with a plain atomic, unless it
+                    // was seen saturated (an erase-only launch only lowers counters, so it still is)
+                    if constexpr (kAway)
+                    {
+                        if (li == 0 && !sat) t.away_add_fast(hb, t.away_slot(k), -1);
                     }
                     if (li == cur) mydel = true;
                 }
@@ -1417,9 +1497,10 @@ k_range_scatter(table_ref<K, M> t, uint64_t total_slots, const int64_t* __restri
 // bad[1] counts stored entries (compared with the size counter on the host).  Each stored key also
 // adds one to want[b] for every bucket b its probe sequence runs through; k_valid_overflow then
 // checks ovf against those counts, and that no bucket with ovf > 0 holds an EMPTY slot.
+// hashes (optional): the caller's hash of each slot's key, for a table that places keys by those
 template <typename K, typename M>
 __global__ __launch_bounds__(256) void
-k_valid(table_ref<K, M> t, uint64_t total_slots, unsigned long long* bad, uint32_t* want)
+k_valid(table_ref<K, M> t, uint64_t total_slots, unsigned long long* bad, uint32_t* want, const uint64_t* hashes)
 {
     const uint32_t flags = *t.flags;
     const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
@@ -1430,10 +1511,10 @@ k_valid(table_ref<K, M> t, uint64_t total_slots, unsigned long long* bad, uint32
         if (!slot_stored(t, s, flags)) continue;
         ++nstored;
         const K k = slot_traits<K, M>::key(t.slots[s]);
-        if (t.find(k) != t.slots + s) ++nbad;
+        const uint64_t h = t.home_of_hash((hashes && s < ns) ? hashes[s] : static_cast<uint64_t>(k));
+        if (t.find_at(k, h) != t.slots + s) ++nbad;
         if (s < ns)
         {
-            const uint64_t h = home_bucket(static_cast<uint64_t>(k), t.nbuckets);
             const uint64_t dist = t.distance(h, s / t.W);
             uint64_t b = h;
             for (uint64_t d = 0; d < dist; ++d)
@@ -1461,6 +1542,45 @@ k_valid_overflow(table_ref<K, M> t, const uint32_t* __restrict__ want, unsigned 
         {
             for (uint32_t j = 0; j < t.W; ++j)
                 if (*t.key_ptr(b * t.W + j) == empty_key<K>()) ++nbad;
+        }
+    }
+    if (nbad) atomicAdd(bad, nbad);
+}
+
+// Away counters (in-bucket layouts with W > 1): for every home bucket h, the keys homed at h that are
+// stored in a later bucket, counted per fingerprint, must equal h's counters, except a saturated one,
+// which stands for any count.  Such keys lie in the buckets after h up to the first bucket
+// b whose predecessor has ovf == 0 (each of them counts in ovf of every bucket from h to its own).
+constexpr uint32_t kMaxAwayCounters = 2 * (16 - 1); // buckets of at most 16 slots
+
+template <typename K, typename M>
+__global__ __launch_bounds__(256) void
+k_valid_away(table_ref<K, M> t, unsigned long long* bad, const uint64_t* hashes)
+{
+    const uint32_t nctr = t.away_counters();
+    if (nctr == 0 || nctr > kMaxAwayCounters) return;
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+    unsigned long long nbad = 0;
+    for (uint64_t h = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; h < t.nbuckets; h += stride)
+    {
+        uint32_t cnt[kMaxAwayCounters];
+        for (uint32_t f = 0; f < nctr; ++f) cnt[f] = 0;
+        uint64_t b = h;
+        for (uint64_t step = 1; step < t.nbuckets && t.overflow(b) != 0; ++step)
+        {
+            b = next_bucket(b, t.nbuckets);
+            for (uint32_t j = 0; j < t.W; ++j)
+            {
+                const K k = *t.key_ptr(b * t.W + j);
+                if (k == empty_key<K>() || k == tomb_key<K>()) continue;
+                if (t.home_of_hash(hashes ? hashes[b * t.W + j] : static_cast<uint64_t>(k)) != h) continue;
+                ++cnt[away_index(static_cast<uint64_t>(k), nctr)];
+            }
+        }
+        for (uint32_t f = 0; f < nctr; ++f)
+        {
+            const uint32_t have = (*t.away_word(h, f) >> (16 * (f & 1))) & 0xFFFFU;
+            if (have < kAwaySaturated && have != cnt[f]) ++nbad;
         }
     }
     if (nbad) atomicAdd(bad, nbad);

@@ -300,7 +300,7 @@ struct table_ops
         return HTAB_OK;
     }
 
-    static int valid(htab* h, int32_t* out, hipStream_t s)
+    static int valid(htab* h, const uint64_t* hashes, int32_t* out, hipStream_t s)
     {
         const T t = T::from_view(h->view);
         // scratch: two counters, then one expected ovf count per bucket
@@ -319,7 +319,8 @@ struct table_ops
         const uint64_t total = t.nslots() + 2;
         uint64_t grid = (total + 255) / 256;
         if (grid > 16384) grid = 16384;
-        hipLaunchKernelGGL((k_valid<K, M>), dim3(static_cast<uint32_t>(grid)), dim3(256), 0, s, t, total, bad, want);
+        hipLaunchKernelGGL((k_valid<K, M>), dim3(static_cast<uint32_t>(grid)), dim3(256), 0, s, t, total, bad, want,
+                           hashes);
         int rc = launch_status("valid launch");
         if (rc) return rc;
         uint64_t bgrid = (t.nbuckets + 255) / 256;
@@ -327,6 +328,13 @@ struct table_ops
         hipLaunchKernelGGL((k_valid_overflow<K, M>), dim3(static_cast<uint32_t>(bgrid)), dim3(256), 0, s, t, want, bad);
         rc = launch_status("valid overflow launch");
         if (rc) return rc;
+        if (t.away_counters() != 0)
+        {
+            hipLaunchKernelGGL((k_valid_away<K, M>), dim3(static_cast<uint32_t>(bgrid)), dim3(256), 0, s, t, bad,
+                               hashes);
+            rc = launch_status("valid away launch");
+            if (rc) return rc;
+        }
         unsigned long long res[2] = {0, 0};
         int64_t stripes[kMaxStripes * kStripeStride];
         HTAB_HIP_TRY(hipMemcpyAsync(res, bad, sizeof(res), hipMemcpyDeviceToHost, s), "valid copy");
@@ -419,7 +427,7 @@ dispatch_kind(htab_call& c)
         case call_op::clear: return table_ops<K, M>::clear(c.h, c.s);
         case call_op::range: return table_ops<K, M>::range(c.h, c.idx_out, const_cast<void*>(c.keys), c.vals_out, c.cap,
                                                            c.n_out, c.s, c.scratch_ext, c.count_dev);
-        case call_op::valid: return table_ops<K, M>::valid(c.h, c.valid_out, c.s);
+        case call_op::valid: return table_ops<K, M>::valid(c.h, c.hashes, c.valid_out, c.s);
         default: break;
     }
     const int lanes = c.h ? c.h->lanes : c.b->lanes;

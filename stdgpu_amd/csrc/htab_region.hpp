@@ -415,6 +415,25 @@ region_insert_one(typename slot_traits<K, M>::value_type* lslots, uint32_t* lovf
         {
             store_mapped<K, M>(reinterpret_cast<value_type*>(kp(cb, cj)), val);
             for (uint32_t x = lb; x < cb; ++x) atomicAdd(ovp(x), 1U);
+            if constexpr (kInline && W > 1)
+            {
+                // stored past its home bucket: the home bucket's away counter of the key's fingerprint
+                // (slot 1 + f/2's padding word, half f mod 2; it stops at saturation, no carry)
+                if (cb != lb)
+                {
+                    const uint32_t f = away_index(static_cast<uint64_t>(key), 2U * (W - 1));
+                    uint32_t* w = reinterpret_cast<uint32_t*>(lbytes + lds_piece<G>(lb, 1 + f / 2) * 16 +
+                                                              slot_traits<K, M>::ovf_offset);
+                    const uint32_t sh = 16 * (f & 1);
+                    uint32_t cur = lds_load(w);
+                    while (((cur >> sh) & 0xFFFFU) < kAwaySaturated)
+                    {
+                        const uint32_t got = lds_cas(w, cur, cur + (1U << sh));
+                        if (got == cur) break;
+                        cur = got;
+                    }
+                }
+            }
             return region_result::inserted;
         }
         if (old == key) return region_result::present;

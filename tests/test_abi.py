@@ -38,9 +38,9 @@ def test_python_binding_covers_header():
 
 def test_struct_layouts_match_header():
     # field order/size of the C structs as seen by ctypes (plain 64-bit ABI)
-    assert ctypes.sizeof(_lib.htab_tuning) == 20
+    assert ctypes.sizeof(_lib.htab_tuning) == 28
     assert ctypes.sizeof(_lib.htab_info) == 8 + 8 * 8
-    assert ctypes.sizeof(_lib.htab_view) == 3 * 8 + 2 * 8 + 2 * 4 + 2 * 8 + 8
+    assert ctypes.sizeof(_lib.htab_view) == 3 * 8 + 2 * 8 + 2 * 4 + 2 * 8 + 8 + 2 * 8
     assert ctypes.sizeof(_lib.htab_deque_view) == 7 * 8
     assert ctypes.sizeof(_lib.htab_vector_view) == 3 * 8
     assert ctypes.sizeof(_lib.htab_bitset_view) == 2 * 8
@@ -86,13 +86,19 @@ def test_restore_rejects_what_is_not_a_snapshot():
     lib = _lib.load()
     h = ctypes.c_void_p()
     hs = ctypes.sizeof(_lib.htab_snapshot_header)
-    assert hs == 8 + 6 * 4 + 5 * 8
+    assert hs == 8 + 6 * 4 + 6 * 8 + 2 * 4
     junk = np.zeros(4096, dtype=np.uint8)
     assert lib.htab_restore(junk.ctypes.data_as(ctypes.c_void_p), junk.size, ctypes.byref(h)) == _lib.HTAB_E_ARG
     assert lib.htab_restore(junk.ctypes.data_as(ctypes.c_void_p), 10, ctypes.byref(h)) == _lib.HTAB_E_ARG
-    hd = _lib.htab_snapshot_header(b"HTABSNP1", 1, _lib.HTAB_MAP_U64_U32, 8, 256, 0, 0, 1000, 125, 0,
-                                   (125 * 8 + 2) * 16, 0)
+    hd = _lib.htab_snapshot_header(b"HTABSNP1", 2, _lib.HTAB_MAP_U64_U32, 8, 256, 0, 0, 1000, 125, 0,
+                                   (125 * 8 + 2) * 16, 0, 0, 0, 0)
     raw = np.frombuffer(bytes(hd), dtype=np.uint8)
+    # version 1 predates the away counters that this layout keeps in its slot padding
+    hd.version = 1
+    old = np.frombuffer(bytes(hd), dtype=np.uint8)
+    assert lib.htab_restore(old.ctypes.data_as(ctypes.c_void_p), old.size, ctypes.byref(h)) == _lib.HTAB_E_ARG
+    assert b"version 1" in lib.htab_last_error()
+    hd.version = 2
     # a header that promises more bytes than the buffer holds
     assert lib.htab_restore(raw.ctypes.data_as(ctypes.c_void_p), raw.size, ctypes.byref(h)) == _lib.HTAB_E_ARG
     assert b"inconsistent" in lib.htab_last_error()

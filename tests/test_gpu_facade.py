@@ -1,6 +1,8 @@
 """Runs the C++ source-compatibility tests of the stdgpu facade (tests/cpp/test_facade.hip,
-test_atomic.hip, test_deque.hip and test_bitset.hip, built by `python -m stdgpu_amd.build`) on the GPU: user kernels call the native device API, host code the
-agnostic API, as in the reference's own test suite."""
+test_atomic.hip, test_deque.hip, test_bitset.hip, test_vector.hip and test_reference_fixture.hip, built
+by `python -m stdgpu_amd.build`) on the GPU: user kernels call the native device API, host code the
+agnostic API, as in the reference's own test suite (test_reference_fixture.hip on the reference's own
+fixture types)."""
 import os
 import subprocess
 
@@ -12,7 +14,8 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 BUILD = os.path.join(ROOT, "tests", "cpp", "_build")
 
 
-@pytest.mark.parametrize("name", ["test_facade", "test_atomic", "test_deque", "test_bitset", "test_vector"])
+@pytest.mark.parametrize("name", ["test_facade", "test_atomic", "test_deque", "test_bitset", "test_vector",
+                                  "test_reference_fixture"])
 def test_cpp_facade_cases(name):
     exe = os.path.join(BUILD, name)
     assert os.path.exists(exe), "build the C++ tests first: python -m stdgpu_amd.build"

@@ -6,7 +6,9 @@ overflow counts, the size stripes, found through htab_get_view), one invariant a
   * ovf_count   - one bucket's overflow count raised by one;
   * empty_in_overflowed_bucket - a TOMB turned into EMPTY in a bucket that keys run through;
   * size        - the size counter one above the number of stored keys;
-  * unreachable - a key moved to an EMPTY slot of a bucket off its probe sequence.
+  * unreachable - a key moved to an EMPTY slot of a bucket off its probe sequence;
+  * away_count  - (map<u64, u32> with buckets of several slots) a nonzero away counter lowered by
+    one, so a key stored past its home bucket is no longer announced there.
 Each must make htab_valid report 0, and writing the original bytes back must make it report 1
 again, so the corruption and not the harness made the difference.
 """
@@ -97,6 +99,15 @@ class Image:
             off = b * self.W * self.sb + 12
             self.slots[off: off + 4] = word
 
+    def away_words(self):
+        """padding words of every slot (bytes 12-15), shape (nbuckets, W); in-bucket layouts only"""
+        rows = self.slots[: self.nslots * self.sb].reshape(self.nb, self.W, self.sb)
+        return rows[:, :, 12:16].copy().view(np.uint32).reshape(self.nb, self.W)
+
+    def set_away_field(self, b, f, value):
+        off = (b * self.W + 1 + f // 2) * self.sb + 12 + 2 * (f % 2)
+        self.slots[off: off + 2] = np.array([value], dtype=np.uint16).view(np.uint8)
+
     def add_size(self, d):
         self.stripe = (self.stripe.view(np.int64) + d).view(np.uint8)
 
@@ -156,6 +167,18 @@ def corrupt(img, what):
                 img.set_key(int(s), EMPTY[kdt])
                 return True
         return False
+    if what == "away_count":
+        if img.ovf_arr is not None or img.W < 2:
+            return False  # counters live in slot padding: map<u64, u32> with buckets of several slots
+        words = img.away_words()
+        nctr = 2 * (img.W - 1)
+        for b in np.nonzero(ovf > 0)[0]:
+            for f in range(nctr):
+                c = int((words[b, 1 + f // 2] >> (16 * (f % 2))) & 0xFFFF)
+                if 0 < c < 0x8000:
+                    img.set_away_field(int(b), f, c - 1)
+                    return True
+        return False
     if what == "size":
         img.add_size(1)
         return True
@@ -173,7 +196,7 @@ def corrupt(img, what):
     raise ValueError(what)
 
 
-CASES = ["duplicate", "ovf_count", "empty_in_overflowed_bucket", "size", "unreachable"]
+CASES = ["duplicate", "ovf_count", "empty_in_overflowed_bucket", "size", "unreachable", "away_count"]
 
 
 @pytest.mark.parametrize("name", list(KINDS))
@@ -199,4 +222,57 @@ def test_corruption_cases_apply_to_the_benchmarked_kind():
     for what in CASES:
         img = Image(tab, kdt, sb)
         assert corrupt(img, what), what
+    type(tab).destroyDeviceObject(tab)
+
+
+def away_recount(img):
+    """away counters recomputed on the host from the stored keys: keys homed at bucket h and stored in
+    another bucket, by fingerprint (bits 8-31 of (lo32 ^ hi32) * 0x9E3779B1 mod 2^32, scaled to 2(W-1)
+    counters)"""
+    keys = img.key_words()
+    stored = (keys != EMPTY[img.kdt]) & (keys != TOMB[img.kdt])
+    slots = np.nonzero(stored)[0]
+    k = keys[slots].astype(np.uint64)
+    homes = home_bucket(k, img.nb)
+    away = (slots // img.W) != homes
+    nctr = 2 * (img.W - 1)
+    ka = k[away]
+    x = ((ka & np.uint64(0xFFFFFFFF)) ^ (ka >> np.uint64(32))) * np.uint64(0x9E3779B1) & np.uint64(0xFFFFFFFF)
+    f = ((x >> np.uint64(8)) * np.uint64(nctr)) >> np.uint64(24)
+    cnt = np.zeros((img.nb, nctr), dtype=np.int64)
+    np.add.at(cnt, (homes[away], f.astype(np.int64)), 1)
+    return cnt, int(away.sum())
+
+
+def away_fields(img):
+    words = np.ascontiguousarray(img.away_words()[:, 1:])  # slots 1..W-1
+    return words.view(np.uint16).reshape(img.nb, -1).astype(np.int64)
+
+
+@pytest.mark.parametrize("lanes", [4, 8, 16])
+def test_away_counters_equal_a_host_recount(lanes):
+    """after inserts, erases of a quarter and re-inserts (bulk), every away counter equals the count of
+    stored keys it stands for, recomputed on the host from the slot bytes alone"""
+    cls, kdt, sdt, vdt, sb = KINDS["map_u64_u32"]
+    cap = 1 << 15
+    rng = np.random.default_rng(11)
+    keys = np.unique(rng.integers(1, 2**62, size=cap, dtype=np.uint64))[: int(cap * 0.9)]
+    rng.shuffle(keys)
+    tab = cls.createDeviceObject(cap, sa.Tuning(bucket_lanes=lanes))
+    k = torch.from_numpy(keys.view(sdt)).cuda()
+    v = torch.from_numpy(value_of(keys, vdt)).cuda()
+    tab.insert(k, v)
+    quarter = torch.from_numpy(keys[::4].copy().view(sdt)).cuda()
+    tab.erase(quarter)
+    img = Image(tab, kdt, sb)
+    want, n_away = away_recount(img)
+    assert n_away > 0
+    assert (away_fields(img) == want).all()
+    tab.insert(quarter, torch.from_numpy(value_of(keys[::4], vdt)).cuda())
+    img = Image(tab, kdt, sb)
+    want, n_away = away_recount(img)
+    assert (away_fields(img) == want).all()
+    assert tab.size() == len(keys) and tab.valid()
+    hits, got = tab.find(k)
+    assert bool(hits.all())
     type(tab).destroyDeviceObject(tab)
