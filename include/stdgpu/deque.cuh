@@ -1,15 +1,15 @@
-// Source-compatible stdgpu::deque<T> (src/stdgpu/deque.cuh) for 4-byte trivially copyable T, the
-// free-list container of voxel-block pipelines, on gfx950.
+// Source-compatible stdgpu::deque<T> (src/stdgpu/deque.cuh) for trivially copyable T of up to 256
+// bytes, on gfx950.
 //
 // Host: createDeviceObject / destroyDeviceObject, size / empty / full / capacity / max_size, clear,
-// valid, device_range (front to back over the ring).  Device (user kernels, per thread): push_back /
-// emplace_back, pop_front, size / empty / full, operator[] / at / front / back
-// (include/stdgpu_amd/deque_device.hpp).  The calling lanes of a
+// valid, shrink_to_fit, data, device_range (front to back over the ring).  Device (user kernels, per
+// thread): push_back / emplace_back / push_front / emplace_front / pop_back / pop_front, size / empty /
+// full, operator[] / at / front / back (include/stdgpu_amd/ring_device.hpp).  The calling lanes of a
 // wavefront reserve from the published-element or released-slot count with one atomic and take
-// consecutive ring positions with one more; a pop whose element is still being written, or a push
-// whose slot is still being read, waits on that slot's sequence word as the reference waits on its
-// per-slot lock.  Pushes and pops may be mixed in one kernel.  push_front and pop_back are not
-// provided.  Bulk pushes and pops of many values per call: htab_deque_* in <stdgpu_amd/htab.h>.
+// consecutive positions with one more; a push or pop then claims its slot by compare-and-swap on the
+// slot's state, waiting while another call on the slot finishes, as the reference waits on its
+// per-slot lock.  Pushes and pops of both ends may be mixed in one kernel.  Bulk pushes and pops of
+// many elements per call: htab_deque_* in <stdgpu_amd/htab.h>.
 #ifndef STDGPU_DEQUE_CUH
 #define STDGPU_DEQUE_CUH
 
@@ -19,7 +19,7 @@
 #include <utility>
 
 #include <stdgpu/impl/amd_core.h>
-#include <stdgpu_amd/deque_device.hpp>
+#include <stdgpu_amd/ring_device.hpp>
 
 namespace stdgpu
 {
@@ -91,8 +91,8 @@ template <typename T, typename Allocator = safe_device_allocator<T>>
 class deque
 {
 public:
-    static_assert(std::is_trivially_copyable_v<T> && sizeof(T) == 4,
-                  "stdgpu::deque (MI355X library): 4-byte trivially copyable value types");
+    static_assert(std::is_trivially_copyable_v<T> && sizeof(T) <= 256,
+                  "stdgpu::deque (MI355X library): trivially copyable value types of at most 256 bytes");
 
     using value_type = T;
     using allocator_type = Allocator;
@@ -115,7 +115,8 @@ public:
         if (capacity <= 0) throw std::invalid_argument("stdgpu::deque: capacity must be positive");
         deque d;
         d._allocator = allocator;
-        detail::check(htab_deque_create(capacity, &d._h), "deque::createDeviceObject");
+        detail::check(htab_deque_create_sized(capacity, static_cast<std::int64_t>(sizeof(T)), &d._h),
+                      "deque::createDeviceObject");
         detail::check(htab_deque_get_view(d._h, &d._view), "deque::createDeviceObject");
         return d;
     }
@@ -135,8 +136,8 @@ public:
     [[nodiscard]] __host__ __device__ allocator_type get_allocator() const noexcept { return _allocator; }
 
     // ---------------------------------------------------------------- device element access
-    __device__ reference at(const index_type n) { return *reinterpret_cast<pointer>(ref().at(n)); }
-    __device__ const_reference at(const index_type n) const { return *reinterpret_cast<const_pointer>(ref().at(n)); }
+    __device__ reference at(const index_type n) { return *ref().template at<T>(n); }
+    __device__ const_reference at(const index_type n) const { return *ref().template at<T>(n); }
     __device__ reference operator[](const index_type n) { return at(n); }
     __device__ const_reference operator[](const index_type n) const { return at(n); }
     __device__ reference front() { return at(0); }
@@ -145,7 +146,8 @@ public:
     __device__ const_reference back() const { return at(static_cast<index_type>(ref().size()) - 1); }
 
     // ---------------------------------------------------------------- device modifiers
-    __device__ bool push_back(const T& element) { return ref().push_back(__builtin_bit_cast(std::uint32_t, element)); }
+    __device__ bool push_back(const T& element) { return ref().push_back(element); }
+    __device__ bool push_front(const T& element) { return ref().push_front(element); }
 
     template <class... Args>
     __device__ bool emplace_back(Args&&... args)
@@ -153,12 +155,24 @@ public:
         return push_back(T(std::forward<Args>(args)...));
     }
 
-    // the popped element and true, or T() and false (deque.cuh:234-239)
+    template <class... Args>
+    __device__ bool emplace_front(Args&&... args)
+    {
+        return push_front(T(std::forward<Args>(args)...));
+    }
+
+    // the popped element and true, or T() and false (deque.cuh:213-239)
+    __device__ pair<T, bool> pop_back()
+    {
+        T v{};
+        const bool ok = ref().pop_back(&v);
+        return pair<T, bool>(ok ? v : T(), ok);
+    }
     __device__ pair<T, bool> pop_front()
     {
-        std::uint32_t bits = 0;
-        if (ref().pop_front(&bits)) return pair<T, bool>(__builtin_bit_cast(T, bits), true);
-        return pair<T, bool>(T(), false);
+        T v{};
+        const bool ok = ref().pop_front(&v);
+        return pair<T, bool>(ok ? v : T(), ok);
     }
 
     // ---------------------------------------------------------------- capacity
@@ -211,21 +225,24 @@ public:
         detail::check(hipStreamSynchronize(policy.stream) == hipSuccess ? HTAB_OK : HTAB_E_HIP, "deque::clear");
     }
 
-    // the counters describe at most capacity elements (deque_detail.cuh:637-670 checks its own layout)
+    // the capacity stays (the reference may or may not shrink, deque.inc shrink_to_fit)
+    void shrink_to_fit() {}
+
+    template <typename ExecutionPolicy, std::enable_if_t<is_execution_policy_v<ExecutionPolicy>, int> = 0>
+    void shrink_to_fit([[maybe_unused]] ExecutionPolicy&& policy)
+    {
+    }
+
+    // the elements' slots full, every other slot free, the counts consistent (deque_detail.cuh:514-573)
     [[nodiscard]] bool valid() const { return valid(execution::device); }
 
     template <typename ExecutionPolicy, std::enable_if_t<is_execution_policy_v<ExecutionPolicy>, int> = 0>
     [[nodiscard]] bool valid(ExecutionPolicy&& policy) const
     {
         if (!_h) return true;
-        std::uint64_t c[2] = { 0, 0 };
-        if (hipMemcpyAsync(&c[0], _view.begin, sizeof(std::uint64_t), hipMemcpyDeviceToHost, policy.stream) !=
-                    hipSuccess ||
-            hipMemcpyAsync(&c[1], _view.end, sizeof(std::uint64_t), hipMemcpyDeviceToHost, policy.stream) !=
-                    hipSuccess ||
-            hipStreamSynchronize(policy.stream) != hipSuccess)
-            return false;
-        return c[0] <= c[1] && c[1] - c[0] <= static_cast<std::uint64_t>(_view.capacity);
+        std::int32_t ok = 0;
+        detail::check(htab_deque_valid(_h, &ok, policy.stream), "deque::valid");
+        return ok != 0;
     }
 
     // the elements front to back (a host call: it reads the front counter and the size)
@@ -249,7 +266,7 @@ public:
     htab_deque* native_handle() const { return _h; }
 
 private:
-    __host__ __device__ htab_dev::deque_ref ref() const { return htab_dev::deque_ref::from_view(_view); }
+    __host__ __device__ htab_dev::ring_ref ref() const { return htab_dev::ring_ref::from_view(_view); }
 
     std::uint64_t front_position(hipStream_t s) const
     {

@@ -1,15 +1,15 @@
-// Source-compatible stdgpu::vector<T> (src/stdgpu/vector.cuh) for 4-byte trivially copyable T, on
-// gfx950.
+// Source-compatible stdgpu::vector<T> (src/stdgpu/vector.cuh) for trivially copyable T of up to 256
+// bytes, on gfx950.
 //
 // Host: createDeviceObject / destroyDeviceObject, insert at the end and erase up to the end from
 // device ranges (the only positions the reference implements, vector.cuh:73-75), clear, size / empty
-// / full / capacity / max_size, shrink_to_fit, device_begin / device_end / device_range, valid.  Device (user
-// kernels, per thread): push_back / emplace_back / pop_back, size / empty / full, operator[] / at /
-// front / back (include/stdgpu_amd/vector_device.hpp).  The calling lanes of a wavefront take their
-// positions with one atomic.  A kernel pushes or pops a given vector, not both: a pop could take a
-// position whose push has not written it yet (the reference waits there on a per-slot lock);
-// deque<T> takes pushes and pops mixed in one kernel.  Bulk pushes and pops of many values per call:
-// htab_vector_* in <stdgpu_amd/htab.h>.
+// / full / capacity / max_size, shrink_to_fit, device_begin / device_end / device_range, valid.  Device
+// (user kernels, per thread): push_back / emplace_back / pop_back, size / empty / full, operator[] /
+// at / front / back (include/stdgpu_amd/ring_device.hpp: the back end of the ring, which never
+// wraps).  The calling lanes of a wavefront take their positions with one reservation and one atomic;
+// each push or pop then claims its slot by compare-and-swap on the slot's state, as the reference
+// claims its per-slot lock, so pushes and pops may be mixed in one kernel.  Bulk pushes and pops of
+// many elements per call: htab_vector_* in <stdgpu_amd/htab.h>.
 #ifndef STDGPU_VECTOR_CUH
 #define STDGPU_VECTOR_CUH
 
@@ -19,7 +19,7 @@
 #include <utility>
 
 #include <stdgpu/impl/amd_core.h>
-#include <stdgpu_amd/vector_device.hpp>
+#include <stdgpu_amd/ring_device.hpp>
 
 namespace stdgpu
 {
@@ -28,8 +28,8 @@ template <typename T, typename Allocator = safe_device_allocator<T>>
 class vector
 {
 public:
-    static_assert(std::is_trivially_copyable_v<T> && sizeof(T) == 4,
-                  "stdgpu::vector (MI355X library): 4-byte trivially copyable value types");
+    static_assert(std::is_trivially_copyable_v<T> && sizeof(T) <= 256,
+                  "stdgpu::vector (MI355X library): trivially copyable value types of at most 256 bytes");
 
     using value_type = T;
     using allocator_type = Allocator;
@@ -52,7 +52,8 @@ public:
         if (capacity <= 0) throw std::invalid_argument("stdgpu::vector: capacity must be positive");
         vector v;
         v._allocator = allocator;
-        detail::check(htab_vector_create(capacity, &v._h), "vector::createDeviceObject");
+        detail::check(htab_vector_create_sized(capacity, static_cast<std::int64_t>(sizeof(T)), &v._h),
+                      "vector::createDeviceObject");
         detail::check(htab_vector_get_view(v._h, &v._view), "vector::createDeviceObject");
         return v;
     }
@@ -82,7 +83,7 @@ public:
     __device__ const_reference back() const { return at(size() - 1); }
 
     // ---------------------------------------------------------------- device modifiers
-    __device__ bool push_back(const T& element) { return ref().push_back(__builtin_bit_cast(std::uint32_t, element)); }
+    __device__ bool push_back(const T& element) { return ref().push_back(element); }
 
     template <class... Args>
     __device__ bool emplace_back(Args&&... args)
@@ -93,9 +94,9 @@ public:
     // the popped element and true, or T() and false (vector.cuh:234-239)
     __device__ pair<T, bool> pop_back()
     {
-        std::uint32_t bits = 0;
-        if (ref().pop_back(&bits)) return pair<T, bool>(__builtin_bit_cast(T, bits), true);
-        return pair<T, bool>(T(), false);
+        T v{};
+        const bool ok = ref().pop_back(&v);
+        return pair<T, bool>(ok ? v : T(), ok);
     }
 
     // ---------------------------------------------------------------- host modifiers
@@ -116,7 +117,7 @@ public:
         if (position.get() != data() + n0) return;
         const std::int64_t n = static_cast<std::int64_t>(last - first);
         if (n <= 0 || n0 + n > capacity()) return;
-        detail::check(htab_vector_push_back(_h, reinterpret_cast<const std::uint32_t*>(detail::raw(first)), n, nullptr,
+        detail::check(htab_vector_push_back(_h, static_cast<const void*>(detail::raw(first)), n, nullptr,
                                             policy.stream),
                       "vector::insert");
         sync(policy.stream, "vector::insert");
@@ -219,24 +220,22 @@ public:
         return stdgpu::device_range<const T>(data(), size(std::forward<ExecutionPolicy>(policy)));
     }
 
-    // the size word lies in [0, capacity] (vector_detail.cuh:616-640 checks its own layout)
+    // the elements' slots full, every other slot free, the counts consistent (vector_detail.cuh:514-577)
     [[nodiscard]] bool valid() const { return valid(execution::device); }
 
     template <typename ExecutionPolicy, std::enable_if_t<is_execution_policy_v<ExecutionPolicy>, int> = 0>
     [[nodiscard]] bool valid(ExecutionPolicy&& policy) const
     {
         if (!_h) return true;
-        std::int64_t n = 0;
-        if (hipMemcpyAsync(&n, _view.size, sizeof(n), hipMemcpyDeviceToHost, policy.stream) != hipSuccess ||
-            hipStreamSynchronize(policy.stream) != hipSuccess)
-            return false;
-        return n >= 0 && n <= _view.capacity;
+        std::int32_t ok = 0;
+        detail::check(htab_vector_valid(_h, &ok, policy.stream), "vector::valid");
+        return ok != 0;
     }
 
     htab_vector* native_handle() const { return _h; }
 
 private:
-    __host__ __device__ htab_dev::vector_ref ref() const { return htab_dev::vector_ref::from_view(_view); }
+    __host__ __device__ htab_dev::ring_ref ref() const { return htab_dev::ring_ref::from_view(_view); }
 
     static void sync(hipStream_t s, const char* what)
     {

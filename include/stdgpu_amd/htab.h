@@ -304,37 +304,55 @@ int htab_snapshot(htab* table, void* host_buffer, int64_t bytes, void* stream);
 int htab_restore(const void* host_buffer, int64_t bytes, htab** out);
 
 /* ------------------------------------------------------------------------------------------
- * vector<uint32_t> used as a stack, the container of the reference's excess free list
- * (src/stdgpu/vector.cuh; push_back / pop_back vector_detail.cuh:181-302, clear_iota :361-368).
- * One bulk call only pushes or only pops; positions are claimed with one atomic per wavefront.
+ * vector<T> and deque<T> for elements of 1..256 bytes (src/stdgpu/vector.cuh, deque.cuh), on one ring
+ * of slots: the elements are the positions [begin, end), position p in slot p mod capacity (a vector
+ * keeps begin at 0, so its positions are the indices of its data).  Bulk calls push or pop n elements
+ * at one end and reserve their positions at once, so their result is that of a loop of the reference's
+ * single calls: push_back puts element i at end + i, pop_back takes end - 1 - i, push_front puts it at
+ * begin - 1 - i, pop_front takes begin + i; size follows the reference's clamped size().  User kernels
+ * push and pop per thread, at both ends and mixed in one kernel, through the view
+ * (include/stdgpu_amd/ring_device.hpp; the C++ stdgpu::vector and stdgpu::deque).
  * ------------------------------------------------------------------------------------------ */
-typedef struct htab_vector htab_vector;
+typedef struct htab_ring_view
+{
+    void* data;          /* capacity elements of elem_bytes */
+    uint32_t* state;     /* capacity slot states: the low two bits are free, writing, full, reading */
+    uint64_t* begin;     /* front position */
+    uint64_t* end;       /* back position; size = end - begin */
+    int64_t* ready;      /* elements published and not yet reserved by a pop */
+    int64_t* free_slots; /* slots released and not yet reserved by a push */
+    int64_t capacity;
+    int64_t elem_bytes;
+} htab_ring_view;
 
+/* vector<T> (vector_detail.cuh): push_back / pop_back :181-302, clear_iota :361-368, size :454-512 */
+typedef struct htab_vector htab_vector;
+typedef htab_ring_view htab_vector_view;
+
+/* createDeviceObject(capacity) of a vector of elem_bytes-byte elements (vector_detail.cuh:40-80) */
+int htab_vector_create_sized(int64_t capacity, int64_t elem_bytes, htab_vector** out);
+/* the same with 4-byte elements (the excess free list's vector<index_t>) */
 int htab_vector_create(int64_t capacity, htab_vector** out);
 int htab_vector_destroy(htab_vector* vec);
-/* push_back of n values; pushed[i] (optional) = 1 iff value i found room (vector_detail.cuh:181-240) */
-int htab_vector_push_back(htab_vector* vec, const uint32_t* values, int64_t n, uint8_t* pushed, void* stream);
-/* pop_back of up to n values from the top: out[i] = the value, or 0xFFFFFFFF when the stack ran empty
- * (out may be null: the values are dropped, as erase(end - n, end) does); popped[i] (optional) = 1 iff
- * element i got a value (vector_detail.cuh:242-302) */
-int htab_vector_pop_back(htab_vector* vec, uint32_t* out, int64_t n, uint8_t* popped, void* stream);
-/* size = 0 (vector_detail.cuh:579-614 clear); one kernel, no memset (safe inside a captured graph) */
+/* push_back of n elements; pushed[i] (optional) = 1 iff element i found room (vector_detail.cuh:181-240) */
+int htab_vector_push_back(htab_vector* vec, const void* values, int64_t n, uint8_t* pushed, void* stream);
+/* pop_back of up to n elements from the top: out[i] = the element, or all-ones bytes (0xFFFFFFFF for
+ * 4-byte elements) when the vector ran empty (out may be null: the elements are dropped, as
+ * erase(end - n, end) does); popped[i] (optional) = 1 iff element i got one (vector_detail.cuh:242-302) */
+int htab_vector_pop_back(htab_vector* vec, void* out, int64_t n, uint8_t* popped, void* stream);
+/* size = 0 (vector_detail.cuh:579-614 clear); kernels only, no memset (safe inside a captured graph) */
 int htab_vector_clear(htab_vector* vec, void* stream);
-/* device addresses for user kernels (include/stdgpu_amd/vector_device.hpp): the elements, the int64 size
- * word, and the capacity */
-typedef struct htab_vector_view
-{
-    uint32_t* data;
-    int64_t* size;
-    int64_t capacity;
-} htab_vector_view;
+/* device addresses for user kernels (include/stdgpu_amd/ring_device.hpp) */
 int htab_vector_get_view(htab_vector* vec, htab_vector_view* out);
-/* vector_clear_iota: element i = first + i, size = capacity (vector_detail.cuh:361-368) */
+/* vector_clear_iota of 4-byte elements: element i = first + i, size = capacity (vector_detail.cuh:361-368) */
 int htab_vector_fill_iota(htab_vector* vec, uint32_t first, void* stream);
 /* size() clamped to [0, capacity] (vector_detail.cuh:454-512) */
 int htab_vector_size(htab_vector* vec, int64_t* out, void* stream);
-/* the elements from bottom to top into out (device memory, capacity entries); *n_out = size */
-int htab_vector_contents(htab_vector* vec, uint32_t* out, int64_t* n_out, void* stream);
+/* the elements from bottom to top into out (device memory, capacity elements); *n_out = size */
+int htab_vector_contents(htab_vector* vec, void* out, int64_t* n_out, void* stream);
+/* valid() (vector_detail.cuh:514-577): the positions [0, size) full, every other slot free, the counts
+ * consistent; for a vector no kernel is changing */
+int htab_vector_valid(htab_vector* vec, int32_t* out, void* stream);
 
 /* ------------------------------------------------------------------------------------------
  * bitset over 32-bit blocks (src/stdgpu/bitset.cuh).  Bulk forms of the per-bit device calls:
@@ -420,43 +438,37 @@ int htab_atomic_store(htab_atomic* a, int64_t index, const void* value, void* st
 const char* htab_last_error(void);
 
 /* ------------------------------------------------------------------------------------------
- * Free-list deque<uint32_t> (deque.cuh; deque_detail.cuh:202-260 push_back, 390-448 pop_front),
- * used by voxel-block pipelines to hand out slot numbers.  A bulk call only pushes or only pops and
- * reserves its positions once, so its result is that of a loop of single calls; size follows the
- * reference's clamped size().  User kernels push and pop per thread through the view
- * (include/stdgpu_amd/deque_device.hpp, the C++ stdgpu::deque): ring position p lives in slot
- * p mod capacity, with seq[slot] = 2t (free), 2t + 1 (holds position p) for turn t = p / capacity.
+ * deque<T> (deque.cuh; deque_detail.cuh:202-448 push_back / pop_back / push_front / pop_front,
+ * :482-512 size), the free list of voxel-block pipelines among others; see htab_ring_view above.
  * ------------------------------------------------------------------------------------------ */
 typedef struct htab_deque htab_deque;
-
-typedef struct htab_deque_view
-{
-    uint32_t* data;      /* capacity values */
-    uint32_t* seq;       /* capacity slot states */
-    uint64_t* begin;     /* front counter */
-    uint64_t* end;       /* back counter; size = end - begin */
-    int64_t* ready;      /* elements published and not yet reserved by a pop */
-    int64_t* free_slots; /* slots released and not yet reserved by a push */
-    int64_t capacity;
-} htab_deque_view;
+typedef htab_ring_view htab_deque_view;
 
 /* device pointers of the deque for user kernels (no reference counterpart: the reference's deque
  * object is itself the device handle) */
 int htab_deque_get_view(const htab_deque* q, htab_deque_view* out);
 /* clear() (deque_detail.cuh:575-635): no elements, every slot free */
 int htab_deque_clear(htab_deque* q, void* stream);
-
+/* createDeviceObject(capacity) of a deque of elem_bytes-byte elements (deque_detail.cuh:40-80) */
+int htab_deque_create_sized(int64_t capacity, int64_t elem_bytes, htab_deque** out);
+/* the same with 4-byte elements (a free list of slot numbers) */
 int htab_deque_create(int64_t capacity, htab_deque** out);
 int htab_deque_destroy(htab_deque* q);
-/* push_back of n values; pushed[i] (optional) = 1 iff value i was stored (0 when full). */
-int htab_deque_push_back(htab_deque* q, const uint32_t* values, int64_t n, uint8_t* pushed, void* stream);
-/* pop_front of n values; popped[i] = 1 iff out[i] received a value (0 when empty). */
-int htab_deque_pop_front(htab_deque* q, uint32_t* out, int64_t n, uint8_t* popped, void* stream);
-/* Fill with 0, 1, ..., capacity-1 (front to back). */
+/* push_back / push_front of n elements; pushed[i] (optional) = 1 iff element i found room */
+int htab_deque_push_back(htab_deque* q, const void* values, int64_t n, uint8_t* pushed, void* stream);
+int htab_deque_push_front(htab_deque* q, const void* values, int64_t n, uint8_t* pushed, void* stream);
+/* pop_front / pop_back of up to n elements: out[i] = the element, or all-ones bytes when the deque ran
+ * empty (out may be null); popped[i] (optional) = 1 iff element i got one */
+int htab_deque_pop_front(htab_deque* q, void* out, int64_t n, uint8_t* popped, void* stream);
+int htab_deque_pop_back(htab_deque* q, void* out, int64_t n, uint8_t* popped, void* stream);
+/* 4-byte elements: the deque holds 0, 1, ..., capacity - 1 front to back (the voxel free list's start) */
 int htab_deque_fill_iota(htab_deque* q, void* stream);
 int htab_deque_size(htab_deque* q, int64_t* out, void* stream);
-/* Copies the stored values front to back into out (capacity entries); *n_out = size. */
-int htab_deque_contents(htab_deque* q, uint32_t* out, int64_t* n_out, void* stream);
+/* the elements front to back into out (device memory, capacity elements); *n_out = size */
+int htab_deque_contents(htab_deque* q, void* out, int64_t* n_out, void* stream);
+/* valid() (deque_detail.cuh:514-573): the positions [begin, end) full, every other slot free, the counts
+ * consistent; for a deque no kernel is changing */
+int htab_deque_valid(htab_deque* q, int32_t* out, void* stream);
 
 #ifdef __cplusplus
 }

@@ -149,6 +149,23 @@ a_fetch_inc_mod(unsigned int* p, unsigned int capacity)
     return old;
 }
 
+// atomicDec-style wrap (openmp/impl/atomic_detail.h:205-216): stored value goes mod-1,...,1,0,mod-1,...
+// (caller passes capacity, backend gets capacity-1, atomic_detail.cuh:693-699)
+static inline unsigned int
+a_fetch_dec_mod(unsigned int* p, unsigned int capacity)
+{
+    unsigned int old;
+    const unsigned int arg = capacity - 1;
+    seq_cst_fence();
+#pragma omp critical
+    {
+        old = *p;
+        *p = (old == 0U || old > arg) ? arg : old - 1U;
+    }
+    seq_cst_fence();
+    return old;
+}
+
 // ---------------------------------------------------------------- bit helpers (bit_detail.h)
 static inline std::uint64_t
 bit_ceil(std::uint64_t x)
@@ -789,6 +806,54 @@ struct u32_deque
         }
         return true;
     }
+    // deque_detail.cuh:262-320: the position is the stored end minus one, modulo the capacity
+    bool pop_back(unsigned int* out)
+    {
+        if (size() == 0) return false;
+        const index_t cur = a_fetch_sub(&size_, 1);
+        if (cur <= 0) return false;
+        index_t pos = static_cast<index_t>(a_fetch_dec_mod(&end_, static_cast<unsigned int>(cap)));
+        pos = (pos == 0) ? cap - 1 : pos - 1;
+        bool popped = false;
+        while (!popped)
+        {
+            if (locks.try_lock(pos))
+            {
+                if (occ.test(pos))
+                {
+                    occ.reset(pos);
+                    *out = data[pos];
+                    popped = true;
+                }
+                locks.unlock(pos);
+            }
+        }
+        return true;
+    }
+    // deque_detail.cuh:330-388: the position is the stored begin minus one, modulo the capacity
+    bool push_front(unsigned int v)
+    {
+        if (size() == cap) return false;
+        const index_t cur = a_fetch_add(&size_, 1);
+        if (cur >= cap) return false;
+        index_t pos = static_cast<index_t>(a_fetch_dec_mod(&begin_, static_cast<unsigned int>(cap)));
+        pos = (pos == 0) ? cap - 1 : pos - 1;
+        bool pushed = false;
+        while (!pushed)
+        {
+            if (locks.try_lock(pos))
+            {
+                if (!occ.test(pos))
+                {
+                    data[pos] = v;
+                    occ.set(pos);
+                    pushed = true;
+                }
+                locks.unlock(pos);
+            }
+        }
+        return true;
+    }
     // deque_detail.cuh:390-448
     bool pop_front(unsigned int* out)
     {
@@ -971,6 +1036,34 @@ orc_deque_pop_front(void* d, std::uint32_t* out, std::int64_t n, std::uint8_t* o
     {
         unsigned int v = 0;
         const bool r = q->pop_front(&v);
+        if (ok) ok[i] = r ? 1 : 0;
+        out[i] = r ? v : 0xFFFFFFFFU;
+    }
+}
+
+// push n values at the front; ok[i] = pushed
+void
+orc_deque_push_front(void* d, const std::uint32_t* v, std::int64_t n, std::uint8_t* ok, int threads)
+{
+    u32_deque* q = static_cast<u32_deque*>(d);
+#pragma omp parallel for num_threads(threads > 0 ? threads : 1) schedule(static)
+    for (std::int64_t i = 0; i < n; ++i)
+    {
+        const bool r = q->push_front(v[i]);
+        if (ok) ok[i] = r ? 1 : 0;
+    }
+}
+
+// pop n values from the back; ok[i] = popped, out[i] = value
+void
+orc_deque_pop_back(void* d, std::uint32_t* out, std::int64_t n, std::uint8_t* ok, int threads)
+{
+    u32_deque* q = static_cast<u32_deque*>(d);
+#pragma omp parallel for num_threads(threads > 0 ? threads : 1) schedule(static)
+    for (std::int64_t i = 0; i < n; ++i)
+    {
+        unsigned int v = 0;
+        const bool r = q->pop_back(&v);
         if (ok) ok[i] = r ? 1 : 0;
         out[i] = r ? v : 0xFFFFFFFFU;
     }

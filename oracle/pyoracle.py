@@ -68,6 +68,8 @@ def load():
         "orc_deque_size": (I64, [P]),
         "orc_deque_push_back": (None, [P, P, I64, P, I32]),
         "orc_deque_pop_front": (None, [P, P, I64, P, I32]),
+        "orc_deque_push_front": (None, [P, P, I64, P, I32]),
+        "orc_deque_pop_back": (None, [P, P, I64, P, I32]),
         "orc_deque_contents": (I64, [P, P]),
         "orc_voxel_insert": (I64, [P, P, P, I64]),
     }
@@ -226,7 +228,7 @@ class Vector:
 
 
 class Deque:
-    """deque<uint32_t> on the CPU, reference algorithm (push_back / pop_front)."""
+    """deque<uint32_t> on the CPU, reference algorithm (push_back / pop_front / push_front / pop_back)."""
 
     def __init__(self, capacity: int):
         self._d = load().orc_deque_create(capacity)
@@ -253,6 +255,18 @@ class Deque:
         out = np.zeros(n, dtype=np.uint32)
         ok = np.zeros(n, dtype=np.uint8)
         load().orc_deque_pop_front(self._d, _p(out), n, _p(ok), threads)
+        return out, ok
+
+    def push_front(self, values, threads: int = 1):
+        v = np.ascontiguousarray(values, dtype=np.uint32)
+        ok = np.zeros(len(v), dtype=np.uint8)
+        load().orc_deque_push_front(self._d, _p(v), len(v), _p(ok), threads)
+        return ok
+
+    def pop_back(self, n: int, threads: int = 1):
+        out = np.zeros(n, dtype=np.uint32)
+        ok = np.zeros(n, dtype=np.uint8)
+        load().orc_deque_pop_back(self._d, _p(out), n, _p(ok), threads)
         return out, ok
 
     def size(self) -> int:

@@ -8,6 +8,7 @@ from .containers import (  # noqa: F401
     atomic,
     atomic_ref_apply,
     bitset,
+    deque,
     deque_u32,
     unordered_map,
     unordered_map_u32_u32,
@@ -16,6 +17,7 @@ from .containers import (  # noqa: F401
     unordered_set,
     unordered_set_u32,
     unordered_set_u64,
+    vector,
     vector_u32,
 )
 from ._lib import HtabError, load  # noqa: F401

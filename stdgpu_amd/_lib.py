@@ -116,24 +116,23 @@ class htab_bitset_view(ctypes.Structure):
     _fields_ = [("blocks", ctypes.c_void_p), ("nbits", ctypes.c_int64)]
 
 
-class htab_deque_view(ctypes.Structure):
+class htab_ring_view(ctypes.Structure):
+    """the device view of a vector or deque (htab_ring_view in htab.h)"""
+
     _fields_ = [
         ("data", ctypes.c_void_p),
-        ("seq", ctypes.c_void_p),
+        ("state", ctypes.c_void_p),
         ("begin", ctypes.c_void_p),
         ("end", ctypes.c_void_p),
         ("ready", ctypes.c_void_p),
         ("free_slots", ctypes.c_void_p),
         ("capacity", ctypes.c_int64),
+        ("elem_bytes", ctypes.c_int64),
     ]
 
 
-class htab_vector_view(ctypes.Structure):
-    _fields_ = [
-        ("data", ctypes.c_void_p),
-        ("size", ctypes.c_void_p),
-        ("capacity", ctypes.c_int64),
-    ]
+htab_deque_view = htab_ring_view
+htab_vector_view = htab_ring_view
 
 
 class htab_view(ctypes.Structure):
@@ -192,9 +191,13 @@ SIGNATURES = {
     "htab_server_run": (_I32, [_P, _P, _P, _I64, _P, _P, _P, _P]),
     "htab_server_replay_plain": (_I32, [_P, _P, _P, _I64, _I32, _P]),
     "htab_deque_create": (_I32, [_I64, ctypes.POINTER(_P)]),
+    "htab_deque_create_sized": (_I32, [_I64, _I64, ctypes.POINTER(_P)]),
     "htab_deque_destroy": (_I32, [_P]),
     "htab_deque_push_back": (_I32, [_P, _P, _I64, _P, _P]),
+    "htab_deque_push_front": (_I32, [_P, _P, _I64, _P, _P]),
     "htab_deque_pop_front": (_I32, [_P, _P, _I64, _P, _P]),
+    "htab_deque_pop_back": (_I32, [_P, _P, _I64, _P, _P]),
+    "htab_deque_valid": (_I32, [_P, ctypes.POINTER(ctypes.c_int32), _P]),
     "htab_deque_fill_iota": (_I32, [_P, _P]),
     "htab_deque_size": (_I32, [_P, ctypes.POINTER(_I64), _P]),
     "htab_deque_contents": (_I32, [_P, _P, ctypes.POINTER(_I64), _P]),
@@ -218,6 +221,8 @@ SIGNATURES = {
     "htab_atomic_load": (_I32, [_P, _I64, _P, _P]),
     "htab_atomic_store": (_I32, [_P, _I64, _P, _P]),
     "htab_vector_create": (_I32, [_I64, ctypes.POINTER(_P)]),
+    "htab_vector_create_sized": (_I32, [_I64, _I64, ctypes.POINTER(_P)]),
+    "htab_vector_valid": (_I32, [_P, ctypes.POINTER(ctypes.c_int32), _P]),
     "htab_vector_destroy": (_I32, [_P]),
     "htab_vector_push_back": (_I32, [_P, _P, _I64, _P, _P]),
     "htab_vector_pop_back": (_I32, [_P, _P, _I64, _P, _P]),

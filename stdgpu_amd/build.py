@@ -14,14 +14,13 @@ PKG = os.path.join(ROOT, "stdgpu_amd")
 CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(PKG, "libstdgpu_amd.so")
 ROOFLINE_LIB = os.path.join(PKG, "libstdgpu_amd_roofline.so")
-SOURCES = ["htab.hip", "noalloc.hip", "deque.hip", "vector.hip", "bitset.hip", "atomic.hip", "server.hip", "kind_set_u32.hip", "kind_set_u64.hip", "kind_map_u64_u32.hip",
+SOURCES = ["htab.hip", "noalloc.hip", "ring.hip", "bitset.hip", "atomic.hip", "server.hip", "kind_set_u32.hip", "kind_set_u64.hip", "kind_map_u64_u32.hip",
            "kind_map_u32_u32.hip", "kind_map_u64_u64.hip"]
 HEADERS = [
     os.path.join(ROOT, "include", "stdgpu_amd", "htab.h"),
     os.path.join(ROOT, "include", "stdgpu_amd", "htab_device.hpp"),
     os.path.join(ROOT, "include", "stdgpu_amd", "atomic_device.hpp"),
-    os.path.join(ROOT, "include", "stdgpu_amd", "deque_device.hpp"),
-    os.path.join(ROOT, "include", "stdgpu_amd", "vector_device.hpp"),
+    os.path.join(ROOT, "include", "stdgpu_amd", "ring_device.hpp"),
     os.path.join(CSRC, "htab_kernels.hpp"),
     os.path.join(CSRC, "wave.hpp"),
     os.path.join(CSRC, "htab_internal.hpp"),

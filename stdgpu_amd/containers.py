@@ -271,125 +271,131 @@ def unordered_set(key: str = "u32"):
     return {"u32": unordered_set_u32, "u64": unordered_set_u64}[key]
 
 
-class deque_u32:
-    """deque<uint32_t> free list (deque.cuh) with bulk push_back / pop_front."""
+class _ring:
+    """Shared host side of vector and deque: bulk calls over a ring of elem_bytes-byte elements.
+    Elements travel as tensors of `dtype` (one element per entry) or, for records that no dtype
+    covers, as uint8 tensors of shape (n, elem_bytes)."""
 
-    def __init__(self, capacity: int):
+    _prefix = ""
+
+    def __init__(self, capacity: int, dtype=None, elem_bytes: int | None = None):
+        torch = _torch()
+        self.dtype = dtype if dtype is not None else (torch.int32 if elem_bytes in (None, 4) else torch.uint8)
+        item = torch.empty((), dtype=self.dtype).element_size()
+        self.elem_bytes = int(elem_bytes) if elem_bytes is not None else item
+        if self.elem_bytes != item and self.dtype != torch.uint8:
+            raise ValueError("elem_bytes must match the dtype's size (or use uint8 records)")
         h = ctypes.c_void_p()
-        check(_lib.load().htab_deque_create(int(capacity), ctypes.byref(h)), "htab_deque_create")
+        fn = getattr(_lib.load(), f"htab_{self._prefix}_create_sized")
+        check(fn(int(capacity), self.elem_bytes, ctypes.byref(h)), f"htab_{self._prefix}_create_sized")
         self._h = h
         self.capacity = capacity
 
+    def _call(self, name, *args):
+        check(getattr(_lib.load(), f"htab_{self._prefix}_{name}")(self._h, *args), f"htab_{self._prefix}_{name}")
+
+    def _records(self, n):
+        torch = _torch()
+        item = torch.empty((), dtype=self.dtype).element_size()
+        shape = (n,) if item == self.elem_bytes else (n, self.elem_bytes)
+        return torch.empty(shape, dtype=self.dtype, device="cuda")
+
     def destroy(self):
         if self._h:
-            check(_lib.load().htab_deque_destroy(self._h), "htab_deque_destroy")
+            check(getattr(_lib.load(), f"htab_{self._prefix}_destroy")(self._h), f"htab_{self._prefix}_destroy")
             self._h = ctypes.c_void_p()
+
+    def clear(self, stream=None):
+        self._call("clear", _stream_ptr(stream))
+
+    def view(self) -> _lib.htab_ring_view:
+        """device pointers for kernels that push and pop per thread (include/stdgpu_amd/ring_device.hpp)"""
+        v = _lib.htab_ring_view()
+        self._call("get_view", ctypes.byref(v))
+        return v
+
+    def _push(self, name, values, pushed, stream):
+        n = values.shape[0] if values.dim() > 0 else 0
+        self._call(name, _ptr(values), int(n), _ptr(pushed), _stream_ptr(stream))
+        return pushed
+
+    def _pop(self, name, n, out, popped, stream):
+        torch = _torch()
+        if out is None:
+            out = self._records(n)
+        if popped is None:
+            popped = torch.empty(n, dtype=torch.uint8, device="cuda")
+        self._call(name, _ptr(out), int(n), _ptr(popped), _stream_ptr(stream))
+        return out, popped
+
+    def size(self, stream=None) -> int:
+        out = ctypes.c_int64()
+        self._call("size", ctypes.byref(out), _stream_ptr(stream))
+        return out.value
+
+    def valid(self, stream=None) -> bool:
+        out = ctypes.c_int32()
+        self._call("valid", ctypes.byref(out), _stream_ptr(stream))
+        return out.value != 0
+
+    def contents(self, stream=None):
+        """the elements front (bottom) to back (top)"""
+        out = self._records(max(self.capacity, 1))
+        n = ctypes.c_int64()
+        self._call("contents", _ptr(out), ctypes.byref(n), _stream_ptr(stream))
+        return out[: n.value]
+
+
+class deque(_ring):
+    """deque<T> (deque.cuh) with bulk push_back / push_front / pop_back / pop_front; a bulk call
+    reserves its positions once and returns what a loop of single calls would."""
+
+    _prefix = "deque"
 
     def fill_iota(self, stream=None):
-        check(_lib.load().htab_deque_fill_iota(self._h, _stream_ptr(stream)), "htab_deque_fill_iota")
-
-    def clear(self, stream=None):
-        check(_lib.load().htab_deque_clear(self._h, _stream_ptr(stream)), "htab_deque_clear")
-
-    def view(self) -> _lib.htab_deque_view:
-        """device pointers for kernels that push and pop per thread (include/stdgpu_amd/deque_device.hpp)"""
-        v = _lib.htab_deque_view()
-        check(_lib.load().htab_deque_get_view(self._h, ctypes.byref(v)), "htab_deque_get_view")
-        return v
+        self._call("fill_iota", _stream_ptr(stream))
 
     def push_back(self, values, pushed=None, stream=None):
-        check(
-            _lib.load().htab_deque_push_back(self._h, _ptr(values), values.numel(), _ptr(pushed), _stream_ptr(stream)),
-            "htab_deque_push_back",
-        )
-        return pushed
+        return self._push("push_back", values, pushed, stream)
+
+    def push_front(self, values, pushed=None, stream=None):
+        return self._push("push_front", values, pushed, stream)
 
     def pop_front(self, n: int, out=None, popped=None, stream=None):
-        torch = _torch()
-        if out is None:
-            out = torch.empty(n, dtype=torch.int32, device="cuda")
-        if popped is None:
-            popped = torch.empty(n, dtype=torch.uint8, device="cuda")
-        check(
-            _lib.load().htab_deque_pop_front(self._h, _ptr(out), int(n), _ptr(popped), _stream_ptr(stream)),
-            "htab_deque_pop_front",
-        )
-        return out, popped
-
-    def size(self, stream=None) -> int:
-        out = ctypes.c_int64()
-        check(_lib.load().htab_deque_size(self._h, ctypes.byref(out), _stream_ptr(stream)), "htab_deque_size")
-        return out.value
-
-    def contents(self, stream=None):
-        torch = _torch()
-        out = torch.empty(max(self.capacity, 1), dtype=torch.int32, device="cuda")
-        n = ctypes.c_int64()
-        check(_lib.load().htab_deque_contents(self._h, _ptr(out), ctypes.byref(n), _stream_ptr(stream)),
-              "htab_deque_contents")
-        return out[: n.value]
-
-
-class vector_u32:
-    """vector<uint32_t> used as a stack (vector.cuh; the excess free list's container) with bulk
-    push_back / pop_back; a bulk call reserves its positions once and returns what a loop of single
-    calls would."""
-
-    def __init__(self, capacity: int):
-        h = ctypes.c_void_p()
-        check(_lib.load().htab_vector_create(int(capacity), ctypes.byref(h)), "htab_vector_create")
-        self._h = h
-        self.capacity = capacity
-
-    def destroy(self):
-        if self._h:
-            check(_lib.load().htab_vector_destroy(self._h), "htab_vector_destroy")
-            self._h = ctypes.c_void_p()
-
-    def fill_iota(self, first: int = 0, stream=None):
-        check(_lib.load().htab_vector_fill_iota(self._h, int(first), _stream_ptr(stream)), "htab_vector_fill_iota")
-
-    def push_back(self, values, pushed=None, stream=None):
-        check(
-            _lib.load().htab_vector_push_back(self._h, _ptr(values), values.numel(), _ptr(pushed), _stream_ptr(stream)),
-            "htab_vector_push_back",
-        )
-        return pushed
+        return self._pop("pop_front", n, out, popped, stream)
 
     def pop_back(self, n: int, out=None, popped=None, stream=None):
-        torch = _torch()
-        if out is None:
-            out = torch.empty(n, dtype=torch.int32, device="cuda")
-        if popped is None:
-            popped = torch.empty(n, dtype=torch.uint8, device="cuda")
-        check(
-            _lib.load().htab_vector_pop_back(self._h, _ptr(out), int(n), _ptr(popped), _stream_ptr(stream)),
-            "htab_vector_pop_back",
-        )
-        return out, popped
+        return self._pop("pop_back", n, out, popped, stream)
 
-    def size(self, stream=None) -> int:
-        out = ctypes.c_int64()
-        check(_lib.load().htab_vector_size(self._h, ctypes.byref(out), _stream_ptr(stream)), "htab_vector_size")
-        return out.value
 
-    def clear(self, stream=None):
-        check(_lib.load().htab_vector_clear(self._h, _stream_ptr(stream)), "htab_vector_clear")
+class vector(_ring):
+    """vector<T> (vector.cuh) with bulk push_back / pop_back; as a stack it is the excess free list's
+    container."""
 
-    def view(self) -> _lib.htab_vector_view:
-        """device addresses for user kernels (elements, size word, capacity)"""
-        v = _lib.htab_vector_view()
-        check(_lib.load().htab_vector_get_view(self._h, ctypes.byref(v)), "htab_vector_get_view")
-        return v
+    _prefix = "vector"
 
-    def contents(self, stream=None):
-        """elements from bottom to top (int32 device tensor of uint32 bits)"""
-        torch = _torch()
-        out = torch.empty(max(self.capacity, 1), dtype=torch.int32, device="cuda")
-        n = ctypes.c_int64()
-        check(_lib.load().htab_vector_contents(self._h, _ptr(out), ctypes.byref(n), _stream_ptr(stream)),
-              "htab_vector_contents")
-        return out[: n.value]
+    def fill_iota(self, first: int = 0, stream=None):
+        self._call("fill_iota", ctypes.c_uint32(int(first)), _stream_ptr(stream))
+
+    def push_back(self, values, pushed=None, stream=None):
+        return self._push("push_back", values, pushed, stream)
+
+    def pop_back(self, n: int, out=None, popped=None, stream=None):
+        return self._pop("pop_back", n, out, popped, stream)
+
+
+class deque_u32(deque):
+    """deque<uint32_t>, the voxel pipeline's free list of slot numbers"""
+
+    def __init__(self, capacity: int):
+        super().__init__(capacity, elem_bytes=4)
+
+
+class vector_u32(vector):
+    """vector<uint32_t> used as a stack"""
+
+    def __init__(self, capacity: int):
+        super().__init__(capacity, elem_bytes=4)
 
 
 class bitset:

@@ -416,6 +416,7 @@ empty_and_allocator()
 int
 main()
 {
+    std::setvbuf(stdout, nullptr, _IOLBF, 0); // a case that never ends still names itself
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
     {

@@ -200,6 +200,7 @@ whole_set_operations_and_reference()
 int
 main()
 {
+    std::setvbuf(stdout, nullptr, _IOLBF, 0); // a case that never ends still names itself
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
     {

@@ -1377,6 +1377,7 @@ bucket_size_is_the_reference_count()
 int
 main()
 {
+    std::setvbuf(stdout, nullptr, _IOLBF, 0); // a case that never ends still names itself
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
     {

@@ -746,6 +746,7 @@ run_all()
 int
 main()
 {
+    std::setvbuf(stdout, nullptr, _IOLBF, 0); // a case that never ends still names itself
     run_all<fx_map>();
     run_all<fx_set>();
     const int before = g_failures;

@@ -1,8 +1,8 @@
 // Source-compatibility tests of stdgpu::vector used from user kernels and host code, written as the
-// reference's own cases are (tests/stdgpu/vector.inc): per-thread push_back / emplace_back / pop_back
-// in kernels that push or pop, insert at and erase up to device_end(), clear, element access, the
-// capacity limits.  Every case checks the elements against what the calls reported.  Prints one line
-// per case; exits non-zero if any case fails.
+// reference's own cases are (tests/stdgpu/vector.inc): per-thread push_back / emplace_back / pop_back,
+// also mixed in one kernel, insert at and erase up to device_end(), clear, element access, the
+// capacity limits, element types of other sizes.  Every case checks the elements against what the
+// calls reported.  Prints one line per case; exits non-zero if any case fails.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -421,9 +421,113 @@ per_thread_rates()
     ivec::destroyDeviceObject(v);
 }
 
+
+// simultaneous_push_back_and_pop_back (vector.inc:985-1077): each thread pushes its value, pops one,
+// and pushes what it popped onto a second vector
+template <typename T>
+__global__ void
+k_simultaneous(stdgpu::vector<T> pool, stdgpu::vector<T> validation, std::size_t n)
+{
+    const std::size_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    pool.push_back(static_cast<T>(i + 1));
+    const stdgpu::pair<T, bool> r = pool.pop_back();
+    if (r.second) validation.push_back(r.first);
+}
+
+template <typename T>
+static void
+simultaneous_push_back_and_pop_back()
+{
+    g_case = "simultaneous_push_back_and_pop_back (vector.inc:985-1077)";
+    const std::size_t N = 100000;
+    auto pool = stdgpu::vector<T>::createDeviceObject(static_cast<stdgpu::index_t>(N));
+    auto validation = stdgpu::vector<T>::createDeviceObject(static_cast<stdgpu::index_t>(N));
+    hipLaunchKernelGGL(k_simultaneous<T>, dim3(static_cast<unsigned>((N + 255) / 256)), dim3(256), 0, 0, pool,
+                       validation, N);
+    CHECK(hipDeviceSynchronize() == hipSuccess);
+    CHECK(pool.size() == 0 && pool.empty() && !pool.full() && pool.valid());
+    CHECK(validation.size() == static_cast<stdgpu::index_t>(N) && !validation.empty() && validation.full() &&
+          validation.valid());
+    std::vector<T> h(N);
+    CHECK(hipMemcpy(h.data(), validation.data(), N * sizeof(T), hipMemcpyDeviceToHost) == hipSuccess);
+    std::sort(h.begin(), h.end());
+    for (std::size_t i = 0; i < N; ++i) CHECK(h[i] == static_cast<T>(i + 1));
+    stdgpu::vector<T>::destroyDeviceObject(pool);
+    stdgpu::vector<T>::destroyDeviceObject(validation);
+}
+
+// a 16-byte element pushed and popped per thread, and inserted from the host
+struct quad
+{
+    std::int32_t v[4];
+};
+
+__global__ void
+k_push_quads(stdgpu::vector<quad> q, std::size_t n, int* ok)
+{
+    const std::size_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const std::int32_t x = static_cast<std::int32_t>(i);
+    ok[i] = q.emplace_back(quad{ { x, x + 1, x * 2, -x } }) ? 1 : 0;
+}
+
+__global__ void
+k_pop_quads(stdgpu::vector<quad> q, std::size_t n, quad* out, int* ok)
+{
+    const std::size_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const auto r = q.pop_back();
+    out[i] = r.first;
+    ok[i] = r.second ? 1 : 0;
+}
+
+static void
+sixteen_byte_elements()
+{
+    g_case = "vector<16-byte struct>: push_back_some / _too_many, pop_back_all / _too_many, insert, erase";
+    const std::size_t C = 50000;
+    auto q = stdgpu::vector<quad>::createDeviceObject(static_cast<stdgpu::index_t>(C));
+    int* ok = dev_alloc<int>(2 * C);
+    quad* out = dev_alloc<quad>(2 * C);
+    hipLaunchKernelGGL(k_push_quads, dim3(static_cast<unsigned>((2 * C + 255) / 256)), dim3(256), 0, 0, q, 2 * C, ok);
+    std::vector<int> hok = to_host(ok, 2 * C);
+    CHECK(std::accumulate(hok.begin(), hok.end(), 0) == static_cast<int>(C) && q.full() && q.valid());
+    std::vector<quad> h(C);
+    CHECK(hipMemcpy(h.data(), q.data(), C * sizeof(quad), hipMemcpyDeviceToHost) == hipSuccess);
+    std::vector<int> seen;
+    for (const quad& x : h)
+    {
+        CHECK(x.v[1] == x.v[0] + 1 && x.v[2] == 2 * x.v[0] && x.v[3] == -x.v[0]);
+        CHECK(hok[static_cast<std::size_t>(x.v[0])] == 1);
+        seen.push_back(x.v[0]);
+    }
+    std::sort(seen.begin(), seen.end());
+    CHECK(std::adjacent_find(seen.begin(), seen.end()) == seen.end());
+    // erase the top half from the host, then pop everything and more
+    q.erase(q.device_begin() + static_cast<std::ptrdiff_t>(C / 2), q.device_end());
+    CHECK(q.size() == static_cast<stdgpu::index_t>(C / 2) && q.valid());
+    hipLaunchKernelGGL(k_pop_quads, dim3(static_cast<unsigned>((C + 255) / 256)), dim3(256), 0, 0, q, C, out, ok);
+    hok = to_host(ok, C);
+    CHECK(std::accumulate(hok.begin(), hok.end(), 0) == static_cast<int>(C / 2) && q.empty() && q.valid());
+    // insert a host range back at the end
+    quad* src = dev_alloc<quad>(C);
+    CHECK(hipMemcpy(src, h.data(), C * sizeof(quad), hipMemcpyHostToDevice) == hipSuccess);
+    q.insert(q.device_end(), stdgpu::make_device(src), stdgpu::make_device(src + C));
+    CHECK(q.size() == static_cast<stdgpu::index_t>(C) && q.valid());
+    std::vector<quad> back(C);
+    CHECK(hipMemcpy(back.data(), q.data(), C * sizeof(quad), hipMemcpyDeviceToHost) == hipSuccess);
+    for (std::size_t i = 0; i < C; ++i) CHECK(back[i].v[0] == h[i].v[0] && back[i].v[3] == h[i].v[3]);
+    (void)hipFree(ok);
+    (void)hipFree(out);
+    (void)hipFree(src);
+    stdgpu::vector<quad>::destroyDeviceObject(q);
+}
+
 int
 main()
 {
+    std::setvbuf(stdout, nullptr, _IOLBF, 0); // a case that never ends still names itself
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
     {
@@ -431,9 +535,13 @@ main()
         return 2;
     }
     void (*cases[])() = { empty_container, pop_back_cases,    push_back_cases,        diverged_lanes,
-                          insert_erase_clear, element_access, allocator_and_policies, per_thread_rates };
+                          insert_erase_clear, element_access, allocator_and_policies, per_thread_rates,
+                          sixteen_byte_elements, simultaneous_push_back_and_pop_back<int>,
+                          simultaneous_push_back_and_pop_back<std::int64_t> };
+    int index = 0;
     for (auto c : cases)
     {
+        std::printf("case %d starts\n", index++);
         const int before = g_failures;
         try
         {

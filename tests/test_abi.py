@@ -41,8 +41,8 @@ def test_struct_layouts_match_header():
     assert ctypes.sizeof(_lib.htab_tuning) == 28
     assert ctypes.sizeof(_lib.htab_info) == 8 + 8 * 8
     assert ctypes.sizeof(_lib.htab_view) == 3 * 8 + 2 * 8 + 2 * 4 + 2 * 8 + 8 + 2 * 8
-    assert ctypes.sizeof(_lib.htab_deque_view) == 7 * 8
-    assert ctypes.sizeof(_lib.htab_vector_view) == 3 * 8
+    assert ctypes.sizeof(_lib.htab_ring_view) == 8 * 8
+    assert _lib.htab_deque_view is _lib.htab_ring_view and _lib.htab_vector_view is _lib.htab_ring_view
     assert ctypes.sizeof(_lib.htab_bitset_view) == 2 * 8
 
 

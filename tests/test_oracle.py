@@ -320,6 +320,41 @@ def test_deque_multiset_conservation():
     assert sorted(np.concatenate([got[ok == 1], out2]).tolist()) == list(range(cap))
 
 
+def test_deque_both_ends_equal_a_plain_deque():
+    """push_front / pop_back (deque_detail.cuh:262-388, positions from atomicDec-style wraps of begin
+    and end) with push_back / pop_front, single-threaded: every result, refusal and the contents front
+    to back equal a bounded double-ended queue (collections.deque), also across the ring's ends"""
+    import collections
+
+    rng = np.random.default_rng(5)
+    cap = 37
+    q = po.Deque(cap)
+    model = collections.deque()
+    for step in range(400):
+        op = int(rng.integers(0, 4))
+        n = int(rng.integers(0, 25))
+        if op < 2:
+            vals = rng.integers(0, 1 << 32, size=n, dtype=np.uint64).astype(np.uint32)
+            ok = (q.push_back if op == 0 else q.push_front)(vals)
+            want = []
+            for x in vals:
+                if len(model) < cap:
+                    (model.append if op == 0 else model.appendleft)(int(x))
+                    want.append(1)
+                else:
+                    want.append(0)
+            assert ok.tolist() == want, step
+        else:
+            out, ok = (q.pop_back if op == 2 else q.pop_front)(n)
+            for i in range(n):
+                if model:
+                    assert ok[i] == 1 and int(out[i]) == (model.pop() if op == 2 else model.popleft()), step
+                else:
+                    assert ok[i] == 0 and out[i] == 0xFFFFFFFF
+        assert q.size() == len(model)
+        assert q.contents().tolist() == list(model), step
+
+
 def test_vector_stack_semantics():
     """vector<uint32_t> as the excess stack (vector_detail.cuh:181-302, 361-368): LIFO order single-
     threaded, refusal when full / empty, clamped size, iota fill, multiset conservation with threads."""
