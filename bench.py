@@ -25,16 +25,16 @@ HBM_PEAK_GBS = 8000.0
 # 64 B random sector per bucket probe (+ the 16 B slot write for insert)
 BYTES_PER_OP = {"insert": 16 + 64 + 16 + 1, "find": 8 + 1 + 4 + 64, "erase": 8 + 64 + 8 + 1, "reinsert": 16 + 64 + 16 + 1}
 # HBM bytes per launch measured with rocprofv3 on MI355X for the default workload (2^26 keys,
-# 8-slot buckets, 256-thread workgroups): FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000 KiB, the scales
+# 8-slot buckets, 64-thread workgroups): FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000 KiB, the scales
 # that make both counters read a 2 GiB copy kernel's bytes exactly (profiles/map_u64_u32_2e26_bulk_kernels.md)
 # 128 B lines a find reads per key (TCC misses per dispatch / keys), same profile and workload
-MEASURED_FIND_LINES = {(26, 8, 64): 89417762 / (1 << 26), (26, 8, 256): 89039255 / (1 << 26)}
+MEASURED_FIND_LINES = {(26, 8, 64): 84176200 / (1 << 26)}
 # The kernel sources those counters were read from: the figures are reported only while the
 # digest of these files equals MEASURED_SOURCES_SHA256 (a kernel change makes them null until the
 # profile is taken again).
 KERNEL_SOURCES = ["stdgpu_amd/csrc/htab_kernels.hpp", "stdgpu_amd/csrc/htab_ops.hpp", "stdgpu_amd/csrc/wave.hpp",
                   "stdgpu_amd/csrc/kind_map_u64_u32.hip", "include/stdgpu_amd/htab_device.hpp"]
-MEASURED_SOURCES_SHA256 = None
+MEASURED_SOURCES_SHA256 = "bf90283ef3d044889bad1c6e76d0c241847fac0f547b3b1bf73ab3cd56900b6c"
 
 
 def sources_digest() -> str:
@@ -48,10 +48,8 @@ def sources_digest() -> str:
     return h.hexdigest()
 
 
-MEASURED_TRAFFIC = {(26, 8, 64): {"insert": (2 * 5253340 + 6735788) * 1024, "find": (2 * 5392030 + 327680) * 1024,
-                                  "erase": (2 * 1271969 + 1152885) * 1024, "reinsert": (2 * 2372024 + 1676292) * 1024},
-                    (26, 8, 256): {"insert": (2 * 5252402 + 6709960) * 1024, "find": (2 * 5391956 + 336602) * 1024,
-                                   "erase": (2 * 1272247 + 1146738) * 1024, "reinsert": (2 * 2368420 + 1667805) * 1024}}
+MEASURED_TRAFFIC = {(26, 8, 64): {"insert": (2 * 5249797 + 6874031) * 1024, "find": (2 * 5064417 + 327680) * 1024,
+                                  "erase": (2 * 1271976 + 1188531) * 1024, "reinsert": (2 * 1445214 + 1722347) * 1024}}
 
 
 def parse():
