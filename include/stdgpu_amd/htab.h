@@ -470,6 +470,41 @@ int htab_deque_contents(htab_deque* q, void* out, int64_t* n_out, void* stream);
  * consistent; for a deque no kernel is changing */
 int htab_deque_valid(htab_deque* q, int32_t* out, void* stream);
 
+/* ------------------------------------------------------------------------------------------
+ * Free-list calls between a map and a deque of its mapped values: the voxel-hashing caller of the
+ * reference's containers (per point: deque::pop_front a slot, insert (block, slot), push the slot back
+ * when the block existed; later, erase the blocks out of range and deque::push_back their slots) as
+ * two bulk calls with no host synchronisation.  No reference counterpart as library calls: each
+ * replaces a caller loop over deque_detail.cuh:390-448 (pop_front), unordered_base_detail.cuh:653-794
+ * (insert / erase) and deque_detail.cuh:204-260 (push_back).  Maps only, the deque's element size
+ * equal to the mapped size; like the deque's bulk calls, each runs alone on its table and deque.
+ * ------------------------------------------------------------------------------------------ */
+
+/* Keys that pack three block coordinates of `bits` bits each (x highest, z lowest), each stored as
+ * coordinate + offset; a block's centre is (coordinate + 0.5) * block_size on each axis. */
+typedef struct htab_block_region
+{
+    int32_t bits;      /* 1..21 */
+    int32_t reserved;  /* 0 */
+    int64_t offset;
+    double block_size;
+    double center[3];
+    double radius;     /* blocks whose centre is farther than this from center are selected */
+} htab_block_region;
+
+/* After a bulk insert wrote its per-element flags (htab_insert's `inserted`): every keys[i] with
+ * flags[i] == 1 gets the front element of q as its mapped value, popped in rank order; flags[i]
+ * becomes 2 when q ran empty first (the key keeps the value its insert wrote) and 3 when keys[i] is
+ * not in the table (nothing popped for it).  count (device int64, optional) receives the number of
+ * elements popped. */
+int htab_assign_from_deque(htab* table, const void* keys, uint8_t* flags, int64_t n, htab_deque* q, int64_t* count,
+                           void* stream);
+/* Erases every entry whose key's block centre lies farther than region->radius from region->center
+ * and pushes its mapped value onto the back of q; once q is full, the remaining such entries stay.
+ * count (device int64, optional) receives the number erased (= pushed). */
+int htab_erase_far_blocks(htab* table, const htab_block_region* region, htab_deque* q, int64_t* count,
+                          void* stream);
+
 #ifdef __cplusplus
 }
 #endif

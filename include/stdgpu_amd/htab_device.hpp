@@ -302,6 +302,17 @@ struct table_ref
         return nbuckets < kOrderedClaimBuckets ? claim_offset(static_cast<uint64_t>(key), W) : 0U;
     }
 
+    // The keys at scan positions j0 .. j0 + kScanBatch - 1 of bucket b (scan order starting at slot o),
+    // loaded back to back so that one memory round trip serves them all; a loop that compared each key
+    // before loading the next would wait for every load in turn.  With W below kScanBatch the
+    // positions past W repeat slots already loaded.
+    static constexpr uint32_t kScanBatch = 8;
+    __device__ void load_scan(uint64_t b, uint32_t o, uint32_t j0, K (&k)[kScanBatch]) const
+    {
+#pragma unroll
+        for (uint32_t u = 0; u < kScanBatch; ++u) k[u] = load_agent(key_ptr(b * W + ((o + j0 + u) & (W - 1))));
+    }
+
     // find (unordered_base_detail.cuh:653-682): pointer to the entry or end_ptr().
     __device__ value_type* find(K key) const { return find_at(key, home_bucket(static_cast<uint64_t>(key), nbuckets)); }
 
@@ -322,12 +333,16 @@ struct table_ref
         {
             bool saw_empty = false;
             value_type* base = slots + b * W;
-            for (uint32_t j = 0; j < W; ++j)
+            for (uint32_t j0 = 0; j0 < W; j0 += kScanBatch)
             {
-                const uint32_t x = (o + j) & (W - 1);
-                const K k = load_agent(key_ptr(b * W + x));
-                if (k == key) return base + x;
-                saw_empty |= (k == empty_key<K>());
+                K k[kScanBatch];
+                load_scan(b, o, j0, k);
+#pragma unroll
+                for (uint32_t u = 0; u < kScanBatch; ++u)
+                {
+                    if (k[u] == key) return base + ((o + j0 + u) & (W - 1));
+                    saw_empty |= (k[u] == empty_key<K>());
+                }
             }
             if (saw_empty || overflow(b) == 0) break;
             // not in its home bucket and no key of that bucket with its fingerprint stored further on
@@ -351,15 +366,20 @@ struct table_ref
         for (uint64_t step = 0; step < nbuckets; ++step)
         {
             bool saw_empty = false;
-            for (uint32_t j = 0; j < W; ++j)
+            for (uint32_t j0 = 0; j0 < W; j0 += kScanBatch)
             {
-                const K k = load_agent(key_ptr(b * W + j));
-                if (k == empty_key<K>())
+                K k[kScanBatch];
+                load_scan(b, 0, j0, k);
+#pragma unroll
+                for (uint32_t u = 0; u < kScanBatch; ++u)
                 {
-                    saw_empty = true;
-                    continue;
+                    if (k[u] == empty_key<K>())
+                    {
+                        saw_empty = true;
+                        continue;
+                    }
+                    if (k[u] != tomb_key<K>() && match(k[u])) return slots + b * W + ((j0 + u) & (W - 1));
                 }
-                if (k != tomb_key<K>() && match(k)) return slots + b * W + j;
             }
             if (saw_empty || overflow(b) == 0) break;
             b = next_bucket(b, nbuckets);
@@ -525,19 +545,26 @@ struct table_ref
         for (; step < nbuckets; ++step)
         {
             bool saw_empty = false;
-            for (uint32_t j = 0; j < W; ++j)
+            for (uint32_t j0 = 0; j0 < W; j0 += kScanBatch)
             {
-                const uint64_t s = b * W + ((o + j) & (W - 1));
-                const K k = load_agent(key_ptr(s));
-                if (k == key) return end_ptr();
-                if (fslot == kNone && (k == empty_key<K>() || k == tomb_key<K>()))
+                K kk[kScanBatch];
+                load_scan(b, o, j0, kk);
+#pragma unroll
+                for (uint32_t u = 0; u < kScanBatch; ++u)
                 {
-                    fslot = s;
-                    fstep = step;
-                    fj = j;
-                    fseen = k;
+                    const uint32_t j = j0 + u;
+                    if (j >= W) break; // positions past W repeat loaded slots (scan order matters here)
+                    const K k = kk[u];
+                    if (k == key) return end_ptr();
+                    if (fslot == kNone && (k == empty_key<K>() || k == tomb_key<K>()))
+                    {
+                        fslot = b * W + ((o + j) & (W - 1));
+                        fstep = step;
+                        fj = j;
+                        fseen = k;
+                    }
+                    saw_empty |= (k == empty_key<K>());
                 }
-                saw_empty |= (k == empty_key<K>());
             }
             if (saw_empty || overflow(b) == 0) break;
             // the away count ends the scan at the home bucket as in find()
@@ -641,19 +668,25 @@ struct table_ref
         for (uint64_t step = 0; step < nbuckets; ++step)
         {
             bool saw_empty = false;
-            for (uint32_t j = 0; j < W; ++j)
+            uint64_t hit = ~uint64_t(0);
+            for (uint32_t j0 = 0; j0 < W; j0 += kScanBatch)
             {
-                const uint64_t s = b * W + ((o + j) & (W - 1));
-                const K k = load_agent(key_ptr(s));
-                if (k == key)
+                K k[kScanBatch];
+                load_scan(b, o, j0, k);
+#pragma unroll
+                for (uint32_t u = 0; u < kScanBatch; ++u)
                 {
-                    if (cas_agent(key_ptr(s), key, tomb_key<K>()) != key) return false;
-                    count_add(-1);
-                    overflow_add(h, step, -1);
-                    if (step != 0 && away_counters() != 0) away_add(h, away_slot(key), -1);
-                    return true;
+                    if (k[u] == key) hit = b * W + ((o + j0 + u) & (W - 1));
+                    saw_empty |= (k[u] == empty_key<K>());
                 }
-                saw_empty |= (k == empty_key<K>());
+            }
+            if (hit != ~uint64_t(0))
+            {
+                if (cas_agent(key_ptr(hit), key, tomb_key<K>()) != key) return false;
+                count_add(-1);
+                overflow_add(h, step, -1);
+                if (step != 0 && away_counters() != 0) away_add(h, away_slot(key), -1);
+                return true;
             }
             if (saw_empty || overflow(b) == 0) break;
             if (step == 0 && away_counters() != 0 && away(b, away_slot(key)) == 0) break;

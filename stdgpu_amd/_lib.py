@@ -135,6 +135,19 @@ htab_deque_view = htab_ring_view
 htab_vector_view = htab_ring_view
 
 
+class htab_block_region(ctypes.Structure):
+    """packed 3-D block keys and a sphere around a point (htab_block_region in htab.h)"""
+
+    _fields_ = [
+        ("bits", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+        ("offset", ctypes.c_int64),
+        ("block_size", ctypes.c_double),
+        ("center", ctypes.c_double * 3),
+        ("radius", ctypes.c_double),
+    ]
+
+
 class htab_view(ctypes.Structure):
     _fields_ = [
         ("slots", ctypes.c_void_p),
@@ -231,6 +244,8 @@ SIGNATURES = {
     "htab_vector_contents": (_I32, [_P, _P, ctypes.POINTER(_I64), _P]),
     "htab_vector_clear": (_I32, [_P, _P]),
     "htab_vector_get_view": (_I32, [_P, ctypes.POINTER(htab_vector_view)]),
+    "htab_assign_from_deque": (_I32, [_P, _P, _P, _I64, _P, _P, _P]),
+    "htab_erase_far_blocks": (_I32, [_P, ctypes.POINTER(htab_block_region), _P, _P, _P]),
 }
 
 _lib = None

@@ -183,6 +183,30 @@ class _unordered_base:
                                       _stream_ptr(stream)), "htab_assign")
         return found
 
+    def assign_from_deque(self, keys, flags, deque, count=None, stream=None):
+        """Free-list take (maps): every keys[i] with flags[i] == 1 (the flags a bulk insert wrote) gets
+        the front element of `deque` as its value; flags[i] becomes 2 when the deque ran empty first, 3
+        when keys[i] is absent.  `count` (int64 device tensor of one element, optional) receives the
+        number popped.  No host synchronisation."""
+        self._check_keys(keys)
+        check(_lib.load().htab_assign_from_deque(self._h, _ptr(keys), _ptr(flags), keys.numel(), deque._h, _ptr(count),
+                                                 _stream_ptr(stream)), "htab_assign_from_deque")
+        return flags
+
+    def erase_far_blocks(self, center, radius: float, block_size: float, deque, bits: int = 21,
+                         offset: int = 1 << 20, count=None, stream=None):
+        """Free-list evict (maps): erases every entry whose key, read as a packed 3-D block coordinate
+        (`bits` bits per axis, x highest, each stored + `offset`), has its block centre farther than
+        `radius` from `center`, and pushes its value onto the back of `deque` (stopping when the deque
+        is full).  `count` (int64 device tensor of one element, optional) receives the number erased."""
+        r = _lib.htab_block_region()
+        r.bits, r.reserved, r.offset, r.block_size = int(bits), 0, int(offset), float(block_size)
+        for a in range(3):
+            r.center[a] = float(center[a])
+        r.radius = float(radius)
+        check(_lib.load().htab_erase_far_blocks(self._h, ctypes.byref(r), deque._h, _ptr(count), _stream_ptr(stream)),
+              "htab_erase_far_blocks")
+
     def clear(self, stream=None):
         check(_lib.load().htab_clear(self._h, _stream_ptr(stream)), "htab_clear")
 

@@ -548,6 +548,61 @@ htab_assign(htab* h, const void* keys, const void* vals, int64_t n, uint8_t* fou
     return dispatch(h->kind, c);
 }
 
+// the free-list calls' common checks: a map whose mapped size is the deque's element size
+static int
+freelist_args(htab* h, htab_deque* q, htab_ring_view* view, const char* who)
+{
+    if (!h || !q) return fail(HTAB_E_ARG, who);
+    if (refuse_caller_hashed(h, who)) return HTAB_E_ARG;
+    if (htab_deque_get_view(q, view) != HTAB_OK) return fail(HTAB_E_ARG, who);
+    const int mb = mapped_bytes_of(h->kind);
+    if (mb <= 0 || view->elem_bytes != mb) return fail(HTAB_E_ARG, who);
+    return HTAB_OK;
+}
+
+int
+htab_assign_from_deque(htab* h, const void* keys, uint8_t* flags, int64_t n, htab_deque* q, int64_t* count,
+                       void* stream)
+{
+    htab_ring_view view{};
+    if (freelist_args(h, q, &view, "htab_assign_from_deque: needs a map and a deque of its mapped size"))
+        return HTAB_E_ARG;
+    if (n < 0) return fail(HTAB_E_ARG, "htab_assign_from_deque: negative count");
+    if (n > 0 && (!keys || !flags)) return fail(HTAB_E_ARG, "htab_assign_from_deque: null keys or flags");
+    if (check_alignment(h->kind, nullptr, keys, nullptr, "htab_assign_from_deque: keys not aligned to their size"))
+        return HTAB_E_ARG;
+    htab_call c;
+    c.op = call_op::take;
+    c.h = h;
+    c.keys = keys;
+    c.flags = flags;
+    c.n = n;
+    c.ring = &view;
+    c.count_dev = count;
+    c.s = static_cast<hipStream_t>(stream);
+    return dispatch(h->kind, c);
+}
+
+int
+htab_erase_far_blocks(htab* h, const htab_block_region* region, htab_deque* q, int64_t* count, void* stream)
+{
+    htab_ring_view view{};
+    if (freelist_args(h, q, &view, "htab_erase_far_blocks: needs a map and a deque of its mapped size"))
+        return HTAB_E_ARG;
+    if (!region || region->bits < 1 || region->bits > 21 || region->reserved != 0 || !(region->radius >= 0.0))
+        return fail(HTAB_E_ARG, "htab_erase_far_blocks: bad region (bits 1..21, reserved 0, radius >= 0)");
+    if (key_bytes_of(h->kind) * 8 < 3 * region->bits)
+        return fail(HTAB_E_ARG, "htab_erase_far_blocks: the key holds fewer than 3 * bits bits");
+    htab_call c;
+    c.op = call_op::evict;
+    c.h = h;
+    c.ring = &view;
+    c.region = region;
+    c.count_dev = count;
+    c.s = static_cast<hipStream_t>(stream);
+    return dispatch(h->kind, c);
+}
+
 int
 htab_size(htab* h, int64_t* out, void* stream)
 {

@@ -33,6 +33,7 @@ void reference_sizing(int64_t capacity, int64_t* bucket_count, int64_t* max_size
 constexpr size_t kCtlBytes = 8192; // stripes (32 x 128 B) + flags + scratch words
 constexpr size_t kFlagsOffset = 4096;
 constexpr size_t kPlanOffset = 4224; // one-stripe tables: whether a bulk insert can reach the limit
+constexpr size_t kFreelistOffset = 4352; // three int64 words of a free-list call (htab_freelist.hpp)
 
 #define HTAB_HIP_TRY(expr, what)                                                                                       \
     do                                                                                                                 \
@@ -118,7 +119,9 @@ enum class call_op
     batch_run_offsets,
     batch_run_segments,
     serve,
-    serve_sizes
+    serve_sizes,
+    take,
+    evict
 };
 
 // One request to a kind dispatcher; fields unused by an operation stay zero.
@@ -148,6 +151,8 @@ struct htab_call
     int64_t* scratch_ext = nullptr;                    // range: tile scratch given by the caller (no allocation)
     int64_t* count_dev = nullptr;                      // range: entry count written to device memory (no sync)
     const uint64_t* hashes = nullptr;                  // valid: the caller's hash of each slot's key (or null)
+    const htab_ring_view* ring = nullptr;              // take / evict: the deque's device view
+    const htab_block_region* region = nullptr;         // evict: the blocks to erase
     hipStream_t s = nullptr;
 };
 

@@ -7,6 +7,7 @@
 #include <new>
 #include <type_traits>
 
+#include "htab_freelist.hpp"
 #include "htab_internal.hpp"
 
 namespace htab_host
@@ -419,6 +420,44 @@ dispatch_kind(htab_call& c)
                                    c.s, table_ref<K, M>::from_view(c.h->view), static_cast<const K*>(c.keys),
                                    static_cast<const M*>(c.vals), c.n, c.flags);
                 return launch_status("assign launch");
+            }
+        case call_op::take:
+        case call_op::evict:
+            if constexpr (std::is_void_v<M>)
+            {
+                return fail(HTAB_E_ARG, "free-list calls: sets have no mapped values");
+            }
+            else
+            {
+                const table_ref<K, M> t = table_ref<K, M>::from_view(c.h->view);
+                int64_t* w = reinterpret_cast<int64_t*>(static_cast<char*>(c.h->mem) + kFreelistOffset);
+                const bool take = c.op == call_op::take;
+                hipLaunchKernelGGL(k_freelist_begin<>, dim3(1), dim3(64), 0, c.s, *c.ring, w, take ? 0 : 1);
+                int rc = launch_status("free-list start");
+                if (rc) return rc;
+                // at most 2048 workgroups (one counter atomic each) unless a workgroup would own more
+                // than kFreelistTiles tiles
+                const uint64_t work = take ? static_cast<uint64_t>(c.n) : t.nslots();
+                const uint64_t tiles = (work + kFreelistThreads - 1) / kFreelistThreads;
+                uint64_t grid = tiles < 2048 ? tiles : 2048;
+                if (grid == 0) grid = 1;
+                if ((tiles + grid - 1) / grid > static_cast<uint64_t>(kFreelistTiles))
+                    grid = (tiles + kFreelistTiles - 1) / kFreelistTiles;
+                if (grid > 0x7FFFFFFFULL) return fail(HTAB_E_LIMIT, "free-list call: too many elements");
+                if (take)
+                    hipLaunchKernelGGL((k_take_values<K, M>), dim3(static_cast<uint32_t>(grid)), dim3(kFreelistThreads), 0,
+                                       c.s, t, static_cast<const K*>(c.keys), c.flags, c.n, *c.ring, w);
+                else
+                    hipLaunchKernelGGL((k_evict_far<K, M>), dim3(static_cast<uint32_t>(grid)), dim3(kFreelistThreads), 0,
+                                       c.s, t, *c.region, *c.ring, w);
+                rc = launch_status(take ? "take launch" : "evict launch");
+                if (rc) return rc;
+                if (take)
+                    hipLaunchKernelGGL(k_take_end<>, dim3(1), dim3(64), 0, c.s, *c.ring, w, c.count_dev);
+                else
+                    hipLaunchKernelGGL((k_evict_end<K, M>), dim3(1), dim3(64), 0, c.s, t, *c.region, *c.ring, w,
+                                       c.count_dev);
+                return launch_status("free-list end");
             }
         case call_op::serve_sizes:
             hipLaunchKernelGGL((k_serve_sizes<K, M>), dim3(1), dim3(256), 0, c.s,

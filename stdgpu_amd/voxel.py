@@ -6,10 +6,11 @@ to the cube).  Each frame samples `npoints` surface points, area-weighted, on th
 centre lies within the 16 m sensor range of the camera, quantises them at 8 mm voxels into 8^3-voxel blocks (64 mm)
 and packs the signed block coordinates into 63 bits (21 bits per axis, offset 2^20).
 
-Frame pipeline (GPU, bulk calls): insert the frame's block keys (duplicates included) -> the keys
-that were new receive a slot popped from the free-list deque<uint32_t> (htab_assign) -> find every
-point's block -> erase the blocks whose centre is more than 20 m from the camera and push their
-slots back onto the free list.  The reference's caller does the same per point: pop a slot, insert
+Frame pipeline (GPU, bulk calls, no host synchronisation): insert the frame's block keys (duplicates
+included, per-element flags) -> the keys that were new receive a slot from the front of the free-list
+deque<uint32_t> (htab_assign_from_deque) -> find every point's block -> erase the blocks whose centre
+is more than 20 m from the camera and push their slots onto the back of the free list in the same
+pass over the table (htab_erase_far_blocks).  The reference's caller does the same per point: pop a slot, insert
 (key, slot), push the slot back when the key existed.
 """
 from __future__ import annotations
@@ -125,31 +126,47 @@ def far_mask(keys, cam):
     return (x - cam[0]) ** 2 + (y - cam[1]) ** 2 + (z - cam[2]) ** 2 > RADIUS * RADIUS
 
 
-def run_frame(tab, freelist, keys, cam, stats: dict | None = None, check: bool = True):
-    """One frame on the GPU.  keys: int64 device tensor.  Returns a dict of per-step counts."""
+def frame_buffers(n: int, device) -> dict:
+    """The device arrays one frame of n points writes: reused from frame to frame, so a frame
+    allocates and fills nothing before its library calls."""
     import torch
 
+    return {
+        "vals": torch.full((n,), -1, dtype=torch.int32, device=device),  # the value a new block holds until its slot
+        "ins": torch.empty(n, dtype=torch.uint8, device=device),
+        "hits": torch.empty(n, dtype=torch.uint8, device=device),
+        "found": torch.empty(n, dtype=torch.int32, device=device),
+        "counts": torch.zeros(2, dtype=torch.int64, device=device),  # slots popped, blocks erased
+    }
+
+
+def run_frame(tab, freelist, keys, cam, bufs: dict | None = None, check: bool = True, counts: bool = True):
+    """One frame on the GPU, all library calls on the current stream with no host synchronisation:
+    bulk insert of the frame's block keys (per-element flags), a slot from the front of the free list
+    for every key the insert added (htab_assign_from_deque), find of every point's block, erase of
+    the blocks whose centre lies more than 20 m from the camera with their slots pushed onto the back
+    of the free list (htab_erase_far_blocks).  keys: int64 device tensor.  Returns the frame's counts
+    (`counts`: read back, which waits for the frame; otherwise the device tensor "counts" of
+    [new blocks, erased blocks]); `check` adds the pipeline's own checks (also waiting)."""
     n = keys.numel()
-    dev = keys.device
-    ins = torch.empty(n, dtype=torch.uint8, device=dev)
-    tab.insert(keys, torch.full((n,), -1, dtype=torch.int32, device=dev), inserted=ins)
-    new = keys[ins.bool()]
-    m = new.numel()
-    slots, popped = freelist.pop_front(m)
-    found = torch.empty(m, dtype=torch.uint8, device=dev)
-    tab.assign(new, slots, found=found)
-    hits, vals = tab.find(keys)
-    ck, cv = tab.contents()
-    far = far_mask(ck, cam)
-    ek, es = ck[far], cv[far]
-    er = torch.empty(ek.numel(), dtype=torch.uint8, device=dev)
-    tab.erase(ek, erased=er)
-    freelist.push_back(es)
-    out = {"points": n, "new_blocks": m, "erased": int(ek.numel())}
+    b = bufs if bufs is not None else frame_buffers(n, keys.device)
+    ins, c = b["ins"][:n], b["counts"]
+    far_before = None
+    tab.insert(keys, b["vals"][:n], inserted=ins)
+    tab.assign_from_deque(keys, ins, freelist, count=c[0:1])
+    hits, _ = tab.find(keys, hits=b["hits"][:n], values_out=b["found"][:n])
     if check:
-        out["popped_all"] = bool(popped.all()) if m else True
+        ck, _ = tab.contents()
+        far_before = int(far_mask(ck, cam).sum())
+    tab.erase_far_blocks(cam, RADIUS, BLOCK, freelist, bits=21, offset=OFFSET, count=c[1:2])
+    out = {"points": n, "counts": c}
+    if counts or check:
+        new_blocks, erased = (int(x) for x in c.cpu())
+        out.update(new_blocks=new_blocks, erased=erased)
+    if check:
+        out["popped_all"] = bool((ins <= 1).all())
         out["all_found"] = bool(hits.all())
-        out["erased_all"] = bool(er.all()) if ek.numel() else True
+        out["erased_all"] = out["erased"] == far_before
     return out
 
 
@@ -179,7 +196,8 @@ def measure(frames: int = 30, log2_points: int = 24, capacity_log2: int = 23) ->
     # temporaries are not part of a frame
     wtab, wfl = sa.unordered_map_u64_u32.createDeviceObject(cap), sa.deque_u32(cap)
     wfl.fill_iota()
-    run_frame(wtab, wfl, frame_keys_device(scene, 0, npts, "cuda"), scene.cams[0], check=False)
+    bufs = frame_buffers(npts, "cuda")
+    run_frame(wtab, wfl, frame_keys_device(scene, 0, npts, "cuda"), scene.cams[0], bufs, check=False)
     torch.cuda.synchronize()
     wfl.destroy()
     sa.unordered_map_u64_u32.destroyDeviceObject(wtab)
@@ -193,12 +211,13 @@ def measure(frames: int = 30, log2_points: int = 24, capacity_log2: int = 23) ->
         torch.cuda.synchronize()
         a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         a.record(stream)
-        out = run_frame(tab, fl, keys, scene.cams[f], check=False)
+        out = run_frame(tab, fl, keys, scene.cams[f], bufs, check=False, counts=False)
         b.record(stream)
         b.synchronize()
         per_frame_ms.append(a.elapsed_time(b))
-        new_blocks.append(out["new_blocks"])
-        erased.append(out["erased"])
+        nb, er = (int(x) for x in out["counts"].cpu())
+        new_blocks.append(nb)
+        erased.append(er)
     ok = free_list_invariant(tab, fl, cap) and tab.valid()
     ops = frames * 2 * npts + sum(erased)
     res = {

@@ -44,6 +44,7 @@ def test_struct_layouts_match_header():
     assert ctypes.sizeof(_lib.htab_ring_view) == 8 * 8
     assert _lib.htab_deque_view is _lib.htab_ring_view and _lib.htab_vector_view is _lib.htab_ring_view
     assert ctypes.sizeof(_lib.htab_bitset_view) == 2 * 8
+    assert ctypes.sizeof(_lib.htab_block_region) == 4 + 4 + 8 + 8 + 3 * 8 + 8
 
 
 def test_argument_errors_without_gpu():
@@ -59,6 +60,10 @@ def test_argument_errors_without_gpu():
     assert lib.htab_destroy(None) == _lib.HTAB_E_ARG
     assert lib.htab_find(None, None, 0, None, None, None) == _lib.HTAB_E_ARG
     assert b"null" in lib.htab_last_error()
+    # the free-list calls refuse a missing table or deque before any device work
+    r = _lib.htab_block_region(21, 0, 1 << 20, 0.064)
+    assert lib.htab_assign_from_deque(None, None, None, 0, None, None, None) == _lib.HTAB_E_ARG
+    assert lib.htab_erase_far_blocks(None, ctypes.byref(r), None, None, None) == _lib.HTAB_E_ARG
 
 
 def test_missing_library_fails_loudly(tmp_path):

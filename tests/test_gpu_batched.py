@@ -349,3 +349,51 @@ def test_mixed_operations_on_one_table_and_find_without_hits_are_refused():
     assert lib.htab_batch_run_per_table(batch._b, _lib.HTAB_OP_FIND, off, ctypes.c_void_p(base), None, None, None,
                                         ctypes.c_void_p(s)) == E
     batch.destroy()
+
+
+@pytest.mark.parametrize("load", [0.5, 0.8])
+def test_tuned_persistent_launch_over_4096_tables_equals_plain_calls(load):
+    """The benchmark's full batched shape: 4096 unordered_map<u64,u32> tables of 2^8..2^18 keys
+    (log-uniform), insert all, find (70 % hits), erase 30 %, each phase one graph-captured persistent
+    launch with the tuning the autotuner picks on this workload (8-slot buckets, probe batch 8,
+    64-thread workgroups), at that load and at 0.8.  Every flag, hit, value, table size and the
+    contents of sampled tables equal one plain call per table on tables of the default tuning."""
+    from stdgpu_amd import autotune as at
+    from stdgpu_amd import workloads_device as wd
+
+    cfg = {"bucket_lanes": 8, "probe_batch": 8, "probe_group": 64, "block_size": 64, "load": load}
+    w = wd.batched_maps("cuda", ntables=4096)
+    tabs = at.build_tables(w["sizes"], cfg)
+    batch = TableBatch(tabs)
+    _, out = at.run_phases(batch, w)
+    plain = at.build_tables(w["sizes"], {"bucket_lanes": 8, "probe_batch": 0, "block_size": sa.Tuning().block_size,
+                                         "load": load})
+    pbatch = TableBatch(plain)
+    res = {}
+    for op, name, flag in (("insert", "ins", "inserted"), ("find", "find", "hits"), ("erase", "erase", "erased")):
+        n = w[f"{name}_keys"].numel()
+        f = torch.zeros(n, dtype=torch.uint8, device="cuda")
+        vo = torch.zeros(n, dtype=torch.int32, device="cuda") if op == "find" else None
+        pbatch.run_per_table(op, w[f"{name}_offsets"].cpu().numpy(), w[f"{name}_keys"],
+                             w["ins_vals"] if op == "insert" else None, f, vo)
+        res[flag], res["values"] = f, vo if vo is not None else res.get("values")
+    torch.cuda.synchronize()
+    for flag in ("inserted", "hits", "erased"):
+        assert torch.equal(out[flag], res[flag]), flag
+    hit = res["hits"] == 1
+    assert torch.equal(out["values"][hit], res["values"][hit])
+    assert int(hit.sum()) >= int(0.69 * w["find_keys"].numel())
+    e_off = w["erase_offsets"].cpu().numpy()
+    want = w["sizes"] - np.diff(e_off)
+    for t in range(len(tabs)):
+        assert tabs[t].size() == plain[t].size() == want[t], t
+    for t in (0, 1, 777, 2048, 4095):
+        ka, va = sorted_contents(tabs[t])
+        kb, vb = sorted_contents(plain[t])
+        np.testing.assert_array_equal(ka, kb)
+        np.testing.assert_array_equal(va, vb)
+        assert tabs[t].valid() and plain[t].valid()
+    batch.destroy()
+    pbatch.destroy()
+    for t in tabs + plain:
+        sa.unordered_map_u64_u32.destroyDeviceObject(t)
