@@ -500,19 +500,29 @@ insert_body(const table_ref<K, M>& t, const void* __restrict__ values, const voi
         C q[R];
         K kr[R];
         bool act[R];
-        uint32_t hb[R]; // home buckets (a table has fewer than 2^32 buckets)
+        // With up to 4 elements in flight the home buckets are recomputed from the keys where they
+        // are needed rather than kept in registers across the claims (k_insert<.., 8, 4> then stays
+        // within 64 VGPRs, 8 wavefronts per SIMD); with more, the kernel is past that budget anyway
+        // and keeping them saves the arithmetic.
+        constexpr bool kKeepHome = R > 4;
+        uint32_t hbk[kKeepHome ? R : 1];
+        const auto home_of = [&](int r) -> uint64_t {
+            if constexpr (kKeepHome)
+                return hbk[r];
+            else
+                return home_bucket(static_cast<uint64_t>(kr[r]), t.nbuckets);
+        };
 #pragma unroll
         for (int r = 0; r < R; ++r)
         {
             kr[r] = shfl(mykey, gbase + rb + r);
             act[r] = shfl32(probe ? 1U : 0U, gbase + rb + r) != 0;
-            hb[r] = 0;
+            if constexpr (kKeepHome) hbk[r] = static_cast<uint32_t>(home_bucket(static_cast<uint64_t>(kr[r]), t.nbuckets));
             if (act[r])
             {
-                hb[r] = static_cast<uint32_t>(home_bucket(static_cast<uint64_t>(kr[r]), t.nbuckets));
                 // a stale view is harmless here: a slot seen free but taken fails its CAS, and an
                 // insert-only launch never frees a slot (the in-bucket count comes with the load)
-                q[r] = C::load_plain(t.slots + static_cast<uint64_t>(hb[r]) * W, li);
+                q[r] = C::load_plain(t.slots + home_of(r) * W, li);
             }
         }
         // Decide every element's claim first, then issue the R compare-and-swaps together so that
@@ -598,9 +608,23 @@ insert_body(const table_ref<K, M>& t, const void* __restrict__ values, const voi
                                    ((q[r].key(j) == tomb_key<K>()) ? kWalkTomb : 0U);
                         }
                         info |= shfl32(part, fl);
+                        if (info & kWalkAbsent)
+                        {
+                            // absent past the home bucket and a free slot in it: claim there at once,
+                            // with the other elements' claims (as when the probe sequence ends here)
+                            flp = (flp & ~(pack_t(0xFF) << (8 * r))) | (static_cast<pack_t>(fl) << (8 * r));
+                            if (lane == fl)
+                            {
+                                sjp |= static_cast<pack_t>(part & 0xFFU) << (8 * r);
+                                if (part & kWalkTomb) tombp |= 1U << r;
+                            }
+                        }
                     }
-                    if (li == rb + r) mywalk = info;
-                    pend |= 1U << (rb + r);
+                    if (!(info & kWalkAbsent) || !(info & kWalkFree))
+                    {
+                        if (li == rb + r) mywalk = info;
+                        pend |= 1U << (rb + r);
+                    }
                 }
             }
         }
@@ -613,7 +637,7 @@ insert_body(const table_ref<K, M>& t, const void* __restrict__ values, const voi
             if (lane == fl)
             {
                 const K o = ((tombp >> r) & 1U) ? tomb_key<K>() : empty_key<K>();
-                got[r] = cas_agent(t.key_ptr(static_cast<uint64_t>(hb[r]) * W + static_cast<uint32_t>((sjp >> (8 * r)) & 0xFFU)), o, kr[r]);
+                got[r] = cas_agent(t.key_ptr(home_of(r) * W + static_cast<uint32_t>((sjp >> (8 * r)) & 0xFFU)), o, kr[r]);
             }
         }
 #pragma unroll
@@ -630,12 +654,12 @@ insert_body(const table_ref<K, M>& t, const void* __restrict__ values, const voi
                 {
                     const K o = ((tombp >> r) & 1U) ? tomb_key<K>() : empty_key<K>();
                     seen = (got[r] == o) ? 1 : ((got[r] == kr[r]) ? 2 : 0);
-                    if (seen == 1) store_mapped<K, M>(t.slots + static_cast<uint64_t>(hb[r]) * W + j, v);
+                    if (seen == 1) store_mapped<K, M>(t.slots + home_of(r) * W + j, v);
                 }
                 seen = shfl64(seen, fl);
                 if (seen == 1)
                 {
-                    const uint64_t slot = static_cast<uint64_t>(hb[r]) * W + shfl32(j, fl);
+                    const uint64_t slot = home_of(r) * W + shfl32(j, fl);
                     if (li == rb + r)
                     {
                         myins = true;
