@@ -11,14 +11,17 @@
 // erases that arrived before them.
 //
 // A launch is one kernel per 96 segments (k_serve), whose segments travel in the kernel arguments,
-// then one single-workgroup kernel that stores the sizes of the launch's tables into host-mapped
-// memory, and an event record: no copies and no planning kernel.
+// then, for tables near their limit, one single-workgroup kernel that stores their sizes into
+// host-mapped memory, and an event record: no copies and no planning kernel.
 //
 // Admission: inserts are coalesced while the table's known size plus the inserts in flight plus
-// their n stay within the table's limit.  An insert that may reach the limit (n counts every key,
-// also those already present) waits until the table has no launch in flight and then runs with no
-// other request of that table; its kernel ranks every new key against the exact limit, so it gets
-// what a plain call would (keys past the limit are not inserted) and no request is ever refused.
+// their n stay within the table's limit.  The known size is an upper bound: the last size read back
+// plus every key inserted since (erases ignored, keys already present counted).  A launch reads back
+// the exact size only of a table whose bound has passed 7/8 of its limit, so tables far from their
+// limit cost no size kernel (and no kernel boundary) per launch.  An insert that may reach the limit
+// waits until the table has no launch in flight and then runs with no other request of that table
+// (its size is read back); its kernel ranks every new key against the exact limit, so it gets what a
+// plain call would (keys past the limit are not inserted) and no request is ever refused.
 //
 // No reference counterpart: the reference serves one blocking bulk call at a time.
 #include <hip/hip_runtime.h>
@@ -50,6 +53,8 @@ struct launch_rec
     hipEvent_t done = nullptr;
     std::vector<int64_t> reqs;   // request indices
     std::vector<int32_t> tables; // distinct tables
+    std::vector<int64_t> ins_n;  // per table of `tables`: keys its inserts carry in this launch
+    std::vector<uint8_t> sized;  // per table of `tables`: this launch reads its size back
     bool live = false;
 };
 
@@ -179,7 +184,6 @@ htab_server_run(htab_server* s, const htab_segment* reqs, const int64_t* arrival
     htab_dev::serve_args args{};
     htab_dev::serve_tables tabs{};
     std::vector<int32_t> sized;
-    std::vector<int64_t> run_first(static_cast<size_t>(ntables), -1); // first request of a table's run
     while (finished < nreq && error == HTAB_OK)
     {
         bool progressed = false;
@@ -203,10 +207,15 @@ htab_server_run(htab_server* s, const htab_segment* reqs, const int64_t* arrival
                 status[r] = HTAB_OK;
                 if (reqs[r].op == HTAB_OP_INSERT) inflight_ins[static_cast<size_t>(reqs[r].table)] -= reqs[r].n;
             }
-            for (int32_t tb : l.tables)
+            for (size_t k = 0; k < l.tables.size(); ++k)
             {
-                // the latest completed launch on this table wrote its size
-                known[static_cast<size_t>(tb)] = __atomic_load_n(s->sizes_host + tb, __ATOMIC_ACQUIRE);
+                const int32_t tb = l.tables[k];
+                // a launch that read the size back gives it exactly; otherwise the bound grows by the
+                // launch's inserts (the table's launches complete in their order)
+                if (l.sized[k])
+                    known[static_cast<size_t>(tb)] = __atomic_load_n(s->sizes_host + tb, __ATOMIC_ACQUIRE);
+                else
+                    known[static_cast<size_t>(tb)] += l.ins_n[k];
                 --table_launches[static_cast<size_t>(tb)];
                 if (last_launch[static_cast<size_t>(tb)] == li) last_launch[static_cast<size_t>(tb)] = -1;
             }
@@ -214,6 +223,8 @@ htab_server_run(htab_server* s, const htab_segment* reqs, const int64_t* arrival
             --stream_launches[static_cast<size_t>(l.stream)];
             l.reqs.clear();
             l.tables.clear();
+            l.ins_n.clear();
+            l.sized.clear();
             l.live = false;
             --live;
             progressed = true;
@@ -241,7 +252,8 @@ htab_server_run(htab_server* s, const htab_segment* reqs, const int64_t* arrival
                 {
                     auto& q = fifo[static_cast<size_t>(t)];
                     const int op = reqs[q.front()].op;
-                    bool took = false;
+                    bool took = false, alone_run = false;
+                    int64_t run_ins = 0;
                     while (!q.empty() && reqs[q.front()].op == op && static_cast<int32_t>(segs.size()) < kMaxLaunchSegs)
                     {
                         const int64_t r = q.front();
@@ -261,15 +273,29 @@ htab_server_run(htab_server* s, const htab_segment* reqs, const int64_t* arrival
                             alone = true;
                         }
                         q.pop_front();
-                        if (!took) run_first[static_cast<size_t>(t)] = r;
                         segs.push_back(reqs[r]);
                         l.reqs.push_back(r);
-                        if (op == HTAB_OP_INSERT) inflight_ins[static_cast<size_t>(t)] += n;
+                        if (op == HTAB_OP_INSERT)
+                        {
+                            inflight_ins[static_cast<size_t>(t)] += n;
+                            run_ins += n;
+                        }
                         elems += n;
                         took = true;
+                        alone_run = alone_run || alone;
                         if (alone) break;
                     }
-                    if (took) l.tables.push_back(t);
+                    if (took)
+                    {
+                        l.tables.push_back(t);
+                        l.ins_n.push_back(run_ins);
+                        const __int128 bound = static_cast<__int128>(known[static_cast<size_t>(t)]) +
+                                               inflight_ins[static_cast<size_t>(t)];
+                        l.sized.push_back(op != HTAB_OP_FIND &&
+                                                  (alone_run || bound * 8 > static_cast<__int128>(s->limit[static_cast<size_t>(t)]) * 7)
+                                          ? 1
+                                          : 0);
+                    }
                     if (elems >= s->max_launch_elems || static_cast<int32_t>(segs.size()) >= kMaxLaunchSegs) break;
                 }
             }
@@ -318,8 +344,8 @@ htab_server_run(htab_server* s, const htab_segment* reqs, const int64_t* arrival
             }
             // sizes of the tables whose run inserts or erases (a find leaves the size alone)
             sized.clear();
-            for (int32_t t : l.tables)
-                if (reqs[run_first[static_cast<size_t>(t)]].op != HTAB_OP_FIND) sized.push_back(t);
+            for (size_t k = 0; k < l.tables.size(); ++k)
+                if (l.sized[k]) sized.push_back(l.tables[k]);
             for (size_t g = 0; g < sized.size() && rc == HTAB_OK; g += htab_dev::kServeTables)
             {
                 const size_t m = std::min(sized.size() - g, static_cast<size_t>(htab_dev::kServeTables));
