@@ -45,9 +45,9 @@ to_dev(memory_order o)
 }
 
 inline void
-check_hip(hipError_t e, const char* what)
+check_rt(int rc, const char* what)
 {
-    if (e != hipSuccess) throw std::runtime_error(std::string("stdgpu::atomic: ") + what + ": " + hipGetErrorString(e));
+    if (rc != HTAB_OK) throw std::runtime_error(std::string("stdgpu::atomic: ") + what + ": " + htab_last_error());
 }
 } // namespace detail
 
@@ -115,8 +115,7 @@ public:
     {
         if (_value == nullptr) return T(0);
         T v{};
-        detail::check_hip(hipMemcpyAsync(&v, _value, sizeof(T), hipMemcpyDeviceToHost, policy.stream), "load");
-        detail::check_hip(hipStreamSynchronize(policy.stream), "load");
+        detail::check_rt(htab_rt_copy(&v, _value, sizeof(T), 0, policy.stream), "load");
         return v;
     }
 
@@ -137,8 +136,7 @@ public:
     {
         if (_value == nullptr) return;
         T v = desired;
-        detail::check_hip(hipMemcpyAsync(_value, &v, sizeof(T), hipMemcpyHostToDevice, policy.stream), "store");
-        detail::check_hip(hipStreamSynchronize(policy.stream), "store");
+        detail::check_rt(htab_rt_copy(_value, &v, sizeof(T), 1, policy.stream), "store");
     }
 
     __host__ __device__ T operator=(const T desired) // NOLINT(misc-unconventional-assign-operator)
@@ -293,8 +291,8 @@ public:
     {
         atomic result(allocator);
         T* p = result._allocator.allocate(1);
-        detail::check_hip(hipMemsetAsync(p, 0, sizeof(T), policy.stream), "createDeviceObject");
-        detail::check_hip(hipStreamSynchronize(policy.stream), "createDeviceObject");
+        detail::check_rt(htab_rt_zero(p, sizeof(T), policy.stream), "createDeviceObject");
+        detail::check_rt(htab_rt_sync(policy.stream), "createDeviceObject");
         result._value = p;
         return result;
     }

@@ -198,7 +198,7 @@ private:
     {
         if (!_h) return;
         detail::check(htab_bitset_all_bits(_h, op, s), what);
-        detail::check(hipStreamSynchronize(s) == hipSuccess ? HTAB_OK : HTAB_E_HIP, what);
+        detail::check(htab_rt_sync(s), what);
     }
 
     htab_bitset* _h = nullptr;

@@ -222,7 +222,7 @@ public:
     {
         if (!_h) return;
         detail::check(htab_deque_clear(_h, policy.stream), "deque::clear");
-        detail::check(hipStreamSynchronize(policy.stream) == hipSuccess ? HTAB_OK : HTAB_E_HIP, "deque::clear");
+        detail::check(htab_rt_sync(policy.stream), "deque::clear");
     }
 
     // the capacity stays (the reference may or may not shrink, deque.inc shrink_to_fit)
@@ -273,11 +273,7 @@ private:
         std::uint64_t b = 0;
         if (_h)
         {
-            detail::check(hipMemcpyAsync(&b, _view.begin, sizeof(b), hipMemcpyDeviceToHost, s) == hipSuccess &&
-                                          hipStreamSynchronize(s) == hipSuccess
-                                  ? HTAB_OK
-                                  : HTAB_E_HIP,
-                          "deque::device_range");
+            detail::check(htab_rt_copy(&b, _view.begin, sizeof(b), 0, s), "deque::device_range");
         }
         return b;
     }

@@ -107,14 +107,14 @@ struct safe_device_allocator
     {
         if (n < 0) throw std::invalid_argument("stdgpu: negative allocation size");
         void* p = nullptr;
-        if (hipMalloc(&p, static_cast<std::size_t>(n) * sizeof(T) + (n == 0 ? 1 : 0)) != hipSuccess)
+        if (htab_rt_alloc(n * static_cast<std::int64_t>(sizeof(T)), &p) != HTAB_OK)
             throw std::runtime_error("stdgpu: device allocation failed");
         return static_cast<T*>(p);
     }
 
     void deallocate(T* p, std::int64_t /*n*/)
     {
-        if (p) (void)hipFree(p);
+        if (p) (void)htab_rt_free(p);
     }
 };
 

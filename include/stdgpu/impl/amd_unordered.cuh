@@ -382,11 +382,11 @@ public:
             if (n > 0)
                 hipLaunchKernelGGL((k_facade_insert<amd_unordered_base, value_type>), dim3(grid_of(n)), dim3(256), 0,
                                    policy.stream, *this, static_cast<const value_type*>(raw(first)), n);
-            check(static_cast<int>(hipGetLastError()) == 0 ? HTAB_OK : HTAB_E_HIP, "insert launch");
+            check(htab_rt_launch_status(), "insert launch");
         }
         else
             check(htab_insert(_h, raw(first), n, nullptr, policy.stream), "insert");
-        check(static_cast<int>(hipStreamSynchronize(policy.stream)) == 0 ? HTAB_OK : HTAB_E_HIP, "insert sync");
+        check(htab_rt_sync(policy.stream), "insert sync");
     }
 
     template <typename KeyIterator>
@@ -405,11 +405,11 @@ public:
             if (n > 0)
                 hipLaunchKernelGGL((k_facade_erase<amd_unordered_base, key_type>), dim3(grid_of(n)), dim3(256), 0,
                                    policy.stream, *this, static_cast<const key_type*>(raw(first)), n);
-            check(static_cast<int>(hipGetLastError()) == 0 ? HTAB_OK : HTAB_E_HIP, "erase launch");
+            check(htab_rt_launch_status(), "erase launch");
         }
         else
             check(htab_erase(_h, raw(first), n, nullptr, policy.stream), "erase");
-        check(static_cast<int>(hipStreamSynchronize(policy.stream)) == 0 ? HTAB_OK : HTAB_E_HIP, "erase sync");
+        check(htab_rt_sync(policy.stream), "erase sync");
     }
 
     // Bulk lookups over a key range (the reference has none: its callers run find per thread in a
@@ -467,7 +467,7 @@ public:
     {
         if (!_h) return;
         check(htab_clear(_h, policy.stream), "clear");
-        check(static_cast<int>(hipStreamSynchronize(policy.stream)) == 0 ? HTAB_OK : HTAB_E_HIP, "clear sync");
+        check(htab_rt_sync(policy.stream), "clear sync");
     }
 
     bool valid() const { return valid(execution::device); }
@@ -483,7 +483,7 @@ public:
             check(htab_hash_buffer(_h, &hashes), "valid");
             hipLaunchKernelGGL((k_facade_slot_hashes<amd_unordered_base>), dim3(1024), dim3(256), 0, policy.stream, *this,
                                hashes);
-            check(static_cast<int>(hipGetLastError()) == 0 ? HTAB_OK : HTAB_E_HIP, "valid launch");
+            check(htab_rt_launch_status(), "valid launch");
             check(htab_valid_hashed(_h, hashes, &ok, policy.stream), "valid");
         }
         else
@@ -600,11 +600,11 @@ protected:
             if (n > 0)
                 hipLaunchKernelGGL((k_facade_find<amd_unordered_base, key_type, M>), dim3(grid_of(n)), dim3(256), 0, s,
                                    *this, static_cast<const key_type*>(raw(first)), n, hits, static_cast<M*>(values));
-            check(static_cast<int>(hipGetLastError()) == 0 ? HTAB_OK : HTAB_E_HIP, what);
+            check(htab_rt_launch_status(), what);
         }
         else
             check(htab_find(_h, raw(first), n, hits, values, s), what);
-        check(static_cast<int>(hipStreamSynchronize(s)) == 0 ? HTAB_OK : HTAB_E_HIP, what);
+        check(htab_rt_sync(s), what);
     }
 
     __host__ __device__ static const Key& key_of(const value_type& v)

@@ -239,7 +239,7 @@ private:
 
     static void sync(hipStream_t s, const char* what)
     {
-        detail::check(hipStreamSynchronize(s) == hipSuccess ? HTAB_OK : HTAB_E_HIP, what);
+        detail::check(htab_rt_sync(s), what);
     }
 
     htab_vector* _h = nullptr;

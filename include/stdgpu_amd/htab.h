@@ -438,6 +438,25 @@ int htab_atomic_store(htab_atomic* a, int64_t index, const void* value, void* st
 const char* htab_last_error(void);
 
 /* ------------------------------------------------------------------------------------------
+ * Runtime helpers for the header-only C++ facade (include/stdgpu), so that its host code reaches
+ * HIP only through this library (the facade still launches its own kernels, which are instantiated
+ * for the caller's key and value types).  No reference counterpart: the reference's facade calls its
+ * backend directly (memory_detail.cuh, the backends' memory.h / device.h).
+ * ------------------------------------------------------------------------------------------ */
+/* device allocation of bytes (at least one byte); *out = null on failure (HTAB_E_OOM) */
+int htab_rt_alloc(int64_t bytes, void** out);
+int htab_rt_free(void* p);
+/* copy of bytes between host and device memory, then a wait for the stream: kind 0 device to host,
+ * 1 host to device, 2 device to device */
+int htab_rt_copy(void* dst, const void* src, int64_t bytes, int32_t kind, void* stream);
+/* bytes of device memory set to zero by a kernel (no memset node: safe inside a captured graph) */
+int htab_rt_zero(void* p, int64_t bytes, void* stream);
+/* waits for the stream's work; HTAB_E_HIP with the runtime's message if any of it failed */
+int htab_rt_sync(void* stream);
+/* the status of the calling thread's last kernel launch (HTAB_OK or HTAB_E_HIP) */
+int htab_rt_launch_status(void);
+
+/* ------------------------------------------------------------------------------------------
  * deque<T> (deque.cuh; deque_detail.cuh:202-448 push_back / pop_back / push_front / pop_front,
  * :482-512 size), the free list of voxel-block pipelines among others; see htab_ring_view above.
  * ------------------------------------------------------------------------------------------ */

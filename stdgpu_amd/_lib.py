@@ -246,6 +246,12 @@ SIGNATURES = {
     "htab_vector_get_view": (_I32, [_P, ctypes.POINTER(htab_vector_view)]),
     "htab_assign_from_deque": (_I32, [_P, _P, _P, _I64, _P, _P, _P]),
     "htab_erase_far_blocks": (_I32, [_P, ctypes.POINTER(htab_block_region), _P, _P, _P]),
+    "htab_rt_alloc": (_I32, [_I64, ctypes.POINTER(_P)]),
+    "htab_rt_free": (_I32, [_P]),
+    "htab_rt_copy": (_I32, [_P, _P, _I64, ctypes.c_int32, _P]),
+    "htab_rt_zero": (_I32, [_P, _I64, _P]),
+    "htab_rt_sync": (_I32, [_P]),
+    "htab_rt_launch_status": (_I32, []),
 }
 
 _lib = None

@@ -56,6 +56,22 @@ launch_status(const char* what)
     return HTAB_OK;
 }
 
+// bytes set to zero (htab_rt_zero): 16-byte stores where the range allows, bytes at its ends
+template <int = 0>
+__global__ __launch_bounds__(256) void
+k_zero_bytes(unsigned char* __restrict__ p, uint64_t n)
+{
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+    const uint64_t t = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const uint64_t head = (16 - (reinterpret_cast<uintptr_t>(p) & 15)) & 15;
+    const uint64_t h = head < n ? head : n;
+    const uint64_t words = (n - h) / 16;
+    uint4* w = reinterpret_cast<uint4*>(p + h);
+    for (uint64_t i = t; i < words; i += stride) w[i] = make_uint4(0, 0, 0, 0);
+    for (uint64_t i = t; i < h; i += stride) p[i] = 0;
+    for (uint64_t i = h + words * 16 + t; i < n; i += stride) p[i] = 0;
+}
+
 int
 value_bytes_of(int kind)
 {
@@ -601,6 +617,65 @@ htab_erase_far_blocks(htab* h, const htab_block_region* region, htab_deque* q, i
     c.count_dev = count;
     c.s = static_cast<hipStream_t>(stream);
     return dispatch(h->kind, c);
+}
+
+// ------------------------------------------------------------------ runtime helpers (C++ facade)
+int
+htab_rt_alloc(int64_t bytes, void** out)
+{
+    if (!out || bytes < 0) return fail(HTAB_E_ARG, "htab_rt_alloc: bad argument");
+    *out = nullptr;
+    if (hipMalloc(out, static_cast<size_t>(bytes > 0 ? bytes : 1)) != hipSuccess)
+    {
+        *out = nullptr;
+        return fail(HTAB_E_OOM, "htab_rt_alloc: device allocation failed");
+    }
+    return HTAB_OK;
+}
+
+int
+htab_rt_free(void* p)
+{
+    if (p && hipFree(p) != hipSuccess) return fail(HTAB_E_HIP, "htab_rt_free");
+    return HTAB_OK;
+}
+
+int
+htab_rt_copy(void* dst, const void* src, int64_t bytes, int32_t kind, void* stream)
+{
+    if (bytes < 0 || kind < 0 || kind > 2 || (bytes > 0 && (!dst || !src))) return fail(HTAB_E_ARG, "htab_rt_copy: bad argument");
+    if (bytes == 0) return HTAB_OK;
+    const hipMemcpyKind k = kind == 0 ? hipMemcpyDeviceToHost : (kind == 1 ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    HTAB_HIP_TRY(hipMemcpyAsync(dst, src, static_cast<size_t>(bytes), k, s), "htab_rt_copy");
+    HTAB_HIP_TRY(hipStreamSynchronize(s), "htab_rt_copy: sync");
+    return HTAB_OK;
+}
+
+int
+htab_rt_zero(void* p, int64_t bytes, void* stream)
+{
+    if (bytes < 0 || (bytes > 0 && !p)) return fail(HTAB_E_ARG, "htab_rt_zero: bad argument");
+    if (bytes == 0) return HTAB_OK;
+    const uint64_t n = static_cast<uint64_t>(bytes);
+    uint64_t grid = (n + 255) / 256;
+    if (grid > 65536) grid = 65536;
+    hipLaunchKernelGGL(k_zero_bytes<>, dim3(static_cast<uint32_t>(grid)), dim3(256), 0, static_cast<hipStream_t>(stream),
+                       static_cast<unsigned char*>(p), n);
+    return launch_status("htab_rt_zero");
+}
+
+int
+htab_rt_sync(void* stream)
+{
+    HTAB_HIP_TRY(hipStreamSynchronize(static_cast<hipStream_t>(stream)), "htab_rt_sync");
+    return HTAB_OK;
+}
+
+int
+htab_rt_launch_status(void)
+{
+    return launch_status("kernel launch");
 }
 
 int

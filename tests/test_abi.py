@@ -60,6 +60,12 @@ def test_argument_errors_without_gpu():
     assert lib.htab_destroy(None) == _lib.HTAB_E_ARG
     assert lib.htab_find(None, None, 0, None, None, None) == _lib.HTAB_E_ARG
     assert b"null" in lib.htab_last_error()
+    # runtime helpers refuse bad arguments before any device work
+    out = ctypes.c_void_p()
+    assert lib.htab_rt_alloc(-1, ctypes.byref(out)) == _lib.HTAB_E_ARG
+    assert lib.htab_rt_copy(None, None, 8, 0, None) == _lib.HTAB_E_ARG
+    assert lib.htab_rt_copy(ctypes.c_void_p(0x1000), ctypes.c_void_p(0x1000), 8, 3, None) == _lib.HTAB_E_ARG
+    assert lib.htab_rt_zero(None, 8, None) == _lib.HTAB_E_ARG
     # the free-list calls refuse a missing table or deque before any device work
     r = _lib.htab_block_region(21, 0, 1 << 20, 0.064)
     assert lib.htab_assign_from_deque(None, None, None, 0, None, None, None) == _lib.HTAB_E_ARG
