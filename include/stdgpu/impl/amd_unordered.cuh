@@ -10,7 +10,12 @@
 // is their bits (unique object representations: structs of integers without padding, whose operator==
 // compares every member, such as the reference tests' 6-byte vec3int16), compared with equal_to<Key>
 // or the transparent equal_to<>; any hasher callable with a Key.  A key is stored in a 4- or 8-byte
-// slot key, zero-extended.  Mapped types: any trivially copyable type, empty ones included.
+// slot key, zero-extended.  Keys of more than 8 bytes whose copy is a byte copy (a trivial copy
+// assignment; the reference tests' 12-byte vec3int32, whose destructor is user-provided) are kept whole in the slot's payload record
+// and compared with key_equal; the slot key is then an id unique to the entry, the table is placed by
+// the hasher's value, and presence is decided under the home bucket's lock (htab_device.hpp
+// insert_if / find_if / erase_if).  Payload records are bytes: no destructor runs on them.  Mapped
+// types: any trivially copyable type, empty ones included.
 //
 // Two storage forms:
 //   * aliased: value_type is laid out exactly as the table's slot (4- or 8-byte keys with 4- or 8-byte
@@ -142,11 +147,14 @@ public:
     using iterator = pointer;
     using const_iterator = const_pointer;
 
-    static_assert((std::is_integral_v<Key> ||
-                   (std::is_trivially_copyable_v<Key> && std::has_unique_object_representations_v<Key>)) &&
-                          sizeof(Key) <= 8,
-                  "stdgpu (MI355X library): keys are integers, or trivially copyable types without padding "
-                  "bits, of at most 8 bytes");
+    // keys wider than a slot key: whole in the payload record, an entry id in the slot
+    static constexpr bool kWide = sizeof(Key) > 8;
+    static_assert(((std::is_integral_v<Key> ||
+                    (std::is_trivially_copyable_v<Key> && std::has_unique_object_representations_v<Key>)) &&
+                   sizeof(Key) <= 8) ||
+                          (kWide && std::is_copy_constructible_v<Key> && std::is_trivially_copy_assignable_v<Key>),
+                  "stdgpu (MI355X library): keys are integers, trivially copyable types without padding bits of "
+                  "at most 8 bytes, or wider types whose copy is a byte copy");
     static constexpr bool mapped_ok()
     {
         if constexpr (std::is_void_v<Mapped>)
@@ -155,7 +163,8 @@ public:
             return std::is_trivially_copyable_v<Mapped>;
     }
     static_assert(mapped_ok(), "stdgpu (MI355X library): mapped types are trivially copyable");
-    static_assert(std::is_trivially_copyable_v<Value> || !std::is_void_v<Mapped>,
+    static_assert(std::is_trivially_copyable_v<Value> || !std::is_void_v<Mapped> ||
+                          (kWide && std::is_trivially_copy_assignable_v<Value>),
                   "stdgpu (MI355X library): value types are trivially copyable");
     static_assert(std::is_same_v<KeyEqual, equal_to<Key>> || std::is_same_v<KeyEqual, equal_to<void>>,
                   "stdgpu (MI355X library): keys compare with stdgpu::equal_to<Key> or equal_to<>");
@@ -178,7 +187,7 @@ public:
     static constexpr bool keylike_ok = kTransparent && std::is_integral_v<Key> && std::is_integral_v<KeyLike> &&
                                        !std::is_same_v<KeyLike, Key>;
     // a table placed by the hasher's value: transparent lookups of a non-integer key
-    static constexpr bool kHashed = kTransparent && !std::is_integral_v<Key>;
+    static constexpr bool kHashed = (kTransparent && !std::is_integral_v<Key>) || kWide;
     template <typename KeyLike>
     static constexpr bool keylike_hashed_ok =
             kHashed && !std::is_same_v<std::remove_cv_t<KeyLike>, Key> &&
@@ -219,6 +228,15 @@ public:
         out._h = h;
         check(htab_get_view(h, &out._view), "createDeviceObject");
         out._allocator = allocator;
+        if constexpr (kWide)
+        {
+            // the entry ids: a 64-bit counter, first id 1 (the busy claim value 0 is never an id)
+            void* ids = nullptr;
+            check(htab_rt_alloc(sizeof(std::uint64_t), &ids), "createDeviceObject");
+            out._ids = static_cast<std::uint64_t*>(ids);
+            check(htab_rt_zero(ids, sizeof(std::uint64_t), nullptr), "createDeviceObject");
+            check(htab_rt_sync(nullptr), "createDeviceObject");
+        }
     }
 
     // takes over a table made through the C ABI (for the tunings createDeviceObject has no parameter
@@ -237,7 +255,9 @@ public:
     static void destroy(amd_unordered_base& obj)
     {
         if (obj._h) check(htab_destroy(obj._h), "destroyDeviceObject");
+        if (obj._ids) check(htab_rt_free(obj._ids), "destroyDeviceObject");
         obj._h = nullptr;
+        obj._ids = nullptr;
         obj._view = htab_view{};
     }
 
@@ -316,8 +336,18 @@ public:
                 slot->second = bits;
             }
         };
-        const skey k = to_skey(key_of(value));
-        slot_type* p = kHashed ? e.insert_at(k, home_of(key_of(value)), write, &ins) : e.insert(k, write, &ins);
+        slot_type* p = nullptr;
+        if constexpr (kWide)
+        {
+            const skey id = static_cast<skey>(
+                    __hip_atomic_fetch_add(_ids, std::uint64_t(1), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1);
+            p = e.insert_if(id, kBusy, home_of(key_of(value)), holds(key_of(value)), write, &ins);
+        }
+        else
+        {
+            const skey k = to_skey(key_of(value));
+            p = kHashed ? e.insert_at(k, home_of(key_of(value)), write, &ins) : e.insert(k, write, &ins);
+        }
         return pair<iterator, bool>(entry(p), ins);
     }
 
@@ -330,7 +360,11 @@ public:
     __device__ index_type erase(const key_type& key)
     {
         const engine e = eng();
-        const bool done = kHashed ? e.erase_at(to_skey(key), home_of(key)) : e.erase(to_skey(key));
+        bool done = false;
+        if constexpr (kWide)
+            done = e.erase_if(home_of(key), holds(key));
+        else
+            done = kHashed ? e.erase_at(to_skey(key), home_of(key)) : e.erase(to_skey(key));
         return done ? 1 : 0;
     }
 
@@ -356,12 +390,19 @@ public:
         {
             const skey k = keys[s * step];
             if (k == htab_dev::empty_key<skey>() || k == htab_dev::tomb_key<skey>()) continue;
-            c += bucket(from_skey(k)) == n ? 1 : 0;
+            if constexpr (kWide)
+                c += (k != kBusy && bucket(key_of(entries()[s])) == n) ? 1 : 0;
+            else
+                c += bucket(from_skey(k)) == n ? 1 : 0;
         }
-        // the two reserved key values live in side slots with a presence bit each
-        const std::uint32_t f = htab_dev::load_agent(e.flags);
-        if ((f & 1U) && bucket(from_skey(htab_dev::empty_key<skey>())) == n) ++c;
-        if ((f & 2U) && bucket(from_skey(htab_dev::tomb_key<skey>())) == n) ++c;
+        if constexpr (!kWide)
+        {
+            // the two reserved key values live in side slots with a presence bit each (entry ids of
+            // wide keys never take them)
+            const std::uint32_t f = htab_dev::load_agent(e.flags);
+            if ((f & 1U) && bucket(from_skey(htab_dev::empty_key<skey>())) == n) ++c;
+            if ((f & 2U) && bucket(from_skey(htab_dev::tomb_key<skey>())) == n) ++c;
+        }
         return c;
     }
 
@@ -548,7 +589,10 @@ public:
     {
         const skey k = *eng().key_ptr(s);
         if (k == htab_dev::empty_key<skey>() || k == htab_dev::tomb_key<skey>()) return 0;
-        return static_cast<std::uint64_t>(hasher{}(from_skey(k)));
+        if constexpr (kWide)
+            return static_cast<std::uint64_t>(hasher{}(key_of(entries()[s])));
+        else
+            return static_cast<std::uint64_t>(hasher{}(from_skey(k)));
     }
 
 protected:
@@ -563,15 +607,60 @@ protected:
     __device__ slot_type* find_slot(const key_type& key) const
     {
         const engine e = eng();
-        return kHashed ? e.find_at(to_skey(key), home_of(key)) : e.find(to_skey(key));
+        if constexpr (kWide)
+            return e.find_if(home_of(key), holds(key));
+        else
+            return kHashed ? e.find_at(to_skey(key), home_of(key)) : e.find(to_skey(key));
     }
 
     template <typename KeyLike>
     __device__ slot_type* find_like(const KeyLike& k) const
     {
         const engine e = eng();
-        return e.find_match(e.home_of_hash(static_cast<std::uint64_t>(hasher{}(k))),
-                            [&k](skey bits) { return key_equal{}(from_skey(bits), k); });
+        const std::uint64_t h = e.home_of_hash(static_cast<std::uint64_t>(hasher{}(k)));
+        if constexpr (kWide)
+            return e.find_if(h, holds(k));
+        else
+            return e.find_match(h, [&k](skey bits) { return key_equal{}(from_skey(bits), k); });
+    }
+
+    // wide keys: the slot key an entry holds while its record is written (never an entry id), and
+    // whether occupied slot s (slot key k) holds a key comparing equal to `key` (a Key or a KeyLike).
+    // The record's key is read with agent-scope loads: a plain load may be served by this CU's L1
+    // from before another call wrote the record, while the entry id that call stored after writing it
+    // (a release store) is only ever read from L2, and so is the record then.
+    static constexpr skey kBusy = 0;
+    template <typename KeyLike>
+    __device__ auto holds(const KeyLike& key) const
+    {
+        const value_type* rec = entries();
+        return [rec, &key](std::uint64_t s, skey k) {
+            if (k == kBusy) return false;
+            alignas(Key) unsigned char buf[sizeof(Key)];
+            load_coherent(buf, &key_of(rec[s]));
+            return key_equal{}(*reinterpret_cast<const Key*>(buf), key);
+        };
+    }
+
+    // the bytes of *p read with agent-scope loads, four at a time where the key's alignment allows
+    __device__ static void load_coherent(unsigned char* out, const Key* p)
+    {
+        if constexpr (sizeof(Key) % 4 == 0 && alignof(Key) % 4 == 0)
+        {
+            const std::uint32_t* w = reinterpret_cast<const std::uint32_t*>(p);
+#pragma unroll
+            for (std::size_t i = 0; i < sizeof(Key) / 4; ++i)
+            {
+                const std::uint32_t v = __hip_atomic_load(w + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __builtin_memcpy(out + 4 * i, &v, 4);
+            }
+        }
+        else
+        {
+            const unsigned char* b = reinterpret_cast<const unsigned char*>(p);
+            for (std::size_t i = 0; i < sizeof(Key); ++i)
+                out[i] = __hip_atomic_load(b + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
     }
 
     // the entry of a slot: the slot itself (aliased) or its payload record
@@ -649,6 +738,7 @@ protected:
     htab* _h = nullptr;
     htab_view _view{};
     Allocator _allocator{};
+    std::uint64_t* _ids = nullptr; // wide keys: the entry-id counter
 };
 
 } // namespace stdgpu::detail

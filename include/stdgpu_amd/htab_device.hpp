@@ -362,6 +362,15 @@ struct table_ref
         const uint32_t f = load_agent(flags);
         if ((f & 1U) && match(empty_key<K>())) return slots + side_index(empty_key<K>());
         if ((f & 2U) && match(tomb_key<K>())) return slots + side_index(tomb_key<K>());
+        return find_if(home, [&match](uint64_t, K k) { return match(k); });
+    }
+
+    // The stored entry for which present(s, k) holds, where s is an occupied slot and k its slot key,
+    // looked up from home bucket `home` to the end of the probe sequence (no away count applies: the
+    // caller's key need not be the slot key; the side slots are not searched).
+    template <typename Present>
+    __device__ value_type* find_if(uint64_t home, const Present& present) const
+    {
         uint64_t b = home;
         for (uint64_t step = 0; step < nbuckets; ++step)
         {
@@ -378,13 +387,59 @@ struct table_ref
                         saw_empty = true;
                         continue;
                     }
-                    if (k[u] != tomb_key<K>() && match(k[u])) return slots + b * W + ((j0 + u) & (W - 1));
+                    const uint64_t s = b * W + ((j0 + u) & (W - 1));
+                    if (k[u] != tomb_key<K>() && present(s, k[u])) return slots + s;
                 }
             }
             if (saw_empty || overflow(b) == 0) break;
             b = next_bucket(b, nbuckets);
         }
         return end_ptr();
+    }
+
+    // erase() of the entry for which present(s, k) holds (see find_if and insert_if): the slot
+    // becomes TOMB, and the counts along its probe sequence come down as in erase_at
+    template <typename Present>
+    __device__ bool erase_if(uint64_t h, const Present& present) const
+    {
+        uint64_t b = h;
+        for (uint64_t step = 0; step < nbuckets; ++step)
+        {
+            bool saw_empty = false;
+            uint64_t hit = ~uint64_t(0);
+            K hk = 0;
+            for (uint32_t j0 = 0; j0 < W; j0 += kScanBatch)
+            {
+                K k[kScanBatch];
+                load_scan(b, 0, j0, k);
+#pragma unroll
+                for (uint32_t u = 0; u < kScanBatch; ++u)
+                {
+                    if (k[u] == empty_key<K>())
+                    {
+                        saw_empty = true;
+                        continue;
+                    }
+                    const uint64_t s = b * W + ((j0 + u) & (W - 1));
+                    if (hit == ~uint64_t(0) && k[u] != tomb_key<K>() && present(s, k[u]))
+                    {
+                        hit = s;
+                        hk = k[u];
+                    }
+                }
+            }
+            if (hit != ~uint64_t(0))
+            {
+                if (cas_agent(key_ptr(hit), hk, tomb_key<K>()) != hk) return false;
+                count_add(-1);
+                overflow_add(h, step, -1);
+                if (step != 0 && away_counters() != 0) away_add(h, away_slot(hk), -1);
+                return true;
+            }
+            if (saw_empty || overflow(b) == 0) break;
+            b = next_bucket(b, nbuckets);
+        }
+        return false;
     }
 
     // Adds delta (+1 or -1, the same at each call site) to the size for every lane of the wavefront
@@ -483,6 +538,36 @@ struct table_ref
             return p;
         }
         if (find_at(key, h) != end_ptr()) return end_ptr();
+        return locked_claim(
+                h, [&]() { return find_at(key, h) != end_ptr(); },
+                [&]() { return claim_locked(key, key, h, [key](uint64_t, K k) { return k == key; }, true, write_value, inserted); });
+    }
+
+    // insert() of an entry whose key the caller compares itself: keys wider than the slot key (the C++
+    // facade keeps such a key in the slot's payload record and gives each entry a unique slot key).
+    // present(s, k) says whether occupied slot s, holding slot key k, holds the entry's key; `key` is
+    // the slot key the new entry gets (neither reserved value), `busy` a value no entry ever has: the
+    // free slot is claimed as busy, write_value fills it, and only then is `key` stored (release), so a
+    // scan never compares a record before it is written.  Inserts of one home bucket take its lock, so
+    // two calls of one key never both claim.
+    template <typename Present, typename WriteValue>
+    __device__ value_type* insert_if(K key, K busy, uint64_t h, const Present& present, const WriteValue& write_value,
+                                     bool* inserted) const
+    {
+        *inserted = false;
+        if (nstripes == 1 && load_agent(stripes) >= static_cast<int64_t>(limit)) return end_ptr();
+        if (find_if(h, present) != end_ptr()) return end_ptr();
+        return locked_claim(
+                h, [&]() { return find_if(h, present) != end_ptr(); },
+                [&]() { return claim_locked(key, busy, h, present, false, write_value, inserted); });
+    }
+
+    // The home-bucket lock around claim(): the calls of a wavefront that wait on one lock word try it
+    // through their lowest lane; found() lets a waiting call leave once another call has inserted its
+    // key meanwhile.
+    template <typename Found, typename Claim>
+    __device__ value_type* locked_claim(uint64_t h, const Found& found, const Claim& claim) const
+    {
         uint32_t* lock = ovf_ptr(h);
         const int lane = static_cast<int>(__builtin_amdgcn_mbcnt_hi(~0U, __builtin_amdgcn_mbcnt_lo(~0U, 0U)));
         value_type* result = end_ptr();
@@ -496,8 +581,7 @@ struct table_ref
             bool ready = false;
             if (waiting)
             {
-                if (!first && ((nstripes == 1 && load_agent(stripes) >= static_cast<int64_t>(limit)) ||
-                               find_at(key, h) != end_ptr()))
+                if (!first && ((nstripes == 1 && load_agent(stripes) >= static_cast<int64_t>(limit)) || found()))
                     waiting = false;
                 else
                     ready = !(load_agent(lock) & kOvfLock);
@@ -518,7 +602,7 @@ struct table_ref
                 got = !(old & kOvfLock);
                 if (got)
                 {
-                    result = claim_locked(key, h, write_value, inserted);
+                    result = claim();
                     settle();
                     __hip_atomic_fetch_and(lock, ~kOvfLock, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                     waiting = false;
@@ -531,8 +615,12 @@ struct table_ref
     }
 
     // the body of insert() under the home-bucket lock
-    template <typename WriteValue>
-    __device__ value_type* claim_locked(K key, uint64_t h, const WriteValue& write_value, bool* inserted) const
+    // key: the entry's slot key; claim: what the free slot is claimed as (key itself, or a busy value
+    // that write_value's record precedes, see insert_if); present(s, k): occupied slot s holds the key;
+    // keyed: present is k == key, so the key's away count may end the scan at its home bucket
+    template <typename Present, typename WriteValue>
+    __device__ value_type* claim_locked(K key, K claim, uint64_t h, const Present& present, bool keyed,
+                                        const WriteValue& write_value, bool* inserted) const
     {
         constexpr uint64_t kNone = ~uint64_t(0);
         if (nstripes == 1 && load_agent(stripes) >= static_cast<int64_t>(limit)) return end_ptr();
@@ -555,7 +643,8 @@ struct table_ref
                     const uint32_t j = j0 + u;
                     if (j >= W) break; // positions past W repeat loaded slots (scan order matters here)
                     const K k = kk[u];
-                    if (k == key) return end_ptr();
+                    if (k != empty_key<K>() && k != tomb_key<K>() && present(b * W + ((o + j) & (W - 1)), k))
+                        return end_ptr();
                     if (fslot == kNone && (k == empty_key<K>() || k == tomb_key<K>()))
                     {
                         fslot = b * W + ((o + j) & (W - 1));
@@ -568,7 +657,7 @@ struct table_ref
             }
             if (saw_empty || overflow(b) == 0) break;
             // the away count ends the scan at the home bucket as in find()
-            if (step == 0 && away_counters() != 0 && away(b, away_slot(key)) == 0) break;
+            if (keyed && step == 0 && away_counters() != 0 && away(b, away_slot(key)) == 0) break;
             b = next_bucket(b, nbuckets);
         }
         // 2. absent: claim the first free slot from the scan's first one on (the next bucket on
@@ -625,7 +714,7 @@ struct table_ref
                 // lock bit)
                 settle();
             }
-            const K seen = cas_agent(key_ptr(fslot), fseen, key);
+            const K seen = cas_agent(key_ptr(fslot), fseen, claim);
             if (seen == fseen)
             {
                 if (!count_add(1))
@@ -636,6 +725,7 @@ struct table_ref
                     return end_ptr();
                 }
                 write_value(slots + fslot);
+                if (claim != key) __hip_atomic_store(key_ptr(fslot), key, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
                 *inserted = true;
                 return slots + fslot;
             }

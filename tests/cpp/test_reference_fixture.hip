@@ -7,8 +7,12 @@
 // the set's 6-byte value_type lives in a payload record per slot, and both tables are placed by
 // vec_hash's value so that a vec3int32 finds its vec3int16.
 //
-// Every case of the reference file is here except clear_nontrivial, whose table type keys by the
-// 12-byte vec3int32 (keys are at most 8 bytes in this library).  Where the reference asserts that a
+// Every case of the reference file is here, clear_nontrivial included, whose tables key by the 12-byte
+// vec3int32 (unordered_map<vec3int32, dummy, vec_hash, equal_to<>> and unordered_set<vec3int32,
+// vec_hash, equal_to<>>, unordered_map.inc:238, unordered_set.inc:233): keys wider than 8 bytes live
+// whole in a payload record per slot, under a unique entry id.  A further case drives those tables
+// from many threads (duplicates, one key from every thread, find, erase, bulk calls, valid()) against
+// a host model.  Where the reference asserts that a
 // key without a collision sits at begin() + bucket(key) (an artefact of its layout, which this
 // library does not share), the case asserts that the key is found and that bucket() agrees for the
 // pinned colliding pairs.  Prints one line per case; exits non-zero if any case fails.
@@ -21,6 +25,7 @@
 #include <numeric>
 #include <random>
 #include <set>
+#include <tuple>
 #include <vector>
 
 #include <stdgpu/unordered_map.cuh>
@@ -726,6 +731,215 @@ custom_allocator_map()
     cmap_t::destroyDeviceObject(c);
 }
 
+// ------------------------------------------------------------------ keys wider than 8 bytes
+using nt_map = stdgpu::unordered_map<vec3int32, dummy, vec_hash, stdgpu::equal_to<>>;
+using nt_set = stdgpu::unordered_set<vec3int32, vec_hash, stdgpu::equal_to<>>;
+static_assert(sizeof(vec3int32) == 12, "the reference fixture's vec3int32");
+
+__host__ __device__ inline vec3int32
+key_of_value(const vec3int32& v)
+{
+    return v;
+}
+__host__ __device__ inline vec3int32
+key_of_value(const stdgpu::pair<const vec3int32, dummy>& v)
+{
+    return v.first;
+}
+
+template <typename Table>
+__device__ typename Table::value_type
+wide_value(const vec3int32& k)
+{
+    if constexpr (std::is_same_v<typename Table::value_type, vec3int32>)
+        return k;
+    else
+        return typename Table::value_type(k, dummy{});
+}
+
+template <typename Table>
+__global__ void
+k_wide_insert(Table t, const vec3int32* keys, std::size_t n, int* inserted)
+{
+    const std::size_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) inserted[i] = t.insert(wide_value<Table>(keys[i])).second ? 1 : 0;
+}
+
+template <typename Table>
+__global__ void
+k_wide_find(Table t, const vec3int32* keys, std::size_t n, int* found)
+{
+    const std::size_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const auto it = t.find(keys[i]);
+    // the entry found holds the key looked up; contains() and count() agree
+    const bool hit = it != t.end();
+    found[i] = hit ? ((key_of_value(*it) == keys[i] && t.contains(keys[i]) && t.count(keys[i]) == 1) ? 1 : 2)
+                   : (t.contains(keys[i]) ? 2 : 0);
+}
+
+template <typename Table>
+__global__ void
+k_wide_erase(Table t, const vec3int32* keys, std::size_t n, int* erased)
+{
+    const std::size_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) erased[i] = static_cast<int>(t.erase(keys[i]));
+}
+
+template <typename Range>
+__global__ void
+k_wide_range_keys(Range r, vec3int32* out)
+{
+    const std::size_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < static_cast<std::size_t>(r.size())) out[i] = key_of_value(*(r.begin() + i));
+}
+
+static std::vector<vec3int32>
+random_wide(std::size_t n, unsigned seed)
+{
+    std::mt19937 rng(seed);
+    std::uniform_int_distribution<std::int32_t> d(-2000000000, 2000000000);
+    std::vector<vec3int32> out;
+    out.reserve(n);
+    for (std::size_t i = 0; i < n; ++i) out.emplace_back(d(rng), d(rng), d(rng));
+    return out;
+}
+
+static std::tuple<std::int32_t, std::int32_t, std::int32_t>
+tup(const vec3int32& k)
+{
+    return { k.x, k.y, k.z };
+}
+
+// clear_nontrivial (unordered_datastructure.inc:2461-2478)
+template <typename Table>
+static void
+case_clear_nontrivial()
+{
+    g_case = "clear_nontrivial (vec3int32 keys)";
+    Table t = Table::createDeviceObject(100000);
+    const std::vector<vec3int32> position = { vec3int32(-7, -3, 15) };
+    vec3int32* d = to_dev(position);
+    int* ins = dev_alloc<int>(1);
+    hipLaunchKernelGGL(k_wide_insert<Table>, dim3(1), dim3(64), 0, 0, t, d, 1, ins);
+    CHECK(to_host(ins, 1)[0] == 1);
+    CHECK(t.size() == 1);
+    CHECK(t.valid());
+    t.clear();
+    CHECK(t.size() == 0);
+    CHECK(t.valid());
+    (void)hipFree(d);
+    (void)hipFree(ins);
+    Table::destroyDeviceObject(t);
+}
+
+// many threads: 40000 distinct keys inserted twice each in one kernel (exactly one insert of each key
+// reports true), one key from 4096 threads at once, finds of present and absent keys, an erase of
+// half, re-inserts, bulk insert / erase, valid() after every step, contents against a host model
+template <typename Table>
+static void
+case_wide_many()
+{
+    g_case = "vec3int32 keys from many threads (payload records, entry ids)";
+    const std::size_t N = 40000;
+    Table t = Table::createDeviceObject(100000);
+    std::vector<vec3int32> keys = random_wide(N, 7);
+    std::set<std::tuple<std::int32_t, std::int32_t, std::int32_t>> model;
+    for (const auto& k : keys) model.insert(tup(k));
+    CHECK(model.size() == N); // distinct (drawn from 2^96)
+    std::vector<vec3int32> twice(keys);
+    twice.insert(twice.end(), keys.begin(), keys.end());
+    std::shuffle(twice.begin(), twice.end(), std::mt19937(3));
+    vec3int32* d2 = to_dev(twice);
+    int* flag = dev_alloc<int>(2 * N + 4096);
+    hipLaunchKernelGGL(k_wide_insert<Table>, dim3(blocks(2 * N)), dim3(256), 0, 0, t, d2, 2 * N, flag);
+    std::vector<int> f = to_host(flag, 2 * N);
+    if (sum(f) != static_cast<int>(N) || t.size() != static_cast<stdgpu::index_t>(N))
+        std::printf("  inserted %d of %zu keys, size %lld\n", sum(f), N, static_cast<long long>(t.size()));
+    CHECK(sum(f) == static_cast<int>(N) && t.size() == static_cast<stdgpu::index_t>(N) && t.valid());
+    // each key's two inserts: exactly one reported true
+    std::set<std::tuple<std::int32_t, std::int32_t, std::int32_t>> won;
+    for (std::size_t i = 0; i < 2 * N; ++i)
+        if (f[i]) CHECK(won.insert(tup(twice[i])).second);
+    CHECK(won == model);
+    // one new key from 4096 threads at once
+    const vec3int32 hot(123, -456, 789);
+    std::vector<vec3int32> same(4096, hot);
+    vec3int32* ds = to_dev(same);
+    hipLaunchKernelGGL(k_wide_insert<Table>, dim3(blocks(4096)), dim3(256), 0, 0, t, ds, 4096, flag);
+    CHECK(sum(to_host(flag, 4096)) == 1 && t.size() == static_cast<stdgpu::index_t>(N + 1) && t.valid());
+    model.insert(tup(hot));
+    // finds: every stored key and as many absent ones
+    std::vector<vec3int32> probe(keys);
+    const std::vector<vec3int32> absent = random_wide(N, 99);
+    probe.insert(probe.end(), absent.begin(), absent.end());
+    vec3int32* dp = to_dev(probe);
+    int* found = dev_alloc<int>(2 * N);
+    hipLaunchKernelGGL(k_wide_find<Table>, dim3(blocks(2 * N)), dim3(256), 0, 0, t, dp, 2 * N, found);
+    std::vector<int> fr = to_host(found, 2 * N);
+    for (std::size_t i = 0; i < 2 * N; ++i) CHECK(fr[i] == (model.count(tup(probe[i])) ? 1 : 0));
+    // erase the first half (and each key of it a second time: one reports 1), then re-insert a quarter
+    std::vector<vec3int32> half(keys.begin(), keys.begin() + N / 2);
+    std::vector<vec3int32> half2(half);
+    half2.insert(half2.end(), half.begin(), half.end());
+    vec3int32* dh = to_dev(half2);
+    hipLaunchKernelGGL(k_wide_erase<Table>, dim3(blocks(N)), dim3(256), 0, 0, t, dh, N, flag);
+    CHECK(sum(to_host(flag, N)) == static_cast<int>(N / 2) && t.size() == static_cast<stdgpu::index_t>(N / 2 + 1) &&
+          t.valid());
+    for (const auto& k : half) model.erase(tup(k));
+    std::vector<vec3int32> quarter(keys.begin(), keys.begin() + N / 4);
+    vec3int32* dq = to_dev(quarter);
+    hipLaunchKernelGGL(k_wide_insert<Table>, dim3(blocks(N / 4)), dim3(256), 0, 0, t, dq, N / 4, flag);
+    CHECK(sum(to_host(flag, N / 4)) == static_cast<int>(N / 4) && t.valid());
+    for (const auto& k : quarter) model.insert(tup(k));
+    // bulk erase of the hot key and the quarter through the host API, then a bulk insert of them back
+    std::vector<vec3int32> bulk(quarter);
+    bulk.push_back(hot);
+    vec3int32* db = to_dev(bulk);
+    t.erase(stdgpu::make_device(db), stdgpu::make_device(db + bulk.size()));
+    for (const auto& k : bulk) model.erase(tup(k));
+    CHECK(t.size() == static_cast<stdgpu::index_t>(model.size()) && t.valid());
+    // contents: the device range holds exactly the model's keys
+    {
+        auto range = t.device_range();
+        const std::size_t n = static_cast<std::size_t>(range.size());
+        CHECK(n == model.size());
+        vec3int32* dk = dev_alloc<vec3int32>(n + 1);
+        if (n) hipLaunchKernelGGL((k_wide_range_keys<decltype(range)>), dim3(blocks(n)), dim3(256), 0, 0, range, dk);
+        std::set<std::tuple<std::int32_t, std::int32_t, std::int32_t>> have;
+        for (const auto& k : to_host(dk, n)) have.insert(tup(k));
+        (void)hipFree(dk);
+        CHECK(have == model);
+    }
+    hipLaunchKernelGGL(k_wide_find<Table>, dim3(blocks(2 * N)), dim3(256), 0, 0, t, dp, 2 * N, found);
+    fr = to_host(found, 2 * N);
+    for (std::size_t i = 0; i < 2 * N; ++i) CHECK(fr[i] == (model.count(tup(probe[i])) ? 1 : 0));
+    (void)hipFree(d2);
+    (void)hipFree(ds);
+    (void)hipFree(dp);
+    (void)hipFree(dh);
+    (void)hipFree(dq);
+    (void)hipFree(db);
+    (void)hipFree(flag);
+    (void)hipFree(found);
+    Table::destroyDeviceObject(t);
+}
+
+template <typename Table>
+static void
+run_wide(const char* name)
+{
+    std::printf("-- %s\n", name);
+    using fn = void (*)();
+    const fn cases[] = { case_clear_nontrivial<Table>, case_wide_many<Table> };
+    for (fn c : cases)
+    {
+        const int before = g_failures;
+        c();
+        std::printf("%s %s\n", g_failures == before ? "ok  " : "FAIL", g_case);
+    }
+}
+
 template <typename Table>
 static void
 run_all()
@@ -749,6 +963,8 @@ main()
     std::setvbuf(stdout, nullptr, _IOLBF, 0); // a case that never ends still names itself
     run_all<fx_map>();
     run_all<fx_set>();
+    run_wide<nt_map>("unordered_map<vec3int32, dummy, vec_hash, equal_to<>>");
+    run_wide<nt_set>("unordered_set<vec3int32, vec_hash, equal_to<>>");
     const int before = g_failures;
     custom_allocator_map();
     std::printf("%s %s\n", g_failures == before ? "ok  " : "FAIL", g_case);
