@@ -50,11 +50,11 @@ k_freelist_begin(htab_ring_view r, int64_t* w, int push)
 
 // Ranking: each workgroup owns a contiguous run of at most kFreelistTiles tiles of kFreelistThreads
 // elements.  Pass 1 reads all of them at once (the tile loop is unrolled, so the loads of every tile
-// are in flight together and what pass 2 needs stays in registers) and records which threads of each
-// tile want a rank (one ballot mask per wavefront per tile, in LDS); thread 0 turns the masks into
-// prefix counts and takes the workgroup's run with one atomic on the call's counter; pass 2 gives each
-// wanting thread its rank.  So a call makes at most one counter atomic per workgroup, however sparse
-// or dense the wanting threads are.
+// are in flight together) and records which elements of each tile want a rank (one mask per
+// wavefront per tile, in LDS); thread 0 turns the masks into prefix counts and takes the workgroup's
+// run with one atomic on the call's counter; pass 2 works on each wanting element with the rank its
+// position gives it.  So a call makes at most one counter atomic per workgroup, however sparse or
+// dense the wanting elements are.
 constexpr int kFreelistTiles = 8;
 constexpr int kFreelistWaves = kFreelistThreads / 64;
 // both kernels are built for 8 wavefronts per SIMD (two workgroups per CU): their work is latency of
@@ -78,14 +78,6 @@ block_tiles(uint64_t tiles, uint64_t* t0, uint32_t* nt)
     *nt = *t0 < t1 ? static_cast<uint32_t>(t1 - *t0) : 0U;
 }
 
-// pass 1, one tile: record this thread's want
-__device__ inline void
-rank_note(rank_lds& L, uint32_t tile, bool want)
-{
-    const uint64_t b = ballot(want);
-    if (lane_id() == 0) L.mask[tile][threadIdx.x / 64] = b;
-}
-
 // between the passes: prefix counts and the workgroup's base (every thread calls it)
 __device__ inline void
 rank_settle(rank_lds& L, uint32_t ntiles, int64_t* counter)
@@ -106,18 +98,36 @@ rank_settle(rank_lds& L, uint32_t ntiles, int64_t* counter)
     __syncthreads();
 }
 
-// pass 2, one tile: this thread's rank (call only for a thread that wanted one)
-__device__ inline int64_t
-rank_of(const rank_lds& L, uint32_t tile)
+// Dealing a wavefront's sparse work: the elements marked in the wave-uniform per-tile masks m are
+// numbered in tile order, then lane order, and element c goes to lane c % 64 in round c / 64, so a
+// wavefront with a few marked elements in each of its tiles runs their dependent memory accesses in
+// one round instead of one round per tile.  dealt() gives this lane's element of round rd: its tile
+// and the lane it belongs to.
+__device__ inline bool
+dealt(const uint64_t (&m)[kFreelistTiles], uint32_t rd, uint32_t* tile, uint32_t* src)
 {
-    const int lane = lane_id();
-    const int v = static_cast<int>(threadIdx.x) / 64;
-    return L.base + L.prefix[tile][v] + __builtin_popcountll(L.mask[tile][v] & ((uint64_t(1) << lane) - 1));
+    uint32_t c = rd * 64U + static_cast<uint32_t>(lane_id());
+#pragma unroll
+    for (int u = 0; u < kFreelistTiles; ++u)
+    {
+        const uint32_t cnt = static_cast<uint32_t>(__builtin_popcountll(m[u]));
+        if (c < cnt)
+        {
+            uint64_t mm = m[u];
+            for (uint32_t i = 0; i < c; ++i) mm &= mm - 1; // the c-th marked lane of tile u
+            *tile = static_cast<uint32_t>(u);
+            *src = static_cast<uint32_t>(__builtin_ctzll(mm));
+            return true;
+        }
+        c -= cnt;
+    }
+    return false;
 }
 
 // ---------------------------------------------------------------- take
 // flags[i] == 1: keys[i] gets the value at deque position w[1] + rank (2 when the deque ran out
-// first; 3 when keys[i] is not in the table, nothing popped for it)
+// first; 3 when keys[i] is not in the table, nothing popped for it).  The finds of a wavefront's
+// flagged elements are dealt one per lane; the lane that finds an element also assigns it.
 template <typename K, typename M>
 __global__ __launch_bounds__(kFreelistThreads) __attribute__((amdgpu_waves_per_eu(8, 8))) void
 k_take_values(table_ref<K, M> t, const K* __restrict__ keys, uint8_t* __restrict__ flags, int64_t n,
@@ -132,46 +142,63 @@ k_take_values(table_ref<K, M> t, const K* __restrict__ keys, uint8_t* __restrict
     uint64_t t0;
     uint32_t nt;
     block_tiles((static_cast<uint64_t>(n) + kFreelistThreads - 1) / kFreelistThreads, &t0, &nt);
+    const int wave = static_cast<int>(threadIdx.x) / 64;
     const int64_t i0 = static_cast<int64_t>(t0 * kFreelistThreads + threadIdx.x);
-    uint8_t f[kFreelistTiles];
+    const int64_t w0 = static_cast<int64_t>(t0 * kFreelistThreads) + wave * 64; // lane 0 of this wavefront, tile 0
+    // the elements the insert added, one ballot per tile
+    uint64_t m[kFreelistTiles];
 #pragma unroll
     for (int u = 0; u < kFreelistTiles; ++u)
     {
         const int64_t i = i0 + static_cast<int64_t>(u) * kFreelistThreads;
-        f[u] = (static_cast<uint32_t>(u) < nt && i < n) ? flags[i] : 0;
+        m[u] = ballot(static_cast<uint32_t>(u) < nt && i < n && flags[i] == 1);
     }
-    value_type* p[kFreelistTiles];
-#pragma unroll
-    for (int u = 0; u < kFreelistTiles; ++u)
+    if (lane_id() == 0)
     {
-        p[u] = nullptr;
-        if (f[u] == 1)
+#pragma unroll
+        for (int u = 0; u < kFreelistTiles; ++u) L.mask[u][wave] = 0;
+    }
+    // their finds, dealt one per lane; a found element is marked for ranking (an LDS or into its
+    // tile's mask; the zeroing above precedes it in this wavefront's order) and kept as one word per
+    // round: its slot << 11 | tile << 6 | lane (kNone: nothing found this round)
+    constexpr uint64_t kNone = ~uint64_t(0);
+    uint64_t found[kFreelistTiles];
+#pragma unroll
+    for (int rd = 0; rd < kFreelistTiles; ++rd)
+    {
+        found[rd] = kNone;
+        uint32_t u = 0, src = 0;
+        if (dealt(m, static_cast<uint32_t>(rd), &u, &src))
         {
-            const int64_t i = i0 + static_cast<int64_t>(u) * kFreelistThreads;
-            p[u] = t.find(keys[i]);
-            if (p[u] == t.end_ptr())
+            const int64_t i = w0 + static_cast<int64_t>(u) * kFreelistThreads + src;
+            value_type* q = t.find(keys[i]);
+            if (q == t.end_ptr())
             {
-                p[u] = nullptr;
                 flags[i] = 3;
             }
+            else
+            {
+                found[rd] = (static_cast<uint64_t>(q - t.slots) << 11) | (u << 6) | src;
+                atomicOr(reinterpret_cast<unsigned long long*>(&L.mask[u][wave]), 1ULL << src);
+            }
         }
-        if (static_cast<uint32_t>(u) < nt) rank_note(L, u, p[u] != nullptr);
     }
     rank_settle(L, nt, w);
 #pragma unroll
-    for (int u = 0; u < kFreelistTiles; ++u)
+    for (int rd = 0; rd < kFreelistTiles; ++rd)
     {
-        if (!p[u]) continue;
-        const int64_t rank = rank_of(L, u);
+        if (found[rd] == kNone) continue;
+        const uint32_t u = static_cast<uint32_t>(found[rd] >> 6) & 31U, src = static_cast<uint32_t>(found[rd]) & 63U;
+        const int64_t rank = L.base + L.prefix[u][wave] + __builtin_popcountll(L.mask[u][wave] & ((uint64_t(1) << src) - 1));
         if (rank < avail)
         {
             const uint64_t slot = (pos0 + static_cast<uint64_t>(rank)) % cap;
-            p[u]->second = data[slot];
+            t.slots[found[rd] >> 11].second = data[slot];
             r.state[slot] += 2; // full -> free: the call runs alone on the deque
         }
         else
         {
-            flags[i0 + static_cast<int64_t>(u) * kFreelistThreads] = 2;
+            flags[w0 + static_cast<int64_t>(u) * kFreelistThreads + src] = 2;
         }
     }
 }
@@ -223,9 +250,11 @@ k_evict_far(table_ref<K, M> t, htab_block_region g, htab_ring_view r, int64_t* w
     uint64_t t0;
     uint32_t nt;
     block_tiles((ns + kFreelistThreads - 1) / kFreelistThreads, &t0, &nt);
+    const int wave = static_cast<int>(threadIdx.x) / 64;
     const uint64_t s0 = t0 * kFreelistThreads + threadIdx.x;
+    const uint64_t w0 = t0 * kFreelistThreads + static_cast<uint64_t>(wave) * 64; // lane 0 of this wavefront, tile 0
     // plain loads: nothing but this launch's erases changes the table meanwhile, and each slot is read
-    // by one thread
+    // by one thread here and erased by at most one
     K k[kFreelistTiles];
 #pragma unroll
     for (int u = 0; u < kFreelistTiles; ++u)
@@ -233,12 +262,12 @@ k_evict_far(table_ref<K, M> t, htab_block_region g, htab_ring_view r, int64_t* w
         const uint64_t s = s0 + static_cast<uint64_t>(u) * kFreelistThreads;
         k[u] = (static_cast<uint32_t>(u) < nt && s < ns) ? *t.key_ptr(s) : empty_key<K>();
     }
-    uint32_t want = 0;
+    uint64_t m[kFreelistTiles];
 #pragma unroll
     for (int u = 0; u < kFreelistTiles; ++u)
     {
-        if (!table_ref<K, M>::is_reserved(k[u]) && far_block(g, static_cast<uint64_t>(k[u]))) want |= 1U << u;
-        if (static_cast<uint32_t>(u) < nt) rank_note(L, u, (want >> u) & 1U);
+        m[u] = ballot(!table_ref<K, M>::is_reserved(k[u]) && far_block(g, static_cast<uint64_t>(k[u])));
+        if (static_cast<uint32_t>(u) < nt && lane_id() == 0) L.mask[u][wave] = m[u];
     }
     rank_settle(L, nt, w);
     if (threadIdx.x == 0)
@@ -251,13 +280,16 @@ k_evict_far(table_ref<K, M> t, htab_block_region g, htab_ring_view r, int64_t* w
             __hip_atomic_fetch_add(t.stripes + static_cast<uint64_t>(stripe) * kStripeStride, -e, __ATOMIC_RELAXED,
                                    __HIP_MEMORY_SCOPE_AGENT);
     }
+    // the erases of a wavefront's far slots, dealt one per lane
 #pragma unroll
-    for (int u = 0; u < kFreelistTiles; ++u)
+    for (int rd = 0; rd < kFreelistTiles; ++rd)
     {
-        if (!((want >> u) & 1U)) continue;
-        const int64_t rank = rank_of(L, u);
+        uint32_t u = 0, src = 0;
+        if (!dealt(m, static_cast<uint32_t>(rd), &u, &src)) continue;
+        const int64_t rank = L.base + L.prefix[u][wave] + __builtin_popcountll(m[u] & ((uint64_t(1) << src) - 1));
         if (rank >= room) continue;
-        const uint64_t s = s0 + static_cast<uint64_t>(u) * kFreelistThreads;
+        const uint64_t s = w0 + static_cast<uint64_t>(u) * kFreelistThreads + src;
+        const K key = *t.key_ptr(s);
         const uint64_t b = s / t.W;
         // EMPTY when no key runs through the bucket (an EMPTY slot in it says so too), else TOMB; the
         // bucket's keys and count are loaded together
@@ -272,13 +304,13 @@ k_evict_far(table_ref<K, M> t, htab_block_region g, htab_ring_view r, int64_t* w
         }
         __hip_atomic_store(t.key_ptr(s), ends ? empty_key<K>() : tomb_key<K>(), __ATOMIC_RELAXED,
                            __HIP_MEMORY_SCOPE_AGENT);
-        const uint64_t h = home_bucket(static_cast<uint64_t>(k[u]), t.nbuckets);
+        const uint64_t h = home_bucket(static_cast<uint64_t>(key), t.nbuckets);
         if (h != b)
         {
             t.overflow_add(h, t.distance(h, b), -1);
             if (t.away_counters() != 0)
             {
-                const uint32_t f = t.away_slot(k[u]);
+                const uint32_t f = t.away_slot(key);
                 if (t.away(h, f) < kAwaySaturated) t.away_add_fast(h, f, -1);
             }
         }

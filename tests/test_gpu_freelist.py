@@ -32,7 +32,9 @@ def deque_list(q):
 
 def table_dict(t):
     k, v = t.contents()
-    return dict(zip(k.cpu().numpy().view(np.uint64).tolist(), (v.cpu().numpy().view(np.uint32)).tolist()))
+    kn = k.cpu().numpy()
+    kn = kn.view(np.uint64) if kn.dtype.itemsize == 8 else kn.view(np.uint32)
+    return dict(zip(kn.tolist(), (v.cpu().numpy().view(np.uint32)).tolist()))
 
 
 @pytest.mark.parametrize("deque_len", [1000, 5000])
@@ -162,3 +164,49 @@ def test_free_list_calls_refuse_sets_and_mismatched_deques():
         mm.erase_far_blocks((0, 0, 0), 1.0, 1.0, q, bits=22)
     assert _lib.load().htab_last_error()
     q.destroy()
+
+
+def test_free_list_calls_on_u32_keys():
+    """unordered_map<u32,u32> (8-byte slots, 16-slot buckets): blocks of 10 bits per axis; the take
+    gives every added key one of the deque's front elements, the evict erases exactly numpy's far
+    selection and pushes its values, and the oracle's table after the same calls holds the same
+    entries."""
+    rng = np.random.default_rng(21)
+    c = rng.integers(-500, 500, size=(6000, 3), dtype=np.int64)
+    off = 1 << 9
+    k = np.unique(((c[:, 0] + off) << 20) | ((c[:, 1] + off) << 10) | (c[:, 2] + off)).astype(np.uint32)
+    n = len(k)
+    cap = int(n / 0.85)
+    tab = sa.unordered_map_u32_u32.createDeviceObject(cap)
+    ref = po.Table("u32", "u32", cap)
+    keys = torch.from_numpy(k.view(np.int32)).cuda()
+    flags = torch.empty(n, dtype=torch.uint8, device="cuda")
+    tab.insert(keys, torch.full((n,), -1, dtype=torch.int32, device="cuda"), inserted=flags)
+    assert bool((flags == 1).all())
+    q = sa.deque_u32(n + 100)
+    q.fill_iota()
+    count = torch.zeros(1, dtype=torch.int64, device="cuda")
+    tab.assign_from_deque(keys, flags, q, count=count)
+    assert int(count.item()) == n and deque_list(q) == list(range(n, n + 100))
+    got = table_dict(tab)
+    assert sorted(got.values()) == list(range(n))
+    ref.insert(k, np.array([got[int(x)] for x in k], dtype=np.uint32))
+    # the far test on 10-bit fields, block size 1.0, centre (3, -2, 7), radius 300
+    fx = ((k >> 20) & 1023).astype(np.int64) - off
+    fy = ((k >> 10) & 1023).astype(np.int64) - off
+    fz = (k & 1023).astype(np.int64) - off
+    d2 = (fx + 0.5 - 3.0) ** 2 + (fy + 0.5 + 2.0) ** 2 + (fz + 0.5 - 7.0) ** 2
+    far = d2 > 300.0 * 300.0
+    assert 500 < int(far.sum()) < n - 500
+    tab.erase_far_blocks((3.0, -2.0, 7.0), 300.0, 1.0, q, bits=10, offset=off, count=count)
+    assert int(count.item()) == int(far.sum())
+    after = table_dict(tab)
+    assert set(after) == set(int(x) for x in k[~far])
+    pushed = deque_list(q)[100:]
+    assert sorted(pushed) == sorted(got[int(x)] for x in k[far])
+    ref.erase(k[far])
+    rk, rv = ref.sorted_contents()
+    assert dict(zip(rk.tolist(), rv.tolist())) == after
+    assert tab.valid() and q.valid()
+    q.destroy()
+    sa.unordered_map_u32_u32.destroyDeviceObject(tab)
