@@ -8,7 +8,13 @@
 // scatter; the keys a full bin cannot take (a hot key of a skewed input, or bad luck) spill to a list.
 // Then one workgroup per region copies the region into LDS, inserts the region's keys there with LDS
 // atomics (the claim protocol of the per-key kernel, so copies of one key in a launch still leave one
-// stored key) and writes the region back.  A key whose probe sequence runs past the region's last
+// stored key) and writes the region back.  Maps with 8-byte keys claim a slot by a 32-bit
+// compare-and-swap on the low word of its value instead of a 64-bit one on its key: gfx950 runs the
+// 64-bit LDS compare-and-swap far slower (at 2^26 keys it took 3.2 ms of this kernel's 4.0 ms; the
+// 32-bit form and the 64-bit exchange take none of that).  The copy marks every free slot's value word
+// unclaimed when the region arrives; the winner of the word stores its key, then its value; a thread
+// that loses the word reads the slot again until the winner's key is there.  A stored value may equal
+// the mark: a claim that wins the word of a slot already holding a key puts the mark back.  A key whose probe sequence runs past the region's last
 // bucket (into a region another workgroup owns meanwhile) is deferred.  The per-key kernel then
 // inserts the deferred keys, the spilled keys and the two reserved key values on the whole table.
 //
@@ -315,6 +321,24 @@ lds_cas(T* p, T expected, T desired)
     return expected;
 }
 
+// the value word of a free slot in the LDS copy of a region (maps with 8-byte keys), and of a slot
+// whose claim is under way
+constexpr uint32_t kRegionUnclaimed = 0xFFFFFFFFU;
+constexpr uint32_t kRegionClaimed = 0xFFFFFFFEU;
+
+template <typename K, typename M>
+struct region_claim
+{
+    // the claim goes through the low word of the value (offset 8 of the slot)
+    static constexpr bool by_value = sizeof(K) == 8 && sizeof(M) >= 4;
+};
+
+template <typename K>
+struct region_claim<K, void>
+{
+    static constexpr bool by_value = false;
+};
+
 enum class region_result
 {
     inserted,
@@ -387,31 +411,74 @@ region_insert_one(typename slot_traits<K, M>::value_type* lslots, uint32_t* lovf
         if (b + 1 >= nb) return region_result::deferred; // the sequence runs on into the next region
         ++b;
     }
+    // claims: the first free slot from (cb, cj) on in the key's order; after a lost claim the bucket is
+    // read again, all its keys in flight together (a walk one slot at a time puts a chain of dependent
+    // LDS round trips on every lost claim, and a wavefront waits for its longest chain)
     uint32_t cb = fb >= 0 ? static_cast<uint32_t>(fb) : b + 1;
     uint32_t cj = fb >= 0 ? fj : 0U;
     while (cb < nb)
     {
-        bool found = false;
+        int32_t nj = -1;
         K seen = empty_key<K>();
-        for (; cj < W; ++cj)
+        for (uint32_t j0 = 0; j0 < W; j0 += kSeg)
         {
-            const K x = lds_load(kp(cb, cj));
-            if (x == key) return region_result::present;
-            if (x == empty_key<K>() || x == tomb_key<K>())
+            K kk[kSeg];
+#pragma unroll
+            for (uint32_t j = 0; j < kSeg; ++j) kk[j] = lds_load(kp(cb, j0 + j));
+            bool hit = false;
+#pragma unroll
+            for (uint32_t j = 0; j < kSeg; ++j)
             {
-                seen = x;
-                found = true;
-                break;
+                hit |= kk[j] == key;
+                if (nj < 0 && j0 + j >= cj && (kk[j] == empty_key<K>() || kk[j] == tomb_key<K>()))
+                {
+                    nj = static_cast<int32_t>(j0 + j);
+                    seen = kk[j];
+                }
             }
+            if (hit) return region_result::present;
         }
-        if (!found)
+        if (nj < 0)
         {
             ++cb;
             cj = 0;
             continue;
         }
-        const K old = lds_cas(kp(cb, cj), seen, key);
-        if (old == seen)
+        cj = static_cast<uint32_t>(nj);
+        K old;
+        bool won;
+        if constexpr (region_claim<K, M>::by_value)
+        {
+            uint32_t* vw = reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(kp(cb, cj)) + sizeof(K));
+            old = seen;
+            won = false;
+            if (lds_cas(vw, kRegionUnclaimed, kRegionClaimed) == kRegionUnclaimed)
+            {
+                old = lds_load(kp(cb, cj));
+                if (old == empty_key<K>() || old == tomb_key<K>())
+                {
+                    __hip_atomic_store(kp(cb, cj), key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                    // the key before the value (which may equal the mark): a wavefront's LDS operations
+                    // are performed in the order they are issued, so only the compiler must keep it
+                    asm volatile("" ::: "memory");
+                    old = seen;
+                    won = true;
+                }
+                else
+                {
+                    // the slot already holds a key whose value's low word is the mark: put it back
+                    __hip_atomic_store(vw, kRegionUnclaimed, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                }
+            }
+            // lost the word: old is still a free-slot key until the winner stores its key, and the slot is
+            // read again (every claimer stores its key in the same pass as its compare-and-swap)
+        }
+        else
+        {
+            old = lds_cas(kp(cb, cj), seen, key);
+            won = old == seen;
+        }
+        if (won)
         {
             store_mapped<K, M>(reinterpret_cast<value_type*>(kp(cb, cj)), val);
             for (uint32_t x = lb; x < cb; ++x) atomicAdd(ovp(x), 1U);
@@ -437,7 +504,7 @@ region_insert_one(typename slot_traits<K, M>::value_type* lslots, uint32_t* lovf
             return region_result::inserted;
         }
         if (old == key) return region_result::present;
-        ++cj;
+        if (old != empty_key<K>() && old != tomb_key<K>()) ++cj;
     }
     return region_result::deferred; // no free slot left in the region
 }
@@ -512,6 +579,18 @@ k_region_insert(table_ref<K, M> t, region_scratch sc)
         for (uint32_t i = threadIdx.x; i < nb; i += kRegionThreads) lovf[i] = t.ovf[b0 + i];
     __builtin_amdgcn_s_waitcnt(0); // the direct loads count as vector memory operations
     __syncthreads();
+    if constexpr (region_claim<K, M>::by_value)
+    {
+        // every free slot of the copy starts unclaimed
+        for (uint32_t i = threadIdx.x; i < nb * W; i += kRegionThreads)
+        {
+            const uint32_t b = i / W, x = i - b * W;
+            char* sp = reinterpret_cast<char*>(s.slots) + lds_piece<G>(b, x / (W / G)) * 16 + (x % (W / G)) * sizeof(value_type);
+            const K kk = *reinterpret_cast<const K*>(sp);
+            if (kk == empty_key<K>() || kk == tomb_key<K>()) *reinterpret_cast<uint32_t*>(sp + sizeof(K)) = kRegionUnclaimed;
+        }
+        __syncthreads();
+    }
     const bool ordered = t.nbuckets < kOrderedClaimBuckets;
     int64_t added = 0;
     auto one = [&](uint32_t q, K k, uint64_t v) {
