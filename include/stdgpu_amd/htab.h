@@ -51,9 +51,12 @@ typedef struct htab_tuning
     int32_t probe_batch;       /* elements a lane group probes together: 2, 4 or 8 (at most bucket_lanes);
                                 * default: 8 for find, 4 for insert and erase (1 with one-lane buckets) */
     int32_t insert_path;       /* bulk inserts without per-element flags on tables whose limit is their
-                                * slot count: 0 (default) and 1 = one compare-and-swap per key,
-                                * 2 = keys partitioned by table region and each region inserted in
-                                * LDS (tables of at most 65536 regions of 32 KB) */
+                                * slot count: 1 = one compare-and-swap per key; 2 = keys partitioned
+                                * by table region and each region inserted in LDS (tables of at most
+                                * 65536 regions of 32 KB; the launch allocates its scratch the first
+                                * time, about 40 bytes per key for unordered_map<u64, u32>); 0 (default) = 2 for a launch of half
+                                * to all of the slots of a table larger than 256 MB (falling back to
+                                * 1 if its scratch cannot be allocated), else 1 */
     int32_t caller_hashes;     /* nonzero: keys are placed by a 64-bit hash their caller computes (the C++
                                 * facade's hasher, for transparent lookups of struct keys), so only the
                                 * native per-thread calls (which take that hash) and htab_valid_hashed

@@ -38,13 +38,23 @@ struct ops
     using T = table_ref<K, M>;
 
     // The partitioned path (htab_region.hpp) for a launch without per-element flags on a table whose
-    // limit is its slot count, when the table asks for it.  It is not the default: at 2^26 keys into
-    // 2^23 buckets its five launches take 5.3 ms where one CAS per key takes 4.1 ms.
+    // limit is its slot count: always when the table asks for it (insert_path 2), and by default
+    // (insert_path 0) for a launch of half to all of the slots of a table larger than the 256 MB
+    // Infinity Cache, whose per-key compare-and-swaps go to HBM.  At 2^26 keys into 2^23 buckets of
+    // map<u64,u32> the partitioned launches take 3.5 ms against 4.3 ms; a launch of a fifth of the slots
+    // (2^24 keys) takes 1.6 ms against 1.3 ms, since the region copy streams the whole table whatever
+    // the launch holds.  A launch of more keys than slots (copies of keys, which the per-key kernel
+    // finds in cache) and tables in the cache keep one CAS per key.
     static bool use_regions(const htab* h, int64_t n, const uint8_t* ins)
     {
-        if (ins || h->view.nstripes == 1 || n <= 0 || n > 0x7FFFFFFFLL || h->insert_path != 2) return false;
+        if (ins || h->view.nstripes == 1 || n <= 0 || n > 0x7FFFFFFFLL || h->insert_path == 1) return false;
         const uint64_t nreg = (h->view.nbuckets + region_buckets<G>() - 1) / region_buckets<G>();
-        return nreg <= kMaxRegions;
+        if (nreg > kMaxRegions) return false;
+        if (h->insert_path == 2) return true;
+        const uint64_t slots = T::from_view(h->view).nslots();
+        const uint64_t bytes = slots * sizeof(typename T::value_type);
+        const uint64_t keys = static_cast<uint64_t>(n);
+        return bytes > kRegionDefaultMinBytes && 2 * keys >= slots && keys <= slots;
     }
 
     static int insert_regions(htab* h, const void* values, const void* keys, const void* vals, int64_t n, hipStream_t s)
@@ -86,7 +96,14 @@ struct ops
             if (h->part) (void)hipFree(h->part);
             h->part = nullptr;
             h->part_bytes = 0;
-            HTAB_HIP_TRY(hipMalloc(&h->part, off), "partitioned insert scratch");
+            if (hipMalloc(&h->part, off) != hipSuccess)
+            {
+                (void)hipGetLastError();
+                h->part = nullptr;
+                // the default choice falls back to one compare-and-swap per key
+                if (h->insert_path == 0) return kRegionsDeclined;
+                return fail(HTAB_E_OOM, "partitioned insert scratch");
+            }
             h->part_bytes = off;
         }
         char* base = static_cast<char*>(h->part);
@@ -107,6 +124,10 @@ struct ops
         if (rgrid < 2) rgrid = 2;
         if (rgrid > 1024) rgrid = 1024;
         hipLaunchKernelGGL(k_region_reset<>, dim3(rgrid), dim3(256), 0, s, ctl, sc.region_fill, static_cast<uint32_t>(nrs));
+        // the default choice first samples the launch for copies of keys (k_region_sample)
+        const bool sampled = h->insert_path == 0;
+        if (sampled)
+            hipLaunchKernelGGL((k_region_sample<K, M>), dim3(1), dim3(256), 0, s, values, keys, vals, n, ctl);
         hipLaunchKernelGGL((k_part1_scatter<K, M>), dim3(nchunks), dim3(kPartThreads), 0, s, t, values, keys, vals, n,
                            rbk, nsuper, sc);
         hipLaunchKernelGGL((k_part2_scatter<K, M>), dim3(nsuper * chunks_per_super), dim3(kPartThreads), 0, s, t, rbk,
@@ -118,13 +139,26 @@ struct ops
                            &ctl->deferred);
         hipLaunchKernelGGL((k_insert_list<K, M, G, R>), dim3(1024), dim3(256), 0, s, t, sc.s_keys, sc.s_vals,
                            &ctl->spilled);
+        if (sampled)
+        {
+            // one-wavefront workgroups, as the per-key kernel runs best, at most 65536 of them: with one
+            // workgroup per 64 keys the launch that finds nothing to do cost a uniform 2^26-key insert
+            // 0.2 ms
+            const int64_t blocks = (n + 63) / 64;
+            hipLaunchKernelGGL((k_insert_per_key_if<K, M, G, R>), dim3(static_cast<uint32_t>(blocks < 65536 ? blocks : 65536)),
+                               dim3(64), 0, s, t, values, keys, vals, n, ctl);
+        }
         return launch_status("partitioned insert launch");
     }
 
     static int insert(htab* h, const void* values, const void* keys, const void* vals, int64_t n, uint8_t* ins,
                       hipStream_t s)
     {
-        if (use_regions(h, n, ins)) return insert_regions(h, values, keys, vals, n, s);
+        if (use_regions(h, n, ins))
+        {
+            const int rc = insert_regions(h, values, keys, vals, n, s);
+            if (rc != kRegionsDeclined) return rc;
+        }
         const int64_t grid = (n + h->block - 1) / h->block;
         const T t = T::from_view(h->view);
         const bool one_stripe = h->view.nstripes == 1;

@@ -44,6 +44,10 @@ constexpr int kPartThreads = 256;
 constexpr int kPartItems = 8;                             // elements per thread of a scatter chunk
 constexpr int kPartChunk = kPartThreads * kPartItems;     // 2048
 constexpr int kRegionThreads = 256;
+// the default insert takes the partitioned path only on tables larger than this (the Infinity Cache)
+constexpr uint64_t kRegionDefaultMinBytes = 256ULL << 20;
+// insert_regions' answer when the default choice could not get its scratch memory
+constexpr int kRegionsDeclined = 1;
 
 template <int G>
 __host__ __device__ constexpr uint32_t
@@ -58,6 +62,8 @@ struct region_ctl
     uint32_t super_fill[kMaxSuper]; // elements offered to each super-bin (may pass its capacity)
     uint32_t spilled;               // elements in the spill list
     uint32_t deferred;              // elements the region kernel left (in buffer A once pass 2 is done)
+    uint32_t per_key;               // nonzero: the launch's sample found copies of keys, so the per-key
+                                    // kernel inserts the whole launch and the partitioned kernels exit
 };
 
 struct region_scratch
@@ -147,6 +153,7 @@ k_region_reset(region_ctl* ctl, uint32_t* region_fill, uint32_t nregion_slots)
     {
         ctl->spilled = 0;
         ctl->deferred = 0;
+        ctl->per_key = 0;
     }
     for (uint32_t r = i; r < nregion_slots; r += gridDim.x * blockDim.x) region_fill[r] = 0;
 }
@@ -248,6 +255,7 @@ k_part1_scatter(table_ref<K, M> t, const void* __restrict__ values, const void* 
                 const void* __restrict__ vals, int64_t n, uint32_t rb, uint32_t nsuper, region_scratch sc)
 {
     __shared__ scatter_lds<K, M, kMaxSuper> s;
+    if (sc.ctl->per_key) return;
     const int64_t base = static_cast<int64_t>(blockIdx.x) * kPartChunk;
     K k[kPartItems];
     uint64_t v[kPartItems];
@@ -282,6 +290,7 @@ __global__ __launch_bounds__(kPartThreads) void
 k_part2_scatter(table_ref<K, M> t, uint32_t rb, uint32_t chunks_per_super, region_scratch sc)
 {
     __shared__ scatter_lds<K, M, kSubBins> s;
+    if (sc.ctl->per_key) return;
     const uint32_t d = blockIdx.x / chunks_per_super;
     const uint32_t c = blockIdx.x % chunks_per_super;
     const uint32_t f = sc.ctl->super_fill[d];
@@ -543,6 +552,7 @@ k_region_insert(table_ref<K, M> t, region_scratch sc)
     constexpr bool kInline = slot_traits<K, M>::ovf_in_bucket;
     constexpr uint32_t kPieces = static_cast<uint32_t>(kRegionBytes / 16 / kRegionThreads); // 16 B per thread each
     __shared__ L s;
+    if (sc.ctl->per_key) return;
     const uint32_t r = blockIdx.x;
     const uint32_t f = sc.region_fill[r];
     const uint32_t count = f < sc.cap2 ? f : sc.cap2;
@@ -670,6 +680,70 @@ k_insert_list(table_ref<K, M> t, const void* __restrict__ keys, const void* __re
         const int64_t j = base + static_cast<int64_t>(threadIdx.x) / per;
         const bool mine = (sparse ? (lane & (G - 1)) == 0 : true) && j < cnt;
         insert_body<K, M, G, R>(t, nullptr, keys, vals, j, mine, nullptr, false);
+    }
+}
+
+// Whether a default partitioned insert goes ahead: kSampleKeys keys spread evenly over the launch go
+// into a small LDS table, and when more than one in eight of them repeats one sampled before (a skewed
+// input: a few keys hold much of the launch and would overflow their bins) the launch is left to the
+// per-key kernel, whose compare-and-swaps on those keys hit in cache.  A uniform input of distinct keys
+// shows no repeat.
+constexpr uint32_t kSampleKeys = 1024;
+
+template <typename K, typename M>
+__global__ __launch_bounds__(256) void
+k_region_sample(const void* __restrict__ values, const void* __restrict__ keys, const void* __restrict__ vals,
+                int64_t n, region_ctl* ctl)
+{
+    constexpr uint32_t kSlots = 2 * kSampleKeys;
+    __shared__ uint64_t tab[kSlots];
+    __shared__ uint32_t repeats;
+    constexpr uint64_t kFree = ~static_cast<uint64_t>(0);
+    for (uint32_t i = threadIdx.x; i < kSlots; i += blockDim.x) tab[i] = kFree;
+    if (threadIdx.x == 0) repeats = 0;
+    __syncthreads();
+    const uint32_t ns = n < static_cast<int64_t>(kSampleKeys) ? static_cast<uint32_t>(n) : kSampleKeys;
+    uint32_t mine = 0;
+    for (uint32_t j = threadIdx.x; j < ns; j += blockDim.x)
+    {
+        // n < 2^31 on this path, so j * n fits
+        const int64_t e = static_cast<int64_t>(j) * n / ns;
+        K k;
+        uint64_t v;
+        fetch_element<K, M>(values, keys, vals, e, &k, &v);
+        const uint64_t x = static_cast<uint64_t>(k);
+        if (x == kFree) continue; // the table's own free mark: not sampled
+        uint32_t h = static_cast<uint32_t>(mix64(x)) & (kSlots - 1);
+        while (true)
+        {
+            const uint64_t got = lds_cas(&tab[h], kFree, x);
+            if (got == kFree) break;
+            if (got == x)
+            {
+                ++mine;
+                break;
+            }
+            h = (h + 1) & (kSlots - 1);
+        }
+    }
+    if (mine) atomicAdd(&repeats, mine);
+    __syncthreads();
+    if (threadIdx.x == 0) ctl->per_key = repeats * 8 > ns ? 1U : 0U;
+}
+
+// The whole launch by the per-key insert when the sample chose it (else nothing): a grid-stride loop,
+// so the launch that finds nothing to do stays small.
+template <typename K, typename M, int G, int R>
+__global__ __launch_bounds__(64) void
+k_insert_per_key_if(table_ref<K, M> t, const void* __restrict__ values, const void* __restrict__ keys,
+                    const void* __restrict__ vals, int64_t n, const region_ctl* __restrict__ ctl)
+{
+    if (ctl->per_key == 0) return;
+    const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+    for (int64_t base = static_cast<int64_t>(blockIdx.x) * blockDim.x; base < n; base += stride)
+    {
+        const int64_t i = base + threadIdx.x;
+        insert_body<K, M, G, R>(t, values, keys, vals, i, i < n, nullptr, false);
     }
 }
 

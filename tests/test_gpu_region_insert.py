@@ -201,3 +201,52 @@ def test_partitioned_full_size_sequence():
     tab.find(w["keys"], hits=hits, values_out=got)
     assert bool(hits.all()) and torch.equal(got, w["values"])
     sa.unordered_map_u64_u32.destroyDeviceObject(tab)
+
+
+def _sorted_contents(t):
+    k, v = t.contents()
+    o = torch.argsort(k)
+    return k[o], v[o]
+
+
+@pytest.mark.parametrize("hot", [False, True])
+def test_default_insert_chooses_by_size_and_sample(hot):
+    """Default tuning (insert_path 0) on a table past 256 MB (2^25 slots of 16 B): a launch of 0.6 of the
+    slots in distinct keys takes the partitioned path; the same launch with every other key replaced by
+    one hot key has 1 in 2 of its sampled keys repeat, so the sample hands it to the per-key kernel (the
+    partitioned kernels exit).  Either way the contents, values and size equal a twin table that
+    inserts with one compare-and-swap per key (insert_path 1), and a later re-insert of a fifth of the
+    slots (below the size rule: per-key) and finds agree too."""
+    from stdgpu_amd import workloads_device as wd
+
+    cap = 1 << 25
+    n = int(cap * 0.6)
+    keys = wd.stream(91, n, "cuda")
+    if hot:
+        keys[::2] = keys[1]
+    vals = wd.value_of(keys)
+    a = sa.unordered_map_u64_u32.createDeviceObject(cap)
+    b = sa.unordered_map_u64_u32.createDeviceObject(cap, sa.Tuning(insert_path=1))
+    assert a.info.physical_slots * 16 > (256 << 20)
+    for t in (a, b):
+        t.insert(keys, vals)
+    distinct = int(torch.unique(keys).numel())
+    assert a.size() == b.size() == distinct
+    ka, va = _sorted_contents(a)
+    kb, vb = _sorted_contents(b)
+    assert torch.equal(ka, kb) and torch.equal(va, vb)
+    assert a.valid() and b.valid()
+    more = wd.stream(92, cap // 5, "cuda")
+    for t in (a, b):
+        t.erase(keys[: n // 4])
+        t.insert(more, wd.value_of(more))
+    assert a.size() == b.size()
+    ka, va = _sorted_contents(a)
+    kb, vb = _sorted_contents(b)
+    assert torch.equal(ka, kb) and torch.equal(va, vb)
+    h, _ = a.find(keys)
+    h2, _ = b.find(keys)
+    assert torch.equal(h, h2)
+    assert a.valid()
+    for t in (a, b):
+        sa.unordered_map_u64_u32.destroyDeviceObject(t)
