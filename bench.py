@@ -24,17 +24,20 @@ HBM_PEAK_GBS = 8000.0
 # algorithmic bytes per operation (u64 -> u32 map): streamed input + streamed output + one
 # 64 B random sector per bucket probe (+ the 16 B slot write for insert)
 BYTES_PER_OP = {"insert": 16 + 64 + 16 + 1, "find": 8 + 1 + 4 + 64, "erase": 8 + 64 + 8 + 1, "reinsert": 16 + 64 + 16 + 1}
-# HBM bytes per launch measured with rocprofv3 on MI355X for the default workload (2^26 keys,
-# 8-slot buckets, 64-thread workgroups): FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000 KiB, the scales
+# HBM bytes per call measured with rocprofv3 on MI355X for the default workload (2^26 keys, 8-slot
+# buckets, 64-thread workgroups), summed over the kernels of the call (the 2^26-key insert is the
+# partitioned one: sample, two scatter passes, region kernel, lists): FETCH_SIZE x 2.000 + WRITE_SIZE
+# x 1.000 KiB, the scales
 # that make both counters read a 2 GiB copy kernel's bytes exactly (profiles/map_u64_u32_2e26_bulk_kernels.md)
 # 128 B lines a find reads per key (TCC misses per dispatch / keys), same profile and workload
-MEASURED_FIND_LINES = {(26, 8, 64): 84175740 / (1 << 26)}
+MEASURED_FIND_LINES = {(26, 8, 64): 84150492 / (1 << 26)}
 # The kernel sources those counters were read from: the figures are reported only while the
 # digest of these files equals MEASURED_SOURCES_SHA256 (a kernel change makes them null until the
 # profile is taken again).
 KERNEL_SOURCES = ["stdgpu_amd/csrc/htab_kernels.hpp", "stdgpu_amd/csrc/htab_ops.hpp", "stdgpu_amd/csrc/wave.hpp",
-                  "stdgpu_amd/csrc/kind_map_u64_u32.hip", "include/stdgpu_amd/htab_device.hpp"]
-MEASURED_SOURCES_SHA256 = "8febcbdfbba7718e215e5b7294bb25411eb4976035eba9f72545bd10415ff44a"
+                  "stdgpu_amd/csrc/kind_map_u64_u32.hip", "include/stdgpu_amd/htab_device.hpp",
+                  "stdgpu_amd/csrc/htab_region.hpp"]
+MEASURED_SOURCES_SHA256 = "171b515a05678caf164002a6fc204f1ced82d5db597b9947d5bbb9e01d5478c3"
 
 
 def sources_digest() -> str:
@@ -48,8 +51,8 @@ def sources_digest() -> str:
     return h.hexdigest()
 
 
-MEASURED_TRAFFIC = {(26, 8, 64): {"insert": (2 * 5250641 + 6874802) * 1024, "find": (2 * 5064418 + 327680) * 1024,
-                                  "erase": (2 * 1271982 + 1188533) * 1024, "reinsert": (2 * 1439857 + 1715444) * 1024}}
+MEASURED_TRAFFIC = {(26, 8, 64): {"insert": (2 * 1857449 + 3522316) * 1024, "find": (2 * 5063078 + 327680) * 1024,
+                                  "erase": (2 * 1271604 + 1178470) * 1024, "reinsert": (2 * 1438134 + 1715160) * 1024}}
 
 
 def parse():
