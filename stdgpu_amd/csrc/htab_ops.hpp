@@ -69,11 +69,11 @@ struct ops
         // bin capacities: a bin's expected share of uniformly hashed keys, plus 6 % (super-bins) or
         // 12.5 % (regions) and four standard deviations; what a bin cannot take spills
         const double per_region = static_cast<double>(n) / nreg;
-        const double e1 = per_region * kSubBins;
+        const double e1 = per_region * kSubBins / kSuperCopies;
         const uint32_t cap1 = static_cast<uint32_t>(std::ceil(e1 * 1.0625 + 4.0 * std::sqrt(e1) + 64.0));
         const uint32_t cap2 = static_cast<uint32_t>(std::ceil(per_region * 1.125 + 4.0 * std::sqrt(per_region) + 32.0));
         auto up = [](size_t x) { return (x + 255) / 256 * 256; };
-        const size_t na = static_cast<size_t>(nsuper) * cap1, nbb = nrs * cap2, ns = static_cast<size_t>(n);
+        const size_t na = static_cast<size_t>(nsuper) * kSuperCopies * cap1, nbb = nrs * cap2, ns = static_cast<size_t>(n);
         size_t off = up(sizeof(region_ctl));
         const size_t o_fill = off;
         off += up(nrs * 4);
@@ -130,8 +130,8 @@ struct ops
             hipLaunchKernelGGL((k_region_sample<K, M>), dim3(1), dim3(256), 0, s, values, keys, vals, n, ctl);
         hipLaunchKernelGGL((k_part1_scatter<K, M>), dim3(nchunks), dim3(kPartThreads), 0, s, t, values, keys, vals, n,
                            rbk, nsuper, sc);
-        hipLaunchKernelGGL((k_part2_scatter<K, M>), dim3(nsuper * chunks_per_super), dim3(kPartThreads), 0, s, t, rbk,
-                           chunks_per_super, sc);
+        hipLaunchKernelGGL((k_part2_scatter<K, M>), dim3(nsuper * kSuperCopies * chunks_per_super), dim3(kPartThreads), 0,
+                           s, t, rbk, chunks_per_super, nsuper, sc);
         hipLaunchKernelGGL((k_region_insert<K, M, G>), dim3(nreg), dim3(kRegionThreads), 0, s, t, sc);
         // what the regions could not place, what the bins could not take and the two reserved key
         // values, by the per-key kernel

@@ -40,6 +40,10 @@ constexpr int kRegionBytes = 32768;
 constexpr uint32_t kSubBins = 128;   // regions per super-bin (digit of the second pass)
 constexpr uint32_t kMaxSuper = 512;  // super-bins (digit of the first pass)
 constexpr uint32_t kMaxRegions = kSubBins * kMaxSuper;
+// copies of each super-bin: pass-1 chunk c fills copy c % kSuperCopies (a bin of its own, of 1 /
+// kSuperCopies of the super-bin's share), so no fill counter takes every chunk's reservation (one
+// counter per super-bin took 32,768 same-address atomics at 2^26 keys)
+constexpr uint32_t kSuperCopies = 8;
 constexpr int kPartThreads = 256;
 constexpr int kPartItems = 8;                             // elements per thread of a scatter chunk
 constexpr int kPartChunk = kPartThreads * kPartItems;     // 2048
@@ -59,7 +63,8 @@ region_buckets()
 // Device bookkeeping of one partitioned insert.
 struct region_ctl
 {
-    uint32_t super_fill[kMaxSuper]; // elements offered to each super-bin (may pass its capacity)
+    uint32_t super_fill[kMaxSuper * kSuperCopies]; // elements offered to each super-bin copy, copy-major
+                                                   // (may pass its capacity)
     uint32_t spilled;               // elements in the spill list
     uint32_t deferred;              // elements the region kernel left (in buffer A once pass 2 is done)
     uint32_t per_key;               // nonzero: the launch's sample found copies of keys, so the per-key
@@ -148,7 +153,7 @@ __global__ __launch_bounds__(256) void
 k_region_reset(region_ctl* ctl, uint32_t* region_fill, uint32_t nregion_slots)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < kMaxSuper) ctl->super_fill[i] = 0;
+    for (uint32_t j = i; j < kMaxSuper * kSuperCopies; j += gridDim.x * blockDim.x) ctl->super_fill[j] = 0;
     if (i == 0)
     {
         ctl->spilled = 0;
@@ -279,7 +284,9 @@ k_part1_scatter(table_ref<K, M> t, const void* __restrict__ values, const void* 
             ok[j] = false;
         }
     }
-    scatter_chunk<K, M, true>(s, k, v, ok, nsuper, t.nbuckets, rb, sc.ctl->super_fill, 0, sc.cap1, sc.a_keys,
+    const uint32_t copy = blockIdx.x % kSuperCopies;
+    scatter_chunk<K, M, true>(s, k, v, ok, nsuper, t.nbuckets, rb, sc.ctl->super_fill + copy * nsuper,
+                              static_cast<uint64_t>(copy) * nsuper, sc.cap1, sc.a_keys,
                               sc.a_vals, sc.ctl, sc);
 }
 
@@ -287,7 +294,7 @@ k_part1_scatter(table_ref<K, M> t, const void* __restrict__ values, const void* 
 // workgroup c takes chunk c % chunks_per_super of super-bin c / chunks_per_super
 template <typename K, typename M>
 __global__ __launch_bounds__(kPartThreads) void
-k_part2_scatter(table_ref<K, M> t, uint32_t rb, uint32_t chunks_per_super, region_scratch sc)
+k_part2_scatter(table_ref<K, M> t, uint32_t rb, uint32_t chunks_per_super, uint32_t nsuper, region_scratch sc)
 {
     __shared__ scatter_lds<K, M, kSubBins> s;
     if (sc.ctl->per_key) return;
@@ -309,8 +316,10 @@ k_part2_scatter(table_ref<K, M> t, uint32_t rb, uint32_t chunks_per_super, regio
         v[j] = 0;
         if (ok[j]) region_elem<K, M>::get(sc.a_keys, sc.a_vals, static_cast<uint64_t>(d) * sc.cap1 + e, &k[j], &v[j]);
     }
-    scatter_chunk<K, M, false>(s, k, v, ok, kSubBins, t.nbuckets, rb, sc.region_fill + d * kSubBins,
-                               static_cast<uint64_t>(d) * kSubBins, sc.cap2, sc.b_keys, sc.b_vals, sc.ctl, sc);
+    // bin d is copy d / nsuper of super-bin d % nsuper
+    const uint32_t sup = d % nsuper;
+    scatter_chunk<K, M, false>(s, k, v, ok, kSubBins, t.nbuckets, rb, sc.region_fill + sup * kSubBins,
+                               static_cast<uint64_t>(sup) * kSubBins, sc.cap2, sc.b_keys, sc.b_vals, sc.ctl, sc);
 }
 
 // ------------------------------------------------------------------ the region insert
