@@ -55,8 +55,9 @@ typedef struct htab_tuning
                                 * by table region and each region inserted in LDS (tables of at most
                                 * 65536 regions of 32 KB; the launch allocates its scratch the first
                                 * time, about 40 bytes per key for unordered_map<u64, u32>); 0 (default) = 2 for a launch of half
-                                * to all of the slots of a table larger than 256 MB (falling back to
-                                * 1 if its scratch cannot be allocated), else 1 */
+                                * to all of the slots of a table larger than 256 MB, unless a 1024-key
+                                * sample of the launch shows copies of keys (then 1, decided on the
+                                * device) or its scratch cannot be allocated (then 1), else 1 */
     int32_t caller_hashes;     /* nonzero: keys are placed by a 64-bit hash their caller computes (the C++
                                 * facade's hasher, for transparent lookups of struct keys), so only the
                                 * native per-thread calls (which take that hash) and htab_valid_hashed
